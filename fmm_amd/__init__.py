@@ -1,0 +1,281 @@
+"""fmm_amd -- the FMM evaluation hot path of tbetcke/fmm on one MI355X.
+
+Host mirror of the reference's evaluate/plan API (fmm-exec/src/fmm.rs):
+
+    fmm = FmmBuilder(p=8).gradients(True).max_points_per_leaf(128).build(sources, targets)
+    out = fmm.evaluate(charges)        # Output(potential, gradient)
+
+The octree and interaction lists are built on the CPU by the library's C++ host code
+(the role of octree + fmm-plan); every operator of the evaluation runs as hand-written
+HIP kernels for gfx950 behind the C ABI of include/fmm_amd.h.  There is no CPU
+evaluation path: without the built library or a GPU the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import FmmError, build  # noqa: F401
+
+PLAIN_F64 = 0
+MIXED = 1
+PRECISIONS = {"f64": PLAIN_F64, "plain": PLAIN_F64, "mixed": MIXED}
+KINDS = ("p2m", "m2m", "l2l", "m2l", "p2l", "l2p", "m2p", "p2p", "other")
+LISTS = {"near": 0, "v": 1, "w": 2, "x": 3}
+
+
+def device_count() -> int:
+    n = C.c_int32(0)
+    _lib.check(_lib.lib().fmmh_device_count(C.byref(n)))
+    return int(n.value)
+
+
+def tables(p: int):
+    """(M2M[8], L2L[8], M2L[316]) matrices A, output = A @ input (CONVENTIONS §3.12)."""
+    n = (p + 1) ** 2
+    a, b, c = np.zeros(8 * n * n), np.zeros(8 * n * n), np.zeros(316 * n * n)
+    _lib.check(_lib.lib().fmmh_tables(p, _lib.ptr(a), _lib.ptr(b), _lib.ptr(c)))
+    f = lambda x, k: x.reshape(k, n, n).transpose(0, 2, 1).copy()
+    return f(a, 8), f(b, 8), f(c, 316)
+
+
+class Plan:
+    """Tree, box index, lists and leaf-ordered points (one rank), built on the CPU."""
+
+    def __init__(self, sources, targets=None, leaf_points=64, max_level=16, near_rule=None):
+        src = np.ascontiguousarray(sources, dtype=np.float64).reshape(-1, 3)
+        tgt = src if targets is None else np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 3)
+        self.sources, self.targets = src, tgt
+        o = _lib.PlanOptions(int(leaf_points), int(max_level), -1.0, -1.0)
+        if near_rule is not None:
+            o.near_w, o.near_x = float(near_rule[0]), float(near_rule[1])
+        h = C.c_void_p()
+        _lib.check(_lib.lib().fmmh_plan_build(_lib.ptr(src), len(src), _lib.ptr(tgt), len(tgt), C.byref(o), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.lib().fmmh_plan_free(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def nlevels(self) -> int:
+        return _lib.lib().fmmh_plan_nlevels(self._h)
+
+    @property
+    def nleaves(self) -> int:
+        return _lib.lib().fmmh_plan_nleaves(self._h)
+
+    def domain(self):
+        box, side = np.zeros(6), C.c_double()
+        _lib.check(_lib.lib().fmmh_plan_domain(self._h, _lib.ptr(box), C.byref(side)))
+        return box, side.value
+
+    def box_keys(self, level: int) -> np.ndarray:
+        out = np.zeros(_lib.lib().fmmh_plan_nboxes(self._h, level), np.uint64)
+        if len(out):
+            _lib.check(_lib.lib().fmmh_plan_box_keys(self._h, level, _lib.ptr(out)))
+        return out
+
+    def leaf_keys(self) -> np.ndarray:
+        out = np.zeros(self.nleaves, np.uint64)
+        _lib.check(_lib.lib().fmmh_plan_leaf_keys(self._h, _lib.ptr(out)))
+        return out
+
+    def list(self, kind: str, level: int):
+        k = LISTS[kind]
+        L = _lib.lib()
+        total = L.fmmh_plan_list(self._h, k, level, None, None, None)
+        if total < 0:
+            raise FmmError(9, L.fmmh_last_error().decode())
+        if kind in ("near", "w"):
+            rows = int(np.sum((self.leaf_keys() & np.uint64(0x7FFF)) == level))
+        else:
+            rows = len(self.box_keys(level))
+        off = np.zeros(rows + 1, np.uint32)
+        ent = np.zeros(max(total, 1), np.uint32)
+        aux = np.zeros(max(total, 1), np.uint16)
+        L.fmmh_plan_list(self._h, k, level, _lib.ptr(off), _lib.ptr(ent), _lib.ptr(aux))
+        if kind == "v":
+            return off, ent[:total], aux[:total]
+        return off, ent[:total]
+
+    def point_leaves(self, targets=False) -> np.ndarray:
+        n = len(self.targets) if targets else len(self.sources)
+        out = np.zeros(n, np.uint32)
+        _lib.check(_lib.lib().fmmh_plan_point_leaves(self._h, int(targets), _lib.ptr(out)))
+        return out
+
+    def leaf_coordinates(self, targets=False):
+        n = len(self.targets) if targets else len(self.sources)
+        u, order = np.zeros((n, 3)), np.zeros(n, np.uint32)
+        _lib.check(_lib.lib().fmmh_plan_leaf_coordinates(self._h, int(targets), _lib.ptr(u), _lib.ptr(order)))
+        return u, order
+
+
+@dataclass
+class Output:
+    """fmm-exec's Output: potential[nt] and, with gradients, gradient[nt, 3]."""
+
+    potential: np.ndarray
+    gradient: Optional[np.ndarray]
+
+
+class Operator:
+    """The device operator: the level calls of FmmOperator (fmm-plan/src/operator.rs)
+    on the GPU, plus the evaluator's whole order in one call."""
+
+    def __init__(self, plan: Plan, p: int, precision="f64", gradients=True, device=0):
+        self.plan = plan
+        self.p = p
+        self.gradients = gradients
+        self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        o = _lib.OperatorOptions(int(p), self.precision, int(bool(gradients)), int(device))
+        h = C.c_void_p()
+        _lib.check(_lib.lib().fmmh_operator_create(plan.handle, C.byref(o), C.byref(h)))
+        self._h = h
+        self.ntargets = len(plan.targets)
+        self.nsources = len(plan.sources)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.lib().fmmh_operator_free(h)
+            self._h = None
+
+    def stats(self) -> dict:
+        s = _lib.Stats()
+        _lib.check(_lib.lib().fmmh_operator_stats(self._h, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in _lib.Stats._fields_}
+
+    def load_charges(self, charges):
+        if hasattr(charges, "data_ptr") and getattr(charges, "is_cuda", False):
+            _lib.check(_lib.lib().fmmh_load_charges(self._h, C.c_void_p(charges.data_ptr()), 1))
+            return
+        q = np.ascontiguousarray(charges, dtype=np.float64).reshape(-1)
+        if len(q) != self.nsources:
+            raise FmmError(9, f"expected {self.nsources} charges, got {len(q)}")
+        _lib.check(_lib.lib().fmmh_load_charges(self._h, _lib.ptr(q), 0))
+
+    def reset(self):
+        _lib.check(_lib.lib().fmmh_reset(self._h))
+
+    def call(self, kind: str, level: int):
+        _lib.check(getattr(_lib.lib(), "fmmh_" + kind)(self._h, int(level)))
+
+    def evaluate(self):
+        _lib.check(_lib.lib().fmmh_evaluate(self._h))
+
+    def read_output(self) -> Output:
+        phi = np.zeros(self.ntargets)
+        grad = np.zeros((self.ntargets, 3)) if self.gradients else None
+        _lib.check(_lib.lib().fmmh_read_output(self._h, _lib.ptr(phi), _lib.ptr(grad), 0))
+        return Output(phi, grad)
+
+    def read_output_device(self, phi, grad=None):
+        """Output pass into device buffers (e.g. torch tensors on cuda:0)."""
+        g = C.c_void_p(grad.data_ptr()) if grad is not None else None
+        _lib.check(_lib.lib().fmmh_read_output(self._h, C.c_void_p(phi.data_ptr()), g, 1))
+
+    def sync(self):
+        _lib.check(_lib.lib().fmmh_sync(self._h))
+
+    def kind_timing(self, enable=True):
+        _lib.check(_lib.lib().fmmh_kind_timing(self._h, int(enable)))
+
+    def kind_times(self):
+        ms = np.zeros(9)
+        n = np.zeros(9, np.int64)
+        _lib.check(_lib.lib().fmmh_kind_times(self._h, _lib.ptr(ms), _lib.ptr(n)))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KINDS)}
+
+
+class Fmm:
+    """An FMM built for fixed sources and targets (fmm-exec's Fmm)."""
+
+    def __init__(self, plan: Plan, operator: Operator):
+        self.plan = plan
+        self.operator = operator
+
+    def evaluate(self, charges) -> Output:
+        """Potential and gradient at the targets (1/(4 pi) included), caller order."""
+        op = self.operator
+        op.load_charges(charges)
+        op.reset()
+        op.evaluate()
+        return op.read_output()
+
+    def evaluate_levels(self, charges) -> Output:
+        """The same evaluation issued as the reference's level calls, one by one
+        (fmm-plan/src/operator.rs:408-425)."""
+        op, L = self.operator, self.plan.nlevels - 1
+        op.load_charges(charges)
+        op.reset()
+        for level in range(L, -1, -1):
+            op.call("p2m", level)
+            if level >= 1:
+                op.call("m2m", level - 1)
+        for level in range(1, L + 1):
+            for k in ("l2l", "m2l", "p2l"):
+                op.call(k, level)
+        for level in range(0, L + 1):
+            for k in ("l2p", "m2p", "p2p"):
+                op.call(k, level)
+        return op.read_output()
+
+
+class FmmBuilder:
+    """Settings of an FMM (fmm-exec/src/fmm.rs:1261-1760), built with build()."""
+
+    def __init__(self, p: int = 6):
+        self._p = p
+        self._gradients = True
+        self._max_level = 16
+        self._leaf_points = 64
+        self._precision = "f64"
+        self._device = 0
+        self._near = None
+
+    def gradients(self, on: bool):
+        self._gradients = bool(on)
+        return self
+
+    def max_level(self, level: int):
+        self._max_level = int(level)
+        return self
+
+    def max_points_per_leaf(self, n: int):
+        self._leaf_points = int(n)
+        return self
+
+    def precision(self, precision: str):
+        if precision not in PRECISIONS:
+            raise FmmError(3, f"unknown precision {precision!r}")
+        self._precision = precision
+        return self
+
+    def device(self, ordinal: int):
+        self._device = int(ordinal)
+        return self
+
+    def near_rule(self, crossovers):
+        """None (off) or (N_W, N_X): NearRuleChoice::Crossovers."""
+        self._near = crossovers
+        return self
+
+    def build_plan(self, sources, targets=None) -> Plan:
+        return Plan(sources, targets, self._leaf_points, self._max_level, self._near)
+
+    def build(self, sources, targets=None) -> Fmm:
+        plan = self.build_plan(sources, targets)
+        op = Operator(plan, self._p, self._precision, self._gradients, self._device)
+        return Fmm(plan, op)

@@ -1,0 +1,265 @@
+// capi.cpp -- the extern "C" boundary declared in include/fmm_amd.h.
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+
+#include "../../include/fmm_amd.h"
+#include "operator.hpp"
+#include "plan.hpp"
+
+struct fmmh_plan {
+  fmmh::Plan plan;
+};
+
+struct fmmh_operator {
+  std::unique_ptr<fmmh::DeviceOperator> op;
+  int64_t nlevels = 0;
+  int64_t nleaves = 0;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int32_t guard(F &&f) {
+  try {
+    f();
+    g_last_error.clear();
+    return FMMH_OK;
+  } catch (const fmmh::Error &e) {
+    g_last_error = e.what();
+    return static_cast<int32_t>(e.status);
+  } catch (const std::bad_alloc &) {
+    g_last_error = "host out of memory";
+    return FMMH_OUT_OF_MEMORY;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return FMMH_DEVICE;
+  }
+}
+
+void need(bool ok, const char *what) {
+  if (!ok) throw fmmh::Error(fmmh::kInvalidArgument, what);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fmmh_last_error(void) { return g_last_error.c_str(); }
+
+int32_t fmmh_abi_version(void) { return 1; }
+
+int32_t fmmh_plan_build(const double *sources, int64_t nsources, const double *targets, int64_t ntargets,
+                        const fmmh_plan_options *options, fmmh_plan **out) {
+  return guard([&] {
+    need(out != nullptr, "fmmh_plan_build: out is NULL");
+    need(nsources == 0 || sources != nullptr, "fmmh_plan_build: sources is NULL");
+    need(ntargets == 0 || targets != nullptr, "fmmh_plan_build: targets is NULL");
+    fmmh::PlanOptions o;
+    if (options) {
+      o.leaf_points = options->leaf_points;
+      o.max_level = options->max_level;
+      o.near_w = options->near_w;
+      o.near_x = options->near_x;
+    }
+    need(o.max_level >= 0 && o.max_level <= fmmh::kDeepest, "fmmh_plan_build: max_level must lie in 0..16");
+    auto p = std::make_unique<fmmh_plan>();
+    p->plan = fmmh::build_plan(sources, nsources, targets, ntargets, o);
+    *out = p.release();
+  });
+}
+
+void fmmh_plan_free(fmmh_plan *plan) { delete plan; }
+
+int32_t fmmh_plan_nlevels(const fmmh_plan *plan) { return plan ? plan->plan.nlevels : -1; }
+
+int64_t fmmh_plan_nboxes(const fmmh_plan *plan, int32_t level) {
+  if (!plan || level < 0 || level >= plan->plan.nlevels) return 0;
+  return plan->plan.nboxes(level);
+}
+
+int64_t fmmh_plan_nleaves(const fmmh_plan *plan) { return plan ? plan->plan.nleaves() : -1; }
+
+int32_t fmmh_plan_domain(const fmmh_plan *plan, double *box, double *side) {
+  return guard([&] {
+    need(plan && box && side, "fmmh_plan_domain: NULL argument");
+    std::memcpy(box, plan->plan.box.data(), 6 * sizeof(double));
+    *side = plan->plan.side;
+  });
+}
+
+int32_t fmmh_plan_box_keys(const fmmh_plan *plan, int32_t level, uint64_t *out) {
+  return guard([&] {
+    need(plan && out, "fmmh_plan_box_keys: NULL argument");
+    need(level >= 0 && level < plan->plan.nlevels, "fmmh_plan_box_keys: level out of range");
+    const auto &k = plan->plan.keys[level];
+    std::memcpy(out, k.data(), k.size() * sizeof(uint64_t));
+  });
+}
+
+int32_t fmmh_plan_leaf_keys(const fmmh_plan *plan, uint64_t *out) {
+  return guard([&] {
+    need(plan && out, "fmmh_plan_leaf_keys: NULL argument");
+    std::memcpy(out, plan->plan.leaf_key.data(), plan->plan.leaf_key.size() * sizeof(uint64_t));
+  });
+}
+
+int64_t fmmh_plan_list(const fmmh_plan *plan, int32_t kind, int32_t level, uint32_t *offsets, uint32_t *entries,
+                       uint16_t *aux) {
+  if (!plan || level < 0 || level >= plan->plan.nlevels || kind < 0 || kind > 3) {
+    g_last_error = "fmmh_plan_list: bad plan, kind or level";
+    return -1;
+  }
+  const fmmh::Plan &p = plan->plan;
+  const fmmh::Csr &c = kind == 0 ? p.near[level] : kind == 1 ? p.v[level] : kind == 2 ? p.w[level] : p.x[level];
+  if (offsets && !c.off.empty()) std::memcpy(offsets, c.off.data(), c.off.size() * sizeof(uint32_t));
+  if (entries && !c.ent.empty()) std::memcpy(entries, c.ent.data(), c.ent.size() * sizeof(uint32_t));
+  if (aux && kind == 1 && !c.aux.empty()) std::memcpy(aux, c.aux.data(), c.aux.size() * sizeof(uint16_t));
+  return static_cast<int64_t>(c.size());
+}
+
+int32_t fmmh_plan_point_leaves(const fmmh_plan *plan, int32_t targets, uint32_t *out) {
+  return guard([&] {
+    need(plan && out, "fmmh_plan_point_leaves: NULL argument");
+    const fmmh::Plan &p = plan->plan;
+    const auto &start = targets ? p.tgt_start : p.src_start;
+    const auto &order = targets ? p.tgt_order : p.src_order;
+    for (uint32_t l = 0; l < p.nleaves(); ++l)
+      for (uint32_t j = start[l]; j < start[l + 1]; ++j) out[order[j]] = l;
+  });
+}
+
+int32_t fmmh_plan_leaf_coordinates(const fmmh_plan *plan, int32_t targets, double *coords, uint32_t *order) {
+  return guard([&] {
+    need(plan != nullptr, "fmmh_plan_leaf_coordinates: NULL plan");
+    const fmmh::Plan &p = plan->plan;
+    const auto &u = targets ? p.ut : p.us;
+    const auto &o = targets ? p.tgt_order : p.src_order;
+    if (coords) std::memcpy(coords, u.data(), u.size() * sizeof(double));
+    if (order) std::memcpy(order, o.data(), o.size() * sizeof(uint32_t));
+  });
+}
+
+int32_t fmmh_tables(int32_t p, double *m2m, double *l2l, double *m2l) {
+  return guard([&] {
+    std::vector<double> a, b, c;
+    fmmh::build_tables(p, a, b, c);
+    if (m2m) std::memcpy(m2m, a.data(), a.size() * sizeof(double));
+    if (l2l) std::memcpy(l2l, b.data(), b.size() * sizeof(double));
+    if (m2l) std::memcpy(m2l, c.data(), c.size() * sizeof(double));
+  });
+}
+
+int32_t fmmh_device_count(int32_t *count) {
+  return guard([&] {
+    need(count != nullptr, "fmmh_device_count: NULL count");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+  });
+}
+
+int32_t fmmh_operator_create(const fmmh_plan *plan, const fmmh_operator_options *options, fmmh_operator **out) {
+  return guard([&] {
+    need(plan && out, "fmmh_operator_create: NULL argument");
+    fmmh::OperatorOptions o;
+    if (options) {
+      o.p = options->p;
+      o.precision = options->precision;
+      o.gradients = options->gradients;
+      o.device = options->device;
+    }
+    auto h = std::make_unique<fmmh_operator>();
+    h->op = std::make_unique<fmmh::DeviceOperator>(plan->plan, o);
+    h->nlevels = plan->plan.nlevels;
+    h->nleaves = plan->plan.nleaves();
+    *out = h.release();
+  });
+}
+
+void fmmh_operator_free(fmmh_operator *op) { delete op; }
+
+int32_t fmmh_operator_stats(const fmmh_operator *op, fmmh_stats *out) {
+  return guard([&] {
+    need(op && out, "fmmh_operator_stats: NULL argument");
+    out->nlevels = op->op->nlevels();
+    out->nleaves = op->nleaves;
+    out->nboxes = static_cast<int64_t>(op->op->nboxes_total());
+    out->p2p_pairs = op->op->p2p_pairs();
+    out->m2l_pairs = op->op->m2l_pairs();
+    out->m2l_slots = op->op->m2l_slots();
+    out->device_bytes = op->op->device_bytes();
+  });
+}
+
+int32_t fmmh_load_charges(fmmh_operator *op, const double *charges, int32_t on_device) {
+  return guard([&] {
+    need(op && charges, "fmmh_load_charges: NULL argument");
+    op->op->load_charges(charges, on_device != 0);
+  });
+}
+
+int32_t fmmh_reset(fmmh_operator *op) {
+  return guard([&] {
+    need(op != nullptr, "fmmh_reset: NULL operator");
+    op->op->reset();
+  });
+}
+
+#define FMMH_LEVEL_CALL(NAME)                                                      \
+  int32_t fmmh_##NAME(fmmh_operator *op, int32_t level) {                          \
+    return guard([&] {                                                             \
+      need(op != nullptr, "fmmh_" #NAME ": NULL operator");                        \
+      need(level >= 0 && level < op->nlevels, "fmmh_" #NAME ": level out of range"); \
+      op->op->NAME(level);                                                         \
+    });                                                                            \
+  }
+FMMH_LEVEL_CALL(p2m)
+FMMH_LEVEL_CALL(m2m)
+FMMH_LEVEL_CALL(m2l)
+FMMH_LEVEL_CALL(p2l)
+FMMH_LEVEL_CALL(l2l)
+FMMH_LEVEL_CALL(l2p)
+FMMH_LEVEL_CALL(m2p)
+FMMH_LEVEL_CALL(p2p)
+#undef FMMH_LEVEL_CALL
+
+int32_t fmmh_evaluate(fmmh_operator *op) {
+  return guard([&] {
+    need(op != nullptr, "fmmh_evaluate: NULL operator");
+    op->op->evaluate();
+  });
+}
+
+int32_t fmmh_read_output(fmmh_operator *op, double *phi, double *grad, int32_t on_device) {
+  return guard([&] {
+    need(op && phi, "fmmh_read_output: NULL argument");
+    op->op->read_output(phi, grad, on_device != 0);
+  });
+}
+
+int32_t fmmh_sync(fmmh_operator *op) {
+  return guard([&] {
+    need(op != nullptr, "fmmh_sync: NULL operator");
+    op->op->sync();
+  });
+}
+
+int32_t fmmh_kind_timing(fmmh_operator *op, int32_t enable) {
+  return guard([&] {
+    need(op != nullptr, "fmmh_kind_timing: NULL operator");
+    op->op->set_timing(enable != 0);
+  });
+}
+
+int32_t fmmh_kind_times(fmmh_operator *op, double *ms, int64_t *launches) {
+  return guard([&] {
+    need(op != nullptr, "fmmh_kind_times: NULL operator");
+    op->op->kind_times(ms, launches);
+  });
+}
+
+}  // extern "C"

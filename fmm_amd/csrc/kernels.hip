@@ -1,0 +1,662 @@
+// kernels.hip -- HIP kernels of the FMM evaluation for gfx950 (MI355X).
+//
+// Conventions (reference docs/CONVENTIONS.md): real storage of expansions, idx(n, m) =
+// n^2 + n + m (§3.6); scaled coefficients (§3.7); leaf-scaled point coordinates and
+// exact relative frames from integer cells (§3.13); the device P2P rules (§3.13
+// "Device kernels").
+//
+// Layout in HBM:
+//  - points in leaf order as structure of arrays x, y, z, q (sources) and x, y, z
+//    (targets); the points of a leaf are one contiguous range (src_start / tgt_start);
+//  - cells: 4 u32 (level, i, j, k) per leaf and per box;
+//  - expansions of every level in one array, box-major: box gid (level base + box index)
+//    at gid * nc, with nc = (p+1)^2 coefficients contiguous;
+//  - outputs phi^ and g^ in leaf order (f64), turned into phi and grad in caller order by
+//    the output pass.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+
+#include "kernels.hpp"
+
+namespace fmmh {
+namespace dev {
+
+__constant__ double c_inv_reg[kMaxCoefGrad];  // 1 / ((n + m)(n - m)) at idx(n, m), n >= m + 2
+__constant__ double c_inv_2m[kMaxDegree + 3];  // 1 / (2m)
+
+void upload_constants() {
+  double inv_reg[kMaxCoefGrad] = {0};
+  for (int n = 0; n <= kMaxDegree + 1; ++n)
+    for (int m = 0; m <= n; ++m)
+      if (n >= m + 2) inv_reg[n * n + n + m] = 1.0 / double((n + m) * (n - m));
+  double inv2m[kMaxDegree + 3] = {0};
+  for (int m = 1; m <= kMaxDegree + 2; ++m) inv2m[m] = 1.0 / (2.0 * m);
+  hipMemcpyToSymbol(HIP_SYMBOL(c_inv_reg), inv_reg, sizeof(inv_reg));
+  hipMemcpyToSymbol(HIP_SYMBOL(c_inv_2m), inv2m, sizeof(inv2m));
+}
+
+__device__ __forceinline__ int idx(int n, int m) { return n * n + n + m; }
+
+// ---- frames (CONVENTIONS §3.13 "Relative frames") -----------------------------------
+// c^(s|t) = (C_L(s) - C_L(t)) 2^(l_t - L) and r^(s|t) = 2^(l_t - l_s), L = max(l_s, l_t),
+// C_L = (2 i + 1) 2^(L - l): integers below 2^17 times powers of two, exact in f32/f64.
+template <typename T>
+__device__ __forceinline__ void relative_frame(const uint32_t *cs, const uint32_t *ct, T c[3], T &r) {
+  const int ls = int(cs[0]), lt = int(ct[0]);
+  const int L = ls > lt ? ls : lt;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int64_t a = int64_t(2 * cs[k + 1] + 1) << (L - ls);
+    const int64_t b = int64_t(2 * ct[k + 1] + 1) << (L - lt);
+    c[k] = T(ldexp(double(a - b), lt - L));
+  }
+  r = T(ldexp(1.0, lt - ls));
+}
+
+template <typename T>
+__device__ __forceinline__ T rsqrt_t(T x);
+
+template <>
+__device__ __forceinline__ float rsqrt_t<float>(float x) {
+  return __builtin_amdgcn_rsqf(x);
+}
+
+template <>
+__device__ __forceinline__ double rsqrt_t<double>(double x) {
+  // v_rsq_f64 estimate refined by two Newton steps y += y (1 - x y^2) / 2
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-x * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-x * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  return y;
+}
+
+// ---- fills and moves ----------------------------------------------------------------
+
+__global__ void k_fill(double *p, size_t n, double v) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_gather_charges(const double *q, const uint32_t *order, size_t n, double *q64, float *q32) {
+  for (size_t j = blockIdx.x * size_t(blockDim.x) + threadIdx.x; j < n; j += size_t(gridDim.x) * blockDim.x) {
+    const double v = q[order[j]];
+    q64[j] = v;
+    if (q32) q32[j] = float(v);
+  }
+}
+
+// Output pass (fmm-exec/src/fmm.rs:117-145; CONVENTIONS §3.13 "Output"): per leaf,
+// phi = phi^ / (4 pi r_t) and grad = g^ / (4 pi r_t^2) in f64, into caller order.
+__global__ void k_output(const uint32_t *tgt_start, const uint32_t *leaf_cell, const uint32_t *order,
+                         double side, const double *phi_hat, const double *grad_hat, double *phi,
+                         double *grad, int gradients) {
+  const uint32_t lf = blockIdx.x;
+  const uint32_t t0 = tgt_start[lf], t1 = tgt_start[lf + 1];
+  const double rt = ldexp(side, -int(leaf_cell[4 * lf]) - 1);
+  const double s1 = 4.0 * 3.141592653589793 * rt, s2 = s1 * rt;
+  for (uint32_t i = t0 + threadIdx.x; i < t1; i += blockDim.x) {
+    const uint32_t c = order[i];
+    phi[c] = phi_hat[i] / s1;
+    if (gradients) {
+      grad[3 * size_t(c) + 0] = grad_hat[3 * size_t(i) + 0] / s2;
+      grad[3 * size_t(c) + 1] = grad_hat[3 * size_t(i) + 1] / s2;
+      grad[3 * size_t(c) + 2] = grad_hat[3 * size_t(i) + 2] / s2;
+    }
+  }
+}
+
+// ---- P2P (CONVENTIONS §3.13 "Device kernels") ---------------------------------------
+// One workgroup per target leaf, TPT targets per thread in registers; every source leaf
+// of the near row is staged through LDS in tiles of P2P_TILE points, with
+// y^ = fma(r^, u_s, c^) formed once per source (u_s itself for s = t).  A pair adds
+// nothing when r^2 == 0: the select comes before the charge is touched.
+
+template <typename T, bool GRAD, int TPT>
+__global__ __launch_bounds__(P2P_THREADS) void k_p2p(uint32_t leaf_first, const uint32_t *near_off,
+                                                     const uint32_t *near_ent, const uint32_t *leaf_cell,
+                                                     const uint32_t *src_start, const uint32_t *tgt_start,
+                                                     const T *sx, const T *sy, const T *sz, const T *sq,
+                                                     const T *tx, const T *ty, const T *tz, double *phi,
+                                                     double *grad) {
+  __shared__ T ys[4][P2P_TILE];
+  const uint32_t lf = leaf_first + blockIdx.x;
+  const uint32_t t0 = tgt_start[lf], t1 = tgt_start[lf + 1];
+  if (t0 == t1) return;
+  const uint32_t *ct = leaf_cell + 4 * size_t(lf);
+  for (uint32_t tb = t0; tb < t1; tb += P2P_THREADS * TPT) {
+    T ux[TPT], uy[TPT], uz[TPT], ap[TPT], ax[TPT], ay[TPT], az[TPT];
+#pragma unroll
+    for (int a = 0; a < TPT; ++a) {
+      const uint32_t i = tb + threadIdx.x + a * P2P_THREADS;
+      const bool ok = i < t1;
+      ux[a] = ok ? tx[i] : T(0);
+      uy[a] = ok ? ty[i] : T(0);
+      uz[a] = ok ? tz[i] : T(0);
+      ap[a] = ax[a] = ay[a] = az[a] = T(0);
+    }
+    for (uint32_t e = near_off[lf]; e < near_off[lf + 1]; ++e) {
+      const uint32_t sl = near_ent[e];
+      const uint32_t s0 = src_start[sl], s1 = src_start[sl + 1];
+      T c[3], r;
+      const bool self = sl == lf;
+      relative_frame<T>(leaf_cell + 4 * size_t(sl), ct, c, r);
+      for (uint32_t sb = s0; sb < s1; sb += P2P_TILE) {
+        const uint32_t cnt = (s1 - sb) < uint32_t(P2P_TILE) ? (s1 - sb) : uint32_t(P2P_TILE);
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < cnt; j += P2P_THREADS) {
+          const uint32_t s = sb + j;
+          ys[0][j] = self ? sx[s] : fma(r, sx[s], c[0]);
+          ys[1][j] = self ? sy[s] : fma(r, sy[s], c[1]);
+          ys[2][j] = self ? sz[s] : fma(r, sz[s], c[2]);
+          ys[3][j] = sq[s];
+        }
+        __syncthreads();
+        for (uint32_t j = 0; j < cnt; ++j) {
+          const T yx = ys[0][j], yy = ys[1][j], yz = ys[2][j], q = ys[3][j];
+#pragma unroll
+          for (int a = 0; a < TPT; ++a) {
+            const T dx = ux[a] - yx, dy = uy[a] - yy, dz = uz[a] - yz;
+            const T r2 = dx * dx + dy * dy + dz * dz;
+            const T inv = r2 == T(0) ? T(0) : rsqrt_t<T>(r2);
+            const T qi = q * inv;
+            ap[a] += qi;
+            if (GRAD) {
+              const T qi3 = qi * (inv * inv);
+              ax[a] -= qi3 * dx;
+              ay[a] -= qi3 * dy;
+              az[a] -= qi3 * dz;
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < TPT; ++a) {
+      const uint32_t i = tb + threadIdx.x + a * P2P_THREADS;
+      if (i < t1) {
+        phi[i] += double(ap[a]);
+        if (GRAD) {
+          grad[3 * size_t(i) + 0] += double(ax[a]);
+          grad[3 * size_t(i) + 1] += double(ay[a]);
+          grad[3 * size_t(i) + 2] += double(az[a]);
+        }
+      }
+    }
+  }
+}
+
+// ---- expansions from points: P2M (regular, unit frame) and P2L (irregular) ----------
+// One workgroup per output box; the row's source leaves in order, their points in tiles
+// of EXP_TILE.  Phase 1: thread (point, order m) writes column m of q conj X(v) into
+// LDS; phase 2: each thread owns coefficient slots and adds the tile's points in order.
+
+template <bool IRREG>
+__device__ void column_harmonics(int p, double x, double y, double z, double q, int m, double *out) {
+  const double r2 = x * x + y * y + z * z;
+  double dr, di = 0.0;
+  if (IRREG) {
+    const double inv_r2 = 1.0 / r2;
+    dr = rsqrt_t<double>(r2);
+    for (int k = 1; k <= m; ++k) {
+      const double f = (2.0 * k - 1.0) * inv_r2;
+      const double nr = f * (x * dr - y * di), ni = f * (x * di + y * dr);
+      dr = nr;
+      di = ni;
+    }
+    double pr = 0.0, pi = 0.0, cr = dr, ci = di;
+    for (int n = m; n <= p; ++n) {
+      if (n > m) {
+        double nr, ni;
+        if (n == m + 1) {
+          const double f = (2.0 * m + 1.0) * z * inv_r2;
+          nr = f * cr;
+          ni = f * ci;
+        } else {
+          const double a = (2.0 * n - 1.0) * z, b = double((n - 1) * (n - 1) - m * m);
+          nr = (a * cr - b * pr) * inv_r2;
+          ni = (a * ci - b * pi) * inv_r2;
+        }
+        pr = cr;
+        pi = ci;
+        cr = nr;
+        ci = ni;
+      }
+      out[idx(n, m)] = q * cr;
+      if (m > 0) out[idx(n, -m)] = -q * ci;
+    }
+  } else {
+    dr = 1.0;
+    for (int k = 1; k <= m; ++k) {
+      const double f = c_inv_2m[k];
+      const double nr = f * (x * dr - y * di), ni = f * (x * di + y * dr);
+      dr = nr;
+      di = ni;
+    }
+    double pr = 0.0, pi = 0.0, cr = dr, ci = di;
+    for (int n = m; n <= p; ++n) {
+      if (n > m) {
+        double nr, ni;
+        if (n == m + 1) {
+          nr = z * cr;
+          ni = z * ci;
+        } else {
+          const double a = (2.0 * n - 1.0) * z, inv = c_inv_reg[idx(n, m)];
+          nr = (a * cr - r2 * pr) * inv;
+          ni = (a * ci - r2 * pi) * inv;
+        }
+        pr = cr;
+        pi = ci;
+        cr = nr;
+        ci = ni;
+      }
+      out[idx(n, m)] = q * cr;
+      if (m > 0) out[idx(n, -m)] = -q * ci;
+    }
+  }
+}
+
+template <bool IRREG>
+__global__ __launch_bounds__(EXP_THREADS) void k_expand(int p, const uint32_t *row_box, const uint32_t *row_cell,
+                                                        const uint32_t *row_off, const uint32_t *row_ent,
+                                                        const uint32_t *leaf_cell, const uint32_t *src_start,
+                                                        const double *sx, const double *sy, const double *sz,
+                                                        const double *sq, double *coeffs) {
+  __shared__ double H[EXP_TILE * kMaxCoef];
+  const int nc = (p + 1) * (p + 1);
+  const uint32_t row = blockIdx.x;
+  const uint32_t box = row_box[row];
+  constexpr int SLOTS = (kMaxCoef + EXP_THREADS - 1) / EXP_THREADS;
+  double acc[SLOTS];
+#pragma unroll
+  for (int s = 0; s < SLOTS; ++s) acc[s] = 0.0;
+  for (uint32_t e = row_off[row]; e < row_off[row + 1]; ++e) {
+    const uint32_t sl = row_ent[e];
+    const uint32_t s0 = src_start[sl], s1 = src_start[sl + 1];
+    double c[3] = {0.0, 0.0, 0.0}, r = 1.0;
+    // P2L from leaf s into box t: frame (c^(t|s), r^(t|s)), v = (u_s - c) / r
+    if (IRREG) relative_frame<double>(row_cell + 4 * size_t(row), leaf_cell + 4 * size_t(sl), c, r);
+    const double inv_r = 1.0 / r;
+    for (uint32_t sb = s0; sb < s1; sb += EXP_TILE) {
+      const int cnt = int((s1 - sb) < uint32_t(EXP_TILE) ? (s1 - sb) : uint32_t(EXP_TILE));
+      __syncthreads();
+      for (int task = threadIdx.x; task < cnt * (p + 1); task += EXP_THREADS) {
+        const int j = task / (p + 1), m = task % (p + 1);
+        const uint32_t s = sb + j;
+        double x = sx[s], y = sy[s], z = sz[s];
+        if (IRREG) {
+          x = (x - c[0]) * inv_r;
+          y = (y - c[1]) * inv_r;
+          z = (z - c[2]) * inv_r;
+        }
+        column_harmonics<IRREG>(p, x, y, z, sq[s], m, H + j * nc);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < SLOTS; ++s) {
+        const int slot = threadIdx.x + s * EXP_THREADS;
+        if (slot < nc)
+          for (int j = 0; j < cnt; ++j) acc[s] += H[j * nc + slot];
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < SLOTS; ++s) {
+    const int slot = threadIdx.x + s * EXP_THREADS;
+    if (slot < nc) coeffs[size_t(box) * nc + slot] += acc[s];
+  }
+}
+
+// ---- evaluation at targets: L2P (regular) and M2P (irregular) ------------------------
+// One workgroup per target leaf; every entry's coefficients go to LDS with the derived
+// coefficients of the gradient (the ladders of CONVENTIONS §3.11 "L2P and M2P" moved
+// onto the coefficients), then each thread walks the recursion of its target column by
+// column and contracts on the fly.  Regular: grad_k = sum_{n<p} G_k R_n, with
+// Gz_n^m = L_{n+1}^m, Gx_n^m = (L_{n+1}^{m+1} - L_{n+1}^{m-1})/2,
+// Gy_n^m = i (L_{n+1}^{m+1} + L_{n+1}^{m-1})/2.  Irregular: grad_k = sum_{1<=n<=p+1}
+// G_k I_n, with Gz_n^m = -M_{n-1}^m, Gx_n^m = (M_{n-1}^{m+1} - M_{n-1}^{m-1})/2,
+// Gy_n^m = i (M_{n-1}^{m+1} + M_{n-1}^{m-1})/2.
+
+__device__ __forceinline__ void read_order(const double *c, int n, int q, int pmax, double &a, double &b) {
+  if (n < 0 || n > pmax || q > n || -q > n) {
+    a = b = 0.0;
+    return;
+  }
+  if (q >= 0) {
+    a = c[idx(n, q)];
+    b = q > 0 ? c[idx(n, -q)] : 0.0;
+  } else {
+    const double s = ((-q) & 1) ? -1.0 : 1.0;
+    a = s * c[idx(n, -q)];
+    b = -s * c[idx(n, q)];
+  }
+}
+
+template <bool IRREG, bool GRAD>
+__global__ __launch_bounds__(EVAL_THREADS) void k_evaluate(int p, const uint32_t *row_leaf, const uint32_t *row_off,
+                                                           const uint32_t *row_box, const uint32_t *box_cell,
+                                                           const uint32_t *leaf_cell, const uint32_t *tgt_start,
+                                                           const double *tx, const double *ty, const double *tz,
+                                                           const double *coeffs, double *phi, double *grad) {
+  __shared__ double C[kMaxCoef];
+  __shared__ double G[3][kMaxCoefGrad];
+  const int nc = (p + 1) * (p + 1);
+  const uint32_t row = blockIdx.x;
+  const uint32_t lf = row_leaf[row];
+  const uint32_t t0 = tgt_start[lf], t1 = tgt_start[lf + 1];
+  if (t0 == t1) return;
+  const int pg = IRREG ? p + 1 : p - 1;  // degree of the gradient coefficients
+  const int ng = pg >= 0 ? (pg + 1) * (pg + 1) : 0;
+  for (uint32_t e = row_off[row]; e < row_off[row + 1]; ++e) {
+    const uint32_t box = row_box[e];
+    double c[3] = {0.0, 0.0, 0.0}, r = 1.0;
+    if (IRREG) relative_frame<double>(box_cell + 4 * size_t(box), leaf_cell + 4 * size_t(lf), c, r);
+    __syncthreads();
+    for (int k = threadIdx.x; k < nc; k += EVAL_THREADS) C[k] = coeffs[size_t(box) * nc + k];
+    __syncthreads();
+    if (GRAD)
+      for (int k = threadIdx.x; k < ng; k += EVAL_THREADS) {
+        int n = 0;
+        while ((n + 1) * (n + 1) <= k) ++n;
+        const int m = k - n * n - n;
+        if (m < 0) continue;  // slot -m is written with slot +m
+        const int src = IRREG ? n - 1 : n + 1;
+        double ar, ai, br, bi, zr, zi;
+        read_order(C, src, m + 1, p, ar, ai);
+        read_order(C, src, m - 1, p, br, bi);
+        read_order(C, src, m, p, zr, zi);
+        const double xr = 0.5 * (ar - br), xi = 0.5 * (ai - bi);
+        const double yr = -0.5 * (ai + bi), yi = 0.5 * (ar + br);
+        if (IRREG) {
+          zr = -zr;
+          zi = -zi;
+        }
+        G[0][idx(n, m)] = xr;
+        G[1][idx(n, m)] = yr;
+        G[2][idx(n, m)] = zr;
+        if (m > 0) {
+          G[0][idx(n, -m)] = xi;
+          G[1][idx(n, -m)] = yi;
+          G[2][idx(n, -m)] = zi;
+        }
+      }
+    __syncthreads();
+    const double inv_r = 1.0 / r;
+    for (uint32_t i = t0 + threadIdx.x; i < t1; i += EVAL_THREADS) {
+      double x = tx[i], y = ty[i], z = tz[i];
+      if (IRREG) {
+        x = (x - c[0]) * inv_r;
+        y = (y - c[1]) * inv_r;
+        z = (z - c[2]) * inv_r;
+      }
+      const double r2 = x * x + y * y + z * z;
+      const double inv_r2 = IRREG ? 1.0 / r2 : 0.0;
+      double sp = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+      double dr = IRREG ? rsqrt_t<double>(r2) : 1.0, di = 0.0;
+      const int ptop = IRREG ? (GRAD ? p + 1 : p) : p;
+      for (int m = 0; m <= ptop; ++m) {
+        if (m > 0) {
+          const double f = IRREG ? (2.0 * m - 1.0) * inv_r2 : c_inv_2m[m];
+          const double nr = f * (x * dr - y * di), ni = f * (x * di + y * dr);
+          dr = nr;
+          di = ni;
+        }
+        double pr = 0.0, pi = 0.0, cr = dr, ci = di;
+        const double w = m == 0 ? 1.0 : 2.0;
+        for (int n = m; n <= ptop; ++n) {
+          if (n > m) {
+            double nr, ni;
+            if (IRREG) {
+              if (n == m + 1) {
+                const double f = (2.0 * m + 1.0) * z * inv_r2;
+                nr = f * cr;
+                ni = f * ci;
+              } else {
+                const double a = (2.0 * n - 1.0) * z, b = double((n - 1) * (n - 1) - m * m);
+                nr = (a * cr - b * pr) * inv_r2;
+                ni = (a * ci - b * pi) * inv_r2;
+              }
+            } else {
+              if (n == m + 1) {
+                nr = z * cr;
+                ni = z * ci;
+              } else {
+                const double a = (2.0 * n - 1.0) * z, inv = c_inv_reg[idx(n, m)];
+                nr = (a * cr - r2 * pr) * inv;
+                ni = (a * ci - r2 * pi) * inv;
+              }
+            }
+            pr = cr;
+            pi = ci;
+            cr = nr;
+            ci = ni;
+          }
+          if (n <= p) {
+            const double lr = C[idx(n, m)], li = m > 0 ? C[idx(n, -m)] : 0.0;
+            sp += w * (lr * cr - li * ci);
+          }
+          if (GRAD && n <= pg && (!IRREG || n >= 1)) {
+            const int a = idx(n, m), b = idx(n, -m);
+            const double xr = G[0][a], yr = G[1][a], zr = G[2][a];
+            const double xi = m > 0 ? G[0][b] : 0.0, yi = m > 0 ? G[1][b] : 0.0, zi = m > 0 ? G[2][b] : 0.0;
+            sx += w * (xr * cr - xi * ci);
+            sy += w * (yr * cr - yi * ci);
+            sz += w * (zr * cr - zi * ci);
+          }
+        }
+      }
+      phi[i] += sp * inv_r;
+      if (GRAD) {
+        const double s2 = inv_r * inv_r;
+        grad[3 * size_t(i) + 0] += sx * s2;
+        grad[3 * size_t(i) + 1] += sy * s2;
+        grad[3 * size_t(i) + 2] += sz * s2;
+      }
+    }
+  }
+}
+
+// ---- grouped dense contraction on MFMA (M2M, L2L, M2L) -------------------------------
+// Task = COLS output columns (boxes), 8 waves of 16.  For every group of the task (a
+// table and one source box per column) the workgroup streams the table through LDS in
+// chunks of KC k-steps and every wave issues v_mfma_f64_16x16x4f64 over all M-tiles of
+// its 16 columns, accumulating in registers: the groups of a task are added in group
+// order, so every target adds its rows in the order of its list (offset index for M2L,
+// octant for M2M).  Table layout (swizzle_table): [chunk][m-tile][kk][lane] with
+// lane -> (row = 16 mt + lane % 16, k = kk + KC chunk + (lane / 16) KS), zero-padded;
+// the B operand of lane (g = lane / 16, j = lane % 16) is coefficient k of column j's
+// source.  Accumulator element r of lane (g, j) is row 4 g + r of the M-tile.
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+template <int MT>
+__global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, const uint32_t *task_group_off,
+                                                      const uint16_t *group_table, const uint32_t *group_src,
+                                                      const uint32_t *col_out, const double *__restrict__ tsw,
+                                                      const double *__restrict__ X, double *__restrict__ Y) {
+  constexpr int CHUNK = MT * KC * 64;
+  __shared__ double As[CHUNK];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int col = wave * 16 + (lane & 15), g = lane >> 4;
+  const int KS = nchunks * KC;
+  const uint32_t task = blockIdx.x;
+  const uint32_t out = col_out[size_t(task) * COLS + col];
+  d4 acc[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) acc[mt] = d4{0.0, 0.0, 0.0, 0.0};
+  const uint32_t g0 = task_group_off[task], g1 = task_group_off[task + 1];
+  for (uint32_t gi = g0; gi < g1; ++gi) {
+    const int t = group_table[gi];
+    const uint32_t src = group_src[size_t(gi) * COLS + col];
+    const double *xs = src != 0xFFFFFFFFu ? X + size_t(src) * nc : nullptr;
+    const double *tt = tsw + size_t(t) * nchunks * CHUNK;
+    for (int c = 0; c < nchunks; ++c) {
+      __syncthreads();
+      const double2 *gsrc = reinterpret_cast<const double2 *>(tt + size_t(c) * CHUNK);
+      double2 *ldst = reinterpret_cast<double2 *>(As);
+      for (int e = tid; e < CHUNK / 2; e += 512) ldst[e] = gsrc[e];
+      double b[KC];
+#pragma unroll
+      for (int kk = 0; kk < KC; ++kk) {
+        const int k = c * KC + kk + g * KS;
+        b[kk] = (xs != nullptr && k < nc) ? xs[k] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+          acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[(mt * KC + kk) * 64 + lane], b[kk], acc[mt], 0, 0, 0);
+    }
+  }
+  if (out != 0xFFFFFFFFu) {
+    double *y = Y + size_t(out) * nc;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = mt * 16 + 4 * g + r;
+        if (i < nc) y[i] += acc[mt][r];
+      }
+  }
+}
+
+// ---- host wrappers ---------------------------------------------------------------------
+
+static inline unsigned grid_for(size_t n, int threads) {
+  size_t g = (n + threads - 1) / threads;
+  if (g > 65535u * 16u) g = 65535u * 16u;
+  return unsigned(g == 0 ? 1 : g);
+}
+
+void fill(hipStream_t s, double *p, size_t n, double v) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
+}
+
+void gather_charges(hipStream_t s, const double *q, const uint32_t *order, size_t n, double *q64, float *q32) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_gather_charges, dim3(grid_for(n, 256)), dim3(256), 0, s, q, order, n, q64, q32);
+}
+
+void output_pass(hipStream_t s, uint32_t nleaves, const uint32_t *tgt_start, const uint32_t *leaf_cell,
+                 const uint32_t *order, double side, const double *phi_hat, const double *grad_hat,
+                 double *phi, double *grad, int gradients) {
+  if (nleaves == 0) return;
+  hipLaunchKernelGGL(k_output, dim3(nleaves), dim3(128), 0, s, tgt_start, leaf_cell, order, side, phi_hat,
+                     grad_hat, phi, grad, gradients);
+}
+
+template <typename T>
+void p2p(hipStream_t s, uint32_t leaf_first, uint32_t count, int gradients, const uint32_t *near_off,
+         const uint32_t *near_ent, const uint32_t *leaf_cell, const uint32_t *src_start,
+         const uint32_t *tgt_start, const T *sx, const T *sy, const T *sz, const T *sq, const T *tx,
+         const T *ty, const T *tz, double *phi, double *grad) {
+  if (count == 0) return;
+  if (gradients)
+    hipLaunchKernelGGL((k_p2p<T, true, 1>), dim3(count), dim3(P2P_THREADS), 0, s, leaf_first, near_off, near_ent,
+                       leaf_cell, src_start, tgt_start, sx, sy, sz, sq, tx, ty, tz, phi, grad);
+  else
+    hipLaunchKernelGGL((k_p2p<T, false, 1>), dim3(count), dim3(P2P_THREADS), 0, s, leaf_first, near_off, near_ent,
+                       leaf_cell, src_start, tgt_start, sx, sy, sz, sq, tx, ty, tz, phi, grad);
+}
+
+template void p2p<float>(hipStream_t, uint32_t, uint32_t, int, const uint32_t *, const uint32_t *, const uint32_t *,
+                         const uint32_t *, const uint32_t *, const float *, const float *, const float *,
+                         const float *, const float *, const float *, const float *, double *, double *);
+template void p2p<double>(hipStream_t, uint32_t, uint32_t, int, const uint32_t *, const uint32_t *,
+                          const uint32_t *, const uint32_t *, const uint32_t *, const double *, const double *,
+                          const double *, const double *, const double *, const double *, const double *,
+                          double *, double *);
+
+void expand(hipStream_t s, int p, int irregular, uint32_t rows, const uint32_t *row_box, const uint32_t *row_cell,
+            const uint32_t *row_off, const uint32_t *row_ent, const uint32_t *leaf_cell,
+            const uint32_t *src_start, const double *sx, const double *sy, const double *sz, const double *sq,
+            double *coeffs) {
+  if (rows == 0) return;
+  if (irregular)
+    hipLaunchKernelGGL(k_expand<true>, dim3(rows), dim3(EXP_THREADS), 0, s, p, row_box, row_cell, row_off, row_ent,
+                       leaf_cell, src_start, sx, sy, sz, sq, coeffs);
+  else
+    hipLaunchKernelGGL(k_expand<false>, dim3(rows), dim3(EXP_THREADS), 0, s, p, row_box, row_cell, row_off,
+                       row_ent, leaf_cell, src_start, sx, sy, sz, sq, coeffs);
+}
+
+void evaluate(hipStream_t s, int p, int irregular, int gradients, uint32_t rows, const uint32_t *row_leaf,
+              const uint32_t *row_off, const uint32_t *row_box, const uint32_t *box_cell,
+              const uint32_t *leaf_cell, const uint32_t *tgt_start, const double *tx, const double *ty,
+              const double *tz, const double *coeffs, double *phi, double *grad) {
+  if (rows == 0) return;
+#define FMMH_EVAL(I, G)                                                                                          \
+  hipLaunchKernelGGL((k_evaluate<I, G>), dim3(rows), dim3(EVAL_THREADS), 0, s, p, row_leaf, row_off, row_box,    \
+                     box_cell, leaf_cell, tgt_start, tx, ty, tz, coeffs, phi, grad)
+  if (irregular) {
+    if (gradients)
+      FMMH_EVAL(true, true);
+    else
+      FMMH_EVAL(true, false);
+  } else {
+    if (gradients)
+      FMMH_EVAL(false, true);
+    else
+      FMMH_EVAL(false, false);
+  }
+#undef FMMH_EVAL
+}
+
+static int m_tiles(int p) { return ((p + 1) * (p + 1) + 15) / 16; }
+static int k_chunks(int p) { return ((p + 1) * (p + 1) + 4 * KC - 1) / (4 * KC); }
+
+size_t swizzled_table_size(int p) { return size_t(k_chunks(p)) * m_tiles(p) * KC * 64; }
+
+void swizzle_table(int p, const double *a, double *out) {
+  const int nc = (p + 1) * (p + 1), MT = m_tiles(p), NCH = k_chunks(p), KS = NCH * KC;
+  for (int c = 0; c < NCH; ++c)
+    for (int mt = 0; mt < MT; ++mt)
+      for (int kk = 0; kk < KC; ++kk)
+        for (int lane = 0; lane < 64; ++lane) {
+          const int row = mt * 16 + (lane & 15);
+          const int k = c * KC + kk + (lane >> 4) * KS;
+          const double v = (row < nc && k < nc) ? a[row + size_t(k) * nc] : 0.0;
+          out[((size_t(c) * MT + mt) * KC + kk) * 64 + lane] = v;
+        }
+}
+
+void contract_f64(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off, const uint16_t *group_table,
+                  const uint32_t *group_src, const uint32_t *col_out, const double *tsw, const double *X,
+                  double *Y) {
+  if (ntasks == 0) return;
+  const int nc = (p + 1) * (p + 1), nch = k_chunks(p);
+#define FMMH_CONTRACT(MT)                                                                                        \
+  case MT:                                                                                                       \
+    hipLaunchKernelGGL(k_contract_f64<MT>, dim3(ntasks), dim3(512), 0, s, nc, nch, task_group_off, group_table,  \
+                       group_src, col_out, tsw, X, Y);                                                           \
+    break;
+  switch (m_tiles(p)) {
+    FMMH_CONTRACT(1)
+    FMMH_CONTRACT(2)
+    FMMH_CONTRACT(3)
+    FMMH_CONTRACT(4)
+    FMMH_CONTRACT(5)
+    FMMH_CONTRACT(6)
+    FMMH_CONTRACT(7)
+    FMMH_CONTRACT(8)
+    FMMH_CONTRACT(9)
+    FMMH_CONTRACT(10)
+    FMMH_CONTRACT(11)
+    FMMH_CONTRACT(12)
+    FMMH_CONTRACT(13)
+    FMMH_CONTRACT(14)
+    FMMH_CONTRACT(15)
+    FMMH_CONTRACT(16)
+    default:
+      break;
+  }
+#undef FMMH_CONTRACT
+}
+
+}  // namespace dev
+}  // namespace fmmh

@@ -1,0 +1,62 @@
+// kernels.hpp -- launch wrappers of the HIP kernels (kernels.hip) for the operator.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fmmh {
+namespace dev {
+
+constexpr int kMaxDegree = 15;  // device path: p <= 15, (p+1)^2 <= 256 coefficients
+constexpr int kMaxCoef = (kMaxDegree + 1) * (kMaxDegree + 1);
+constexpr int kMaxCoefGrad = (kMaxDegree + 2) * (kMaxDegree + 2);
+constexpr int P2P_THREADS = 128;
+constexpr int P2P_TILE = 128;
+constexpr int EXP_THREADS = 64;
+constexpr int EXP_TILE = 16;
+constexpr int EVAL_THREADS = 128;
+constexpr int COLS = 128;  // output columns (boxes) per contraction task: 8 waves x 16
+constexpr int KC = 4;      // MFMA k-steps per LDS chunk of a table
+
+void upload_constants();
+
+void fill(hipStream_t s, double *p, size_t n, double v);
+void gather_charges(hipStream_t s, const double *q, const uint32_t *order, size_t n, double *q64, float *q32);
+void output_pass(hipStream_t s, uint32_t nleaves, const uint32_t *tgt_start, const uint32_t *leaf_cell,
+                 const uint32_t *order, double side, const double *phi_hat, const double *grad_hat,
+                 double *phi, double *grad, int gradients);
+
+// P2P over the leaves leaf_first .. leaf_first + count (global leaf indices).
+template <typename T>
+void p2p(hipStream_t s, uint32_t leaf_first, uint32_t count, int gradients, const uint32_t *near_off,
+         const uint32_t *near_ent, const uint32_t *leaf_cell, const uint32_t *src_start,
+         const uint32_t *tgt_start, const T *sx, const T *sy, const T *sz, const T *sq, const T *tx,
+         const T *ty, const T *tz, double *phi, double *grad);
+
+// P2M (irregular = 0, row_ent the leaf of each row, unit frame) or P2L (irregular = 1,
+// row_ent the X-list leaves).  rows: output box gid and cell per row, CSR of entries.
+void expand(hipStream_t s, int p, int irregular, uint32_t rows, const uint32_t *row_box,
+            const uint32_t *row_cell, const uint32_t *row_off, const uint32_t *row_ent,
+            const uint32_t *leaf_cell, const uint32_t *src_start, const double *sx, const double *sy,
+            const double *sz, const double *sq, double *coeffs);
+
+// L2P (irregular = 0: one entry per row, the leaf's own box, unit frame) or M2P
+// (irregular = 1: the W-list boxes with their cells) at the targets of each row's leaf.
+void evaluate(hipStream_t s, int p, int irregular, int gradients, uint32_t rows, const uint32_t *row_leaf,
+              const uint32_t *row_off, const uint32_t *row_box, const uint32_t *box_cell,
+              const uint32_t *leaf_cell, const uint32_t *tgt_start, const double *tx, const double *ty,
+              const double *tz, const double *coeffs, double *phi, double *grad);
+
+// Grouped dense contraction on MFMA (f64): for every task (COLS output columns) and
+// every group of the task, Y[out[c]] += T_group X[src[g, c]].
+void contract_f64(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off,
+                  const uint16_t *group_table, const uint32_t *group_src, const uint32_t *col_out,
+                  const double *tables_swizzled, const double *X, double *Y);
+
+// Swizzled table size (doubles) per table at degree p, and the host-side swizzle.
+size_t swizzled_table_size(int p);
+void swizzle_table(int p, const double *table_colmajor, double *out);
+
+}  // namespace dev
+}  // namespace fmmh

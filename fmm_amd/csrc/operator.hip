@@ -1,0 +1,566 @@
+// operator.hip -- the device operator (see operator.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cstring>
+#include <map>
+
+#include "kernels.hpp"
+#include "operator.hpp"
+
+namespace fmmh {
+
+void hip_check(hipError_t e, const char *what) {
+  if (e == hipSuccess) return;
+  const Status s = (e == hipErrorOutOfMemory) ? kOutOfMemory : kDeviceError;
+  throw Error(s, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+
+template <typename T>
+std::vector<T> to_type(const std::vector<double> &v) {
+  return std::vector<T>(v.begin(), v.end());
+}
+
+// Builds one contraction schedule from target columns and their (group, source) lists.
+// cols: output gid per column in task order, COLS per task (kNone = padding);
+// groups_of(c): ascending (table, source gid) pairs of column c.
+struct ScheduleBuilder {
+  std::vector<uint32_t> task_group_off{0}, group_src, col_out;
+  std::vector<uint16_t> group_table;
+  uint64_t pairs = 0, slots = 0;
+
+  void add_task(const std::vector<uint32_t> &cols, const std::vector<std::vector<std::pair<uint16_t, uint32_t>>> &lists) {
+    const int C = dev::COLS;
+    std::map<uint16_t, std::array<uint32_t, dev::COLS>> by_table;
+    for (int c = 0; c < C; ++c) {
+      col_out.push_back(c < int(cols.size()) ? cols[c] : kNone);
+      if (c >= int(cols.size())) continue;
+      for (const auto &e : lists[c]) {
+        auto it = by_table.find(e.first);
+        if (it == by_table.end()) {
+          std::array<uint32_t, dev::COLS> a;
+          a.fill(kNone);
+          it = by_table.emplace(e.first, a).first;
+        }
+        it->second[c] = e.second;
+        ++pairs;
+      }
+    }
+    for (const auto &kv : by_table) {
+      group_table.push_back(kv.first);
+      group_src.insert(group_src.end(), kv.second.begin(), kv.second.end());
+      slots += C;
+    }
+    task_group_off.push_back(static_cast<uint32_t>(group_table.size()));
+  }
+
+  void finish(Schedule &s) {
+    s.ntasks = static_cast<uint32_t>(task_group_off.size() - 1);
+    s.pairs = pairs;
+    s.slots = slots;
+    s.task_group_off.upload(task_group_off);
+    s.group_src.upload(group_src);
+    s.col_out.upload(col_out);
+    s.group_table.upload(group_table);
+  }
+};
+
+void upload_rows(Rows &r, const std::vector<uint32_t> &id, const std::vector<uint32_t> &cell,
+                 const std::vector<uint32_t> &off, const std::vector<uint32_t> &ent) {
+  r.rows = static_cast<uint32_t>(id.size());
+  r.row_id.upload(id);
+  r.row_cell.upload(cell);
+  r.row_off.upload(off);
+  r.row_ent.upload(ent);
+}
+
+}  // namespace
+
+DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options) : opt_(options) {
+  require(opt_.p >= 0 && opt_.p <= dev::kMaxDegree, "the device path supports 0 <= p <= 15", kUnsupportedLayout);
+  require(opt_.precision == kPlainF64 || opt_.precision == kMixed, "unknown precision", kUnsupportedPrecision);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(kNoDevice, "no HIP device");
+  require(opt_.device >= 0 && opt_.device < ndev, "device ordinal out of range", kNoDevice);
+  hip_check(hipSetDevice(opt_.device), "hipSetDevice");
+  hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  dev::upload_constants();
+
+  nlevels_ = plan.nlevels;
+  nc_ = ncoef(opt_.p);
+  side_ = plan.side;
+  ns_ = plan.ns;
+  nt_ = plan.nt;
+  nleaves_ = plan.nleaves();
+  leaf_off_ = plan.leaf_off;
+  box_base_.assign(nlevels_ + 1, 0);
+  for (int l = 0; l < nlevels_; ++l) box_base_[l + 1] = box_base_[l] + plan.nboxes(l);
+  nbox_total_ = box_base_[nlevels_];
+  auto gid = [&](int level, uint32_t b) { return box_base_[level] + b; };
+
+  // cells of leaves and boxes
+  std::vector<uint32_t> leaf_cell(4 * size_t(nleaves_)), box_cell(4 * nbox_total_);
+  for (uint32_t i = 0; i < nleaves_; ++i) {
+    const Cell c = decode(plan.leaf_key[i]);
+    leaf_cell[4 * i] = c.level;
+    leaf_cell[4 * i + 1] = uint32_t(c.i);
+    leaf_cell[4 * i + 2] = uint32_t(c.j);
+    leaf_cell[4 * i + 3] = uint32_t(c.k);
+  }
+  for (int l = 0; l < nlevels_; ++l)
+    for (uint32_t b = 0; b < plan.nboxes(l); ++b) {
+      const Cell c = decode(plan.keys[l][b]);
+      const size_t g = gid(l, b);
+      box_cell[4 * g] = c.level;
+      box_cell[4 * g + 1] = uint32_t(c.i);
+      box_cell[4 * g + 2] = uint32_t(c.j);
+      box_cell[4 * g + 3] = uint32_t(c.k);
+    }
+  leaf_cell_.upload(leaf_cell);
+  box_cell_.upload(box_cell);
+  src_start_.upload(plan.src_start);
+  tgt_start_.upload(plan.tgt_start);
+  src_order_.upload(plan.src_order);
+  tgt_order_.upload(plan.tgt_order);
+
+  // near rows of every leaf, concatenated in leaf order
+  std::vector<uint32_t> near_off(nleaves_ + 1, 0), near_ent;
+  for (int l = 0; l < nlevels_; ++l) {
+    const Csr &N = plan.near[l];
+    for (uint32_t r = 0; r < N.rows(); ++r) {
+      const uint32_t lf = plan.leaf_off[l] + r;
+      near_off[lf + 1] = N.off[r + 1] - N.off[r];
+      for (uint32_t e = N.off[r]; e < N.off[r + 1]; ++e) {
+        near_ent.push_back(N.ent[e]);
+        const uint32_t sl = N.ent[e];
+        p2p_pairs_ += uint64_t(plan.src_start[sl + 1] - plan.src_start[sl]) *
+                      (plan.tgt_start[lf + 1] - plan.tgt_start[lf]);
+      }
+    }
+  }
+  for (uint32_t i = 0; i < nleaves_; ++i) near_off[i + 1] += near_off[i];
+  near_off_.upload(near_off);
+  near_ent_.upload(near_ent);
+
+  // points in leaf order
+  std::vector<double> sx(ns_), sy(ns_), sz(ns_), tx(nt_), ty(nt_), tz(nt_);
+  for (int64_t i = 0; i < ns_; ++i) {
+    sx[i] = plan.us[3 * i];
+    sy[i] = plan.us[3 * i + 1];
+    sz[i] = plan.us[3 * i + 2];
+  }
+  for (int64_t i = 0; i < nt_; ++i) {
+    tx[i] = plan.ut[3 * i];
+    ty[i] = plan.ut[3 * i + 1];
+    tz[i] = plan.ut[3 * i + 2];
+  }
+  sx_.upload(sx);
+  sy_.upload(sy);
+  sz_.upload(sz);
+  tx_.upload(tx);
+  ty_.upload(ty);
+  tz_.upload(tz);
+  q_.alloc(ns_);
+  q_in_.alloc(ns_);
+  if (opt_.precision == kMixed) {
+    // the f32 points are the casts of the f64 leaf-scaled values (CONVENTIONS §3.13)
+    sx32_.upload(to_type<float>(sx));
+    sy32_.upload(to_type<float>(sy));
+    sz32_.upload(to_type<float>(sz));
+    tx32_.upload(to_type<float>(tx));
+    ty32_.upload(to_type<float>(ty));
+    tz32_.upload(to_type<float>(tz));
+    q32_.alloc(ns_);
+  }
+
+  // expansions and outputs
+  mult_.alloc(nbox_total_ * nc_);
+  loc_.alloc(nbox_total_ * nc_);
+  phi_hat_.alloc(nt_);
+  grad_hat_.alloc(opt_.gradients ? 3 * nt_ : 0);
+  phi_.alloc(nt_);
+  grad_.alloc(opt_.gradients ? 3 * nt_ : 0);
+
+  // tables, swizzled for the MFMA contraction
+  {
+    std::vector<double> m2m, l2l, m2l;
+    build_tables(opt_.p, m2m, l2l, m2l);
+    const size_t n2 = size_t(nc_) * nc_, sw = dev::swizzled_table_size(opt_.p);
+    auto swz = [&](const std::vector<double> &t, int count, DevBuf<double> &dst) {
+      std::vector<double> out(sw * count);
+      for (int i = 0; i < count; ++i) dev::swizzle_table(opt_.p, t.data() + i * n2, out.data() + i * sw);
+      dst.upload(out);
+    };
+    swz(m2m, 8, t_m2m_);
+    swz(l2l, 8, t_l2l_);
+    if (nlevels_ > 2) swz(m2l, 316, t_m2l_);
+  }
+
+  // leaf-kernel rows
+  {
+    std::vector<uint32_t> id, cell, off{0}, ent;
+    p2m_level_.assign(nlevels_ + 1, 0);
+    for (int l = 0; l < nlevels_; ++l) {
+      for (uint32_t lf = plan.leaf_off[l]; lf < plan.leaf_off[l + 1]; ++lf) {
+        if (plan.src_start[lf + 1] == plan.src_start[lf]) continue;
+        id.push_back(gid(l, plan.leaf_box[lf]));
+        cell.insert(cell.end(), {leaf_cell[4 * lf], leaf_cell[4 * lf + 1], leaf_cell[4 * lf + 2], leaf_cell[4 * lf + 3]});
+        ent.push_back(lf);
+        off.push_back(static_cast<uint32_t>(ent.size()));
+      }
+      p2m_level_[l + 1] = static_cast<uint32_t>(id.size());
+    }
+    upload_rows(p2m_rows_, id, cell, off, ent);
+  }
+  {
+    std::vector<uint32_t> id, cell, off{0}, ent;
+    l2p_level_.assign(nlevels_ + 1, 0);
+    for (int l = 0; l < nlevels_; ++l) {
+      for (uint32_t lf = plan.leaf_off[l]; lf < plan.leaf_off[l + 1]; ++lf) {
+        if (plan.tgt_start[lf + 1] == plan.tgt_start[lf]) continue;
+        id.push_back(lf);
+        ent.push_back(gid(l, plan.leaf_box[lf]));
+        off.push_back(static_cast<uint32_t>(ent.size()));
+      }
+      l2p_level_[l + 1] = static_cast<uint32_t>(id.size());
+    }
+    upload_rows(l2p_rows_, id, cell, off, ent);
+  }
+  {
+    std::vector<uint32_t> id, cell, off{0}, ent;
+    m2p_level_.assign(nlevels_ + 1, 0);
+    for (int l = 0; l < nlevels_; ++l) {
+      const Csr &W = plan.w[l];
+      for (uint32_t r = 0; r < W.rows(); ++r) {
+        const uint32_t lf = plan.leaf_off[l] + r;
+        if (W.off[r + 1] == W.off[r] || plan.tgt_start[lf + 1] == plan.tgt_start[lf]) continue;
+        id.push_back(lf);
+        for (uint32_t e = W.off[r]; e < W.off[r + 1]; ++e) ent.push_back(gid(l + 1, W.ent[e]));
+        off.push_back(static_cast<uint32_t>(ent.size()));
+      }
+      m2p_level_[l + 1] = static_cast<uint32_t>(id.size());
+    }
+    upload_rows(m2p_rows_, id, cell, off, ent);
+  }
+  p2l_rows_.resize(nlevels_);
+  for (int l = 1; l < nlevels_; ++l) {
+    std::vector<uint32_t> id, cell, off{0}, ent;
+    const Csr &X = plan.x[l];
+    for (uint32_t b = 0; b < X.rows(); ++b) {
+      if (X.off[b + 1] == X.off[b]) continue;
+      const size_t g = gid(l, b);
+      id.push_back(uint32_t(g));
+      cell.insert(cell.end(), {box_cell[4 * g], box_cell[4 * g + 1], box_cell[4 * g + 2], box_cell[4 * g + 3]});
+      for (uint32_t e = X.off[b]; e < X.off[b + 1]; ++e) ent.push_back(X.ent[e]);
+      off.push_back(static_cast<uint32_t>(ent.size()));
+    }
+    upload_rows(p2l_rows_[l], id, cell, off, ent);
+  }
+  build_schedules(plan);
+  hip_check(hipDeviceSynchronize(), "operator build");
+}
+
+DeviceOperator::~DeviceOperator() {
+  for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void DeviceOperator::set_timing(bool enable) {
+  sync();
+  timing_ = enable;
+  if (enable && events_.empty()) {
+    events_.resize(1024);
+    for (auto &e : events_) hip_check(hipEventCreate(&e), "hipEventCreate");
+  }
+  marks_.clear();
+  marks_.reserve(events_.size() / 2);
+  events_used_ = 0;
+  for (int k = 0; k < 16; ++k) {
+    kind_ms_[k] = 0.0;
+    kind_launches_[k] = 0;
+  }
+}
+
+template <class F>
+void DeviceOperator::timed(int kind, F &&f) {
+  if (!timing_ || events_used_ + 2 > events_.size()) {
+    f();
+    return;
+  }
+  Mark m{kind, events_[events_used_], events_[events_used_ + 1]};
+  events_used_ += 2;
+  hip_check(hipEventRecord(m.a, stream_), "hipEventRecord");
+  f();
+  hip_check(hipEventRecord(m.b, stream_), "hipEventRecord");
+  marks_.push_back(m);
+}
+
+void DeviceOperator::collect_timings() {
+  for (const Mark &m : marks_) {
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, m.a, m.b), "hipEventElapsedTime");
+    kind_ms_[m.kind] += ms;
+    kind_launches_[m.kind] += 1;
+  }
+  marks_.clear();
+  events_used_ = 0;
+}
+
+void DeviceOperator::kind_times(double *ms, int64_t *launches) {
+  sync();
+  for (int k = 0; k < 9; ++k) {
+    if (ms) ms[k] = kind_ms_[k];
+    if (launches) launches[k] = kind_launches_[k];
+  }
+}
+
+void DeviceOperator::build_schedules(const Plan &plan) {
+  auto gid = [&](int level, uint32_t b) { return box_base_[level] + b; };
+  const int C = dev::COLS;
+  m2m_.resize(nlevels_);
+  l2l_.resize(nlevels_);
+  m2l_.resize(nlevels_);
+  for (int l = 0; l < nlevels_; ++l) {
+    // M2M into the parents of level l, by octant (fmm-plan/src/operator.rs:256-262)
+    if (l + 1 < nlevels_) {
+      ScheduleBuilder sb;
+      std::vector<uint32_t> cols;
+      std::vector<std::vector<std::pair<uint16_t, uint32_t>>> lists;
+      for (uint32_t b = 0; b < plan.nboxes(l); ++b) {
+        const uint32_t fc = plan.first_child[l][b];
+        if (fc == kNone) continue;
+        cols.push_back(gid(l, b));
+        std::vector<std::pair<uint16_t, uint32_t>> li;
+        for (int o = 0; o < 8; ++o) li.push_back({uint16_t(o), gid(l + 1, fc + o)});
+        lists.push_back(std::move(li));
+        if (int(cols.size()) == C) {
+          sb.add_task(cols, lists);
+          cols.clear();
+          lists.clear();
+        }
+      }
+      if (!cols.empty()) sb.add_task(cols, lists);
+      sb.finish(m2m_[l]);
+    }
+    if (l >= 1) {
+      // L2L into the boxes of level l, grouped by octant so a task shares one table
+      ScheduleBuilder sb;
+      for (int o = 0; o < 8; ++o) {
+        std::vector<uint32_t> cols;
+        std::vector<std::vector<std::pair<uint16_t, uint32_t>>> lists;
+        for (uint32_t b = 0; b < plan.nboxes(l); ++b) {
+          if (octant_of(plan.keys[l][b]) != o) continue;
+          cols.push_back(gid(l, b));
+          lists.push_back({{uint16_t(o), gid(l - 1, plan.parent[l][b])}});
+          if (int(cols.size()) == C) {
+            sb.add_task(cols, lists);
+            cols.clear();
+            lists.clear();
+          }
+        }
+        if (!cols.empty()) sb.add_task(cols, lists);
+      }
+      sb.finish(l2l_[l]);
+    }
+    if (l >= 2) {
+      // M2L: targets of one octant share the 189 offsets their parity allows
+      ScheduleBuilder sb;
+      const Csr &V = plan.v[l];
+      for (int o = 0; o < 8; ++o) {
+        std::vector<uint32_t> cols;
+        std::vector<std::vector<std::pair<uint16_t, uint32_t>>> lists;
+        for (uint32_t b = 0; b < plan.nboxes(l); ++b) {
+          if (octant_of(plan.keys[l][b]) != o || V.off[b + 1] == V.off[b]) continue;
+          cols.push_back(gid(l, b));
+          std::vector<std::pair<uint16_t, uint32_t>> li;
+          for (uint32_t e = V.off[b]; e < V.off[b + 1]; ++e) li.push_back({V.aux[e], gid(l, V.ent[e])});
+          lists.push_back(std::move(li));
+          if (int(cols.size()) == C) {
+            sb.add_task(cols, lists);
+            cols.clear();
+            lists.clear();
+          }
+        }
+        if (!cols.empty()) sb.add_task(cols, lists);
+      }
+      sb.finish(m2l_[l]);
+    }
+  }
+}
+
+uint64_t DeviceOperator::m2l_pairs() const {
+  uint64_t s = 0;
+  for (const auto &x : m2l_) s += x.pairs;
+  return s;
+}
+
+uint64_t DeviceOperator::m2l_slots() const {
+  uint64_t s = 0;
+  for (const auto &x : m2l_) s += x.slots;
+  return s;
+}
+
+size_t DeviceOperator::device_bytes() const {
+  return mult_.bytes() + loc_.bytes() + t_m2l_.bytes() + t_m2m_.bytes() + t_l2l_.bytes() + sx_.bytes() * 8;
+}
+
+void DeviceOperator::launch_contract(const Schedule &s, const double *X, double *Y, const DevBuf<double> &tables) {
+  dev::contract_f64(stream_, opt_.p, s.ntasks, s.task_group_off.ptr, s.group_table.ptr, s.group_src.ptr,
+                    s.col_out.ptr, tables.ptr, X, Y);
+}
+
+void DeviceOperator::load_charges(const double *charges, bool on_device) {
+  if (ns_ == 0) return;
+  const double *src = charges;
+  if (!on_device) {
+    hip_check(hipMemcpyAsync(q_in_.ptr, charges, ns_ * sizeof(double), hipMemcpyHostToDevice, stream_), "charges");
+    src = q_in_.ptr;
+  }
+  dev::gather_charges(stream_, src, src_order_.ptr, ns_, q_.ptr, opt_.precision == kMixed ? q32_.ptr : nullptr);
+}
+
+void DeviceOperator::reset() {
+  dev::fill(stream_, mult_.ptr, mult_.count, 0.0);
+  dev::fill(stream_, loc_.ptr, loc_.count, 0.0);
+  dev::fill(stream_, phi_hat_.ptr, phi_hat_.count, 0.0);
+  dev::fill(stream_, grad_hat_.ptr, grad_hat_.count, 0.0);
+}
+
+void DeviceOperator::p2m(int level) {
+  if (level < 0 || level >= nlevels_) return;
+  timed(0, [&] {
+    const uint32_t r0 = p2m_level_[level], r1 = p2m_level_[level + 1];
+    dev::expand(stream_, opt_.p, 0, r1 - r0, p2m_rows_.row_id.ptr + r0, p2m_rows_.row_cell.ptr + 4 * size_t(r0),
+                p2m_rows_.row_off.ptr + r0, p2m_rows_.row_ent.ptr, leaf_cell_.ptr, src_start_.ptr, sx_.ptr, sy_.ptr,
+                sz_.ptr, q_.ptr, mult_.ptr);
+  });
+}
+
+void DeviceOperator::m2m(int level) {
+  if (level < 0 || level + 1 >= nlevels_) return;
+  timed(1, [&] {
+    launch_contract(m2m_[level], mult_.ptr, mult_.ptr, t_m2m_);
+  });
+}
+
+void DeviceOperator::m2l(int level) {
+  if (level < 2 || level >= nlevels_) return;
+  timed(3, [&] {
+    launch_contract(m2l_[level], mult_.ptr, loc_.ptr, t_m2l_);
+  });
+}
+
+void DeviceOperator::p2l(int level) {
+  if (level < 1 || level >= nlevels_) return;
+  timed(4, [&] {
+    const Rows &r = p2l_rows_[level];
+    dev::expand(stream_, opt_.p, 1, r.rows, r.row_id.ptr, r.row_cell.ptr, r.row_off.ptr, r.row_ent.ptr, leaf_cell_.ptr,
+                src_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, q_.ptr, loc_.ptr);
+  });
+}
+
+void DeviceOperator::l2l(int level) {
+  if (level < 1 || level >= nlevels_) return;
+  timed(2, [&] {
+    launch_contract(l2l_[level], loc_.ptr, loc_.ptr, t_l2l_);
+  });
+}
+
+void DeviceOperator::l2p(int level) {
+  if (level < 0 || level >= nlevels_) return;
+  timed(5, [&] {
+    const uint32_t r0 = l2p_level_[level], r1 = l2p_level_[level + 1];
+    dev::evaluate(stream_, opt_.p, 0, opt_.gradients, r1 - r0, l2p_rows_.row_id.ptr + r0, l2p_rows_.row_off.ptr + r0,
+                  l2p_rows_.row_ent.ptr, box_cell_.ptr, leaf_cell_.ptr, tgt_start_.ptr, tx_.ptr, ty_.ptr, tz_.ptr,
+                  loc_.ptr, phi_hat_.ptr, grad_hat_.ptr);
+  });
+}
+
+void DeviceOperator::m2p(int level) {
+  if (level < 0 || level >= nlevels_) return;
+  timed(6, [&] {
+    const uint32_t r0 = m2p_level_[level], r1 = m2p_level_[level + 1];
+    dev::evaluate(stream_, opt_.p, 1, opt_.gradients, r1 - r0, m2p_rows_.row_id.ptr + r0, m2p_rows_.row_off.ptr + r0,
+                  m2p_rows_.row_ent.ptr, box_cell_.ptr, leaf_cell_.ptr, tgt_start_.ptr, tx_.ptr, ty_.ptr, tz_.ptr,
+                  mult_.ptr, phi_hat_.ptr, grad_hat_.ptr);
+  });
+}
+
+void DeviceOperator::p2p(int level) {
+  if (level < 0 || level >= nlevels_) return;
+  timed(7, [&] {
+    const uint32_t l0 = leaf_off_[level], l1 = leaf_off_[level + 1];
+    if (opt_.precision == kMixed)
+      dev::p2p<float>(stream_, l0, l1 - l0, opt_.gradients, near_off_.ptr, near_ent_.ptr, leaf_cell_.ptr,
+                      src_start_.ptr, tgt_start_.ptr, sx32_.ptr, sy32_.ptr, sz32_.ptr, q32_.ptr, tx32_.ptr,
+                      ty32_.ptr, tz32_.ptr, phi_hat_.ptr, grad_hat_.ptr);
+    else
+      dev::p2p<double>(stream_, l0, l1 - l0, opt_.gradients, near_off_.ptr, near_ent_.ptr, leaf_cell_.ptr,
+                       src_start_.ptr, tgt_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, q_.ptr, tx_.ptr, ty_.ptr, tz_.ptr,
+                       phi_hat_.ptr, grad_hat_.ptr);
+  });
+}
+
+void DeviceOperator::evaluate() {
+  const int L = nlevels_ - 1;
+  // upward: P2M of every leaf (leaves have no M2M rows), then M2M from the deepest
+  // parents up (the per-level order p2m(l), m2m(l-1) gives the same sums)
+  timed(0, [&] {
+    dev::expand(stream_, opt_.p, 0, p2m_rows_.rows, p2m_rows_.row_id.ptr, p2m_rows_.row_cell.ptr,
+                p2m_rows_.row_off.ptr, p2m_rows_.row_ent.ptr, leaf_cell_.ptr, src_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr,
+                q_.ptr, mult_.ptr);
+  });
+  for (int l = L - 1; l >= 0; --l) m2m(l);
+  // downward
+  for (int l = 1; l <= L; ++l) {
+    l2l(l);
+    m2l(l);
+    p2l(l);
+  }
+  // leaves: each kind over every level at once; a target still receives L2P, M2P and
+  // P2P contributions in that order
+  timed(5, [&] {
+    dev::evaluate(stream_, opt_.p, 0, opt_.gradients, l2p_rows_.rows, l2p_rows_.row_id.ptr, l2p_rows_.row_off.ptr,
+                  l2p_rows_.row_ent.ptr, box_cell_.ptr, leaf_cell_.ptr, tgt_start_.ptr, tx_.ptr, ty_.ptr, tz_.ptr,
+                  loc_.ptr, phi_hat_.ptr, grad_hat_.ptr);
+  });
+  timed(6, [&] {
+    dev::evaluate(stream_, opt_.p, 1, opt_.gradients, m2p_rows_.rows, m2p_rows_.row_id.ptr, m2p_rows_.row_off.ptr,
+                  m2p_rows_.row_ent.ptr, box_cell_.ptr, leaf_cell_.ptr, tgt_start_.ptr, tx_.ptr, ty_.ptr, tz_.ptr,
+                  mult_.ptr, phi_hat_.ptr, grad_hat_.ptr);
+  });
+  timed(7, [&] {
+  if (opt_.precision == kMixed)
+    dev::p2p<float>(stream_, 0, nleaves_, opt_.gradients, near_off_.ptr, near_ent_.ptr, leaf_cell_.ptr,
+                    src_start_.ptr, tgt_start_.ptr, sx32_.ptr, sy32_.ptr, sz32_.ptr, q32_.ptr, tx32_.ptr,
+                    ty32_.ptr, tz32_.ptr, phi_hat_.ptr, grad_hat_.ptr);
+  else
+    dev::p2p<double>(stream_, 0, nleaves_, opt_.gradients, near_off_.ptr, near_ent_.ptr, leaf_cell_.ptr,
+                     src_start_.ptr, tgt_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, q_.ptr, tx_.ptr, ty_.ptr, tz_.ptr,
+                     phi_hat_.ptr, grad_hat_.ptr);
+  });
+}
+
+void DeviceOperator::read_output(double *phi, double *grad, bool on_device) {
+  dev::output_pass(stream_, nleaves_, tgt_start_.ptr, leaf_cell_.ptr, tgt_order_.ptr, side_, phi_hat_.ptr,
+                   grad_hat_.ptr, on_device ? phi : phi_.ptr, on_device ? grad : grad_.ptr,
+                   opt_.gradients && grad != nullptr);
+  if (!on_device) {
+    hip_check(hipMemcpyAsync(phi, phi_.ptr, nt_ * sizeof(double), hipMemcpyDeviceToHost, stream_), "download phi");
+    if (opt_.gradients && grad)
+      hip_check(hipMemcpyAsync(grad, grad_.ptr, 3 * nt_ * sizeof(double), hipMemcpyDeviceToHost, stream_),
+                "download grad");
+  }
+  sync();
+}
+
+void DeviceOperator::sync() {
+  hip_check(hipStreamSynchronize(stream_), "evaluation");
+  hip_check(hipGetLastError(), "launch");
+  if (!marks_.empty()) collect_timings();
+}
+
+}  // namespace fmmh

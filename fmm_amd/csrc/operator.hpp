@@ -1,0 +1,168 @@
+// operator.hpp -- the device operator: everything an evaluation needs resident on one
+// MI355X, and the level calls of the reference's operator interface
+// (FmmOperator, fmm-plan/src/operator.rs:434-497; DeviceOperator,
+// fmm-exec/src/device.rs:5212-5446).
+//
+// Build once (allocate and upload plan, points and tables), then per evaluation: load
+// charges, reset, the level calls in the evaluator's order (all queued on one stream),
+// and one read of the output (the output pass and the single synchronisation).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "plan.hpp"
+
+namespace fmmh {
+
+enum Precision : int {
+  kPlainF64 = 0,  // every operator in f64
+  kMixed = 1,     // P2P in f32; translations in f64 (split-precision M2L to follow)
+};
+
+void hip_check(hipError_t e, const char *what);
+
+template <typename T>
+struct DevBuf {
+  T *ptr = nullptr;
+  size_t count = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), count(o.count) {
+    o.ptr = nullptr;
+    o.count = 0;
+  }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      ptr = o.ptr;
+      count = o.count;
+      o.ptr = nullptr;
+      o.count = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    count = 0;
+  }
+  void alloc(size_t n) {
+    release();
+    count = n;
+    if (n) hip_check(hipMalloc(&ptr, n * sizeof(T)), "hipMalloc");
+  }
+  void upload(const std::vector<T> &v) {
+    alloc(v.size());
+    if (!v.empty()) hip_check(hipMemcpy(ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+  }
+  size_t bytes() const { return count * sizeof(T); }
+};
+
+// One contraction launch: tasks of COLS output columns, groups of (table, sources).
+struct Schedule {
+  uint32_t ntasks = 0;
+  uint64_t pairs = 0;  // real (column, group) pairs with a source
+  uint64_t slots = 0;  // columns x groups issued
+  DevBuf<uint32_t> task_group_off, group_src, col_out;
+  DevBuf<uint16_t> group_table;
+};
+
+// A row list for the leaf kernels: rows with an output (box or leaf) and CSR entries.
+struct Rows {
+  uint32_t rows = 0;
+  DevBuf<uint32_t> row_id, row_cell, row_off, row_ent;
+};
+
+struct OperatorOptions {
+  int p = 6;
+  int precision = kPlainF64;
+  int gradients = 1;
+  int device = 0;
+};
+
+class DeviceOperator {
+ public:
+  DeviceOperator(const Plan &plan, const OperatorOptions &options);
+  ~DeviceOperator();
+
+  // begin_evaluation: charges in caller order (host or device pointer).
+  void load_charges(const double *charges, bool on_device);
+  void reset();
+  // level calls (FmmOperator, fmm-plan/src/operator.rs:434-497)
+  void p2m(int level);
+  void m2m(int level);
+  void m2l(int level);
+  void p2l(int level);
+  void l2l(int level);
+  void l2p(int level);
+  void m2p(int level);
+  void p2p(int level);
+  // the evaluator's fixed order (fmm-plan/src/operator.rs:408-425), all levels
+  void evaluate();
+  // output pass + download in caller order (phi[nt], grad[3 nt]); the one sync
+  void read_output(double *phi, double *grad, bool on_device);
+  void sync();
+  // per-kind device times (HIP events on this operator's stream)
+  void set_timing(bool enable);
+  void kind_times(double *ms, int64_t *launches);
+
+  int nlevels() const { return nlevels_; }
+  hipStream_t stream() const { return stream_; }
+  const OperatorOptions &options() const { return opt_; }
+  // statistics for reports and the roofline
+  uint64_t p2p_pairs() const { return p2p_pairs_; }
+  uint64_t m2l_pairs() const;
+  uint64_t m2l_slots() const;
+  size_t device_bytes() const;
+  double *device_locals() const { return loc_.ptr; }
+  double *device_multipoles() const { return mult_.ptr; }
+  size_t nboxes_total() const { return nbox_total_; }
+
+ private:
+  void build_schedules(const Plan &plan);
+  template <class F>
+  void timed(int kind, F &&f);
+  void collect_timings();
+  void launch_contract(const Schedule &s, const double *X, double *Y, const DevBuf<double> &tables);
+
+  OperatorOptions opt_;
+  int nlevels_ = 0, nc_ = 0;
+  hipStream_t stream_ = nullptr;
+  double side_ = 0.0;
+  int64_t ns_ = 0, nt_ = 0;
+  uint32_t nleaves_ = 0;
+  size_t nbox_total_ = 0;
+  std::vector<uint32_t> box_base_, leaf_off_;
+  uint64_t p2p_pairs_ = 0;
+
+  DevBuf<uint32_t> leaf_cell_, box_cell_, src_start_, tgt_start_, src_order_, tgt_order_;
+  DevBuf<uint32_t> near_off_, near_ent_;
+  DevBuf<double> sx_, sy_, sz_, tx_, ty_, tz_, q_, q_in_;
+  DevBuf<float> sx32_, sy32_, sz32_, tx32_, ty32_, tz32_, q32_;
+  DevBuf<double> mult_, loc_, phi_hat_, grad_hat_, phi_, grad_;
+  DevBuf<double> t_m2m_, t_l2l_, t_m2l_;
+
+  bool timing_ = false;
+  struct Mark {
+    int kind;
+    hipEvent_t a, b;
+  };
+  std::vector<hipEvent_t> events_;
+  std::vector<Mark> marks_;
+  size_t events_used_ = 0;
+  double kind_ms_[16] = {0};
+  int64_t kind_launches_[16] = {0};
+
+  Rows p2m_rows_, l2p_rows_, m2p_rows_;
+  std::vector<uint32_t> p2m_level_, l2p_level_, m2p_level_;  // row ranges per level
+  std::vector<Rows> p2l_rows_;                               // per level
+  std::vector<Schedule> m2m_, l2l_, m2l_;                    // per level
+};
+
+}  // namespace fmmh

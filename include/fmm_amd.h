@@ -1,0 +1,154 @@
+/*
+ * fmm_amd.h -- C ABI of the MI355X FMM evaluation path.
+ *
+ * The boundary replaces, for one rank and one GPU, the reference's device operator and
+ * its kernel wrappers (tbetcke/fmm):
+ *   - the level-batched operator interface FmmOperator (fmm-plan/src/operator.rs:434-497,
+ *     implemented for the device by DeviceOperator, fmm-exec/src/device.rs:5212-5446):
+ *     fmmh_p2m ... fmmh_p2p below, one call per level and kind, every call adding (+=)
+ *     into its output, with HostData::Reset (operator.rs:326-336) as fmmh_reset;
+ *   - the evaluation entry points of fmm-exec (Fmm::evaluate, fmm-exec/src/fmm.rs:3691;
+ *     DeviceOperator::begin_evaluation, device.rs:4386; read_output, device.rs:4687):
+ *     fmmh_load_charges, fmmh_evaluate, fmmh_read_output;
+ *   - the host build of the tree and lists that stays on the CPU (FmmBuilder::build,
+ *     fmm-exec/src/fmm.rs:1853-1935, on one rank): fmmh_plan_build and its accessors.
+ *
+ * Plain pointers and sizes only.  Every function returns an fmmh_status (0 = success);
+ * fmmh_last_error() describes the last failure of the calling thread.  Status codes
+ * mirror KernelError (fmm-kernels/src/error.rs:12-72).  A failed launch surfaces at the
+ * next synchronising call (fmmh_read_output), as in the reference (device.rs:414-425).
+ * Everything is allocated at fmmh_operator_create; evaluations allocate nothing.
+ * One operator = one HIP stream; calls on one operator must not overlap.
+ */
+#ifndef FMM_AMD_H
+#define FMM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  FMMH_OK = 0,
+  FMMH_NOT_COMPILED = 1,
+  FMMH_NO_DEVICE = 2,
+  FMMH_UNSUPPORTED_PRECISION = 3,
+  FMMH_OUT_OF_MEMORY = 4,
+  FMMH_TOO_LARGE = 5,
+  FMMH_WRONG_DEVICE = 6,
+  FMMH_UNSUPPORTED_LAYOUT = 7,
+  FMMH_DEVICE = 8,
+  FMMH_INVALID_ARGUMENT = 9
+} fmmh_status;
+
+typedef enum {
+  FMMH_PLAIN_F64 = 0, /* every operator in f64 */
+  FMMH_MIXED = 1      /* f32 P2P; f64 translations and accumulation */
+} fmmh_precision;
+
+typedef enum { FMMH_LIST_NEAR = 0, FMMH_LIST_V = 1, FMMH_LIST_W = 2, FMMH_LIST_X = 3 } fmmh_list_kind;
+
+/* Operator kinds for timings: the order of fmm-bench's kind table. */
+typedef enum {
+  FMMH_KIND_P2M = 0,
+  FMMH_KIND_M2M = 1,
+  FMMH_KIND_L2L = 2,
+  FMMH_KIND_M2L = 3,
+  FMMH_KIND_P2L = 4,
+  FMMH_KIND_L2P = 5,
+  FMMH_KIND_M2P = 6,
+  FMMH_KIND_P2P = 7,
+  FMMH_KIND_OTHER = 8,
+  FMMH_NKINDS = 9
+} fmmh_kind;
+
+typedef struct fmmh_plan fmmh_plan;
+typedef struct fmmh_operator fmmh_operator;
+
+/* FmmBuilder settings of the tree (fmm-exec/src/fmm.rs:1302-1760). */
+typedef struct {
+  int64_t leaf_points; /* max_points_per_leaf: refine while a box holds more keys */
+  int32_t max_level;   /* deepest level, at most 16 */
+  double near_w;       /* near rule crossovers (NearRuleChoice::Crossovers); < 0: off */
+  double near_x;
+} fmmh_plan_options;
+
+typedef struct {
+  int32_t p;         /* expansion degree, 0..15 on the device */
+  int32_t precision; /* fmmh_precision */
+  int32_t gradients; /* 1: potential and gradient, 0: potential only */
+  int32_t device;    /* HIP device ordinal */
+} fmmh_operator_options;
+
+typedef struct {
+  int32_t nlevels;
+  int64_t nleaves;
+  int64_t nboxes;
+  uint64_t p2p_pairs;  /* n_s n_t over near pairs, coincident included */
+  uint64_t m2l_pairs;  /* V-list pairs */
+  uint64_t m2l_slots;  /* column x group slots the M2L contraction issues */
+  uint64_t device_bytes;
+} fmmh_stats;
+
+const char *fmmh_last_error(void);
+int32_t fmmh_abi_version(void);
+
+/* ---- plan (host) --------------------------------------------------------------- */
+/* sources/targets: 3 doubles per point (x, y, z); targets may equal sources. */
+int32_t fmmh_plan_build(const double *sources, int64_t nsources, const double *targets, int64_t ntargets,
+                        const fmmh_plan_options *options, fmmh_plan **out);
+void fmmh_plan_free(fmmh_plan *plan);
+int32_t fmmh_plan_nlevels(const fmmh_plan *plan);
+int64_t fmmh_plan_nboxes(const fmmh_plan *plan, int32_t level);
+int64_t fmmh_plan_nleaves(const fmmh_plan *plan);
+/* box[6] = [xmin, ymin, zmin, xmax, ymax, zmax]; *side = w */
+int32_t fmmh_plan_domain(const fmmh_plan *plan, double *box, double *side);
+int32_t fmmh_plan_box_keys(const fmmh_plan *plan, int32_t level, uint64_t *out);
+int32_t fmmh_plan_leaf_keys(const fmmh_plan *plan, uint64_t *out);
+/* Rows of `kind` on `level` (near, W: leaves of the level; V, X: boxes of the level).
+ * Returns the entry count (or -1); fills offsets[rows + 1], entries and, for V, the
+ * offset indices in aux when the pointers are not NULL. */
+int64_t fmmh_plan_list(const fmmh_plan *plan, int32_t kind, int32_t level, uint32_t *offsets, uint32_t *entries,
+                       uint16_t *aux);
+/* Leaf of every point in caller order. */
+int32_t fmmh_plan_point_leaves(const fmmh_plan *plan, int32_t targets, uint32_t *out);
+/* Leaf-scaled coordinates (3 per point, leaf order) and the caller index of each. */
+int32_t fmmh_plan_leaf_coordinates(const fmmh_plan *plan, int32_t targets, double *coords, uint32_t *order);
+
+/* Operator tables of degree p (column-major, 8 / 8 / 316 matrices of (p+1)^4 doubles). */
+int32_t fmmh_tables(int32_t p, double *m2m, double *l2l, double *m2l);
+
+/* ---- device operator -------------------------------------------------------------- */
+int32_t fmmh_device_count(int32_t *count);
+int32_t fmmh_operator_create(const fmmh_plan *plan, const fmmh_operator_options *options, fmmh_operator **out);
+void fmmh_operator_free(fmmh_operator *op);
+int32_t fmmh_operator_stats(const fmmh_operator *op, fmmh_stats *out);
+/* Charges of the sources in caller order, on the host (on_device = 0) or on the GPU. */
+int32_t fmmh_load_charges(fmmh_operator *op, const double *charges, int32_t on_device);
+/* HostData::Reset: zero multipoles, locals and target output. */
+int32_t fmmh_reset(fmmh_operator *op);
+/* FmmOperator level calls (fmm-plan/src/operator.rs:434-485). */
+int32_t fmmh_p2m(fmmh_operator *op, int32_t level);
+int32_t fmmh_m2m(fmmh_operator *op, int32_t level);
+int32_t fmmh_m2l(fmmh_operator *op, int32_t level);
+int32_t fmmh_p2l(fmmh_operator *op, int32_t level);
+int32_t fmmh_l2l(fmmh_operator *op, int32_t level);
+int32_t fmmh_l2p(fmmh_operator *op, int32_t level);
+int32_t fmmh_m2p(fmmh_operator *op, int32_t level);
+int32_t fmmh_p2p(fmmh_operator *op, int32_t level);
+/* Every level call in the evaluator's order (fmm-plan/src/evaluator.rs:873-881). */
+int32_t fmmh_evaluate(fmmh_operator *op);
+/* Output pass and download: phi[nt], grad[3 nt] (may be NULL) in caller order, to the
+ * host (on_device = 0) or to device buffers.  Synchronises. */
+int32_t fmmh_read_output(fmmh_operator *op, double *phi, double *grad, int32_t on_device);
+int32_t fmmh_sync(fmmh_operator *op);
+/* Per-kind device times (HIP events on the operator's stream), summed since the last
+ * call with enable: ms[FMMH_NKINDS]. */
+int32_t fmmh_kind_timing(fmmh_operator *op, int32_t enable);
+int32_t fmmh_kind_times(fmmh_operator *op, double *ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -110,6 +110,8 @@ int64_t orc_fmm_list(const orc_fmm *f, int kind, int level, int64_t *offsets,
                      int64_t *entries, int64_t *aux);
 /* Leaf of every source / target point (leaf numbering), in caller order. */
 void orc_fmm_point_leaves(const orc_fmm *f, int targets, int64_t *out);
+/* Leaf-scaled coordinates in leaf order and the caller index of each position. */
+void orc_fmm_leaf_coordinates(const orc_fmm *f, int targets, double *u, int64_t *order);
 
 #ifdef __cplusplus
 }

@@ -94,6 +94,7 @@ def lib():
         L.orc_fmm_list.argtypes = [C.c_void_p, C.c_int, C.c_int, pd, pd, pd]
         L.orc_fmm_list.restype = C.c_int64
         L.orc_fmm_point_leaves.argtypes = [C.c_void_p, C.c_int, _i64p]
+        L.orc_fmm_leaf_coordinates.argtypes = [C.c_void_p, C.c_int, _f64p, _i64p]
         _lib = L
     return _lib
 
@@ -371,6 +372,12 @@ class Fmm:
         if kind == "v":
             return off, ent[:total], aux[:total]
         return off, ent[:total]
+
+    def leaf_coordinates(self, targets=False):
+        n = len(self.targets) if targets else len(self.sources)
+        u, order = np.zeros((n, 3)), np.zeros(n, np.int64)
+        lib().orc_fmm_leaf_coordinates(self._h, int(targets), u, order)
+        return u, order
 
     def point_leaves(self, targets=False):
         n = len(self.targets) if targets else len(self.sources)

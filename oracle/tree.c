@@ -808,6 +808,12 @@ int64_t orc_fmm_list(const orc_fmm *f, int kind, int level, int64_t *offsets, in
   return total;
 }
 
+void orc_fmm_leaf_coordinates(const orc_fmm *f, int targets, double *u, int64_t *order) {
+  int64_t n = targets ? f->nt : f->ns;
+  memcpy(u, targets ? f->ut : f->us, sizeof(double) * 3 * n);
+  memcpy(order, targets ? f->tgt_order : f->src_order, sizeof(int64_t) * n);
+}
+
 void orc_fmm_point_leaves(const orc_fmm *f, int targets, int64_t *out) {
   memcpy(out, targets ? f->tgt_leaf : f->src_leaf, sizeof(int64_t) * (targets ? f->nt : f->ns));
 }
