@@ -1,0 +1,211 @@
+"""Benchmark of the FMM evaluation hot path on one MI355X.
+
+Workload: the reference's GPU configuration -- N = 4,000,000 points uniform in the unit
+cube from SplitMix64(0x00be9c06) (fmm-bench/src/measure.rs:35-58), charges N(0,1) from the
+same stream (Box-Muller), sources = targets, potential and gradient, at most 128 points
+per leaf, degree p (12 by default: relative error of the potential below 1e-6).
+
+One step = one evaluation with the charges already in HBM: charge gather, reset, every
+operator in the evaluator's order, and the output pass into device buffers.  The
+mixed-precision path is the headline value; the plain f64 path is timed beside it in
+the same run, and the error of both against the f64 direct sum at 4,096 sampled targets
+is reported.  The oracle's serial CPU FMM on a bounded sample is the CPU baseline.
+
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# MI355X published peaks
+PEAK_F64_MFMA = 78.6  # TFLOP/s, dense FP64 matrix
+PEAK_F32 = 157.3  # TFLOP/s, FP32 (vector and dense matrix)
+PEAK_BF16 = 2500.0  # TFLOP/s, dense BF16 MFMA
+PEAK_HBM = 8000.0  # GB/s
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=4_000_000)
+    ap.add_argument("--p", type=int, default=12)
+    ap.add_argument("--leaf", type=int, default=128)
+    ap.add_argument("--sampled", type=int, default=4096)
+    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--near", default="device", help="near rule: off, device, or N_W,N_X")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain f64 leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each box has one GPU)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import fmm_amd
+    from oracle import oracle as O
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    rng = O.Rng(O.BENCH_SEED)
+    pts = rng.cube(args.n)
+    q = rng.normals(args.n)
+    sample = rng.sample(args.n, args.sampled)
+    log(f"problem: N={args.n} drawn in {time.time() - t0:.1f}s")
+
+    if args.near == "off":
+        near = None
+    elif args.near == "device":
+        near = (0.5 * (args.p + 1) ** 2,) * 2
+    else:
+        near = tuple(float(v) for v in args.near.split(","))
+    t0 = time.time()
+    plan = fmm_amd.Plan(pts, leaf_points=args.leaf, near_rule=near)
+    t_plan = time.time() - t0
+    lk = plan.leaf_keys()
+    hist = {int(l): int(c) for l, c in zip(*np.unique(lk & np.uint64(0x7FFF), return_counts=True))}
+    log(f"plan: {plan.nlevels} levels, {plan.nleaves} leaves {hist}, built in {t_plan:.1f}s")
+
+    q_dev = torch.from_numpy(q).to(dev)
+    phi_dev = torch.zeros(args.n, dtype=torch.float64, device=dev)
+    grad_dev = torch.zeros(3 * args.n, dtype=torch.float64, device=dev)
+
+    t0 = time.time()
+    exact_phi, exact_grad = O.direct_sum(pts, q, pts[sample])
+    exact_phi /= 4 * np.pi
+    exact_grad /= 4 * np.pi
+    log(f"direct sum at {len(sample)} targets in {time.time() - t0:.1f}s")
+
+    def run(precision):
+        t0 = time.time()
+        op = fmm_amd.Operator(plan, args.p, precision, True, 0)
+        stats = op.stats()
+        log(f"[{precision}] operator built in {time.time() - t0:.1f}s: {stats}")
+
+        def step():
+            op.load_charges(q_dev)
+            op.reset()
+            op.evaluate()
+            op.read_output_device(phi_dev, grad_dev)
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(args.steps):
+            torch.cuda.synchronize()
+            s = time.perf_counter()
+            step()
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - s)
+        # kind timings over separate evaluations (HIP events on the operator's stream)
+        op.kind_timing(True)
+        for _ in range(max(2, min(args.steps, 5))):
+            step()
+        kinds = op.kind_times()
+        op.kind_timing(False)
+        phi = phi_dev.cpu().numpy()
+        grad = grad_dev.cpu().numpy().reshape(-1, 3)
+        e_phi = O.error_norms(phi[sample], exact_phi)[0]
+        e_grad = O.error_norms(grad[sample], exact_grad, True)[0]
+        med = float(np.median(times))
+        log(f"[{precision}] median {med * 1e3:.2f} ms ({args.n / med / 1e6:.1f} Mpts/s), "
+            f"err phi {e_phi:.3e} grad {e_grad:.3e}")
+        log(f"[{precision}] kinds (ms, launches): " +
+            ", ".join(f"{k} {v[0] / max(v[1], 1):.3f}x{v[1]}" for k, v in kinds.items() if v[1]))
+        res = dict(ms=med * 1e3, times=times, mpts=args.n / med / 1e6, err_phi=e_phi, err_grad=e_grad,
+                   kinds=kinds, stats=stats)
+        del op
+        return res
+
+    mixed = run("mixed")
+    plain = None if args.no_plain else run("f64")
+
+    # roofline of the dominant kernel of the headline (mixed) path
+    stats = mixed["stats"]
+    kinds = mixed["kinds"]
+    n = (args.p + 1) ** 2
+    reps = max(1, kinds["p2p"][1])
+    per_eval = {k: v[0] / max(1, v[1]) * (v[1] / reps) for k, v in kinds.items() if v[1]}
+    dominant = max(per_eval, key=per_eval.get)
+    if dominant == "m2l":
+        flops = 2.0 * n * n * stats["m2l_pairs"]
+        launches_per_eval = kinds["m2l"][1] / reps
+        avg_ms = kinds["m2l"][0] / kinds["m2l"][1]
+        achieved = flops / launches_per_eval / (avg_ms * 1e-3) / 1e12
+        roof = {"kernel": "contract_f64 (M2L)", "bound": "mfma", "achieved": achieved, "peak": PEAK_F64_MFMA,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_F64_MFMA, "traffic": None,
+                "algorithmic": "2 (p+1)^4 flops per V-list pair, f64 MFMA peak"}
+    else:
+        pairs = stats["p2p_pairs"]
+        avg_ms = kinds["p2p"][0] / kinds["p2p"][1]
+        achieved = 30.0 * pairs / (avg_ms * 1e-3) / 1e12
+        roof = {"kernel": "p2p<float> (P2P)", "bound": "mfma", "achieved": achieved, "peak": PEAK_F32,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_F32, "traffic": None,
+                "algorithmic": "15 lane-ops (30 flops) per pair, FP32 peak"}
+    roof["avg_launch_ms"] = avg_ms
+
+    cpu = None
+    if not args.no_cpu:
+        m = min(args.cpu_sample, args.n)
+        t0 = time.time()
+        f = O.Fmm(pts[:m], p=args.p, leaf_points=args.leaf, near_rule=near)
+        t_build = time.time() - t0
+        t0 = time.time()
+        f.evaluate(q[:m])
+        t_eval = time.time() - t0
+        cpu = {"value": m / t_eval / 1e6, "unit": "Mpts/s", "cores": 1, "kind": "port",
+               "sample": f"oracle serial f64 FMM evaluation (dense M2L tables) of the first {m} points of the "
+                         f"same workload, p={args.p}, <= {args.leaf} per leaf; {t_eval:.1f} s "
+                         f"(build {t_build:.1f} s excluded)"}
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": "Laplace FMM Mpts/s at 1e-6 rel err: mixed-prec batched vs plain FP64 GPU",
+        "value": mixed["mpts"],
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 P2P + f64 translations",
+        "data": "synthetic: SplitMix64(0x00be9c06) uniform cube points, N(0,1) charges",
+        "config": {"workload": "laplace-fmm-uniform-cube", "n": args.n, "p": args.p, "leaf_points": args.leaf,
+                   "charges": "normal", "gradients": True, "near_rule": near, "sampled_targets": len(sample),
+                   "leaves": int(plan.nleaves), "levels": int(plan.nlevels), "leaf_levels": hist},
+        "errors": {"mixed": {"phi": mixed["err_phi"], "grad": mixed["err_grad"]}},
+        "kind_ms": {k: round(v[0] / max(1, v[1]) * (v[1] / reps), 4) for k, v in kinds.items() if v[1]},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if plain is not None:
+        line["plain_f64"] = {"value": plain["mpts"], "ms_per_step": plain["ms"], "err_phi": plain["err_phi"],
+                             "err_grad": plain["err_grad"]}
+        line["speedup_vs_plain_f64"] = mixed["mpts"] / plain["mpts"]
+        line["errors"]["plain_f64"] = {"phi": plain["err_phi"], "grad": plain["err_grad"]}
+    line["p2p_pairs"] = stats["p2p_pairs"]
+    line["m2l_pairs"] = stats["m2l_pairs"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

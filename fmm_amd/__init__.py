@@ -189,6 +189,12 @@ class Operator:
     def sync(self):
         _lib.check(_lib.lib().fmmh_sync(self._h))
 
+    def expansions(self, locals_: bool, level: int) -> np.ndarray:
+        n = (self.p + 1) ** 2
+        out = np.zeros((len(self.plan.box_keys(level)), n))
+        _lib.check(_lib.lib().fmmh_read_expansions(self._h, int(locals_), int(level), _lib.ptr(out)))
+        return out
+
     def kind_timing(self, enable=True):
         _lib.check(_lib.lib().fmmh_kind_timing(self._h, int(enable)))
 
@@ -267,9 +273,18 @@ class FmmBuilder:
         self._device = int(ordinal)
         return self
 
-    def near_rule(self, crossovers):
-        """None (off) or (N_W, N_X): NearRuleChoice::Crossovers."""
-        self._near = crossovers
+    def near_rule(self, rule):
+        """The near rule (fmm-exec/src/fmm.rs:1018-1050): None or "off"; (N_W, N_X) named
+        crossovers (NearRuleChoice::Crossovers); or "device", the crossovers of this
+        GPU path's cost ratio, N_W = N_X = (p+1)^2 / 2 points (an M2P or P2L costs about
+        (p+1)^2 / 2 pair interactions per point on the device)."""
+        if rule is None or rule == "off":
+            self._near = None
+        elif rule == "device":
+            c = 0.5 * (self._p + 1) ** 2
+            self._near = (c, c)
+        else:
+            self._near = (float(rule[0]), float(rule[1]))
         return self
 
     def build_plan(self, sources, targets=None) -> Plan:

@@ -34,7 +34,7 @@ EXPORTS = [
     "fmmh_device_count", "fmmh_operator_create", "fmmh_operator_free", "fmmh_operator_stats",
     "fmmh_load_charges", "fmmh_reset", "fmmh_p2m", "fmmh_m2m", "fmmh_m2l", "fmmh_p2l", "fmmh_l2l",
     "fmmh_l2p", "fmmh_m2p", "fmmh_p2p", "fmmh_evaluate", "fmmh_read_output", "fmmh_sync",
-    "fmmh_kind_timing", "fmmh_kind_times",
+    "fmmh_kind_timing", "fmmh_kind_times", "fmmh_read_expansions",
 ]
 
 
@@ -113,6 +113,7 @@ def lib():
     L.fmmh_read_output.argtypes = [vp, vp, vp, i32]
     L.fmmh_sync.argtypes = [vp]
     L.fmmh_kind_timing.argtypes = [vp, i32]
+    L.fmmh_read_expansions.argtypes = [vp, i32, i32, vp]
     L.fmmh_kind_times.argtypes = [vp, vp, vp]
     for name in EXPORTS:
         f = getattr(L, name)

@@ -248,6 +248,14 @@ int32_t fmmh_sync(fmmh_operator *op) {
   });
 }
 
+int32_t fmmh_read_expansions(fmmh_operator *op, int32_t locals, int32_t level, double *out) {
+  return guard([&] {
+    need(op && out, "fmmh_read_expansions: NULL argument");
+    need(level >= 0 && level < op->nlevels, "fmmh_read_expansions: level out of range");
+    op->op->read_expansions(locals != 0, level, out);
+  });
+}
+
 int32_t fmmh_kind_timing(fmmh_operator *op, int32_t enable) {
   return guard([&] {
     need(op != nullptr, "fmmh_kind_timing: NULL operator");

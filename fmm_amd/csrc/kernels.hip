@@ -112,54 +112,86 @@ __global__ void k_output(const uint32_t *tgt_start, const uint32_t *leaf_cell, c
 // ---- P2P (CONVENTIONS §3.13 "Device kernels") ---------------------------------------
 // One workgroup per target leaf, TPT targets per thread in registers; every source leaf
 // of the near row is staged through LDS in tiles of P2P_TILE points, with
-// y^ = fma(r^, u_s, c^) formed once per source (u_s itself for s = t).  A pair adds
+// y^ = fma(r^, u_s, c^) formed once per source in f64 (u_s itself for s = t).  A pair adds
 // nothing when r^2 == 0: the select comes before the charge is touched.
+//
+// T = double: every operation in f64.  T = float (mixed precision): coordinates stay
+// f64 in HBM; each is split into hi + lo floats (hi = fl32(u), lo = fl32(u - hi)) and the
+// separation is formed as (t_hi - y_hi) + (t_lo - y_lo), which keeps it accurate to about
+// 2^-24 of |d| even for close pairs (a plain f32 difference loses 2^-24 of |u| >> |d|);
+// r^2, the inverse square root and the sums are f32, added into the f64 output once per
+// target.  Coincident points give d = 0 exactly in both forms.
 
 template <typename T, bool GRAD, int TPT>
 __global__ __launch_bounds__(P2P_THREADS) void k_p2p(uint32_t leaf_first, const uint32_t *near_off,
                                                      const uint32_t *near_ent, const uint32_t *leaf_cell,
                                                      const uint32_t *src_start, const uint32_t *tgt_start,
-                                                     const T *sx, const T *sy, const T *sz, const T *sq,
-                                                     const T *tx, const T *ty, const T *tz, double *phi,
-                                                     double *grad) {
-  __shared__ T ys[4][P2P_TILE];
+                                                     const double *sx, const double *sy, const double *sz,
+                                                     const double *sq, const double *tx, const double *ty,
+                                                     const double *tz, double *phi, double *grad) {
+  constexpr bool SPLIT = sizeof(T) == 4;
+  constexpr int NY = SPLIT ? 7 : 4;
+  __shared__ T ys[NY][P2P_TILE];
   const uint32_t lf = leaf_first + blockIdx.x;
   const uint32_t t0 = tgt_start[lf], t1 = tgt_start[lf + 1];
   if (t0 == t1) return;
   const uint32_t *ct = leaf_cell + 4 * size_t(lf);
   for (uint32_t tb = t0; tb < t1; tb += P2P_THREADS * TPT) {
-    T ux[TPT], uy[TPT], uz[TPT], ap[TPT], ax[TPT], ay[TPT], az[TPT];
+    T ux[TPT], uy[TPT], uz[TPT], lx[TPT], ly[TPT], lz[TPT], ap[TPT], ax[TPT], ay[TPT], az[TPT];
 #pragma unroll
     for (int a = 0; a < TPT; ++a) {
       const uint32_t i = tb + threadIdx.x + a * P2P_THREADS;
       const bool ok = i < t1;
-      ux[a] = ok ? tx[i] : T(0);
-      uy[a] = ok ? ty[i] : T(0);
-      uz[a] = ok ? tz[i] : T(0);
+      const double x = ok ? tx[i] : 0.0, y = ok ? ty[i] : 0.0, z = ok ? tz[i] : 0.0;
+      ux[a] = T(x);
+      uy[a] = T(y);
+      uz[a] = T(z);
+      lx[a] = T(x - double(ux[a]));
+      ly[a] = T(y - double(uy[a]));
+      lz[a] = T(z - double(uz[a]));
       ap[a] = ax[a] = ay[a] = az[a] = T(0);
     }
     for (uint32_t e = near_off[lf]; e < near_off[lf + 1]; ++e) {
       const uint32_t sl = near_ent[e];
       const uint32_t s0 = src_start[sl], s1 = src_start[sl + 1];
-      T c[3], r;
+      double c[3], r;
       const bool self = sl == lf;
-      relative_frame<T>(leaf_cell + 4 * size_t(sl), ct, c, r);
+      relative_frame<double>(leaf_cell + 4 * size_t(sl), ct, c, r);
       for (uint32_t sb = s0; sb < s1; sb += P2P_TILE) {
         const uint32_t cnt = (s1 - sb) < uint32_t(P2P_TILE) ? (s1 - sb) : uint32_t(P2P_TILE);
         __syncthreads();
         for (uint32_t j = threadIdx.x; j < cnt; j += P2P_THREADS) {
           const uint32_t s = sb + j;
-          ys[0][j] = self ? sx[s] : fma(r, sx[s], c[0]);
-          ys[1][j] = self ? sy[s] : fma(r, sy[s], c[1]);
-          ys[2][j] = self ? sz[s] : fma(r, sz[s], c[2]);
-          ys[3][j] = sq[s];
+          const double yx = self ? sx[s] : fma(r, sx[s], c[0]);
+          const double yy = self ? sy[s] : fma(r, sy[s], c[1]);
+          const double yz = self ? sz[s] : fma(r, sz[s], c[2]);
+          ys[0][j] = T(yx);
+          ys[1][j] = T(yy);
+          ys[2][j] = T(yz);
+          ys[3][j] = T(sq[s]);
+          if (SPLIT) {
+            ys[4][j] = T(yx - double(T(yx)));
+            ys[5][j] = T(yy - double(T(yy)));
+            ys[6][j] = T(yz - double(T(yz)));
+          }
         }
         __syncthreads();
         for (uint32_t j = 0; j < cnt; ++j) {
           const T yx = ys[0][j], yy = ys[1][j], yz = ys[2][j], q = ys[3][j];
+          T yxl = T(0), yyl = T(0), yzl = T(0);
+          if (SPLIT) {
+            yxl = ys[4][j];
+            yyl = ys[5][j];
+            yzl = ys[6][j];
+          }
 #pragma unroll
           for (int a = 0; a < TPT; ++a) {
-            const T dx = ux[a] - yx, dy = uy[a] - yy, dz = uz[a] - yz;
+            T dx = ux[a] - yx, dy = uy[a] - yy, dz = uz[a] - yz;
+            if (SPLIT) {
+              dx += lx[a] - yxl;
+              dy += ly[a] - yyl;
+              dz += lz[a] - yzl;
+            }
             const T r2 = dx * dx + dy * dy + dz * dz;
             const T inv = r2 == T(0) ? T(0) : rsqrt_t<T>(r2);
             const T qi = q * inv;
@@ -468,7 +500,8 @@ __global__ __launch_bounds__(EVAL_THREADS) void k_evaluate(int p, const uint32_t
 // octant for M2M).  Table layout (swizzle_table): [chunk][m-tile][kk][lane] with
 // lane -> (row = 16 mt + lane % 16, k = kk + KC chunk + (lane / 16) KS), zero-padded;
 // the B operand of lane (g = lane / 16, j = lane % 16) is coefficient k of column j's
-// source.  Accumulator element r of lane (g, j) is row 4 g + r of the M-tile.
+// source.  Accumulator element r of lane (g, j) is row 4 r + g of the M-tile (the
+// f64 16x16x4 accumulator layout, checked on gfx950).
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -518,7 +551,7 @@ __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, const
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = mt * 16 + 4 * g + r;
+        const int i = mt * 16 + 4 * r + g;
         if (i < nc) y[i] += acc[mt][r];
       }
   }
@@ -553,8 +586,8 @@ void output_pass(hipStream_t s, uint32_t nleaves, const uint32_t *tgt_start, con
 template <typename T>
 void p2p(hipStream_t s, uint32_t leaf_first, uint32_t count, int gradients, const uint32_t *near_off,
          const uint32_t *near_ent, const uint32_t *leaf_cell, const uint32_t *src_start,
-         const uint32_t *tgt_start, const T *sx, const T *sy, const T *sz, const T *sq, const T *tx,
-         const T *ty, const T *tz, double *phi, double *grad) {
+         const uint32_t *tgt_start, const double *sx, const double *sy, const double *sz, const double *sq,
+         const double *tx, const double *ty, const double *tz, double *phi, double *grad) {
   if (count == 0) return;
   if (gradients)
     hipLaunchKernelGGL((k_p2p<T, true, 1>), dim3(count), dim3(P2P_THREADS), 0, s, leaf_first, near_off, near_ent,
@@ -565,8 +598,8 @@ void p2p(hipStream_t s, uint32_t leaf_first, uint32_t count, int gradients, cons
 }
 
 template void p2p<float>(hipStream_t, uint32_t, uint32_t, int, const uint32_t *, const uint32_t *, const uint32_t *,
-                         const uint32_t *, const uint32_t *, const float *, const float *, const float *,
-                         const float *, const float *, const float *, const float *, double *, double *);
+                         const uint32_t *, const uint32_t *, const double *, const double *, const double *,
+                         const double *, const double *, const double *, const double *, double *, double *);
 template void p2p<double>(hipStream_t, uint32_t, uint32_t, int, const uint32_t *, const uint32_t *,
                           const uint32_t *, const uint32_t *, const uint32_t *, const double *, const double *,
                           const double *, const double *, const double *, const double *, const double *,

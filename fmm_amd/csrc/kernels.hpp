@@ -27,12 +27,13 @@ void output_pass(hipStream_t s, uint32_t nleaves, const uint32_t *tgt_start, con
                  const uint32_t *order, double side, const double *phi_hat, const double *grad_hat,
                  double *phi, double *grad, int gradients);
 
-// P2P over the leaves leaf_first .. leaf_first + count (global leaf indices).
+// P2P over the leaves leaf_first .. leaf_first + count (global leaf indices), computed in
+// T (double, or float with double-float separations); f64 coordinates and charges.
 template <typename T>
 void p2p(hipStream_t s, uint32_t leaf_first, uint32_t count, int gradients, const uint32_t *near_off,
          const uint32_t *near_ent, const uint32_t *leaf_cell, const uint32_t *src_start,
-         const uint32_t *tgt_start, const T *sx, const T *sy, const T *sz, const T *sq, const T *tx,
-         const T *ty, const T *tz, double *phi, double *grad);
+         const uint32_t *tgt_start, const double *sx, const double *sy, const double *sz, const double *sq,
+         const double *tx, const double *ty, const double *tz, double *phi, double *grad);
 
 // P2M (irregular = 0, row_ent the leaf of each row, unit frame) or P2L (irregular = 1,
 // row_ent the X-list leaves).  rows: output box gid and cell per row, CSR of entries.

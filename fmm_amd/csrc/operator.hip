@@ -19,11 +19,6 @@ void hip_check(hipError_t e, const char *what) {
 
 namespace {
 
-template <typename T>
-std::vector<T> to_type(const std::vector<double> &v) {
-  return std::vector<T>(v.begin(), v.end());
-}
-
 // Builds one contraction schedule from target columns and their (group, source) lists.
 // cols: output gid per column in task order, COLS per task (kNone = padding);
 // groups_of(c): ascending (table, source gid) pairs of column c.
@@ -165,16 +160,6 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
   tz_.upload(tz);
   q_.alloc(ns_);
   q_in_.alloc(ns_);
-  if (opt_.precision == kMixed) {
-    // the f32 points are the casts of the f64 leaf-scaled values (CONVENTIONS §3.13)
-    sx32_.upload(to_type<float>(sx));
-    sy32_.upload(to_type<float>(sy));
-    sz32_.upload(to_type<float>(sz));
-    tx32_.upload(to_type<float>(tx));
-    ty32_.upload(to_type<float>(ty));
-    tz32_.upload(to_type<float>(tz));
-    q32_.alloc(ns_);
-  }
 
   // expansions and outputs
   mult_.alloc(nbox_total_ * nc_);
@@ -419,7 +404,7 @@ void DeviceOperator::load_charges(const double *charges, bool on_device) {
     hip_check(hipMemcpyAsync(q_in_.ptr, charges, ns_ * sizeof(double), hipMemcpyHostToDevice, stream_), "charges");
     src = q_in_.ptr;
   }
-  dev::gather_charges(stream_, src, src_order_.ptr, ns_, q_.ptr, opt_.precision == kMixed ? q32_.ptr : nullptr);
+  dev::gather_charges(stream_, src, src_order_.ptr, ns_, q_.ptr, nullptr);
 }
 
 void DeviceOperator::reset() {
@@ -489,18 +474,22 @@ void DeviceOperator::m2p(int level) {
   });
 }
 
+void DeviceOperator::launch_p2p(uint32_t leaf_first, uint32_t count) {
+  if (opt_.precision == kMixed)
+    dev::p2p<float>(stream_, leaf_first, count, opt_.gradients, near_off_.ptr, near_ent_.ptr, leaf_cell_.ptr,
+                    src_start_.ptr, tgt_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, q_.ptr, tx_.ptr, ty_.ptr, tz_.ptr,
+                    phi_hat_.ptr, grad_hat_.ptr);
+  else
+    dev::p2p<double>(stream_, leaf_first, count, opt_.gradients, near_off_.ptr, near_ent_.ptr, leaf_cell_.ptr,
+                     src_start_.ptr, tgt_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, q_.ptr, tx_.ptr, ty_.ptr, tz_.ptr,
+                     phi_hat_.ptr, grad_hat_.ptr);
+}
+
 void DeviceOperator::p2p(int level) {
   if (level < 0 || level >= nlevels_) return;
   timed(7, [&] {
     const uint32_t l0 = leaf_off_[level], l1 = leaf_off_[level + 1];
-    if (opt_.precision == kMixed)
-      dev::p2p<float>(stream_, l0, l1 - l0, opt_.gradients, near_off_.ptr, near_ent_.ptr, leaf_cell_.ptr,
-                      src_start_.ptr, tgt_start_.ptr, sx32_.ptr, sy32_.ptr, sz32_.ptr, q32_.ptr, tx32_.ptr,
-                      ty32_.ptr, tz32_.ptr, phi_hat_.ptr, grad_hat_.ptr);
-    else
-      dev::p2p<double>(stream_, l0, l1 - l0, opt_.gradients, near_off_.ptr, near_ent_.ptr, leaf_cell_.ptr,
-                       src_start_.ptr, tgt_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, q_.ptr, tx_.ptr, ty_.ptr, tz_.ptr,
-                       phi_hat_.ptr, grad_hat_.ptr);
+    launch_p2p(l0, l1 - l0);
   });
 }
 
@@ -532,16 +521,7 @@ void DeviceOperator::evaluate() {
                   m2p_rows_.row_ent.ptr, box_cell_.ptr, leaf_cell_.ptr, tgt_start_.ptr, tx_.ptr, ty_.ptr, tz_.ptr,
                   mult_.ptr, phi_hat_.ptr, grad_hat_.ptr);
   });
-  timed(7, [&] {
-  if (opt_.precision == kMixed)
-    dev::p2p<float>(stream_, 0, nleaves_, opt_.gradients, near_off_.ptr, near_ent_.ptr, leaf_cell_.ptr,
-                    src_start_.ptr, tgt_start_.ptr, sx32_.ptr, sy32_.ptr, sz32_.ptr, q32_.ptr, tx32_.ptr,
-                    ty32_.ptr, tz32_.ptr, phi_hat_.ptr, grad_hat_.ptr);
-  else
-    dev::p2p<double>(stream_, 0, nleaves_, opt_.gradients, near_off_.ptr, near_ent_.ptr, leaf_cell_.ptr,
-                     src_start_.ptr, tgt_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, q_.ptr, tx_.ptr, ty_.ptr, tz_.ptr,
-                     phi_hat_.ptr, grad_hat_.ptr);
-  });
+  timed(7, [&] { launch_p2p(0, nleaves_); });
 }
 
 void DeviceOperator::read_output(double *phi, double *grad, bool on_device) {
@@ -555,6 +535,14 @@ void DeviceOperator::read_output(double *phi, double *grad, bool on_device) {
                 "download grad");
   }
   sync();
+}
+
+void DeviceOperator::read_expansions(bool locals, int level, double *out) {
+  require(level >= 0 && level < nlevels_, "read_expansions: level out of range");
+  sync();
+  const size_t n = size_t(box_base_[level + 1] - box_base_[level]) * nc_;
+  const double *src = (locals ? loc_.ptr : mult_.ptr) + size_t(box_base_[level]) * nc_;
+  hip_check(hipMemcpy(out, src, n * sizeof(double), hipMemcpyDeviceToHost), "read_expansions");
 }
 
 void DeviceOperator::sync() {

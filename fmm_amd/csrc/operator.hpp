@@ -108,6 +108,8 @@ class DeviceOperator {
   // output pass + download in caller order (phi[nt], grad[3 nt]); the one sync
   void read_output(double *phi, double *grad, bool on_device);
   void sync();
+  // copy the multipoles (locals = false) or locals of one level to host, box-major
+  void read_expansions(bool locals, int level, double *out);
   // per-kind device times (HIP events on this operator's stream)
   void set_timing(bool enable);
   void kind_times(double *ms, int64_t *launches);
@@ -129,6 +131,7 @@ class DeviceOperator {
   template <class F>
   void timed(int kind, F &&f);
   void collect_timings();
+  void launch_p2p(uint32_t leaf_first, uint32_t count);
   void launch_contract(const Schedule &s, const double *X, double *Y, const DevBuf<double> &tables);
 
   OperatorOptions opt_;
@@ -144,7 +147,6 @@ class DeviceOperator {
   DevBuf<uint32_t> leaf_cell_, box_cell_, src_start_, tgt_start_, src_order_, tgt_order_;
   DevBuf<uint32_t> near_off_, near_ent_;
   DevBuf<double> sx_, sy_, sz_, tx_, ty_, tz_, q_, q_in_;
-  DevBuf<float> sx32_, sy32_, sz32_, tx32_, ty32_, tz32_, q32_;
   DevBuf<double> mult_, loc_, phi_hat_, grad_hat_, phi_, grad_;
   DevBuf<double> t_m2m_, t_l2l_, t_m2l_;
 

@@ -221,6 +221,55 @@ void assign_points(const double *pts, int64_t n, const Plan &plan,
       leaf_coordinates(pts + 3 * order[j], plan.leaf_key[l], plan, u.data() + 3 * j);
 }
 
+// The near rule with named crossovers (fmm-plan/src/near.rs:15-40): a W entry of a leaf
+// that is itself a leaf joins the near row when it holds fewer than near_w sources; the
+// X entries of a leaf join its near row when it holds fewer than near_x targets.  Near
+// rows keep leaf (level, key) order; the converted entries leave their W and X rows.
+void apply_near_rule(Plan &plan) {
+  const double nw = plan.options.near_w, nx = plan.options.near_x;
+  if (nw < 0 && nx < 0) return;
+  for (int l = 0; l < plan.nlevels; ++l) {
+    const uint32_t l0 = plan.leaf_off[l], nll = plan.leaf_off[l + 1] - l0;
+    Csr &N = plan.near[l];
+    Csr &W = plan.w[l];
+    Csr &X = plan.x[l];
+    Csr nn, nwc, nxc;
+    nn.off.assign(nll + 1, 0);
+    nwc.off.assign(nll + 1, 0);
+    std::vector<bool> drop_x(plan.nboxes(l), false);
+    for (uint32_t r = 0; r < nll; ++r) {
+      const uint32_t lf = l0 + r;
+      std::vector<uint32_t> row(N.ent.begin() + N.off[r], N.ent.begin() + N.off[r + 1]);
+      for (uint32_t e = W.off[r]; e < W.off[r + 1]; ++e) {
+        const uint32_t sb = W.ent[e];
+        const uint32_t sl = l + 1 < plan.nlevels ? plan.box_leaf[l + 1][sb] : kNone;
+        if (nw >= 0 && sl != kNone && double(plan.src_start[sl + 1] - plan.src_start[sl]) < nw)
+          row.push_back(sl);
+        else
+          nwc.ent.push_back(sb);
+      }
+      const uint32_t b = plan.leaf_box[lf];
+      if (nx >= 0 && double(plan.tgt_start[lf + 1] - plan.tgt_start[lf]) < nx) {
+        for (uint32_t e = X.off[b]; e < X.off[b + 1]; ++e) row.push_back(X.ent[e]);
+        drop_x[b] = true;
+      }
+      std::sort(row.begin(), row.end());
+      nn.ent.insert(nn.ent.end(), row.begin(), row.end());
+      nn.off[r + 1] = static_cast<uint32_t>(nn.ent.size());
+      nwc.off[r + 1] = static_cast<uint32_t>(nwc.ent.size());
+    }
+    const uint32_t nb = plan.nboxes(l);
+    nxc.off.assign(nb + 1, 0);
+    for (uint32_t b = 0; b < nb; ++b) {
+      if (!drop_x[b]) nxc.ent.insert(nxc.ent.end(), X.ent.begin() + X.off[b], X.ent.begin() + X.off[b + 1]);
+      nxc.off[b + 1] = static_cast<uint32_t>(nxc.ent.size());
+    }
+    N = std::move(nn);
+    W = std::move(nwc);
+    X = std::move(nxc);
+  }
+}
+
 }  // namespace
 
 Plan build_plan(const double *sources, int64_t ns, const double *targets, int64_t nt,
@@ -510,6 +559,7 @@ Plan build_plan(const double *sources, int64_t ns, const double *targets, int64_
   }
   assign_points(sources, ns, plan, morton, morton_to_leaf, plan.src_start, plan.src_order, plan.us);
   assign_points(targets, nt, plan, morton, morton_to_leaf, plan.tgt_start, plan.tgt_order, plan.ut);
+  apply_near_rule(plan);
   return plan;
 }
 

@@ -143,6 +143,10 @@ int32_t fmmh_evaluate(fmmh_operator *op);
  * host (on_device = 0) or to device buffers.  Synchronises. */
 int32_t fmmh_read_output(fmmh_operator *op, double *phi, double *grad, int32_t on_device);
 int32_t fmmh_sync(fmmh_operator *op);
+/* Multipoles (locals = 0) or locals of one level, box-major (p+1)^2 doubles per box in
+ * the box index's order: the values an operator hands to the host store at the "send"
+ * events of HostData (fmm-plan/src/operator.rs:344-404).  Synchronises. */
+int32_t fmmh_read_expansions(fmmh_operator *op, int32_t locals, int32_t level, double *out);
 /* Per-kind device times (HIP events on the operator's stream), summed since the last
  * call with enable: ms[FMMH_NKINDS]. */
 int32_t fmmh_kind_timing(fmmh_operator *op, int32_t enable);

@@ -87,15 +87,19 @@ uint64_t orc_point_key(const double x[3], const double box[6], int level);
 typedef struct orc_fmm orc_fmm;
 
 /* Build the tree, lists and tables of an FMM over sources and targets
- * (FmmBuilder::build, fmm-exec/src/fmm.rs:1853, on one rank with the near rule off).
+ * (FmmBuilder::build, fmm-exec/src/fmm.rs:1853, on one rank).  near_w / near_x: the
+ * near rule's named crossovers (NearRuleChoice::Crossovers), negative for off.
  * Returns NULL on a degenerate extent. */
 orc_fmm *orc_fmm_new(const double *src, int64_t ns, const double *tgt, int64_t nt, int p,
-                     int64_t leaf_points, int max_level, int gradients);
+                     int64_t leaf_points, int max_level, int gradients, double near_w,
+                     double near_x);
 void orc_fmm_free(orc_fmm *f);
 /* Evaluate with one charge vector; output in caller order: phi[nt], grad[3 nt]. */
 void orc_fmm_evaluate(orc_fmm *f, const double *q, double *phi, double *grad);
 
 /* Inspection for the parity tests. */
+/* Multipoles (locals = 0) or locals of `level` after the last evaluation, box-major. */
+int orc_fmm_expansions(const orc_fmm *f, int locals, int level, double *out);
 int orc_fmm_nlevels(const orc_fmm *f);
 double orc_fmm_domain(const orc_fmm *f, double box[6]);
 int64_t orc_fmm_nboxes(const orc_fmm *f, int level);

@@ -77,7 +77,8 @@ def lib():
         L.orc_bounding_box.argtypes = [_f64p, C.c_int64, _f64p]
         L.orc_point_key.argtypes = [_f64p, _f64p, C.c_int]
         L.orc_point_key.restype = C.c_uint64
-        L.orc_fmm_new.argtypes = [_f64p, C.c_int64, _f64p, C.c_int64, C.c_int, C.c_int64, C.c_int, C.c_int]
+        L.orc_fmm_new.argtypes = [_f64p, C.c_int64, _f64p, C.c_int64, C.c_int, C.c_int64, C.c_int, C.c_int,
+                                  C.c_double, C.c_double]
         L.orc_fmm_new.restype = C.c_void_p
         L.orc_fmm_free.argtypes = [C.c_void_p]
         L.orc_fmm_evaluate.argtypes = [C.c_void_p, _f64p, _f64p, pd]
@@ -95,6 +96,7 @@ def lib():
         L.orc_fmm_list.restype = C.c_int64
         L.orc_fmm_point_leaves.argtypes = [C.c_void_p, C.c_int, _i64p]
         L.orc_fmm_leaf_coordinates.argtypes = [C.c_void_p, C.c_int, _f64p, _i64p]
+        L.orc_fmm_expansions.argtypes = [C.c_void_p, C.c_int, C.c_int, _f64p]
         _lib = L
     return _lib
 
@@ -311,13 +313,14 @@ def adjacent(a, b):
 class Fmm:
     """The oracle FMM (one rank, near rule off): tree, lists and f64 evaluation."""
 
-    def __init__(self, sources, targets=None, p=6, leaf_points=64, max_level=16, gradients=True):
+    def __init__(self, sources, targets=None, p=6, leaf_points=64, max_level=16, gradients=True, near_rule=None):
         self.sources = np.ascontiguousarray(sources, np.float64)
         self.targets = self.sources if targets is None else np.ascontiguousarray(targets, np.float64)
         self.gradients = gradients
         self.p = p
+        nw, nx = (-1.0, -1.0) if near_rule is None else (float(near_rule[0]), float(near_rule[1]))
         self._h = lib().orc_fmm_new(self.sources, len(self.sources), self.targets, len(self.targets),
-                                   p, leaf_points, max_level, int(gradients))
+                                   p, leaf_points, max_level, int(gradients), nw, nx)
         if not self._h:
             raise ValueError("degenerate point set")
 
@@ -378,6 +381,12 @@ class Fmm:
         u, order = np.zeros((n, 3)), np.zeros(n, np.int64)
         lib().orc_fmm_leaf_coordinates(self._h, int(targets), u, order)
         return u, order
+
+    def expansions(self, locals_: bool, level: int):
+        out = np.zeros((len(self.box_keys(level)), nc(self.p)))
+        if lib().orc_fmm_expansions(self._h, int(locals_), level, out) != 0:
+            raise ValueError("no evaluation yet")
+        return out
 
     def point_leaves(self, targets=False):
         n = len(self.targets) if targets else len(self.sources)

@@ -260,6 +260,7 @@ struct orc_fmm {
   int64_t *src_leaf, *tgt_leaf;   /* caller index -> leaf */
   double *us, *ut;                /* leaf-scaled coordinates, leaf order */
   double *m2m, *l2l, *m2l;        /* tables */
+  double **mult, **loc;           /* expansions of the last evaluation, per level */
 };
 
 static int64_t box_index(const orc_fmm *f, uint64_t key) {
@@ -393,6 +394,12 @@ static int64_t leaf_of_point(const orc_fmm *f, uint64_t fine) {
   return -1;
 }
 
+static void csr_free(csr *c) {
+  free(c->off);
+  free(c->ent);
+  free(c->aux);
+}
+
 static void csr_from_lists(csr *c, kvec *rows, int64_t nrows) {
   c->off = malloc(sizeof(int64_t) * (nrows + 1));
   c->off[0] = 0;
@@ -417,8 +424,75 @@ static int64_t *points_to_leaves(const orc_fmm *f, const double *pts, int64_t n,
   return leaf;
 }
 
+/* The near rule (fmm-plan/src/near.rs:15-40) with named crossovers: a W entry of a
+ * leaf that is itself a leaf moves to the near row when it holds fewer than near_w
+ * sources; the X entries of a leaf move to its near row when it holds fewer than
+ * near_x targets.  Near rows stay in leaf (level, key) order.  Negative: off. */
+static void apply_near_rule(orc_fmm *f, double near_w, double near_x) {
+  for (int l = 0; l < f->nlevels; ++l) {
+    int64_t l0 = f->leaf_off[l], nl = f->leaf_off[l + 1] - l0;
+    kvec *near = calloc(nl + 1, sizeof(kvec)), *wk = calloc(nl + 1, sizeof(kvec));
+    for (int64_t r = 0; r < nl; ++r) {
+      int64_t lf = l0 + r;
+      for (int64_t e = f->u[l].off[r]; e < f->u[l].off[r + 1]; ++e) kpush(&near[r], (uint64_t)f->u[l].ent[e]);
+      for (int64_t e = f->w[l].off[r]; e < f->w[l].off[r + 1]; ++e) {
+        int64_t sb = f->w[l].ent[e];
+        int sl = (int)f->w[l].aux[e];
+        int64_t leaf = f->box_leaf[sl][sb];
+        if (near_w >= 0 && leaf >= 0 && (double)(f->src_start[leaf + 1] - f->src_start[leaf]) < near_w)
+          kpush(&near[r], (uint64_t)leaf);
+        else
+          kpush(&wk[r], (uint64_t)e);
+      }
+      /* X row of this leaf's box */
+      int64_t b = f->leaf_box[lf];
+      if (near_x >= 0 && (double)(f->tgt_start[lf + 1] - f->tgt_start[lf]) < near_x) {
+        for (int64_t e = f->x[l].off[b]; e < f->x[l].off[b + 1]; ++e) kpush(&near[r], (uint64_t)f->x[l].ent[e]);
+      }
+      qsort(near[r].v, near[r].n, sizeof(uint64_t), cmp_u64);
+    }
+    /* rebuild near and W rows */
+    csr nu, nw;
+    csr_from_lists(&nu, near, nl);
+    csr_from_lists(&nw, wk, nl);
+    for (int64_t r = 0; r < nl; ++r) {
+      for (int64_t e = 0; e < near[r].n; ++e) nu.ent[nu.off[r] + e] = (int64_t)near[r].v[e];
+      for (int64_t e = 0; e < wk[r].n; ++e) {
+        nw.ent[nw.off[r] + e] = f->w[l].ent[wk[r].v[e]];
+        nw.aux[nw.off[r] + e] = f->w[l].aux[wk[r].v[e]];
+      }
+      free(near[r].v);
+      free(wk[r].v);
+    }
+    free(near);
+    free(wk);
+    csr_free(&f->u[l]);
+    csr_free(&f->w[l]);
+    f->u[l] = nu;
+    f->w[l] = nw;
+    /* X rows of converted leaves become empty */
+    int64_t nb = f->nbox[l];
+    kvec *xr = calloc(nb + 1, sizeof(kvec));
+    for (int64_t b = 0; b < nb; ++b) {
+      int64_t lf = f->box_leaf[l][b];
+      int drop = near_x >= 0 && lf >= 0 && (double)(f->tgt_start[lf + 1] - f->tgt_start[lf]) < near_x;
+      if (!drop)
+        for (int64_t e = f->x[l].off[b]; e < f->x[l].off[b + 1]; ++e) kpush(&xr[b], (uint64_t)f->x[l].ent[e]);
+    }
+    csr nx;
+    csr_from_lists(&nx, xr, nb);
+    for (int64_t b = 0; b < nb; ++b) {
+      for (int64_t e = 0; e < xr[b].n; ++e) nx.ent[nx.off[b] + e] = (int64_t)xr[b].v[e];
+      free(xr[b].v);
+    }
+    free(xr);
+    csr_free(&f->x[l]);
+    f->x[l] = nx;
+  }
+}
+
 orc_fmm *orc_fmm_new(const double *src, int64_t ns, const double *tgt, int64_t nt, int p,
-                     int64_t leaf_points, int max_level, int gradients) {
+                     int64_t leaf_points, int max_level, int gradients, double near_w, double near_x) {
   int64_t n = ns + nt;
   if (n == 0) return NULL;
   double *all = malloc(sizeof(double) * 3 * n);
@@ -588,6 +662,7 @@ orc_fmm *orc_fmm_new(const double *src, int64_t ns, const double *tgt, int64_t n
     for (int64_t i = f->tgt_start[lf]; i < f->tgt_start[lf + 1]; ++i)
       leaf_coords(f, tgt + 3 * f->tgt_order[i], f->leaf_key[lf], f->ut + 3 * i);
   }
+  if (near_w >= 0 || near_x >= 0) apply_near_rule(f, near_w, near_x);
   /* tables */
   size_t n2 = (size_t)nc(p) * nc(p);
   f->m2m = malloc(sizeof(double) * 8 * n2);
@@ -600,14 +675,16 @@ orc_fmm *orc_fmm_new(const double *src, int64_t ns, const double *tgt, int64_t n
   return f;
 }
 
-static void csr_free(csr *c) {
-  free(c->off);
-  free(c->ent);
-  free(c->aux);
-}
-
 void orc_fmm_free(orc_fmm *f) {
   if (!f) return;
+  if (f->mult) {
+    for (int l = 0; l < f->nlevels; ++l) {
+      free(f->mult[l]);
+      free(f->loc[l]);
+    }
+    free(f->mult);
+    free(f->loc);
+  }
   for (int l = 0; l < f->nlevels; ++l) {
     free(f->keys[l]);
     free(f->box_leaf[l]);
@@ -668,10 +745,18 @@ static void matvec_add(int n, const double *a, const double *x, double *y) {
 void orc_fmm_evaluate(orc_fmm *f, const double *q_caller, double *phi, double *grad) {
   int p = f->p, n = nc(p), L = f->nlevels - 1;
   size_t n2 = (size_t)n * n;
-  double **mult = calloc(f->nlevels, sizeof(double *)), **loc = calloc(f->nlevels, sizeof(double *));
+  if (!f->mult) {
+    f->mult = calloc(f->nlevels, sizeof(double *));
+    f->loc = calloc(f->nlevels, sizeof(double *));
+    for (int l = 0; l < f->nlevels; ++l) {
+      f->mult[l] = malloc(((size_t)f->nbox[l] * n + 1) * sizeof(double));
+      f->loc[l] = malloc(((size_t)f->nbox[l] * n + 1) * sizeof(double));
+    }
+  }
+  double **mult = f->mult, **loc = f->loc;
   for (int l = 0; l < f->nlevels; ++l) {
-    mult[l] = calloc((size_t)f->nbox[l] * n + 1, sizeof(double));
-    loc[l] = calloc((size_t)f->nbox[l] * n + 1, sizeof(double));
+    memset(mult[l], 0, ((size_t)f->nbox[l] * n + 1) * sizeof(double));
+    memset(loc[l], 0, ((size_t)f->nbox[l] * n + 1) * sizeof(double));
   }
   double *q = malloc(sizeof(double) * (f->ns + 1));
   for (int64_t i = 0; i < f->ns; ++i) q[i] = q_caller[f->src_order[i]];
@@ -762,12 +847,6 @@ void orc_fmm_evaluate(orc_fmm *f, const double *q_caller, double *phi, double *g
         for (int k = 0; k < 3; ++k) grad[3 * c + k] = gr[3 * i + k] / s2;
     }
   }
-  for (int l = 0; l < f->nlevels; ++l) {
-    free(mult[l]);
-    free(loc[l]);
-  }
-  free(mult);
-  free(loc);
   free(q);
   free(ph);
   free(gr);
@@ -776,6 +855,12 @@ void orc_fmm_evaluate(orc_fmm *f, const double *q_caller, double *phi, double *g
 /* ---- inspection --------------------------------------------------------------------- */
 
 int orc_fmm_nlevels(const orc_fmm *f) { return f->nlevels; }
+
+int orc_fmm_expansions(const orc_fmm *f, int locals, int level, double *out) {
+  if (!f->mult || level < 0 || level >= f->nlevels) return -1;
+  memcpy(out, locals ? f->loc[level] : f->mult[level], sizeof(double) * (size_t)f->nbox[level] * nc(f->p));
+  return 0;
+}
 
 double orc_fmm_domain(const orc_fmm *f, double box[6]) {
   memcpy(box, f->box, sizeof(double) * 6);

@@ -69,6 +69,20 @@ def test_plain_f64_matches_oracle(kind, n, p, leaf):
     assert np.array_equal(out.gradient, out2.gradient)
 
 
+@pytest.mark.parametrize("rule", [(float("inf"), float("inf")), (20.0, 10.0)])
+def test_near_rule_plain_f64_matches_oracle(rule):
+    _log(f"case near rule {rule}")
+    rng = O.Rng(41)
+    pts = np.vstack([rng.plummer(5000, (0.5, 0.5, 0.5), 0.05), rng.cube(3000)])
+    q = rng.normals(len(pts))
+    fmm = fmm_amd.FmmBuilder(7).max_points_per_leaf(24).near_rule(rule).build(pts)
+    out = fmm.evaluate(q)
+    ref = O.Fmm(pts, p=7, leaf_points=24, near_rule=rule)
+    rphi, rgrad = ref.evaluate(q)
+    assert _rel(out.potential, rphi) < 1e-12
+    assert _rel(out.gradient, rgrad, True) < 1e-12
+
+
 def test_level_calls_equal_evaluate():
     _log("case level calls")
     pts = _points("plummer", 6000, 3)
@@ -130,9 +144,9 @@ def test_mixed_precision_within_f32_contract():
     out = fmm.evaluate(q)
     ref = O.Fmm(pts, p=8, leaf_points=48)
     rphi, rgrad = ref.evaluate(q)
-    # f32 P2P: near-field terms to ~1e-7 relative
-    assert _rel(out.potential, rphi) < 1e-6
-    assert _rel(out.gradient, rgrad, True) < 1e-6
+    # f32 P2P with double-float separations: near-field terms to a few 2^-24 relative
+    assert _rel(out.potential, rphi) < 3e-7
+    assert _rel(out.gradient, rgrad, True) < 3e-7
 
 
 def test_published_error_reproduced_on_gpu():

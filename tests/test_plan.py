@@ -66,6 +66,25 @@ def test_plan_matches_oracle_bit_for_bit(name, pts, leaf):
     assert np.array_equal(up, uo)  # same f64 operations, same bits
 
 
+@pytest.mark.parametrize("rule", [(float("inf"), float("inf")), (10.0, 7.0), (0.0, 30.0)])
+def test_near_rule_matches_oracle(rule):
+    rng = O.Rng(77)
+    pts = np.vstack([rng.plummer(3000, (0.5, 0.5, 0.5), 0.05), rng.cube(1500)])
+    orc = O.Fmm(pts, p=1, leaf_points=12, near_rule=rule)
+    plan = fmm_amd.Plan(pts, leaf_points=12, near_rule=rule)
+    ref = fmm_amd.Plan(pts, leaf_points=12)
+    moved = 0
+    for level in range(plan.nlevels):
+        for kind in ("near", "w", "x"):
+            po, pe = plan.list(kind, level)
+            oo, oe = orc.list(kind, level)
+            assert np.array_equal(po.astype(np.int64), oo), (kind, level)
+            assert np.array_equal(pe.astype(np.int64), oe), (kind, level)
+        moved += len(plan.list("near", level)[1]) - len(ref.list("near", level)[1])
+    if rule[0] > 0:
+        assert moved > 0
+
+
 def test_separate_targets_match_oracle():
     rng = O.Rng(7)
     src = rng.plummer(2500, (0.5, 0.5, 0.5), 0.08)
