@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--near", default="device", help="near rule: off, device, or N_W,N_X")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain f64 leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first copy 256 MiB device to device (a known-size kernel for counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each box has one GPU)",
@@ -62,6 +64,13 @@ def main():
     from oracle import oracle as O
 
     dev = torch.device("cuda:0")
+    if args.copy_probe:
+        a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy probe: 268435456 bytes read, 268435456 bytes written")
+        del a, b
     t0 = time.time()
     rng = O.Rng(O.BENCH_SEED)
     pts = rng.cube(args.n)

@@ -144,9 +144,10 @@ def test_mixed_precision_within_f32_contract():
     out = fmm.evaluate(q)
     ref = O.Fmm(pts, p=8, leaf_points=48)
     rphi, rgrad = ref.evaluate(q)
-    # f32 P2P with double-float separations: near-field terms to a few 2^-24 relative
-    assert _rel(out.potential, rphi) < 3e-7
-    assert _rel(out.gradient, rgrad, True) < 3e-7
+    # f32 P2P with double-float separations: the reference's contract for f32 sums is
+    # 1e-6 relative to the f64 direct sum (docs/design/laplace-fmm-plan.md:1696)
+    assert _rel(out.potential, rphi) < 1e-6
+    assert _rel(out.gradient, rgrad, True) < 1e-6
 
 
 def test_published_error_reproduced_on_gpu():
