@@ -1,0 +1,395 @@
+// contract.hip -- grouped dense contractions on MFMA for the translations (M2M, L2L,
+// M2L): for every task of COLS output columns and every group of the task (one table
+// and one source expansion per column), Y[out[c]] += T_group X[src[group, c]].
+//
+// The tables of CONVENTIONS §3.12 are the A operand, staged through LDS in software-
+// pipelined chunks (the next chunk's table slice and source coefficients are fetched into
+// registers while the current one is multiplied); the source expansions are the B
+// operand, read straight from HBM/L2; each of the 8 waves of a workgroup owns 16 columns
+// and all M-tiles of them in registers.  The groups of a task are added in group order,
+// so every output adds its translations in the order of its list (offset index for M2L,
+// octant for M2M), the accumulation rule of fmm-plan/src/operator.rs:34-48.
+//
+// Two arithmetic variants:
+//  - f64: v_mfma_f64_16x16x4f64, every product and sum in f64;
+//  - bf16x3 (mixed precision): table and expansion are each split into three bf16 terms,
+//    x = hi + mid + lo (24 significant bits), and the six products above 2^-24
+//    (hi hi, hi mid, mid hi, hi lo, mid mid, lo hi) go through v_mfma_f32_16x16x32_bf16
+//    with f32 accumulation; the f32 sum of each group (one offset) is added into f64
+//    accumulators, so only one offset's worth of terms is ever summed in f32.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+
+#include "kernels.hpp"
+
+namespace fmmh {
+namespace dev {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+// ---- layouts -------------------------------------------------------------------------
+
+static int m_tiles(int p) { return ((p + 1) * (p + 1) + 15) / 16; }
+// f64: chunks of KC k-steps of 4; k = chunk KC + kk + (lane / 16) KS, KS = chunks KC
+static int k_chunks(int p) { return ((p + 1) * (p + 1) + 4 * KC - 1) / (4 * KC); }
+// bf16: k-steps of 32; k = 32 ks + 8 (lane / 16) + e
+static int k_steps16(int p) { return ((p + 1) * (p + 1) + 31) / 32; }
+
+size_t swizzled_table_size(int p) { return size_t(k_chunks(p)) * m_tiles(p) * KC * 64; }
+size_t swizzled_table_size_bf16(int p) { return size_t(k_steps16(p)) * 3 * m_tiles(p) * 64 * 8; }
+size_t split_expansion_size(int p) { return size_t(3) * k_steps16(p) * 32; }
+
+void swizzle_table(int p, const double *a, double *out) {
+  const int nc = (p + 1) * (p + 1), MT = m_tiles(p), NCH = k_chunks(p), KS = NCH * KC;
+  for (int c = 0; c < NCH; ++c)
+    for (int mt = 0; mt < MT; ++mt)
+      for (int kk = 0; kk < KC; ++kk)
+        for (int lane = 0; lane < 64; ++lane) {
+          const int row = mt * 16 + (lane & 15);
+          const int k = c * KC + kk + (lane >> 4) * KS;
+          const double v = (row < nc && k < nc) ? a[row + size_t(k) * nc] : 0.0;
+          out[((size_t(c) * MT + mt) * KC + kk) * 64 + lane] = v;
+        }
+}
+
+// Round to nearest even, f32 -> bf16 bits (finite inputs).
+__host__ __device__ static inline uint16_t bf16_bits(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+
+__host__ __device__ static inline double bf16_value(uint16_t b) {
+  const uint32_t u = uint32_t(b) << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return double(f);
+}
+
+// x = hi + mid + lo, each a bf16 (residuals formed exactly in f64).
+__host__ __device__ static inline void split3(double x, uint16_t &hi, uint16_t &mid, uint16_t &lo) {
+  hi = bf16_bits(float(x));
+  const double r1 = x - bf16_value(hi);
+  mid = bf16_bits(float(r1));
+  const double r2 = r1 - bf16_value(mid);
+  lo = bf16_bits(float(r2));
+}
+
+// [ks][plane][mt][lane][e]: row = 16 mt + lane % 16, k = 32 ks + 8 (lane / 16) + e
+void swizzle_table_bf16(int p, const double *a, uint16_t *out) {
+  const int nc = (p + 1) * (p + 1), MT = m_tiles(p), KS = k_steps16(p);
+  for (int ks = 0; ks < KS; ++ks)
+    for (int mt = 0; mt < MT; ++mt)
+      for (int lane = 0; lane < 64; ++lane)
+        for (int e = 0; e < 8; ++e) {
+          const int row = mt * 16 + (lane & 15), k = 32 * ks + 8 * (lane >> 4) + e;
+          const double v = (row < nc && k < nc) ? a[row + size_t(k) * nc] : 0.0;
+          uint16_t s[3];
+          split3(v, s[0], s[1], s[2]);
+          for (int pl = 0; pl < 3; ++pl)
+            out[((((size_t(ks) * 3 + pl) * MT + mt) * 64 + lane) * 8) + e] = s[pl];
+        }
+}
+
+// ---- f64 contraction ---------------------------------------------------------------------
+
+template <int MT>
+__global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, const uint32_t *task_group_off,
+                                                      const uint16_t *group_table, const uint32_t *group_src,
+                                                      const uint32_t *col_out, const double *__restrict__ tsw,
+                                                      const double *__restrict__ X, double *__restrict__ Y) {
+  constexpr int CHUNK2 = MT * KC * 32;  // double2 per chunk
+  constexpr int PER = (CHUNK2 + 511) / 512;
+  __shared__ double2 As[2][CHUNK2];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int col = wave * 16 + (lane & 15), g = lane >> 4;
+  const int KS = nchunks * KC;
+  const uint32_t task = blockIdx.x;
+  const uint32_t out = col_out[size_t(task) * COLS + col];
+  const uint32_t g0 = task_group_off[task], g1 = task_group_off[task + 1];
+  const int nsteps = int(g1 - g0) * nchunks;
+  if (nsteps == 0) return;  // uniform over the workgroup
+  d4 acc[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) acc[mt] = d4{0.0, 0.0, 0.0, 0.0};
+  double2 pa[PER];
+  double pb[KC], b[KC];
+  auto fetch = [&](int step) {
+    const uint32_t gi = g0 + uint32_t(step / nchunks);
+    const int c = step % nchunks;
+    const uint32_t t = group_table[gi];
+    const uint32_t src = group_src[size_t(gi) * COLS + col];
+    const double2 *gsrc = reinterpret_cast<const double2 *>(tsw + (size_t(t) * nchunks + c) * (CHUNK2 * 2));
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = tid + i * 512;
+      if (e < CHUNK2) pa[i] = gsrc[e];
+    }
+#pragma unroll
+    for (int kk = 0; kk < KC; ++kk) {
+      const int k = c * KC + kk + g * KS;
+      pb[kk] = (src != 0xFFFFFFFFu && k < nc) ? X[size_t(src) * nc + k] : 0.0;
+    }
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = tid + i * 512;
+      if (e < CHUNK2) As[buf][e] = pa[i];
+    }
+  };
+  fetch(0);
+  stash(0);
+#pragma unroll
+  for (int kk = 0; kk < KC; ++kk) b[kk] = pb[kk];
+  __syncthreads();
+  for (int step = 0; step < nsteps; ++step) {
+    const int buf = step & 1;
+    if (step + 1 < nsteps) fetch(step + 1);
+    const double *a = reinterpret_cast<const double *>(As[buf]);
+#pragma unroll
+    for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+        acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[(mt * KC + kk) * 64 + lane], b[kk], acc[mt], 0, 0, 0);
+    if (step + 1 < nsteps) stash(buf ^ 1);
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KC; ++kk) b[kk] = pb[kk];
+  }
+  if (out != 0xFFFFFFFFu) {
+    double *y = Y + size_t(out) * nc;
+    // accumulator element r of lane (g, j): row 4 r + g of the M-tile (f64 16x16x4)
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = mt * 16 + 4 * r + g;
+        if (i < nc) y[i] += acc[mt][r];
+      }
+  }
+}
+
+// ---- bf16x3 contraction ---------------------------------------------------------------------
+
+__device__ __forceinline__ bf8 as_bf8(uint4 v) {
+  bf8 r;
+  memcpy(&r, &v, 16);
+  return r;
+}
+
+template <int MT>
+__global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, const uint32_t *task_group_off,
+                                                       const uint16_t *group_table, const uint32_t *group_src,
+                                                       const uint32_t *col_out, const uint4 *__restrict__ tsw,
+                                                       const uint4 *__restrict__ X16, double *__restrict__ Y) {
+  constexpr int CHUNK = MT * 3 * 64;  // uint4 per k-step: [plane][mt][lane]
+  constexpr int PER = (CHUNK + 511) / 512;
+  __shared__ uint4 As[2][CHUNK];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int col = wave * 16 + (lane & 15), g = lane >> 4;
+  const int kq = ksteps * 4;  // uint4 per plane of a split expansion
+  const uint32_t task = blockIdx.x;
+  const uint32_t out = col_out[size_t(task) * COLS + col];
+  const uint32_t g0 = task_group_off[task], g1 = task_group_off[task + 1];
+  const int nsteps = int(g1 - g0) * ksteps;
+  if (nsteps == 0) return;
+  f4 acc[MT];
+  double sum[MT][4];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    acc[mt] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int r = 0; r < 4; ++r) sum[mt][r] = 0.0;
+  }
+  uint4 pa[PER], pb[3], b[3];
+  const uint4 zero4 = {0u, 0u, 0u, 0u};
+  auto fetch = [&](int step) {
+    const uint32_t gi = g0 + uint32_t(step / ksteps);
+    const int ks = step % ksteps;
+    const uint32_t t = group_table[gi];
+    const uint32_t src = group_src[size_t(gi) * COLS + col];
+    const uint4 *gsrc = tsw + (size_t(t) * ksteps + ks) * CHUNK;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = tid + i * 512;
+      if (e < CHUNK) pa[i] = gsrc[e];
+    }
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl)
+      pb[pl] = src != 0xFFFFFFFFu ? X16[(size_t(src) * 3 + pl) * kq + ks * 4 + g] : zero4;
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = tid + i * 512;
+      if (e < CHUNK) As[buf][e] = pa[i];
+    }
+  };
+  fetch(0);
+  stash(0);
+#pragma unroll
+  for (int pl = 0; pl < 3; ++pl) b[pl] = pb[pl];
+  __syncthreads();
+  for (int step = 0; step < nsteps; ++step) {
+    const int buf = step & 1;
+    if (step + 1 < nsteps) fetch(step + 1);
+    const bf8 bh = as_bf8(b[0]), bm = as_bf8(b[1]), bl = as_bf8(b[2]);
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const bf8 ah = as_bf8(As[buf][(0 * MT + mt) * 64 + lane]);
+      const bf8 am = as_bf8(As[buf][(1 * MT + mt) * 64 + lane]);
+      const bf8 al = as_bf8(As[buf][(2 * MT + mt) * 64 + lane]);
+      // smallest terms first
+      acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[mt], 0, 0, 0);
+      acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, acc[mt], 0, 0, 0);
+      acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[mt], 0, 0, 0);
+      acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, acc[mt], 0, 0, 0);
+      acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, acc[mt], 0, 0, 0);
+      acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[mt], 0, 0, 0);
+    }
+    if ((step + 1) % ksteps == 0) {
+      // end of a group (one offset): its f32 sum goes into the f64 accumulators
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sum[mt][r] += double(acc[mt][r]);
+        acc[mt] = f4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+    if (step + 1 < nsteps) stash(buf ^ 1);
+    __syncthreads();
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) b[pl] = pb[pl];
+  }
+  if (out != 0xFFFFFFFFu) {
+    double *y = Y + size_t(out) * nc;
+    // accumulator element r of lane (g, j): row 4 g + r of the M-tile (bf16 16x16x32)
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = mt * 16 + 4 * g + r;
+        if (i < nc) y[i] += sum[mt][r];
+      }
+  }
+}
+
+// Split expansions (f64, nc per box) into three bf16 planes of kp = 32 ksteps values.
+__global__ void k_split_bf16(size_t nboxes, int nc, int kp, const double *X, uint16_t *X16) {
+  const size_t total = nboxes * size_t(kp);
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < total; i += size_t(gridDim.x) * blockDim.x) {
+    const size_t box = i / kp;
+    const int k = int(i % kp);
+    uint16_t h = 0, m = 0, l = 0;
+    if (k < nc) split3(X[box * nc + k], h, m, l);
+    X16[(box * 3 + 0) * kp + k] = h;
+    X16[(box * 3 + 1) * kp + k] = m;
+    X16[(box * 3 + 2) * kp + k] = l;
+  }
+}
+
+// Ordered reduction of per-pair results: Y[row_box[r]] += sum_e S[row_ent[e]] in row
+// order, one workgroup per output box, one thread per coefficient.
+__global__ void k_reduce_slots(int nc, const uint32_t *row_box, const uint32_t *row_off, const uint32_t *row_ent,
+                               const double *S, double *Y) {
+  const uint32_t r = blockIdx.x;
+  const uint32_t box = row_box[r];
+  for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+    double acc = Y[size_t(box) * nc + i];
+    for (uint32_t e = row_off[r]; e < row_off[r + 1]; ++e) acc += S[size_t(row_ent[e]) * nc + i];
+    Y[size_t(box) * nc + i] = acc;
+  }
+}
+
+// ---- host wrappers ---------------------------------------------------------------------------
+
+void contract_f64(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off, const uint16_t *group_table,
+                  const uint32_t *group_src, const uint32_t *col_out, const double *tsw, const double *X,
+                  double *Y) {
+  if (ntasks == 0) return;
+  const int nc = (p + 1) * (p + 1), nch = k_chunks(p);
+#define FMMH_CONTRACT(MT)                                                                                        \
+  case MT:                                                                                                       \
+    hipLaunchKernelGGL(k_contract_f64<MT>, dim3(ntasks), dim3(512), 0, s, nc, nch, task_group_off, group_table,  \
+                       group_src, col_out, tsw, X, Y);                                                           \
+    break;
+  switch (m_tiles(p)) {
+    FMMH_CONTRACT(1)
+    FMMH_CONTRACT(2)
+    FMMH_CONTRACT(3)
+    FMMH_CONTRACT(4)
+    FMMH_CONTRACT(5)
+    FMMH_CONTRACT(6)
+    FMMH_CONTRACT(7)
+    FMMH_CONTRACT(8)
+    FMMH_CONTRACT(9)
+    FMMH_CONTRACT(10)
+    FMMH_CONTRACT(11)
+    FMMH_CONTRACT(12)
+    FMMH_CONTRACT(13)
+    FMMH_CONTRACT(14)
+    FMMH_CONTRACT(15)
+    FMMH_CONTRACT(16)
+    default:
+      break;
+  }
+#undef FMMH_CONTRACT
+}
+
+void contract_bf16(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off, const uint16_t *group_table,
+                   const uint32_t *group_src, const uint32_t *col_out, const uint16_t *tsw16, const uint16_t *X16,
+                   double *Y) {
+  if (ntasks == 0) return;
+  const int nc = (p + 1) * (p + 1), ks = k_steps16(p);
+  const uint4 *t4 = reinterpret_cast<const uint4 *>(tsw16);
+  const uint4 *x4 = reinterpret_cast<const uint4 *>(X16);
+#define FMMH_CONTRACT(MT)                                                                                        \
+  case MT:                                                                                                       \
+    hipLaunchKernelGGL(k_contract_bf16<MT>, dim3(ntasks), dim3(512), 0, s, nc, ks, task_group_off, group_table,  \
+                       group_src, col_out, t4, x4, Y);                                                           \
+    break;
+  switch (m_tiles(p)) {
+    FMMH_CONTRACT(1)
+    FMMH_CONTRACT(2)
+    FMMH_CONTRACT(3)
+    FMMH_CONTRACT(4)
+    FMMH_CONTRACT(5)
+    FMMH_CONTRACT(6)
+    FMMH_CONTRACT(7)
+    FMMH_CONTRACT(8)
+    FMMH_CONTRACT(9)
+    FMMH_CONTRACT(10)
+    FMMH_CONTRACT(11)
+    FMMH_CONTRACT(12)
+    FMMH_CONTRACT(13)
+    FMMH_CONTRACT(14)
+    FMMH_CONTRACT(15)
+    FMMH_CONTRACT(16)
+    default:
+      break;
+  }
+#undef FMMH_CONTRACT
+}
+
+void split_bf16(hipStream_t s, int p, size_t nboxes, const double *X, uint16_t *X16) {
+  if (nboxes == 0) return;
+  const int nc = (p + 1) * (p + 1), kp = 32 * k_steps16(p);
+  size_t total = nboxes * size_t(kp), grid = (total + 255) / 256;
+  if (grid > 65536) grid = 65536;
+  hipLaunchKernelGGL(k_split_bf16, dim3(unsigned(grid)), dim3(256), 0, s, nboxes, nc, kp, X, X16);
+}
+
+void reduce_slots(hipStream_t s, int p, uint32_t rows, const uint32_t *row_box, const uint32_t *row_off,
+                  const uint32_t *row_ent, const double *S, double *Y) {
+  if (rows == 0) return;
+  hipLaunchKernelGGL(k_reduce_slots, dim3(rows), dim3(256), 0, s, (p + 1) * (p + 1), row_box, row_off, row_ent, S, Y);
+}
+
+}  // namespace dev
+}  // namespace fmmh

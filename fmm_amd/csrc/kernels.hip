@@ -491,72 +491,6 @@ __global__ __launch_bounds__(EVAL_THREADS) void k_evaluate(int p, const uint32_t
   }
 }
 
-// ---- grouped dense contraction on MFMA (M2M, L2L, M2L) -------------------------------
-// Task = COLS output columns (boxes), 8 waves of 16.  For every group of the task (a
-// table and one source box per column) the workgroup streams the table through LDS in
-// chunks of KC k-steps and every wave issues v_mfma_f64_16x16x4f64 over all M-tiles of
-// its 16 columns, accumulating in registers: the groups of a task are added in group
-// order, so every target adds its rows in the order of its list (offset index for M2L,
-// octant for M2M).  Table layout (swizzle_table): [chunk][m-tile][kk][lane] with
-// lane -> (row = 16 mt + lane % 16, k = kk + KC chunk + (lane / 16) KS), zero-padded;
-// the B operand of lane (g = lane / 16, j = lane % 16) is coefficient k of column j's
-// source.  Accumulator element r of lane (g, j) is row 4 r + g of the M-tile (the
-// f64 16x16x4 accumulator layout, checked on gfx950).
-
-typedef double d4 __attribute__((ext_vector_type(4)));
-
-template <int MT>
-__global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, const uint32_t *task_group_off,
-                                                      const uint16_t *group_table, const uint32_t *group_src,
-                                                      const uint32_t *col_out, const double *__restrict__ tsw,
-                                                      const double *__restrict__ X, double *__restrict__ Y) {
-  constexpr int CHUNK = MT * KC * 64;
-  __shared__ double As[CHUNK];
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int col = wave * 16 + (lane & 15), g = lane >> 4;
-  const int KS = nchunks * KC;
-  const uint32_t task = blockIdx.x;
-  const uint32_t out = col_out[size_t(task) * COLS + col];
-  d4 acc[MT];
-#pragma unroll
-  for (int mt = 0; mt < MT; ++mt) acc[mt] = d4{0.0, 0.0, 0.0, 0.0};
-  const uint32_t g0 = task_group_off[task], g1 = task_group_off[task + 1];
-  for (uint32_t gi = g0; gi < g1; ++gi) {
-    const int t = group_table[gi];
-    const uint32_t src = group_src[size_t(gi) * COLS + col];
-    const double *xs = src != 0xFFFFFFFFu ? X + size_t(src) * nc : nullptr;
-    const double *tt = tsw + size_t(t) * nchunks * CHUNK;
-    for (int c = 0; c < nchunks; ++c) {
-      __syncthreads();
-      const double2 *gsrc = reinterpret_cast<const double2 *>(tt + size_t(c) * CHUNK);
-      double2 *ldst = reinterpret_cast<double2 *>(As);
-      for (int e = tid; e < CHUNK / 2; e += 512) ldst[e] = gsrc[e];
-      double b[KC];
-#pragma unroll
-      for (int kk = 0; kk < KC; ++kk) {
-        const int k = c * KC + kk + g * KS;
-        b[kk] = (xs != nullptr && k < nc) ? xs[k] : 0.0;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int kk = 0; kk < KC; ++kk)
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-          acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[(mt * KC + kk) * 64 + lane], b[kk], acc[mt], 0, 0, 0);
-    }
-  }
-  if (out != 0xFFFFFFFFu) {
-    double *y = Y + size_t(out) * nc;
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = mt * 16 + 4 * r + g;
-        if (i < nc) y[i] += acc[mt][r];
-      }
-  }
-}
-
 // ---- host wrappers ---------------------------------------------------------------------
 
 static inline unsigned grid_for(size_t n, int threads) {
@@ -638,57 +572,6 @@ void evaluate(hipStream_t s, int p, int irregular, int gradients, uint32_t rows,
       FMMH_EVAL(false, false);
   }
 #undef FMMH_EVAL
-}
-
-static int m_tiles(int p) { return ((p + 1) * (p + 1) + 15) / 16; }
-static int k_chunks(int p) { return ((p + 1) * (p + 1) + 4 * KC - 1) / (4 * KC); }
-
-size_t swizzled_table_size(int p) { return size_t(k_chunks(p)) * m_tiles(p) * KC * 64; }
-
-void swizzle_table(int p, const double *a, double *out) {
-  const int nc = (p + 1) * (p + 1), MT = m_tiles(p), NCH = k_chunks(p), KS = NCH * KC;
-  for (int c = 0; c < NCH; ++c)
-    for (int mt = 0; mt < MT; ++mt)
-      for (int kk = 0; kk < KC; ++kk)
-        for (int lane = 0; lane < 64; ++lane) {
-          const int row = mt * 16 + (lane & 15);
-          const int k = c * KC + kk + (lane >> 4) * KS;
-          const double v = (row < nc && k < nc) ? a[row + size_t(k) * nc] : 0.0;
-          out[((size_t(c) * MT + mt) * KC + kk) * 64 + lane] = v;
-        }
-}
-
-void contract_f64(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off, const uint16_t *group_table,
-                  const uint32_t *group_src, const uint32_t *col_out, const double *tsw, const double *X,
-                  double *Y) {
-  if (ntasks == 0) return;
-  const int nc = (p + 1) * (p + 1), nch = k_chunks(p);
-#define FMMH_CONTRACT(MT)                                                                                        \
-  case MT:                                                                                                       \
-    hipLaunchKernelGGL(k_contract_f64<MT>, dim3(ntasks), dim3(512), 0, s, nc, nch, task_group_off, group_table,  \
-                       group_src, col_out, tsw, X, Y);                                                           \
-    break;
-  switch (m_tiles(p)) {
-    FMMH_CONTRACT(1)
-    FMMH_CONTRACT(2)
-    FMMH_CONTRACT(3)
-    FMMH_CONTRACT(4)
-    FMMH_CONTRACT(5)
-    FMMH_CONTRACT(6)
-    FMMH_CONTRACT(7)
-    FMMH_CONTRACT(8)
-    FMMH_CONTRACT(9)
-    FMMH_CONTRACT(10)
-    FMMH_CONTRACT(11)
-    FMMH_CONTRACT(12)
-    FMMH_CONTRACT(13)
-    FMMH_CONTRACT(14)
-    FMMH_CONTRACT(15)
-    FMMH_CONTRACT(16)
-    default:
-      break;
-  }
-#undef FMMH_CONTRACT
 }
 
 }  // namespace dev

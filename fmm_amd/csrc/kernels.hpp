@@ -49,15 +49,27 @@ void evaluate(hipStream_t s, int p, int irregular, int gradients, uint32_t rows,
               const uint32_t *leaf_cell, const uint32_t *tgt_start, const double *tx, const double *ty,
               const double *tz, const double *coeffs, double *phi, double *grad);
 
-// Grouped dense contraction on MFMA (f64): for every task (COLS output columns) and
-// every group of the task, Y[out[c]] += T_group X[src[g, c]].
+// Grouped dense contraction on MFMA (contract.hip): for every task (COLS output columns)
+// and every group of the task, Y[out[c]] += T_group X[src[g, c]]; f64, or bf16x3 split
+// with f32 accumulation per group and f64 across groups.
 void contract_f64(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off,
                   const uint16_t *group_table, const uint32_t *group_src, const uint32_t *col_out,
                   const double *tables_swizzled, const double *X, double *Y);
+void contract_bf16(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off,
+                   const uint16_t *group_table, const uint32_t *group_src, const uint32_t *col_out,
+                   const uint16_t *tables_swizzled, const uint16_t *X16, double *Y);
+// X16: three bf16 planes (hi, mid, lo) of split_expansion_size(p) / 3 values per box
+void split_bf16(hipStream_t s, int p, size_t nboxes, const double *X, uint16_t *X16);
+// Y[row_box[r]] += sum of S rows row_ent[row_off[r] .. row_off[r+1]) in order
+void reduce_slots(hipStream_t s, int p, uint32_t rows, const uint32_t *row_box, const uint32_t *row_off,
+                  const uint32_t *row_ent, const double *S, double *Y);
 
-// Swizzled table size (doubles) per table at degree p, and the host-side swizzle.
+// Swizzled table sizes (elements) per table at degree p, and the host-side swizzles.
 size_t swizzled_table_size(int p);
+size_t swizzled_table_size_bf16(int p);
+size_t split_expansion_size(int p);
 void swizzle_table(int p, const double *table_colmajor, double *out);
+void swizzle_table_bf16(int p, const double *table_colmajor, uint16_t *out);
 
 }  // namespace dev
 }  // namespace fmmh

@@ -52,6 +52,17 @@ struct ScheduleBuilder {
     task_group_off.push_back(static_cast<uint32_t>(group_table.size()));
   }
 
+  void append(const ScheduleBuilder &o) {
+    const uint32_t base = static_cast<uint32_t>(group_table.size());
+    for (size_t t = 1; t < o.task_group_off.size(); ++t) task_group_off.push_back(base + o.task_group_off[t]);
+    group_src.insert(group_src.end(), o.group_src.begin(), o.group_src.end());
+    col_out.insert(col_out.end(), o.col_out.begin(), o.col_out.end());
+    group_table.insert(group_table.end(), o.group_table.begin(), o.group_table.end());
+    pairs += o.pairs;
+    slots += o.slots;
+  }
+  uint32_t ntasks() const { return static_cast<uint32_t>(task_group_off.size() - 1); }
+
   void finish(Schedule &s) {
     s.ntasks = static_cast<uint32_t>(task_group_off.size() - 1);
     s.pairs = pairs;
@@ -161,9 +172,8 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
   q_.alloc(ns_);
   q_in_.alloc(ns_);
 
-  // expansions and outputs
+  // expansions and outputs (loc_ gains the pair-mode scratch rows in build_schedules)
   mult_.alloc(nbox_total_ * nc_);
-  loc_.alloc(nbox_total_ * nc_);
   phi_hat_.alloc(nt_);
   grad_hat_.alloc(opt_.gradients ? 3 * nt_ : 0);
   phi_.alloc(nt_);
@@ -181,7 +191,16 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
     };
     swz(m2m, 8, t_m2m_);
     swz(l2l, 8, t_l2l_);
-    if (nlevels_ > 2) swz(m2l, 316, t_m2l_);
+    if (nlevels_ > 2) {
+      if (opt_.precision == kMixed) {
+        const size_t sw16 = dev::swizzled_table_size_bf16(opt_.p);
+        std::vector<uint16_t> out(sw16 * 316);
+        for (int i = 0; i < 316; ++i) dev::swizzle_table_bf16(opt_.p, m2l.data() + i * n2, out.data() + i * sw16);
+        t_m2l16_.upload(out);
+      } else {
+        swz(m2l, 316, t_m2l_);
+      }
+    }
   }
 
   // leaf-kernel rows
@@ -230,21 +249,26 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
     }
     upload_rows(m2p_rows_, id, cell, off, ent);
   }
-  p2l_rows_.resize(nlevels_);
-  for (int l = 1; l < nlevels_; ++l) {
+  {
     std::vector<uint32_t> id, cell, off{0}, ent;
-    const Csr &X = plan.x[l];
-    for (uint32_t b = 0; b < X.rows(); ++b) {
-      if (X.off[b + 1] == X.off[b]) continue;
-      const size_t g = gid(l, b);
-      id.push_back(uint32_t(g));
-      cell.insert(cell.end(), {box_cell[4 * g], box_cell[4 * g + 1], box_cell[4 * g + 2], box_cell[4 * g + 3]});
-      for (uint32_t e = X.off[b]; e < X.off[b + 1]; ++e) ent.push_back(X.ent[e]);
-      off.push_back(static_cast<uint32_t>(ent.size()));
+    p2l_level_.assign(nlevels_ + 1, 0);
+    for (int l = 0; l < nlevels_; ++l) {
+      const Csr &X = plan.x[l];
+      for (uint32_t b = 0; l >= 1 && b < X.rows(); ++b) {
+        if (X.off[b + 1] == X.off[b]) continue;
+        const size_t g = gid(l, b);
+        id.push_back(uint32_t(g));
+        cell.insert(cell.end(), {box_cell[4 * g], box_cell[4 * g + 1], box_cell[4 * g + 2], box_cell[4 * g + 3]});
+        for (uint32_t e = X.off[b]; e < X.off[b + 1]; ++e) ent.push_back(X.ent[e]);
+        off.push_back(static_cast<uint32_t>(ent.size()));
+      }
+      p2l_level_[l + 1] = static_cast<uint32_t>(id.size());
     }
-    upload_rows(p2l_rows_[l], id, cell, off, ent);
+    upload_rows(p2l_rows_, id, cell, off, ent);
   }
   build_schedules(plan);
+  loc_.alloc((nbox_total_ + nslots_) * nc_);
+  if (opt_.precision == kMixed) mult16_.alloc(nbox_total_ * dev::split_expansion_size(opt_.p));
   hip_check(hipDeviceSynchronize(), "operator build");
 }
 
@@ -307,7 +331,6 @@ void DeviceOperator::build_schedules(const Plan &plan) {
   const int C = dev::COLS;
   m2m_.resize(nlevels_);
   l2l_.resize(nlevels_);
-  m2l_.resize(nlevels_);
   for (int l = 0; l < nlevels_; ++l) {
     // M2M into the parents of level l, by octant (fmm-plan/src/operator.rs:256-262)
     if (l + 1 < nlevels_) {
@@ -350,10 +373,20 @@ void DeviceOperator::build_schedules(const Plan &plan) {
       }
       sb.finish(l2l_[l]);
     }
+  }
+  // M2L of every level in one schedule.  Dense levels: tiles of COLS targets of one
+  // octant, whose groups are the offsets their parity allows.  Sparse levels (most of
+  // a tile's column slots without a source): pair mode, tasks of COLS pairs of one
+  // offset writing scratch rows, then an ordered reduction per target.
+  ScheduleBuilder all;
+  std::vector<uint32_t> red_id, red_cell, red_off{0}, red_ent;
+  m2l_level_task_.assign(nlevels_ + 1, 0);
+  m2l_red_level_.assign(nlevels_ + 1, 0);
+  nslots_ = 0;
+  for (int l = 0; l < nlevels_; ++l) {
     if (l >= 2) {
-      // M2L: targets of one octant share the 189 offsets their parity allows
-      ScheduleBuilder sb;
       const Csr &V = plan.v[l];
+      ScheduleBuilder tile;
       for (int o = 0; o < 8; ++o) {
         std::vector<uint32_t> cols;
         std::vector<std::vector<std::pair<uint16_t, uint32_t>>> lists;
@@ -364,37 +397,90 @@ void DeviceOperator::build_schedules(const Plan &plan) {
           for (uint32_t e = V.off[b]; e < V.off[b + 1]; ++e) li.push_back({V.aux[e], gid(l, V.ent[e])});
           lists.push_back(std::move(li));
           if (int(cols.size()) == C) {
-            sb.add_task(cols, lists);
+            tile.add_task(cols, lists);
             cols.clear();
             lists.clear();
           }
         }
-        if (!cols.empty()) sb.add_task(cols, lists);
+        if (!cols.empty()) tile.add_task(cols, lists);
       }
-      sb.finish(m2l_[l]);
+      if (tile.pairs == 0 || 2 * tile.pairs >= tile.slots) {
+        all.append(tile);
+      } else {
+        // pair mode: slot of every V pair, by offset then target
+        std::vector<std::vector<std::pair<uint32_t, uint32_t>>> by_d(316);  // (V entry, target box)
+        for (uint32_t b = 0; b < plan.nboxes(l); ++b)
+          for (uint32_t e = V.off[b]; e < V.off[b + 1]; ++e) by_d[V.aux[e]].push_back({e, b});
+        std::vector<uint32_t> slot_of(V.ent.size());
+        ScheduleBuilder pairs;
+        for (int d = 0; d < 316; ++d) {
+          const auto &list = by_d[d];
+          for (size_t at = 0; at < list.size(); at += C) {
+            std::vector<uint32_t> cols;
+            std::vector<std::vector<std::pair<uint16_t, uint32_t>>> lists;
+            for (size_t k = at; k < list.size() && k < at + C; ++k) {
+              const uint32_t e = list[k].first;
+              slot_of[e] = static_cast<uint32_t>(nbox_total_ + nslots_++);
+              cols.push_back(slot_of[e]);
+              lists.push_back({{uint16_t(d), gid(l, V.ent[e])}});
+            }
+            pairs.add_task(cols, lists);
+          }
+        }
+        pairs.slots = pairs.ntasks() * uint64_t(C);
+        all.append(pairs);
+        for (uint32_t b = 0; b < plan.nboxes(l); ++b) {
+          if (V.off[b + 1] == V.off[b]) continue;
+          red_id.push_back(gid(l, b));
+          for (uint32_t e = V.off[b]; e < V.off[b + 1]; ++e) red_ent.push_back(slot_of[e]);
+          red_off.push_back(static_cast<uint32_t>(red_ent.size()));
+        }
+      }
     }
+    m2l_level_task_[l + 1] = all.ntasks();
+    m2l_red_level_[l + 1] = static_cast<uint32_t>(red_id.size());
   }
-}
-
-uint64_t DeviceOperator::m2l_pairs() const {
-  uint64_t s = 0;
-  for (const auto &x : m2l_) s += x.pairs;
-  return s;
-}
-
-uint64_t DeviceOperator::m2l_slots() const {
-  uint64_t s = 0;
-  for (const auto &x : m2l_) s += x.slots;
-  return s;
+  all.finish(m2l_);
+  upload_rows(m2l_red_, red_id, red_cell, red_off, red_ent);
 }
 
 size_t DeviceOperator::device_bytes() const {
   return mult_.bytes() + loc_.bytes() + t_m2l_.bytes() + t_m2m_.bytes() + t_l2l_.bytes() + sx_.bytes() * 8;
 }
 
-void DeviceOperator::launch_contract(const Schedule &s, const double *X, double *Y, const DevBuf<double> &tables) {
-  dev::contract_f64(stream_, opt_.p, s.ntasks, s.task_group_off.ptr, s.group_table.ptr, s.group_src.ptr,
-                    s.col_out.ptr, tables.ptr, X, Y);
+void DeviceOperator::launch_contract(const Schedule &s, uint32_t task_first, uint32_t ntasks, const double *X,
+                                     double *Y, const DevBuf<double> &tables) {
+  dev::contract_f64(stream_, opt_.p, ntasks, s.task_group_off.ptr + task_first, s.group_table.ptr, s.group_src.ptr,
+                    s.col_out.ptr + size_t(task_first) * dev::COLS, tables.ptr, X, Y);
+}
+
+// M2L of the levels level_first .. level_last: the contraction tasks, then the ordered
+// reduction of the pair-mode levels.
+void DeviceOperator::launch_m2l(int level_first, int level_last) {
+  const uint32_t t0 = m2l_level_task_[level_first], t1 = m2l_level_task_[level_last + 1];
+  if (t1 > t0) {
+    if (opt_.precision == kMixed) {
+      const size_t b0 = box_base_[level_first], b1 = box_base_[level_last + 1];
+      const size_t sz = dev::split_expansion_size(opt_.p);
+      dev::split_bf16(stream_, opt_.p, b1 - b0, mult_.ptr + b0 * nc_, mult16_.ptr + b0 * sz);
+      dev::contract_bf16(stream_, opt_.p, t1 - t0, m2l_.task_group_off.ptr + t0, m2l_.group_table.ptr,
+                         m2l_.group_src.ptr, m2l_.col_out.ptr + size_t(t0) * dev::COLS, t_m2l16_.ptr, mult16_.ptr,
+                         loc_.ptr);
+    } else {
+      launch_contract(m2l_, t0, t1 - t0, mult_.ptr, loc_.ptr, t_m2l_);
+    }
+  }
+  const uint32_t r0 = m2l_red_level_[level_first], r1 = m2l_red_level_[level_last + 1];
+  if (r1 > r0)
+    dev::reduce_slots(stream_, opt_.p, r1 - r0, m2l_red_.row_id.ptr + r0, m2l_red_.row_off.ptr + r0,
+                      m2l_red_.row_ent.ptr, loc_.ptr, loc_.ptr);
+}
+
+void DeviceOperator::launch_p2l(int level_first, int level_last) {
+  const uint32_t r0 = p2l_level_[level_first], r1 = p2l_level_[level_last + 1];
+  dev::expand(stream_, opt_.p, 1, r1 - r0, p2l_rows_.row_id.ptr + r0, p2l_rows_.row_cell.ptr + 4 * size_t(r0),
+              p2l_rows_.row_off.ptr + r0, p2l_rows_.row_ent.ptr, leaf_cell_.ptr, src_start_.ptr, sx_.ptr, sy_.ptr,
+              sz_.ptr, q_.ptr, loc_.ptr);
 }
 
 void DeviceOperator::load_charges(const double *charges, bool on_device) {
@@ -427,30 +513,26 @@ void DeviceOperator::p2m(int level) {
 void DeviceOperator::m2m(int level) {
   if (level < 0 || level + 1 >= nlevels_) return;
   timed(1, [&] {
-    launch_contract(m2m_[level], mult_.ptr, mult_.ptr, t_m2m_);
+    launch_contract(m2m_[level], 0, m2m_[level].ntasks, mult_.ptr, mult_.ptr, t_m2m_);
   });
 }
 
 void DeviceOperator::m2l(int level) {
   if (level < 2 || level >= nlevels_) return;
   timed(3, [&] {
-    launch_contract(m2l_[level], mult_.ptr, loc_.ptr, t_m2l_);
+    launch_m2l(level, level);
   });
 }
 
 void DeviceOperator::p2l(int level) {
   if (level < 1 || level >= nlevels_) return;
-  timed(4, [&] {
-    const Rows &r = p2l_rows_[level];
-    dev::expand(stream_, opt_.p, 1, r.rows, r.row_id.ptr, r.row_cell.ptr, r.row_off.ptr, r.row_ent.ptr, leaf_cell_.ptr,
-                src_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, q_.ptr, loc_.ptr);
-  });
+  timed(4, [&] { launch_p2l(level, level); });
 }
 
 void DeviceOperator::l2l(int level) {
   if (level < 1 || level >= nlevels_) return;
   timed(2, [&] {
-    launch_contract(l2l_[level], loc_.ptr, loc_.ptr, t_l2l_);
+    launch_contract(l2l_[level], 0, l2l_[level].ntasks, loc_.ptr, loc_.ptr, t_l2l_);
   });
 }
 
@@ -503,12 +585,12 @@ void DeviceOperator::evaluate() {
                 q_.ptr, mult_.ptr);
   });
   for (int l = L - 1; l >= 0; --l) m2m(l);
-  // downward
-  for (int l = 1; l <= L; ++l) {
-    l2l(l);
-    m2l(l);
-    p2l(l);
-  }
+  // downward: M2L and P2L read only multipoles and points, so every level's M2L is one
+  // launch and every level's P2L another; then the L2L chain from the root down.  Each
+  // local receives its M2L, P2L and L2L terms, each kind in the order of its row.
+  if (L >= 2) timed(3, [&] { launch_m2l(2, L); });
+  if (L >= 1) timed(4, [&] { launch_p2l(1, L); });
+  for (int l = 1; l <= L; ++l) l2l(l);
   // leaves: each kind over every level at once; a target still receives L2P, M2P and
   // P2P contributions in that order
   timed(5, [&] {

@@ -20,7 +20,7 @@ namespace fmmh {
 
 enum Precision : int {
   kPlainF64 = 0,  // every operator in f64
-  kMixed = 1,     // P2P in f32; translations in f64 (split-precision M2L to follow)
+  kMixed = 1,     // P2P in f32 (double-float separations), M2L in bf16x3 on MFMA
 };
 
 void hip_check(hipError_t e, const char *what);
@@ -119,8 +119,8 @@ class DeviceOperator {
   const OperatorOptions &options() const { return opt_; }
   // statistics for reports and the roofline
   uint64_t p2p_pairs() const { return p2p_pairs_; }
-  uint64_t m2l_pairs() const;
-  uint64_t m2l_slots() const;
+  uint64_t m2l_pairs() const { return m2l_.pairs; }
+  uint64_t m2l_slots() const { return m2l_.slots; }
   size_t device_bytes() const;
   double *device_locals() const { return loc_.ptr; }
   double *device_multipoles() const { return mult_.ptr; }
@@ -132,7 +132,10 @@ class DeviceOperator {
   void timed(int kind, F &&f);
   void collect_timings();
   void launch_p2p(uint32_t leaf_first, uint32_t count);
-  void launch_contract(const Schedule &s, const double *X, double *Y, const DevBuf<double> &tables);
+  void launch_contract(const Schedule &s, uint32_t task_first, uint32_t ntasks, const double *X, double *Y,
+                       const DevBuf<double> &tables);
+  void launch_m2l(int level_first, int level_last);
+  void launch_p2l(int level_first, int level_last);
 
   OperatorOptions opt_;
   int nlevels_ = 0, nc_ = 0;
@@ -149,6 +152,8 @@ class DeviceOperator {
   DevBuf<double> sx_, sy_, sz_, tx_, ty_, tz_, q_, q_in_;
   DevBuf<double> mult_, loc_, phi_hat_, grad_hat_, phi_, grad_;
   DevBuf<double> t_m2m_, t_l2l_, t_m2l_;
+  DevBuf<uint16_t> t_m2l16_, mult16_;  // mixed precision: split tables and multipoles
+  size_t nslots_ = 0;                  // scratch rows after the boxes in loc_ (pair mode)
 
   bool timing_ = false;
   struct Mark {
@@ -163,8 +168,13 @@ class DeviceOperator {
 
   Rows p2m_rows_, l2p_rows_, m2p_rows_;
   std::vector<uint32_t> p2m_level_, l2p_level_, m2p_level_;  // row ranges per level
-  std::vector<Rows> p2l_rows_;                               // per level
-  std::vector<Schedule> m2m_, l2l_, m2l_;                    // per level
+  Rows p2l_rows_;                                            // every level, by level
+  std::vector<uint32_t> p2l_level_;
+  std::vector<Schedule> m2m_, l2l_;                          // per level
+  Schedule m2l_;                                             // every level, by level
+  std::vector<uint32_t> m2l_level_task_;                     // task range per level
+  Rows m2l_red_;                                             // pair-mode reductions
+  std::vector<uint32_t> m2l_red_level_;
 };
 
 }  // namespace fmmh
