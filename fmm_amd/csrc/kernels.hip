@@ -110,8 +110,8 @@ __global__ void k_output(const uint32_t *tgt_start, const uint32_t *leaf_cell, c
 }
 
 // ---- P2P (CONVENTIONS §3.13 "Device kernels") ---------------------------------------
-// One workgroup per target leaf, TPT targets per thread in registers; every source leaf
-// of the near row is staged through LDS in tiles of P2P_TILE points, with
+// One wave per target leaf, targets in registers; every source leaf of the near row is
+// staged through LDS in tiles of 64 points, with
 // y^ = fma(r^, u_s, c^) formed once per source in f64 (u_s itself for s = t).  A pair adds
 // nothing when r^2 == 0: the select comes before the charge is touched.
 //
@@ -122,27 +122,42 @@ __global__ void k_output(const uint32_t *tgt_start, const uint32_t *leaf_cell, c
 // r^2, the inverse square root and the sums are f32, added into the f64 output once per
 // target.  Coincident points give d = 0 exactly in both forms.
 
-template <typename T, bool GRAD, int TPT>
-__global__ __launch_bounds__(P2P_THREADS) void k_p2p(uint32_t leaf_first, const uint32_t *near_off,
-                                                     const uint32_t *near_ent, const uint32_t *leaf_cell,
-                                                     const uint32_t *src_start, const uint32_t *tgt_start,
-                                                     const double *sx, const double *sy, const double *sz,
-                                                     const double *sq, const double *tx, const double *ty,
-                                                     const double *tz, double *phi, double *grad) {
+// Lane mapping (one wave per target leaf, targets processed in chunks of 128): a chunk of
+// n <= 64 targets uses np = the next power of two >= n lanes per source phase and
+// 64 / np phases, phase f taking the sources j = f, f + 64/np, ... of every tile; a chunk
+// of 65..128 targets puts two targets in each lane.  The phases' partial sums are added
+// by a fixed butterfly of lane shuffles, so results are the same on every run.
+template <typename T, bool GRAD>
+__global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, const uint32_t *near_off, const uint32_t *near_ent,
+                                            const uint32_t *leaf_cell, const uint32_t *src_start,
+                                            const uint32_t *tgt_start, const double *sx, const double *sy,
+                                            const double *sz, const double *sq, const double *tx, const double *ty,
+                                            const double *tz, double *phi, double *grad) {
   constexpr bool SPLIT = sizeof(T) == 4;
   constexpr int NY = SPLIT ? 7 : 4;
-  __shared__ T ys[NY][P2P_TILE];
+  constexpr int TILE = 64;
+  __shared__ T ys[NY][TILE];
+  const int lane = threadIdx.x;
   const uint32_t lf = leaf_first + blockIdx.x;
   const uint32_t t0 = tgt_start[lf], t1 = tgt_start[lf + 1];
   if (t0 == t1) return;
   const uint32_t *ct = leaf_cell + 4 * size_t(lf);
-  for (uint32_t tb = t0; tb < t1; tb += P2P_THREADS * TPT) {
-    T ux[TPT], uy[TPT], uz[TPT], lx[TPT], ly[TPT], lz[TPT], ap[TPT], ax[TPT], ay[TPT], az[TPT];
+  for (uint32_t tb = t0; tb < t1; tb += 128) {
+    const uint32_t n = (t1 - tb) < 128u ? (t1 - tb) : 128u;
+    int np = 64, phases = 1;
+    if (n <= 64) {
+      np = 1;
+      while (np < int(n)) np <<= 1;
+      phases = 64 / np;
+    }
+    const int slot = lane & (np - 1), phase = lane / np;
+    T ux[2], uy[2], uz[2], lx[2], ly[2], lz[2], ap[2], ax[2], ay[2], az[2];
+    bool ok[2];
 #pragma unroll
-    for (int a = 0; a < TPT; ++a) {
-      const uint32_t i = tb + threadIdx.x + a * P2P_THREADS;
-      const bool ok = i < t1;
-      const double x = ok ? tx[i] : 0.0, y = ok ? ty[i] : 0.0, z = ok ? tz[i] : 0.0;
+    for (int a = 0; a < 2; ++a) {
+      const uint32_t i = tb + slot + a * 64;
+      ok[a] = (a == 0) ? (uint32_t(slot) < n) : (n > 64 && uint32_t(slot + 64) < n);
+      const double x = ok[a] ? tx[i] : 0.0, y = ok[a] ? ty[i] : 0.0, z = ok[a] ? tz[i] : 0.0;
       ux[a] = T(x);
       uy[a] = T(y);
       uz[a] = T(z);
@@ -157,26 +172,26 @@ __global__ __launch_bounds__(P2P_THREADS) void k_p2p(uint32_t leaf_first, const 
       double c[3], r;
       const bool self = sl == lf;
       relative_frame<double>(leaf_cell + 4 * size_t(sl), ct, c, r);
-      for (uint32_t sb = s0; sb < s1; sb += P2P_TILE) {
-        const uint32_t cnt = (s1 - sb) < uint32_t(P2P_TILE) ? (s1 - sb) : uint32_t(P2P_TILE);
+      for (uint32_t sb = s0; sb < s1; sb += TILE) {
+        const int cnt = int((s1 - sb) < uint32_t(TILE) ? (s1 - sb) : uint32_t(TILE));
         __syncthreads();
-        for (uint32_t j = threadIdx.x; j < cnt; j += P2P_THREADS) {
-          const uint32_t s = sb + j;
+        if (lane < cnt) {
+          const uint32_t s = sb + lane;
           const double yx = self ? sx[s] : fma(r, sx[s], c[0]);
           const double yy = self ? sy[s] : fma(r, sy[s], c[1]);
           const double yz = self ? sz[s] : fma(r, sz[s], c[2]);
-          ys[0][j] = T(yx);
-          ys[1][j] = T(yy);
-          ys[2][j] = T(yz);
-          ys[3][j] = T(sq[s]);
+          ys[0][lane] = T(yx);
+          ys[1][lane] = T(yy);
+          ys[2][lane] = T(yz);
+          ys[3][lane] = T(sq[s]);
           if (SPLIT) {
-            ys[4][j] = T(yx - double(T(yx)));
-            ys[5][j] = T(yy - double(T(yy)));
-            ys[6][j] = T(yz - double(T(yz)));
+            ys[4][lane] = T(yx - double(T(yx)));
+            ys[5][lane] = T(yy - double(T(yy)));
+            ys[6][lane] = T(yz - double(T(yz)));
           }
         }
         __syncthreads();
-        for (uint32_t j = 0; j < cnt; ++j) {
+        for (int j = phase; j < cnt; j += phases) {
           const T yx = ys[0][j], yy = ys[1][j], yz = ys[2][j], q = ys[3][j];
           T yxl = T(0), yyl = T(0), yzl = T(0);
           if (SPLIT) {
@@ -185,7 +200,8 @@ __global__ __launch_bounds__(P2P_THREADS) void k_p2p(uint32_t leaf_first, const 
             yzl = ys[6][j];
           }
 #pragma unroll
-          for (int a = 0; a < TPT; ++a) {
+          for (int a = 0; a < 2; ++a) {
+            if (a == 1 && n <= 64) break;
             T dx = ux[a] - yx, dy = uy[a] - yy, dz = uz[a] - yz;
             if (SPLIT) {
               dx += lx[a] - yxl;
@@ -206,10 +222,19 @@ __global__ __launch_bounds__(P2P_THREADS) void k_p2p(uint32_t leaf_first, const 
         }
       }
     }
+    // add the phases' partial sums (lanes slot + np f share a target)
+    for (int off = np; off < 64; off <<= 1) {
+      ap[0] += __shfl_xor(ap[0], off);
+      if (GRAD) {
+        ax[0] += __shfl_xor(ax[0], off);
+        ay[0] += __shfl_xor(ay[0], off);
+        az[0] += __shfl_xor(az[0], off);
+      }
+    }
 #pragma unroll
-    for (int a = 0; a < TPT; ++a) {
-      const uint32_t i = tb + threadIdx.x + a * P2P_THREADS;
-      if (i < t1) {
+    for (int a = 0; a < 2; ++a) {
+      const uint32_t i = tb + slot + a * 64;
+      if (ok[a] && phase == 0) {
         phi[i] += double(ap[a]);
         if (GRAD) {
           grad[3 * size_t(i) + 0] += double(ax[a]);
@@ -222,89 +247,31 @@ __global__ __launch_bounds__(P2P_THREADS) void k_p2p(uint32_t leaf_first, const 
 }
 
 // ---- expansions from points: P2M (regular, unit frame) and P2L (irregular) ----------
-// One workgroup per output box; the row's source leaves in order, their points in tiles
-// of EXP_TILE.  Phase 1: thread (point, order m) writes column m of q conj X(v) into
-// LDS; phase 2: each thread owns coefficient slots and adds the tile's points in order.
+// One wave per output box; the row's source leaves in order, their points in tiles of
+// 64.  Per tile, lane j forms the diagonal X_m^m (m = 0..p) of point j into LDS; then lane
+// (order m, phase f) walks column m (degrees m..p) of the points j = f, f + F, ... of the
+// tile (F = 64 / (p+1) phases), adding q conj X_n^m to its column in registers.  The
+// phases' columns meet in LDS and are added in phase order.
 
 template <bool IRREG>
-__device__ void column_harmonics(int p, double x, double y, double z, double q, int m, double *out) {
-  const double r2 = x * x + y * y + z * z;
-  double dr, di = 0.0;
-  if (IRREG) {
-    const double inv_r2 = 1.0 / r2;
-    dr = rsqrt_t<double>(r2);
-    for (int k = 1; k <= m; ++k) {
-      const double f = (2.0 * k - 1.0) * inv_r2;
-      const double nr = f * (x * dr - y * di), ni = f * (x * di + y * dr);
-      dr = nr;
-      di = ni;
-    }
-    double pr = 0.0, pi = 0.0, cr = dr, ci = di;
-    for (int n = m; n <= p; ++n) {
-      if (n > m) {
-        double nr, ni;
-        if (n == m + 1) {
-          const double f = (2.0 * m + 1.0) * z * inv_r2;
-          nr = f * cr;
-          ni = f * ci;
-        } else {
-          const double a = (2.0 * n - 1.0) * z, b = double((n - 1) * (n - 1) - m * m);
-          nr = (a * cr - b * pr) * inv_r2;
-          ni = (a * ci - b * pi) * inv_r2;
-        }
-        pr = cr;
-        pi = ci;
-        cr = nr;
-        ci = ni;
-      }
-      out[idx(n, m)] = q * cr;
-      if (m > 0) out[idx(n, -m)] = -q * ci;
-    }
-  } else {
-    dr = 1.0;
-    for (int k = 1; k <= m; ++k) {
-      const double f = c_inv_2m[k];
-      const double nr = f * (x * dr - y * di), ni = f * (x * di + y * dr);
-      dr = nr;
-      di = ni;
-    }
-    double pr = 0.0, pi = 0.0, cr = dr, ci = di;
-    for (int n = m; n <= p; ++n) {
-      if (n > m) {
-        double nr, ni;
-        if (n == m + 1) {
-          nr = z * cr;
-          ni = z * ci;
-        } else {
-          const double a = (2.0 * n - 1.0) * z, inv = c_inv_reg[idx(n, m)];
-          nr = (a * cr - r2 * pr) * inv;
-          ni = (a * ci - r2 * pi) * inv;
-        }
-        pr = cr;
-        pi = ci;
-        cr = nr;
-        ci = ni;
-      }
-      out[idx(n, m)] = q * cr;
-      if (m > 0) out[idx(n, -m)] = -q * ci;
-    }
-  }
-}
-
-template <bool IRREG>
-__global__ __launch_bounds__(EXP_THREADS) void k_expand(int p, const uint32_t *row_box, const uint32_t *row_cell,
-                                                        const uint32_t *row_off, const uint32_t *row_ent,
-                                                        const uint32_t *leaf_cell, const uint32_t *src_start,
-                                                        const double *sx, const double *sy, const double *sz,
-                                                        const double *sq, double *coeffs) {
-  __shared__ double H[EXP_TILE * kMaxCoef];
-  const int nc = (p + 1) * (p + 1);
+__global__ __launch_bounds__(64) void k_expand(int p, const uint32_t *row_box, const uint32_t *row_cell,
+                                               const uint32_t *row_off, const uint32_t *row_ent,
+                                               const uint32_t *leaf_cell, const uint32_t *src_start,
+                                               const double *sx, const double *sy, const double *sz,
+                                               const double *sq, double *coeffs) {
+  __shared__ double pt[5][64];  // z, q, r2 (regular) or 1/r2 (irregular), and x, y
+  __shared__ double dre[kMaxDegree + 1][64], dim[kMaxDegree + 1][64];
+  __shared__ double red[1024];
+  const int lane = threadIdx.x;
+  const int P1 = p + 1, nc = P1 * P1;
+  const int phases = 64 / P1;
+  const int m = lane % P1, ph = lane / P1;
+  const bool active = ph < phases;
   const uint32_t row = blockIdx.x;
   const uint32_t box = row_box[row];
-  constexpr int SLOTS = (kMaxCoef + EXP_THREADS - 1) / EXP_THREADS;
-  double acc[SLOTS];
+  double are[kMaxDegree + 1], aim[kMaxDegree + 1];
 #pragma unroll
-  for (int s = 0; s < SLOTS; ++s) acc[s] = 0.0;
+  for (int k = 0; k <= kMaxDegree; ++k) are[k] = aim[k] = 0.0;
   for (uint32_t e = row_off[row]; e < row_off[row + 1]; ++e) {
     const uint32_t sl = row_ent[e];
     const uint32_t s0 = src_start[sl], s1 = src_start[sl + 1];
@@ -312,33 +279,95 @@ __global__ __launch_bounds__(EXP_THREADS) void k_expand(int p, const uint32_t *r
     // P2L from leaf s into box t: frame (c^(t|s), r^(t|s)), v = (u_s - c) / r
     if (IRREG) relative_frame<double>(row_cell + 4 * size_t(row), leaf_cell + 4 * size_t(sl), c, r);
     const double inv_r = 1.0 / r;
-    for (uint32_t sb = s0; sb < s1; sb += EXP_TILE) {
-      const int cnt = int((s1 - sb) < uint32_t(EXP_TILE) ? (s1 - sb) : uint32_t(EXP_TILE));
+    for (uint32_t sb = s0; sb < s1; sb += 64) {
+      const int cnt = int((s1 - sb) < 64u ? (s1 - sb) : 64u);
       __syncthreads();
-      for (int task = threadIdx.x; task < cnt * (p + 1); task += EXP_THREADS) {
-        const int j = task / (p + 1), m = task % (p + 1);
-        const uint32_t s = sb + j;
+      if (lane < cnt) {
+        const uint32_t s = sb + lane;
         double x = sx[s], y = sy[s], z = sz[s];
         if (IRREG) {
           x = (x - c[0]) * inv_r;
           y = (y - c[1]) * inv_r;
           z = (z - c[2]) * inv_r;
         }
-        column_harmonics<IRREG>(p, x, y, z, sq[s], m, H + j * nc);
+        const double r2 = x * x + y * y + z * z;
+        double dr, di = 0.0, f2 = 0.0;
+        if (IRREG) {
+          f2 = 1.0 / r2;
+          dr = rsqrt_t<double>(r2);
+        } else {
+          dr = 1.0;
+        }
+        dre[0][lane] = dr;
+        dim[0][lane] = 0.0;
+        for (int k = 1; k <= p; ++k) {
+          const double f = IRREG ? (2.0 * k - 1.0) * f2 : c_inv_2m[k];
+          const double nr = f * (x * dr - y * di), ni = f * (x * di + y * dr);
+          dr = nr;
+          di = ni;
+          dre[k][lane] = dr;
+          dim[k][lane] = di;
+        }
+        pt[0][lane] = z;
+        pt[1][lane] = sq[s];
+        pt[2][lane] = IRREG ? f2 : r2;
       }
       __syncthreads();
+      if (active)
+        for (int j = ph; j < cnt; j += phases) {
+          const double z = pt[0][j], q = pt[1][j], w = pt[2][j];
+          double pr = 0.0, pi = 0.0, cr = dre[m][j], ci = dim[m][j];
 #pragma unroll
-      for (int s = 0; s < SLOTS; ++s) {
-        const int slot = threadIdx.x + s * EXP_THREADS;
-        if (slot < nc)
-          for (int j = 0; j < cnt; ++j) acc[s] += H[j * nc + slot];
-      }
+          for (int k = 0; k <= kMaxDegree; ++k) {
+            const int n = m + k;
+            if (n > p) break;
+            if (k > 0) {
+              double nr, ni;
+              if (IRREG) {
+                if (k == 1) {
+                  const double f = (2.0 * m + 1.0) * z * w;
+                  nr = f * cr;
+                  ni = f * ci;
+                } else {
+                  const double a = (2.0 * n - 1.0) * z, b = double((n - 1) * (n - 1) - m * m);
+                  nr = (a * cr - b * pr) * w;
+                  ni = (a * ci - b * pi) * w;
+                }
+              } else {
+                if (k == 1) {
+                  nr = z * cr;
+                  ni = z * ci;
+                } else {
+                  const double a = (2.0 * n - 1.0) * z, inv = c_inv_reg[idx(n, m)];
+                  nr = (a * cr - w * pr) * inv;
+                  ni = (a * ci - w * pi) * inv;
+                }
+              }
+              pr = cr;
+              pi = ci;
+              cr = nr;
+              ci = ni;
+            }
+            are[k] += q * cr;
+            aim[k] -= q * ci;
+          }
+        }
     }
   }
+  __syncthreads();
+  if (active)
 #pragma unroll
-  for (int s = 0; s < SLOTS; ++s) {
-    const int slot = threadIdx.x + s * EXP_THREADS;
-    if (slot < nc) coeffs[size_t(box) * nc + slot] += acc[s];
+    for (int k = 0; k <= kMaxDegree; ++k) {
+      const int n = m + k;
+      if (n > p) break;
+      red[ph * nc + idx(n, m)] = are[k];
+      if (m > 0) red[ph * nc + idx(n, -m)] = aim[k];
+    }
+  __syncthreads();
+  for (int slot = lane; slot < nc; slot += 64) {
+    double s = 0.0;
+    for (int f = 0; f < phases; ++f) s += red[f * nc + slot];
+    coeffs[size_t(box) * nc + slot] += s;
   }
 }
 
@@ -524,11 +553,11 @@ void p2p(hipStream_t s, uint32_t leaf_first, uint32_t count, int gradients, cons
          const double *tx, const double *ty, const double *tz, double *phi, double *grad) {
   if (count == 0) return;
   if (gradients)
-    hipLaunchKernelGGL((k_p2p<T, true, 1>), dim3(count), dim3(P2P_THREADS), 0, s, leaf_first, near_off, near_ent,
-                       leaf_cell, src_start, tgt_start, sx, sy, sz, sq, tx, ty, tz, phi, grad);
+    hipLaunchKernelGGL((k_p2p<T, true>), dim3(count), dim3(64), 0, s, leaf_first, near_off, near_ent, leaf_cell,
+                       src_start, tgt_start, sx, sy, sz, sq, tx, ty, tz, phi, grad);
   else
-    hipLaunchKernelGGL((k_p2p<T, false, 1>), dim3(count), dim3(P2P_THREADS), 0, s, leaf_first, near_off, near_ent,
-                       leaf_cell, src_start, tgt_start, sx, sy, sz, sq, tx, ty, tz, phi, grad);
+    hipLaunchKernelGGL((k_p2p<T, false>), dim3(count), dim3(64), 0, s, leaf_first, near_off, near_ent, leaf_cell,
+                       src_start, tgt_start, sx, sy, sz, sq, tx, ty, tz, phi, grad);
 }
 
 template void p2p<float>(hipStream_t, uint32_t, uint32_t, int, const uint32_t *, const uint32_t *, const uint32_t *,
@@ -545,11 +574,11 @@ void expand(hipStream_t s, int p, int irregular, uint32_t rows, const uint32_t *
             double *coeffs) {
   if (rows == 0) return;
   if (irregular)
-    hipLaunchKernelGGL(k_expand<true>, dim3(rows), dim3(EXP_THREADS), 0, s, p, row_box, row_cell, row_off, row_ent,
-                       leaf_cell, src_start, sx, sy, sz, sq, coeffs);
+    hipLaunchKernelGGL(k_expand<true>, dim3(rows), dim3(64), 0, s, p, row_box, row_cell, row_off, row_ent, leaf_cell,
+                       src_start, sx, sy, sz, sq, coeffs);
   else
-    hipLaunchKernelGGL(k_expand<false>, dim3(rows), dim3(EXP_THREADS), 0, s, p, row_box, row_cell, row_off,
-                       row_ent, leaf_cell, src_start, sx, sy, sz, sq, coeffs);
+    hipLaunchKernelGGL(k_expand<false>, dim3(rows), dim3(64), 0, s, p, row_box, row_cell, row_off, row_ent, leaf_cell,
+                       src_start, sx, sy, sz, sq, coeffs);
 }
 
 void evaluate(hipStream_t s, int p, int irregular, int gradients, uint32_t rows, const uint32_t *row_leaf,
