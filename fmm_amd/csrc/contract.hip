@@ -183,6 +183,9 @@ __device__ __forceinline__ bf8 as_bf8(uint4 v) {
   return r;
 }
 
+// Waves tile the task two-dimensionally: wave w takes the M-tiles of half w % 2 (MH of
+// them) for the 32 columns of pair w / 2, so each A fragment read from LDS feeds two
+// N-tiles (half the LDS traffic of one column tile per wave).
 template <int MT>
 __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, const uint32_t *task_group_off,
                                                        const uint16_t *group_table, const uint32_t *group_src,
@@ -190,30 +193,39 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, const
                                                        const uint4 *__restrict__ X16, double *__restrict__ Y) {
   constexpr int CHUNK = MT * 3 * 64;  // uint4 per k-step: [plane][mt][lane]
   constexpr int PER = (CHUNK + 511) / 512;
+  constexpr int MH = (MT + 1) / 2;
   __shared__ uint4 As[2][CHUNK];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int col = wave * 16 + (lane & 15), g = lane >> 4;
+  const int half = wave & 1, pair = wave >> 1, g = lane >> 4;
+  const int mt0 = half * MH;
   const int kq = ksteps * 4;  // uint4 per plane of a split expansion
   const uint32_t task = blockIdx.x;
-  const uint32_t out = col_out[size_t(task) * COLS + col];
+  int col[2];
+  uint32_t out[2];
+#pragma unroll
+  for (int nt = 0; nt < 2; ++nt) {
+    col[nt] = pair * 32 + nt * 16 + (lane & 15);
+    out[nt] = col_out[size_t(task) * COLS + col[nt]];
+  }
   const uint32_t g0 = task_group_off[task], g1 = task_group_off[task + 1];
   const int nsteps = int(g1 - g0) * ksteps;
   if (nsteps == 0) return;
-  f4 acc[MT];
-  double sum[MT][4];
+  f4 acc[MH][2];
+  double sum[MH][2][4];
 #pragma unroll
-  for (int mt = 0; mt < MT; ++mt) {
-    acc[mt] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int i = 0; i < MH; ++i)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) sum[mt][r] = 0.0;
-  }
-  uint4 pa[PER], pb[3], b[3];
+    for (int nt = 0; nt < 2; ++nt) {
+      acc[i][nt] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sum[i][nt][r] = 0.0;
+    }
+  uint4 pa[PER], pb[2][3], b[2][3];
   const uint4 zero4 = {0u, 0u, 0u, 0u};
   auto fetch = [&](int step) {
     const uint32_t gi = g0 + uint32_t(step / ksteps);
     const int ks = step % ksteps;
     const uint32_t t = group_table[gi];
-    const uint32_t src = group_src[size_t(gi) * COLS + col];
     const uint4 *gsrc = tsw + (size_t(t) * ksteps + ks) * CHUNK;
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
@@ -221,8 +233,12 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, const
       if (e < CHUNK) pa[i] = gsrc[e];
     }
 #pragma unroll
-    for (int pl = 0; pl < 3; ++pl)
-      pb[pl] = src != 0xFFFFFFFFu ? X16[(size_t(src) * 3 + pl) * kq + ks * 4 + g] : zero4;
+    for (int nt = 0; nt < 2; ++nt) {
+      const uint32_t src = group_src[size_t(gi) * COLS + col[nt]];
+#pragma unroll
+      for (int pl = 0; pl < 3; ++pl)
+        pb[nt][pl] = src != 0xFFFFFFFFu ? X16[(size_t(src) * 3 + pl) * kq + ks * 4 + g] : zero4;
+    }
   };
   auto stash = [&](int buf) {
 #pragma unroll
@@ -234,49 +250,74 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, const
   fetch(0);
   stash(0);
 #pragma unroll
-  for (int pl = 0; pl < 3; ++pl) b[pl] = pb[pl];
+  for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) b[nt][pl] = pb[nt][pl];
   __syncthreads();
   for (int step = 0; step < nsteps; ++step) {
     const int buf = step & 1;
     if (step + 1 < nsteps) fetch(step + 1);
-    const bf8 bh = as_bf8(b[0]), bm = as_bf8(b[1]), bl = as_bf8(b[2]);
+    bf8 bh[2], bm[2], bl[2];
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-      const bf8 ah = as_bf8(As[buf][(0 * MT + mt) * 64 + lane]);
-      const bf8 am = as_bf8(As[buf][(1 * MT + mt) * 64 + lane]);
-      const bf8 al = as_bf8(As[buf][(2 * MT + mt) * 64 + lane]);
-      // smallest terms first
-      acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[mt], 0, 0, 0);
-      acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, acc[mt], 0, 0, 0);
-      acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[mt], 0, 0, 0);
-      acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, acc[mt], 0, 0, 0);
-      acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, acc[mt], 0, 0, 0);
-      acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[mt], 0, 0, 0);
+    for (int nt = 0; nt < 2; ++nt) {
+      bh[nt] = as_bf8(b[nt][0]);
+      bm[nt] = as_bf8(b[nt][1]);
+      bl[nt] = as_bf8(b[nt][2]);
+    }
+#pragma unroll
+    for (int i = 0; i < MH; ++i) {
+      const int mt = mt0 + i;
+      if (mt < MT) {
+        const bf8 ah = as_bf8(As[buf][(0 * MT + mt) * 64 + lane]);
+        const bf8 am = as_bf8(As[buf][(1 * MT + mt) * 64 + lane]);
+        const bf8 al = as_bf8(As[buf][(2 * MT + mt) * 64 + lane]);
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+          // smallest terms first
+          f4 a4 = acc[i][nt];
+          a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[nt], a4, 0, 0, 0);
+          a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm[nt], a4, 0, 0, 0);
+          a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[nt], a4, 0, 0, 0);
+          a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh[nt], a4, 0, 0, 0);
+          a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm[nt], a4, 0, 0, 0);
+          a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[nt], a4, 0, 0, 0);
+          acc[i][nt] = a4;
+        }
+      }
     }
     if ((step + 1) % ksteps == 0) {
       // end of a group (one offset): its f32 sum goes into the f64 accumulators
 #pragma unroll
-      for (int mt = 0; mt < MT; ++mt) {
+      for (int i = 0; i < MH; ++i)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) sum[mt][r] += double(acc[mt][r]);
-        acc[mt] = f4{0.f, 0.f, 0.f, 0.f};
-      }
+        for (int nt = 0; nt < 2; ++nt) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) sum[i][nt][r] += double(acc[i][nt][r]);
+          acc[i][nt] = f4{0.f, 0.f, 0.f, 0.f};
+        }
     }
     if (step + 1 < nsteps) stash(buf ^ 1);
     __syncthreads();
 #pragma unroll
-    for (int pl = 0; pl < 3; ++pl) b[pl] = pb[pl];
-  }
-  if (out != 0xFFFFFFFFu) {
-    double *y = Y + size_t(out) * nc;
-    // accumulator element r of lane (g, j): row 4 g + r of the M-tile (bf16 16x16x32)
+    for (int nt = 0; nt < 2; ++nt)
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
+      for (int pl = 0; pl < 3; ++pl) b[nt][pl] = pb[nt][pl];
+  }
+  // accumulator element r of lane (g, j): row 4 g + r of the M-tile (bf16 16x16x32)
+#pragma unroll
+  for (int nt = 0; nt < 2; ++nt) {
+    if (out[nt] == 0xFFFFFFFFu) continue;
+    double *y = Y + size_t(out[nt]) * nc;
+#pragma unroll
+    for (int i = 0; i < MH; ++i) {
+      const int mt = mt0 + i;
+      if (mt >= MT) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = mt * 16 + 4 * g + r;
-        if (i < nc) y[i] += sum[mt][r];
+        const int row = mt * 16 + 4 * g + r;
+        if (row < nc) y[row] += sum[i][nt][r];
       }
+    }
   }
 }
 
