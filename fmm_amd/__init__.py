@@ -24,7 +24,7 @@ from ._lib import FmmError, build  # noqa: F401
 PLAIN_F64 = 0
 MIXED = 1
 PRECISIONS = {"f64": PLAIN_F64, "plain": PLAIN_F64, "mixed": MIXED}
-KINDS = ("p2m", "m2m", "l2l", "m2l", "p2l", "l2p", "m2p", "p2p", "other")
+KINDS = ("p2m", "m2m", "l2l", "m2l", "p2l", "l2p", "m2p", "p2p", "other", "m2l_contract")
 LISTS = {"near": 0, "v": 1, "w": 2, "x": 3}
 
 
@@ -199,8 +199,8 @@ class Operator:
         _lib.check(_lib.lib().fmmh_kind_timing(self._h, int(enable)))
 
     def kind_times(self):
-        ms = np.zeros(9)
-        n = np.zeros(9, np.int64)
+        ms = np.zeros(len(KINDS))
+        n = np.zeros(len(KINDS), np.int64)
         _lib.check(_lib.lib().fmmh_kind_times(self._h, _lib.ptr(ms), _lib.ptr(n)))
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KINDS)}
 

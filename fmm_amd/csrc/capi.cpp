@@ -50,7 +50,7 @@ extern "C" {
 
 const char *fmmh_last_error(void) { return g_last_error.c_str(); }
 
-int32_t fmmh_abi_version(void) { return 1; }
+int32_t fmmh_abi_version(void) { return 2; }
 
 int32_t fmmh_plan_build(const double *sources, int64_t nsources, const double *targets, int64_t ntargets,
                         const fmmh_plan_options *options, fmmh_plan **out) {

@@ -98,32 +98,38 @@ void swizzle_table_bf16(int p, const double *a, uint16_t *out) {
 
 // ---- f64 contraction ---------------------------------------------------------------------
 
-template <int MT>
+// Wave w owns the columns of group w / WM (16 each, 8 / WM groups, so a task has
+// 128 / WM columns) and the M-tiles mt = w % WM, w % WM + WM, ...: WM = 1 for wide tasks
+// (every wave all M-tiles of its 16 columns), WM = 8 for narrow tasks of 16 columns on
+// the coarse levels, where the waves split the rows instead.
+template <int MT, int WM>
 __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, const uint32_t *task_group_off,
                                                       const uint16_t *group_table, const uint32_t *group_src,
                                                       const uint32_t *col_out, const double *__restrict__ tsw,
                                                       const double *__restrict__ X, double *__restrict__ Y) {
   constexpr int CHUNK2 = MT * KC * 32;  // double2 per chunk
   constexpr int PER = (CHUNK2 + 511) / 512;
+  constexpr int W = 128 / WM;              // columns per task
+  constexpr int MTW = (MT + WM - 1) / WM;  // M-tiles per wave
   __shared__ double2 As[2][CHUNK2];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int col = wave * 16 + (lane & 15), g = lane >> 4;
+  const int wm = wave % WM, col = (wave / WM) * 16 + (lane & 15), g = lane >> 4;
   const int KS = nchunks * KC;
   const uint32_t task = blockIdx.x;
-  const uint32_t out = col_out[size_t(task) * COLS + col];
+  const uint32_t out = col_out[size_t(task) * W + col];
   const uint32_t g0 = task_group_off[task], g1 = task_group_off[task + 1];
   const int nsteps = int(g1 - g0) * nchunks;
   if (nsteps == 0) return;  // uniform over the workgroup
-  d4 acc[MT];
+  d4 acc[MTW];
 #pragma unroll
-  for (int mt = 0; mt < MT; ++mt) acc[mt] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int i = 0; i < MTW; ++i) acc[i] = d4{0.0, 0.0, 0.0, 0.0};
   double2 pa[PER];
   double pb[KC], b[KC];
   auto fetch = [&](int step) {
     const uint32_t gi = g0 + uint32_t(step / nchunks);
     const int c = step % nchunks;
     const uint32_t t = group_table[gi];
-    const uint32_t src = group_src[size_t(gi) * COLS + col];
+    const uint32_t src = group_src[size_t(gi) * W + col];
     const double2 *gsrc = reinterpret_cast<const double2 *>(tsw + (size_t(t) * nchunks + c) * (CHUNK2 * 2));
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
@@ -155,8 +161,11 @@ __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, const
 #pragma unroll
     for (int kk = 0; kk < KC; ++kk)
 #pragma unroll
-      for (int mt = 0; mt < MT; ++mt)
-        acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[(mt * KC + kk) * 64 + lane], b[kk], acc[mt], 0, 0, 0);
+      for (int i = 0; i < MTW; ++i) {
+        const int mt = wm + i * WM;
+        if (mt < MT)
+          acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[(mt * KC + kk) * 64 + lane], b[kk], acc[i], 0, 0, 0);
+      }
     if (step + 1 < nsteps) stash(buf ^ 1);
     __syncthreads();
 #pragma unroll
@@ -166,12 +175,15 @@ __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, const
     double *y = Y + size_t(out) * nc;
     // accumulator element r of lane (g, j): row 4 r + g of the M-tile (f64 16x16x4)
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
+    for (int i = 0; i < MTW; ++i) {
+      const int mt = wm + i * WM;
+      if (mt >= MT) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = mt * 16 + 4 * r + g;
-        if (i < nc) y[i] += acc[mt][r];
+        const int row = mt * 16 + 4 * r + g;
+        if (row < nc) y[row] += acc[i][r];
       }
+    }
   }
 }
 
@@ -350,15 +362,19 @@ __global__ void k_reduce_slots(int nc, const uint32_t *row_box, const uint32_t *
 
 // ---- host wrappers ---------------------------------------------------------------------------
 
-void contract_f64(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off, const uint16_t *group_table,
-                  const uint32_t *group_src, const uint32_t *col_out, const double *tsw, const double *X,
-                  double *Y) {
+void contract_f64(hipStream_t s, int p, int width, uint32_t ntasks, const uint32_t *task_group_off,
+                  const uint16_t *group_table, const uint32_t *group_src, const uint32_t *col_out, const double *tsw,
+                  const double *X, double *Y) {
   if (ntasks == 0) return;
   const int nc = (p + 1) * (p + 1), nch = k_chunks(p);
 #define FMMH_CONTRACT(MT)                                                                                        \
   case MT:                                                                                                       \
-    hipLaunchKernelGGL(k_contract_f64<MT>, dim3(ntasks), dim3(512), 0, s, nc, nch, task_group_off, group_table,  \
-                       group_src, col_out, tsw, X, Y);                                                           \
+    if (width == 16)                                                                                             \
+      hipLaunchKernelGGL((k_contract_f64<MT, 8>), dim3(ntasks), dim3(512), 0, s, nc, nch, task_group_off,        \
+                         group_table, group_src, col_out, tsw, X, Y);                                            \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_contract_f64<MT, 1>), dim3(ntasks), dim3(512), 0, s, nc, nch, task_group_off,        \
+                         group_table, group_src, col_out, tsw, X, Y);                                            \
     break;
   switch (m_tiles(p)) {
     FMMH_CONTRACT(1)

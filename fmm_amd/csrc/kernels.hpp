@@ -52,7 +52,8 @@ void evaluate(hipStream_t s, int p, int irregular, int gradients, uint32_t rows,
 // Grouped dense contraction on MFMA (contract.hip): for every task (COLS output columns)
 // and every group of the task, Y[out[c]] += T_group X[src[g, c]]; f64, or bf16x3 split
 // with f32 accumulation per group and f64 across groups.
-void contract_f64(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off,
+// width: columns per task, COLS (wide) or 16 (narrow: the waves split the rows)
+void contract_f64(hipStream_t s, int p, int width, uint32_t ntasks, const uint32_t *task_group_off,
                   const uint16_t *group_table, const uint32_t *group_src, const uint32_t *col_out,
                   const double *tables_swizzled, const double *X, double *Y);
 void contract_bf16(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off,

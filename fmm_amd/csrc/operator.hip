@@ -23,12 +23,15 @@ namespace {
 // cols: output gid per column in task order, COLS per task (kNone = padding);
 // groups_of(c): ascending (table, source gid) pairs of column c.
 struct ScheduleBuilder {
+  int width = dev::COLS;  // columns per task
   std::vector<uint32_t> task_group_off{0}, group_src, col_out;
   std::vector<uint16_t> group_table;
   uint64_t pairs = 0, slots = 0;
 
+  explicit ScheduleBuilder(int w = dev::COLS) : width(w) {}
+
   void add_task(const std::vector<uint32_t> &cols, const std::vector<std::vector<std::pair<uint16_t, uint32_t>>> &lists) {
-    const int C = dev::COLS;
+    const int C = width;
     std::map<uint16_t, std::array<uint32_t, dev::COLS>> by_table;
     for (int c = 0; c < C; ++c) {
       col_out.push_back(c < int(cols.size()) ? cols[c] : kNone);
@@ -46,7 +49,7 @@ struct ScheduleBuilder {
     }
     for (const auto &kv : by_table) {
       group_table.push_back(kv.first);
-      group_src.insert(group_src.end(), kv.second.begin(), kv.second.end());
+      group_src.insert(group_src.end(), kv.second.begin(), kv.second.begin() + C);
       slots += C;
     }
     task_group_off.push_back(static_cast<uint32_t>(group_table.size()));
@@ -64,6 +67,7 @@ struct ScheduleBuilder {
   uint32_t ntasks() const { return static_cast<uint32_t>(task_group_off.size() - 1); }
 
   void finish(Schedule &s) {
+    s.width = width;
     s.ntasks = static_cast<uint32_t>(task_group_off.size() - 1);
     s.pairs = pairs;
     s.slots = slots;
@@ -320,7 +324,7 @@ void DeviceOperator::collect_timings() {
 
 void DeviceOperator::kind_times(double *ms, int64_t *launches) {
   sync();
-  for (int k = 0; k < 9; ++k) {
+  for (int k = 0; k < kNumKinds; ++k) {
     if (ms) ms[k] = kind_ms_[k];
     if (launches) launches[k] = kind_launches_[k];
   }
@@ -334,7 +338,10 @@ void DeviceOperator::build_schedules(const Plan &plan) {
   for (int l = 0; l < nlevels_; ++l) {
     // M2M into the parents of level l, by octant (fmm-plan/src/operator.rs:256-262)
     if (l + 1 < nlevels_) {
-      ScheduleBuilder sb;
+      // narrow tasks (waves split the rows) while wide ones would leave the GPU idle
+      const int width = plan.nboxes(l) < 64u * dev::COLS ? 16 : dev::COLS;
+      ScheduleBuilder sb(width);
+      const int C = width;
       std::vector<uint32_t> cols;
       std::vector<std::vector<std::pair<uint16_t, uint32_t>>> lists;
       for (uint32_t b = 0; b < plan.nboxes(l); ++b) {
@@ -355,7 +362,9 @@ void DeviceOperator::build_schedules(const Plan &plan) {
     }
     if (l >= 1) {
       // L2L into the boxes of level l, grouped by octant so a task shares one table
-      ScheduleBuilder sb;
+      const int width = plan.nboxes(l) < 64u * dev::COLS ? 16 : dev::COLS;
+      ScheduleBuilder sb(width);
+      const int C = width;
       for (int o = 0; o < 8; ++o) {
         std::vector<uint32_t> cols;
         std::vector<std::vector<std::pair<uint16_t, uint32_t>>> lists;
@@ -450,8 +459,8 @@ size_t DeviceOperator::device_bytes() const {
 
 void DeviceOperator::launch_contract(const Schedule &s, uint32_t task_first, uint32_t ntasks, const double *X,
                                      double *Y, const DevBuf<double> &tables) {
-  dev::contract_f64(stream_, opt_.p, ntasks, s.task_group_off.ptr + task_first, s.group_table.ptr, s.group_src.ptr,
-                    s.col_out.ptr + size_t(task_first) * dev::COLS, tables.ptr, X, Y);
+  dev::contract_f64(stream_, opt_.p, s.width, ntasks, s.task_group_off.ptr + task_first, s.group_table.ptr,
+                    s.group_src.ptr, s.col_out.ptr + size_t(task_first) * s.width, tables.ptr, X, Y);
 }
 
 // M2L of the levels level_first .. level_last: the contraction tasks, then the ordered
@@ -463,11 +472,13 @@ void DeviceOperator::launch_m2l(int level_first, int level_last) {
       const size_t b0 = box_base_[level_first], b1 = box_base_[level_last + 1];
       const size_t sz = dev::split_expansion_size(opt_.p);
       dev::split_bf16(stream_, opt_.p, b1 - b0, mult_.ptr + b0 * nc_, mult16_.ptr + b0 * sz);
-      dev::contract_bf16(stream_, opt_.p, t1 - t0, m2l_.task_group_off.ptr + t0, m2l_.group_table.ptr,
-                         m2l_.group_src.ptr, m2l_.col_out.ptr + size_t(t0) * dev::COLS, t_m2l16_.ptr, mult16_.ptr,
-                         loc_.ptr);
+      timed(kKindM2lContract, [&] {
+        dev::contract_bf16(stream_, opt_.p, t1 - t0, m2l_.task_group_off.ptr + t0, m2l_.group_table.ptr,
+                           m2l_.group_src.ptr, m2l_.col_out.ptr + size_t(t0) * dev::COLS, t_m2l16_.ptr,
+                           mult16_.ptr, loc_.ptr);
+      });
     } else {
-      launch_contract(m2l_, t0, t1 - t0, mult_.ptr, loc_.ptr, t_m2l_);
+      timed(kKindM2lContract, [&] { launch_contract(m2l_, t0, t1 - t0, mult_.ptr, loc_.ptr, t_m2l_); });
     }
   }
   const uint32_t r0 = m2l_red_level_[level_first], r1 = m2l_red_level_[level_last + 1];

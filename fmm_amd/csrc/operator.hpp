@@ -18,6 +18,9 @@
 
 namespace fmmh {
 
+constexpr int kNumKinds = 10;        // timing kinds: the operator kinds, other, M2L contraction
+constexpr int kKindM2lContract = 9;  // the M2L contraction kernel alone
+
 enum Precision : int {
   kPlainF64 = 0,  // every operator in f64
   kMixed = 1,     // P2P in f32 (double-float separations), M2L in bf16x3 on MFMA
@@ -66,6 +69,7 @@ struct DevBuf {
 
 // One contraction launch: tasks of COLS output columns, groups of (table, sources).
 struct Schedule {
+  int width = 128;  // columns per task
   uint32_t ntasks = 0;
   uint64_t pairs = 0;  // real (column, group) pairs with a source
   uint64_t slots = 0;  // columns x groups issued

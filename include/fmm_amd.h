@@ -44,7 +44,7 @@ typedef enum {
 
 typedef enum {
   FMMH_PLAIN_F64 = 0, /* every operator in f64 */
-  FMMH_MIXED = 1      /* f32 P2P; f64 translations and accumulation */
+  FMMH_MIXED = 1      /* f32 P2P (double-float separations), bf16x3 M2L on MFMA; f64 elsewhere */
 } fmmh_precision;
 
 typedef enum { FMMH_LIST_NEAR = 0, FMMH_LIST_V = 1, FMMH_LIST_W = 2, FMMH_LIST_X = 3 } fmmh_list_kind;
@@ -60,7 +60,8 @@ typedef enum {
   FMMH_KIND_M2P = 6,
   FMMH_KIND_P2P = 7,
   FMMH_KIND_OTHER = 8,
-  FMMH_NKINDS = 9
+  FMMH_KIND_M2L_CONTRACT = 9, /* the contraction kernel alone, inside FMMH_KIND_M2L */
+  FMMH_NKINDS = 10
 } fmmh_kind;
 
 typedef struct fmmh_plan fmmh_plan;
