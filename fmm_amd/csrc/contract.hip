@@ -30,6 +30,10 @@ namespace dev {
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+// native vector types (the HIP struct vector types defeat scalar replacement of
+// register arrays, which then live in scratch memory)
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 
 // ---- layouts -------------------------------------------------------------------------
 
@@ -107,11 +111,11 @@ __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, const
                                                       const uint16_t *group_table, const uint32_t *group_src,
                                                       const uint32_t *col_out, const double *__restrict__ tsw,
                                                       const double *__restrict__ X, double *__restrict__ Y) {
-  constexpr int CHUNK2 = MT * KC * 32;  // double2 per chunk
+  constexpr int CHUNK2 = MT * KC * 32;  // dbl2 per chunk
   constexpr int PER = (CHUNK2 + 511) / 512;
   constexpr int W = 128 / WM;              // columns per task
   constexpr int MTW = (MT + WM - 1) / WM;  // M-tiles per wave
-  __shared__ double2 As[2][CHUNK2];
+  __shared__ dbl2 As[2][CHUNK2];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm = wave % WM, col = (wave / WM) * 16 + (lane & 15), g = lane >> 4;
   const int KS = nchunks * KC;
@@ -123,19 +127,17 @@ __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, const
   d4 acc[MTW];
 #pragma unroll
   for (int i = 0; i < MTW; ++i) acc[i] = d4{0.0, 0.0, 0.0, 0.0};
-  double2 pa[PER];
+  dbl2 pa[PER];
   double pb[KC], b[KC];
   auto fetch = [&](int step) {
     const uint32_t gi = g0 + uint32_t(step / nchunks);
     const int c = step % nchunks;
     const uint32_t t = group_table[gi];
     const uint32_t src = group_src[size_t(gi) * W + col];
-    const double2 *gsrc = reinterpret_cast<const double2 *>(tsw + (size_t(t) * nchunks + c) * (CHUNK2 * 2));
+    const dbl2 *gsrc = reinterpret_cast<const dbl2 *>(tsw + (size_t(t) * nchunks + c) * (CHUNK2 * 2));
+    // clamped index: every load is issued unconditionally, so pa stays in registers
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = tid + i * 512;
-      if (e < CHUNK2) pa[i] = gsrc[e];
-    }
+    for (int i = 0; i < PER; ++i) pa[i] = gsrc[min(tid + i * 512, CHUNK2 - 1)];
 #pragma unroll
     for (int kk = 0; kk < KC; ++kk) {
       const int k = c * KC + kk + g * KS;
@@ -189,7 +191,7 @@ __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, const
 
 // ---- bf16x3 contraction ---------------------------------------------------------------------
 
-__device__ __forceinline__ bf8 as_bf8(uint4 v) {
+__device__ __forceinline__ bf8 as_bf8(u4 v) {
   bf8 r;
   memcpy(&r, &v, 16);
   return r;
@@ -201,16 +203,16 @@ __device__ __forceinline__ bf8 as_bf8(uint4 v) {
 template <int MT>
 __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, const uint32_t *task_group_off,
                                                        const uint16_t *group_table, const uint32_t *group_src,
-                                                       const uint32_t *col_out, const uint4 *__restrict__ tsw,
-                                                       const uint4 *__restrict__ X16, double *__restrict__ Y) {
-  constexpr int CHUNK = MT * 3 * 64;  // uint4 per k-step: [plane][mt][lane]
+                                                       const uint32_t *col_out, const u4 *__restrict__ tsw,
+                                                       const u4 *__restrict__ X16, double *__restrict__ Y) {
+  constexpr int CHUNK = MT * 3 * 64;  // u4 per k-step: [plane][mt][lane]
   constexpr int PER = (CHUNK + 511) / 512;
   constexpr int MH = (MT + 1) / 2;
-  __shared__ uint4 As[2][CHUNK];
+  __shared__ u4 As[2][CHUNK];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int half = wave & 1, pair = wave >> 1, g = lane >> 4;
   const int mt0 = half * MH;
-  const int kq = ksteps * 4;  // uint4 per plane of a split expansion
+  const int kq = ksteps * 4;  // u4 per plane of a split expansion
   const uint32_t task = blockIdx.x;
   int col[2];
   uint32_t out[2];
@@ -232,18 +234,16 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, const
 #pragma unroll
       for (int r = 0; r < 4; ++r) sum[i][nt][r] = 0.0;
     }
-  uint4 pa[PER], pb[2][3], b[2][3];
-  const uint4 zero4 = {0u, 0u, 0u, 0u};
+  u4 pa[PER], pb[2][3], b[2][3];
+  const u4 zero4 = {0u, 0u, 0u, 0u};
   auto fetch = [&](int step) {
     const uint32_t gi = g0 + uint32_t(step / ksteps);
     const int ks = step % ksteps;
     const uint32_t t = group_table[gi];
-    const uint4 *gsrc = tsw + (size_t(t) * ksteps + ks) * CHUNK;
+    const u4 *gsrc = tsw + (size_t(t) * ksteps + ks) * CHUNK;
+    // clamped index: every load is issued unconditionally, so pa stays in registers
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = tid + i * 512;
-      if (e < CHUNK) pa[i] = gsrc[e];
-    }
+    for (int i = 0; i < PER; ++i) pa[i] = gsrc[min(tid + i * 512, CHUNK - 1)];
 #pragma unroll
     for (int nt = 0; nt < 2; ++nt) {
       const uint32_t src = group_src[size_t(gi) * COLS + col[nt]];
@@ -404,8 +404,8 @@ void contract_bf16(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_g
                    double *Y) {
   if (ntasks == 0) return;
   const int nc = (p + 1) * (p + 1), ks = k_steps16(p);
-  const uint4 *t4 = reinterpret_cast<const uint4 *>(tsw16);
-  const uint4 *x4 = reinterpret_cast<const uint4 *>(X16);
+  const u4 *t4 = reinterpret_cast<const u4 *>(tsw16);
+  const u4 *x4 = reinterpret_cast<const u4 *>(X16);
 #define FMMH_CONTRACT(MT)                                                                                        \
   case MT:                                                                                                       \
     hipLaunchKernelGGL(k_contract_bf16<MT>, dim3(ntasks), dim3(512), 0, s, nc, ks, task_group_off, group_table,  \

@@ -17,6 +17,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -26,15 +27,16 @@ namespace dev {
 __constant__ double c_inv_reg[kMaxCoefGrad];  // 1 / ((n + m)(n - m)) at idx(n, m), n >= m + 2
 __constant__ double c_inv_2m[kMaxDegree + 3];  // 1 / (2m)
 
-void upload_constants() {
+hipError_t upload_constants() {
   double inv_reg[kMaxCoefGrad] = {0};
   for (int n = 0; n <= kMaxDegree + 1; ++n)
     for (int m = 0; m <= n; ++m)
       if (n >= m + 2) inv_reg[n * n + n + m] = 1.0 / double((n + m) * (n - m));
   double inv2m[kMaxDegree + 3] = {0};
   for (int m = 1; m <= kMaxDegree + 2; ++m) inv2m[m] = 1.0 / (2.0 * m);
-  hipMemcpyToSymbol(HIP_SYMBOL(c_inv_reg), inv_reg, sizeof(inv_reg));
-  hipMemcpyToSymbol(HIP_SYMBOL(c_inv_2m), inv2m, sizeof(inv2m));
+  const hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_inv_reg), inv_reg, sizeof(inv_reg));
+  if (e != hipSuccess) return e;
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_inv_2m), inv2m, sizeof(inv2m));
 }
 
 __device__ __forceinline__ int idx(int n, int m) { return n * n + n + m; }
@@ -125,8 +127,16 @@ __global__ void k_output(const uint32_t *tgt_start, const uint32_t *leaf_cell, c
 // Lane mapping (one wave per target leaf, targets processed in chunks of 128): a chunk of
 // n <= 64 targets uses np = the next power of two >= n lanes per source phase and
 // 64 / np phases, phase f taking the sources j = f, f + 64/np, ... of every tile; a chunk
-// of 65..128 targets puts two targets in each lane.  The phases' partial sums are added
-// by a fixed butterfly of lane shuffles, so results are the same on every run.
+// of 65..128 targets puts two targets in each lane.  A tile is padded with zero charges
+// far away to a multiple of the phase count, so every lane runs the same number of
+// steps; the phases' partial sums are added by a fixed butterfly of lane shuffles, so
+// results are the same on every run.  The next tile's coordinates and charges are
+// loaded into registers while the current tile is computed.
+//
+// Only a target's own leaf can hold a source at r^2 == 0 (equal coordinates give equal
+// leaves): that tile selects 0 for r^2 == 0; other tiles clamp r^2 below at the smallest
+// normal number, which changes nothing for distinct points (r^2 >= 1e-22 in leaf units
+// at the deepest level) and keeps a separation rounded to zero finite.
 template <typename T, bool GRAD>
 __global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, const uint32_t *near_off, const uint32_t *near_ent,
                                             const uint32_t *leaf_cell, const uint32_t *src_start,
@@ -134,14 +144,20 @@ __global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, const uint32_t 
                                             const double *sz, const double *sq, const double *tx, const double *ty,
                                             const double *tz, double *phi, double *grad) {
   constexpr bool SPLIT = sizeof(T) == 4;
-  constexpr int NY = SPLIT ? 7 : 4;
   constexpr int TILE = 64;
-  __shared__ T ys[NY][TILE];
+  constexpr T TINY = sizeof(T) == 4 ? T(1.17549435e-38f) : T(2.2250738585072014e-308);
+  constexpr double FAR = 1024.0;  // padding position, in target-leaf units
+  // a source in LDS: (x, y, z, q) and, for the split form, (x_lo, y_lo, z_lo, 0), so a
+  // lane reads it with one or two 16-byte loads
+  using V4 = typename std::conditional<SPLIT, float4, double4>::type;
+  __shared__ V4 ys[TILE];
+  __shared__ float4 yl[SPLIT ? TILE : 1];
   const int lane = threadIdx.x;
   const uint32_t lf = leaf_first + blockIdx.x;
   const uint32_t t0 = tgt_start[lf], t1 = tgt_start[lf + 1];
   if (t0 == t1) return;
   const uint32_t *ct = leaf_cell + 4 * size_t(lf);
+  const uint32_t e0 = near_off[lf], e1 = near_off[lf + 1];
   for (uint32_t tb = t0; tb < t1; tb += 128) {
     const uint32_t n = (t1 - tb) < 128u ? (t1 - tb) : 128u;
     int np = 64, phases = 1;
@@ -151,12 +167,13 @@ __global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, const uint32_t 
       phases = 64 / np;
     }
     const int slot = lane & (np - 1), phase = lane / np;
+    const bool two = n > 64;
     T ux[2], uy[2], uz[2], lx[2], ly[2], lz[2], ap[2], ax[2], ay[2], az[2];
     bool ok[2];
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
       const uint32_t i = tb + slot + a * 64;
-      ok[a] = (a == 0) ? (uint32_t(slot) < n) : (n > 64 && uint32_t(slot + 64) < n);
+      ok[a] = (a == 0) ? (uint32_t(slot) < n) : (two && uint32_t(slot + 64) < n);
       const double x = ok[a] ? tx[i] : 0.0, y = ok[a] ? ty[i] : 0.0, z = ok[a] ? tz[i] : 0.0;
       ux[a] = T(x);
       uy[a] = T(y);
@@ -166,60 +183,101 @@ __global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, const uint32_t 
       lz[a] = T(z - double(uz[a]));
       ap[a] = ax[a] = ay[a] = az[a] = T(0);
     }
-    for (uint32_t e = near_off[lf]; e < near_off[lf + 1]; ++e) {
-      const uint32_t sl = near_ent[e];
-      const uint32_t s0 = src_start[sl], s1 = src_start[sl + 1];
-      double c[3], r;
-      const bool self = sl == lf;
-      relative_frame<double>(leaf_cell + 4 * size_t(sl), ct, c, r);
-      for (uint32_t sb = s0; sb < s1; sb += TILE) {
-        const int cnt = int((s1 - sb) < uint32_t(TILE) ? (s1 - sb) : uint32_t(TILE));
-        __syncthreads();
-        if (lane < cnt) {
-          const uint32_t s = sb + lane;
-          const double yx = self ? sx[s] : fma(r, sx[s], c[0]);
-          const double yy = self ? sy[s] : fma(r, sy[s], c[1]);
-          const double yz = self ? sz[s] : fma(r, sz[s], c[2]);
-          ys[0][lane] = T(yx);
-          ys[1][lane] = T(yy);
-          ys[2][lane] = T(yz);
-          ys[3][lane] = T(sq[s]);
-          if (SPLIT) {
-            ys[4][lane] = T(yx - double(T(yx)));
-            ys[5][lane] = T(yy - double(T(yy)));
-            ys[6][lane] = T(yz - double(T(yz)));
-          }
-        }
-        __syncthreads();
-        for (int j = phase; j < cnt; j += phases) {
-          const T yx = ys[0][j], yy = ys[1][j], yz = ys[2][j], q = ys[3][j];
-          T yxl = T(0), yyl = T(0), yzl = T(0);
-          if (SPLIT) {
-            yxl = ys[4][j];
-            yyl = ys[5][j];
-            yzl = ys[6][j];
-          }
+    // one source j of the tile against the lane's targets
+    auto step = [&](int j, bool check) {
+      const V4 v = ys[j];
+      const T yx = v.x, yy = v.y, yz = v.z, q = v.w;
+      T yxl = T(0), yyl = T(0), yzl = T(0);
+      if (SPLIT) {
+        const float4 w = yl[j];
+        yxl = w.x;
+        yyl = w.y;
+        yzl = w.z;
+      }
 #pragma unroll
-          for (int a = 0; a < 2; ++a) {
-            if (a == 1 && n <= 64) break;
-            T dx = ux[a] - yx, dy = uy[a] - yy, dz = uz[a] - yz;
-            if (SPLIT) {
-              dx += lx[a] - yxl;
-              dy += ly[a] - yyl;
-              dz += lz[a] - yzl;
-            }
-            const T r2 = dx * dx + dy * dy + dz * dz;
-            const T inv = r2 == T(0) ? T(0) : rsqrt_t<T>(r2);
-            const T qi = q * inv;
-            ap[a] += qi;
-            if (GRAD) {
-              const T qi3 = qi * (inv * inv);
-              ax[a] -= qi3 * dx;
-              ay[a] -= qi3 * dy;
-              az[a] -= qi3 * dz;
-            }
-          }
+      for (int a = 0; a < 2; ++a) {
+        if (a == 1 && !two) break;
+        T dx = ux[a] - yx, dy = uy[a] - yy, dz = uz[a] - yz;
+        if (SPLIT) {
+          dx += lx[a] - yxl;
+          dy += ly[a] - yyl;
+          dz += lz[a] - yzl;
         }
+        const T r2 = dx * dx + dy * dy + dz * dz;
+        const T inv = check ? (r2 == T(0) ? T(0) : rsqrt_t<T>(r2)) : rsqrt_t<T>(r2 > TINY ? r2 : TINY);
+        const T qi = q * inv;
+        ap[a] += qi;
+        if (GRAD) {
+          const T qi3 = qi * (inv * inv);
+          ax[a] -= qi3 * dx;
+          ay[a] -= qi3 * dy;
+          az[a] -= qi3 * dz;
+        }
+      }
+    };
+    // tile cursor over the near row: entry e, sources sb .. s1 of leaf sl
+    uint32_t e = e0, sl = 0, sb = 0, s1 = 0;
+    auto seek = [&](uint32_t from) {
+      for (e = from; e < e1; ++e) {
+        sl = near_ent[e];
+        sb = src_start[sl];
+        s1 = src_start[sl + 1];
+        if (sb < s1) return;
+      }
+    };
+    double px = 0.0, py = 0.0, pz = 0.0, pq = 0.0;
+    auto prefetch = [&]() {
+      const uint32_t s = sb + uint32_t(lane);
+      if (s < s1) {
+        px = sx[s];
+        py = sy[s];
+        pz = sz[s];
+        pq = sq[s];
+      }
+    };
+    seek(e0);
+    if (e < e1) prefetch();
+    while (e < e1) {
+      const int cnt = int((s1 - sb) < uint32_t(TILE) ? (s1 - sb) : uint32_t(TILE));
+      const int steps = (cnt + phases - 1) / phases;
+      const bool self = sl == lf;
+      double c[3], r;
+      relative_frame<double>(leaf_cell + 4 * size_t(sl), ct, c, r);
+      __syncthreads();
+      if (lane < steps * phases) {
+        double yx = FAR, yy = FAR, yz = FAR, q = 0.0;
+        if (lane < cnt) {
+          yx = self ? px : fma(r, px, c[0]);
+          yy = self ? py : fma(r, py, c[1]);
+          yz = self ? pz : fma(r, pz, c[2]);
+          q = pq;
+        }
+        V4 v;
+        v.x = T(yx);
+        v.y = T(yy);
+        v.z = T(yz);
+        v.w = T(q);
+        ys[lane] = v;
+        if (SPLIT)
+          yl[lane] = make_float4(float(yx - double(float(yx))), float(yy - double(float(yy))),
+                                 float(yz - double(float(yz))), 0.0f);
+      }
+      __syncthreads();
+      // advance the cursor and start the next tile's loads before computing this one
+      if (sb + TILE < s1)
+        sb += TILE;
+      else
+        seek(e + 1);
+      if (e < e1) prefetch();
+      if (self) {
+        for (int k = 0; k < steps; ++k) step(phase + k * phases, true);
+      } else {
+        int k = 0;
+        for (; k + 2 <= steps; k += 2) {
+          step(phase + k * phases, false);
+          step(phase + (k + 1) * phases, false);
+        }
+        if (k < steps) step(phase + k * phases, false);
       }
     }
     // add the phases' partial sums (lanes slot + np f share a target)
@@ -248,10 +306,11 @@ __global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, const uint32_t 
 
 // ---- expansions from points: P2M (regular, unit frame) and P2L (irregular) ----------
 // One wave per output box; the row's source leaves in order, their points in tiles of
-// 64.  Per tile, lane j forms the diagonal X_m^m (m = 0..p) of point j into LDS; then lane
-// (order m, phase f) walks column m (degrees m..p) of the points j = f, f + F, ... of the
-// tile (F = 64 / (p+1) phases), adding q conj X_n^m to its column in registers.  The
-// phases' columns meet in LDS and are added in phase order.
+// 32.  Per tile, lane j < 32 forms the diagonal X_m^m (m = 0..p) of point j into LDS; then
+// lane (order m, phase f) walks column m (degrees m..p) of the points j = f, f + F, ... of
+// the tile (F = 64 / (p+1) phases), adding q conj X_n^m to its column in registers.  The
+// phases' columns meet in LDS (over the diagonals' space) and are added in phase order.
+// About 10 KB of LDS per wave, so 16 waves share a CU.
 
 template <bool IRREG>
 __global__ __launch_bounds__(64) void k_expand(int p, const uint32_t *row_box, const uint32_t *row_cell,
@@ -259,9 +318,13 @@ __global__ __launch_bounds__(64) void k_expand(int p, const uint32_t *row_box, c
                                                const uint32_t *leaf_cell, const uint32_t *src_start,
                                                const double *sx, const double *sy, const double *sz,
                                                const double *sq, double *coeffs) {
-  __shared__ double pt[5][64];  // z, q, r2 (regular) or 1/r2 (irregular), and x, y
-  __shared__ double dre[kMaxDegree + 1][64], dim[kMaxDegree + 1][64];
-  __shared__ double red[1024];
+  constexpr int TP = 32;                       // points per tile
+  constexpr int DW = (kMaxDegree + 1) * TP;    // doubles per diagonal plane
+  __shared__ double pt[3][TP];                 // z, q, r2 (regular) or 1/r2 (irregular)
+  __shared__ double diag[2 * DW];              // Re, Im of X_m^m at [m][j]; then the phase sums
+  __shared__ double inv_tab[kMaxDegree + 1][kMaxDegree + 1];  // 1 / ((n + m)(n - m)) at [n - m][m]
+  static_assert(2 * DW >= 1024, "phase sums need 64 (p + 1) <= 1024 doubles");
+  double *dre = diag, *dim = diag + DW, *red = diag;
   const int lane = threadIdx.x;
   const int P1 = p + 1, nc = P1 * P1;
   const int phases = 64 / P1;
@@ -269,6 +332,11 @@ __global__ __launch_bounds__(64) void k_expand(int p, const uint32_t *row_box, c
   const bool active = ph < phases;
   const uint32_t row = blockIdx.x;
   const uint32_t box = row_box[row];
+  if (!IRREG)
+    for (int t = lane; t < (kMaxDegree + 1) * (kMaxDegree + 1); t += 64) {
+      const int kk = t / (kMaxDegree + 1), mm = t % (kMaxDegree + 1);
+      inv_tab[kk][mm] = kk >= 2 ? 1.0 / double((2 * mm + kk) * kk) : 0.0;
+    }
   double are[kMaxDegree + 1], aim[kMaxDegree + 1];
 #pragma unroll
   for (int k = 0; k <= kMaxDegree; ++k) are[k] = aim[k] = 0.0;
@@ -279,8 +347,8 @@ __global__ __launch_bounds__(64) void k_expand(int p, const uint32_t *row_box, c
     // P2L from leaf s into box t: frame (c^(t|s), r^(t|s)), v = (u_s - c) / r
     if (IRREG) relative_frame<double>(row_cell + 4 * size_t(row), leaf_cell + 4 * size_t(sl), c, r);
     const double inv_r = 1.0 / r;
-    for (uint32_t sb = s0; sb < s1; sb += 64) {
-      const int cnt = int((s1 - sb) < 64u ? (s1 - sb) : 64u);
+    for (uint32_t sb = s0; sb < s1; sb += TP) {
+      const int cnt = int((s1 - sb) < uint32_t(TP) ? (s1 - sb) : uint32_t(TP));
       __syncthreads();
       if (lane < cnt) {
         const uint32_t s = sb + lane;
@@ -298,15 +366,15 @@ __global__ __launch_bounds__(64) void k_expand(int p, const uint32_t *row_box, c
         } else {
           dr = 1.0;
         }
-        dre[0][lane] = dr;
-        dim[0][lane] = 0.0;
+        dre[lane] = dr;
+        dim[lane] = 0.0;
         for (int k = 1; k <= p; ++k) {
           const double f = IRREG ? (2.0 * k - 1.0) * f2 : c_inv_2m[k];
           const double nr = f * (x * dr - y * di), ni = f * (x * di + y * dr);
           dr = nr;
           di = ni;
-          dre[k][lane] = dr;
-          dim[k][lane] = di;
+          dre[k * TP + lane] = dr;
+          dim[k * TP + lane] = di;
         }
         pt[0][lane] = z;
         pt[1][lane] = sq[s];
@@ -316,7 +384,7 @@ __global__ __launch_bounds__(64) void k_expand(int p, const uint32_t *row_box, c
       if (active)
         for (int j = ph; j < cnt; j += phases) {
           const double z = pt[0][j], q = pt[1][j], w = pt[2][j];
-          double pr = 0.0, pi = 0.0, cr = dre[m][j], ci = dim[m][j];
+          double pr = 0.0, pi = 0.0, cr = dre[m * TP + j], ci = dim[m * TP + j];
 #pragma unroll
           for (int k = 0; k <= kMaxDegree; ++k) {
             const int n = m + k;
@@ -338,7 +406,7 @@ __global__ __launch_bounds__(64) void k_expand(int p, const uint32_t *row_box, c
                   nr = z * cr;
                   ni = z * ci;
                 } else {
-                  const double a = (2.0 * n - 1.0) * z, inv = c_inv_reg[idx(n, m)];
+                  const double a = (2.0 * n - 1.0) * z, inv = inv_tab[k][m];
                   nr = (a * cr - w * pr) * inv;
                   ni = (a * ci - w * pi) * inv;
                 }

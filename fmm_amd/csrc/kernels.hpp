@@ -19,7 +19,7 @@ constexpr int EVAL_THREADS = 128;
 constexpr int COLS = 128;  // output columns (boxes) per contraction task: 8 waves x 16
 constexpr int KC = 4;      // MFMA k-steps per LDS chunk of a table
 
-void upload_constants();
+hipError_t upload_constants();
 
 void fill(hipStream_t s, double *p, size_t n, double v);
 void gather_charges(hipStream_t s, const double *q, const uint32_t *order, size_t n, double *q64, float *q32);

@@ -97,7 +97,7 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
   require(opt_.device >= 0 && opt_.device < ndev, "device ordinal out of range", kNoDevice);
   hip_check(hipSetDevice(opt_.device), "hipSetDevice");
   hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
-  dev::upload_constants();
+  hip_check(dev::upload_constants(), "upload constants");
 
   nlevels_ = plan.nlevels;
   nc_ = ncoef(opt_.p);
