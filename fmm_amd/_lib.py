@@ -10,7 +10,8 @@ import os
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libfmm_amd.so")
+# FMM_AMD_LIBRARY selects another build of the same sources (e.g. a tuning variant)
+LIB_PATH = os.environ.get("FMM_AMD_LIBRARY") or os.path.join(HERE, "libfmm_amd.so")
 CSRC = os.path.join(HERE, "csrc")
 
 STATUS = {

@@ -107,10 +107,11 @@ void swizzle_table_bf16(int p, const double *a, uint16_t *out) {
 // (every wave all M-tiles of its 16 columns), WM = 8 for narrow tasks of 16 columns on
 // the coarse levels, where the waves split the rows instead.
 template <int MT, int WM>
-__global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, const uint32_t *task_group_off,
+__global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, uint32_t ntasks, const uint32_t *task_group_off,
                                                       const uint16_t *group_table, const uint32_t *group_src,
                                                       const uint32_t *col_out, const double *__restrict__ tsw,
-                                                      const double *__restrict__ X, double *__restrict__ Y) {
+                                                      const double *__restrict__ X, double *__restrict__ Y,
+                                                      uint32_t first_slot, double *__restrict__ S) {
   constexpr int CHUNK2 = MT * KC * 32;  // dbl2 per chunk
   constexpr int PER = (CHUNK2 + 511) / 512;
   constexpr int W = 128 / WM;              // columns per task
@@ -119,7 +120,8 @@ __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, const
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm = wave % WM, col = (wave / WM) * 16 + (lane & 15), g = lane >> 4;
   const int KS = nchunks * KC;
-  const uint32_t task = blockIdx.x;
+  const uint32_t task = xcd_index(ntasks);
+  if (task >= ntasks) return;  // uniform over the workgroup
   const uint32_t out = col_out[size_t(task) * W + col];
   const uint32_t g0 = task_group_off[task], g1 = task_group_off[task + 1];
   const int nsteps = int(g1 - g0) * nchunks;
@@ -174,7 +176,9 @@ __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, const
     for (int kk = 0; kk < KC; ++kk) b[kk] = pb[kk];
   }
   if (out != 0xFFFFFFFFu) {
-    double *y = Y + size_t(out) * nc;
+    // a box row adds; a scratch slot (pair mode: one group, one writer) is stored
+    const bool slot = out >= first_slot;
+    double *y = slot ? S + size_t(out - first_slot) * nc : Y + size_t(out) * nc;
     // accumulator element r of lane (g, j): row 4 r + g of the M-tile (f64 16x16x4)
 #pragma unroll
     for (int i = 0; i < MTW; ++i) {
@@ -183,7 +187,7 @@ __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, const
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = mt * 16 + 4 * r + g;
-        if (row < nc) y[row] += acc[i][r];
+        if (row < nc) y[row] = slot ? acc[i][r] : y[row] + acc[i][r];
       }
     }
   }
@@ -201,10 +205,11 @@ __device__ __forceinline__ bf8 as_bf8(u4 v) {
 // them) for the 32 columns of pair w / 2, so each A fragment read from LDS feeds two
 // N-tiles (half the LDS traffic of one column tile per wave).
 template <int MT>
-__global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, const uint32_t *task_group_off,
+__global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, uint32_t ntasks, const uint32_t *task_group_off,
                                                        const uint16_t *group_table, const uint32_t *group_src,
                                                        const uint32_t *col_out, const u4 *__restrict__ tsw,
-                                                       const u4 *__restrict__ X16, double *__restrict__ Y) {
+                                                       const u4 *__restrict__ X16, double *__restrict__ Y,
+                                                       uint32_t first_slot, float *__restrict__ S) {
   constexpr int CHUNK = MT * 3 * 64;  // u4 per k-step: [plane][mt][lane]
   constexpr int PER = (CHUNK + 511) / 512;
   constexpr int MH = (MT + 1) / 2;
@@ -213,7 +218,8 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, const
   const int half = wave & 1, pair = wave >> 1, g = lane >> 4;
   const int mt0 = half * MH;
   const int kq = ksteps * 4;  // u4 per plane of a split expansion
-  const uint32_t task = blockIdx.x;
+  const uint32_t task = xcd_index(ntasks);
+  if (task >= ntasks) return;  // uniform over the workgroup
   int col[2];
   uint32_t out[2];
 #pragma unroll
@@ -319,6 +325,21 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, const
 #pragma unroll
   for (int nt = 0; nt < 2; ++nt) {
     if (out[nt] == 0xFFFFFFFFu) continue;
+    if (out[nt] >= first_slot) {
+      // scratch slot of pair mode: the task's one group, whose f32 sum is exact in f32
+      float *s = S + size_t(out[nt] - first_slot) * nc;
+#pragma unroll
+      for (int i = 0; i < MH; ++i) {
+        const int mt = mt0 + i;
+        if (mt >= MT) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = mt * 16 + 4 * g + r;
+          if (row < nc) s[row] = float(sum[i][nt][r]);
+        }
+      }
+      continue;
+    }
     double *y = Y + size_t(out[nt]) * nc;
 #pragma unroll
     for (int i = 0; i < MH; ++i) {
@@ -349,13 +370,15 @@ __global__ void k_split_bf16(size_t nboxes, int nc, int kp, const double *X, uin
 
 // Ordered reduction of per-pair results: Y[row_box[r]] += sum_e S[row_ent[e]] in row
 // order, one workgroup per output box, one thread per coefficient.
-__global__ void k_reduce_slots(int nc, const uint32_t *row_box, const uint32_t *row_off, const uint32_t *row_ent,
-                               const double *S, double *Y) {
-  const uint32_t r = blockIdx.x;
+template <typename T>
+__global__ void k_reduce_slots(int nc, uint32_t rows, const uint32_t *row_box, const uint32_t *row_off,
+                               const uint32_t *row_ent, uint32_t first_slot, const T *S, double *Y) {
+  const uint32_t r = xcd_index(rows);
+  if (r >= rows) return;
   const uint32_t box = row_box[r];
   for (int i = threadIdx.x; i < nc; i += blockDim.x) {
     double acc = Y[size_t(box) * nc + i];
-    for (uint32_t e = row_off[r]; e < row_off[r + 1]; ++e) acc += S[size_t(row_ent[e]) * nc + i];
+    for (uint32_t e = row_off[r]; e < row_off[r + 1]; ++e) acc += double(S[size_t(row_ent[e] - first_slot) * nc + i]);
     Y[size_t(box) * nc + i] = acc;
   }
 }
@@ -364,17 +387,19 @@ __global__ void k_reduce_slots(int nc, const uint32_t *row_box, const uint32_t *
 
 void contract_f64(hipStream_t s, int p, int width, uint32_t ntasks, const uint32_t *task_group_off,
                   const uint16_t *group_table, const uint32_t *group_src, const uint32_t *col_out, const double *tsw,
-                  const double *X, double *Y) {
+                  const double *X, double *Y, uint32_t first_slot, double *S) {
   if (ntasks == 0) return;
   const int nc = (p + 1) * (p + 1), nch = k_chunks(p);
 #define FMMH_CONTRACT(MT)                                                                                        \
   case MT:                                                                                                       \
     if (width == 16)                                                                                             \
-      hipLaunchKernelGGL((k_contract_f64<MT, 8>), dim3(ntasks), dim3(512), 0, s, nc, nch, task_group_off,        \
-                         group_table, group_src, col_out, tsw, X, Y);                                            \
+      hipLaunchKernelGGL((k_contract_f64<MT, 8>), dim3(xcd_grid(ntasks)), dim3(512), 0, s, nc, nch, ntasks,      \
+                         task_group_off,                                                                         \
+                         group_table, group_src, col_out, tsw, X, Y, first_slot, S);                             \
     else                                                                                                         \
-      hipLaunchKernelGGL((k_contract_f64<MT, 1>), dim3(ntasks), dim3(512), 0, s, nc, nch, task_group_off,        \
-                         group_table, group_src, col_out, tsw, X, Y);                                            \
+      hipLaunchKernelGGL((k_contract_f64<MT, 1>), dim3(xcd_grid(ntasks)), dim3(512), 0, s, nc, nch, ntasks,      \
+                         task_group_off,                                                                         \
+                         group_table, group_src, col_out, tsw, X, Y, first_slot, S);                             \
     break;
   switch (m_tiles(p)) {
     FMMH_CONTRACT(1)
@@ -401,15 +426,16 @@ void contract_f64(hipStream_t s, int p, int width, uint32_t ntasks, const uint32
 
 void contract_bf16(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off, const uint16_t *group_table,
                    const uint32_t *group_src, const uint32_t *col_out, const uint16_t *tsw16, const uint16_t *X16,
-                   double *Y) {
+                   double *Y, uint32_t first_slot, float *S) {
   if (ntasks == 0) return;
   const int nc = (p + 1) * (p + 1), ks = k_steps16(p);
   const u4 *t4 = reinterpret_cast<const u4 *>(tsw16);
   const u4 *x4 = reinterpret_cast<const u4 *>(X16);
 #define FMMH_CONTRACT(MT)                                                                                        \
   case MT:                                                                                                       \
-    hipLaunchKernelGGL(k_contract_bf16<MT>, dim3(ntasks), dim3(512), 0, s, nc, ks, task_group_off, group_table,  \
-                       group_src, col_out, t4, x4, Y);                                                           \
+    hipLaunchKernelGGL(k_contract_bf16<MT>, dim3(xcd_grid(ntasks)), dim3(512), 0, s, nc, ks, ntasks,             \
+                       task_group_off, group_table,                                                              \
+                       group_src, col_out, t4, x4, Y, first_slot, S);                                            \
     break;
   switch (m_tiles(p)) {
     FMMH_CONTRACT(1)
@@ -443,9 +469,15 @@ void split_bf16(hipStream_t s, int p, size_t nboxes, const double *X, uint16_t *
 }
 
 void reduce_slots(hipStream_t s, int p, uint32_t rows, const uint32_t *row_box, const uint32_t *row_off,
-                  const uint32_t *row_ent, const double *S, double *Y) {
+                  const uint32_t *row_ent, uint32_t first_slot, const double *S, const float *S32, double *Y) {
   if (rows == 0) return;
-  hipLaunchKernelGGL(k_reduce_slots, dim3(rows), dim3(256), 0, s, (p + 1) * (p + 1), row_box, row_off, row_ent, S, Y);
+  const int nc = (p + 1) * (p + 1);
+  if (S32)
+    hipLaunchKernelGGL(k_reduce_slots<float>, dim3(xcd_grid(rows)), dim3(256), 0, s, nc, rows, row_box, row_off,
+                       row_ent, first_slot, S32, Y);
+  else
+    hipLaunchKernelGGL(k_reduce_slots<double>, dim3(xcd_grid(rows)), dim3(256), 0, s, nc, rows, row_box, row_off,
+                       row_ent, first_slot, S, Y);
 }
 
 }  // namespace dev

@@ -93,10 +93,11 @@ __global__ void k_gather_charges(const double *q, const uint32_t *order, size_t 
 
 // Output pass (fmm-exec/src/fmm.rs:117-145; CONVENTIONS §3.13 "Output"): per leaf,
 // phi = phi^ / (4 pi r_t) and grad = g^ / (4 pi r_t^2) in f64, into caller order.
-__global__ void k_output(const uint32_t *tgt_start, const uint32_t *leaf_cell, const uint32_t *order,
-                         double side, const double *phi_hat, const double *grad_hat, double *phi,
-                         double *grad, int gradients) {
-  const uint32_t lf = blockIdx.x;
+__global__ void k_output(uint32_t nleaves, const uint32_t *tgt_start, const uint32_t *leaf_cell,
+                         const uint32_t *order, double side, const double *phi_hat, const double *grad_hat,
+                         double *phi, double *grad, int gradients) {
+  const uint32_t lf = xcd_index(nleaves);
+  if (lf >= nleaves) return;
   const uint32_t t0 = tgt_start[lf], t1 = tgt_start[lf + 1];
   const double rt = ldexp(side, -int(leaf_cell[4 * lf]) - 1);
   const double s1 = 4.0 * 3.141592653589793 * rt, s2 = s1 * rt;
@@ -138,7 +139,8 @@ __global__ void k_output(const uint32_t *tgt_start, const uint32_t *leaf_cell, c
 // normal number, which changes nothing for distinct points (r^2 >= 1e-22 in leaf units
 // at the deepest level) and keeps a separation rounded to zero finite.
 template <typename T, bool GRAD>
-__global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, const uint32_t *near_off, const uint32_t *near_ent,
+__global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, uint32_t count, const uint32_t *near_off,
+                                            const uint32_t *near_ent,
                                             const uint32_t *leaf_cell, const uint32_t *src_start,
                                             const uint32_t *tgt_start, const double *sx, const double *sy,
                                             const double *sz, const double *sq, const double *tx, const double *ty,
@@ -153,7 +155,9 @@ __global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, const uint32_t 
   __shared__ V4 ys[TILE];
   __shared__ float4 yl[SPLIT ? TILE : 1];
   const int lane = threadIdx.x;
-  const uint32_t lf = leaf_first + blockIdx.x;
+  const uint32_t li = xcd_index(count);
+  if (li >= count) return;
+  const uint32_t lf = leaf_first + li;
   const uint32_t t0 = tgt_start[lf], t1 = tgt_start[lf + 1];
   if (t0 == t1) return;
   const uint32_t *ct = leaf_cell + 4 * size_t(lf);
@@ -313,7 +317,7 @@ __global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, const uint32_t 
 // About 10 KB of LDS per wave, so 16 waves share a CU.
 
 template <bool IRREG>
-__global__ __launch_bounds__(64) void k_expand(int p, const uint32_t *row_box, const uint32_t *row_cell,
+__global__ __launch_bounds__(64) void k_expand(int p, uint32_t rows, const uint32_t *row_box, const uint32_t *row_cell,
                                                const uint32_t *row_off, const uint32_t *row_ent,
                                                const uint32_t *leaf_cell, const uint32_t *src_start,
                                                const double *sx, const double *sy, const double *sz,
@@ -330,7 +334,8 @@ __global__ __launch_bounds__(64) void k_expand(int p, const uint32_t *row_box, c
   const int phases = 64 / P1;
   const int m = lane % P1, ph = lane / P1;
   const bool active = ph < phases;
-  const uint32_t row = blockIdx.x;
+  const uint32_t row = xcd_index(rows);
+  if (row >= rows) return;
   const uint32_t box = row_box[row];
   if (!IRREG)
     for (int t = lane; t < (kMaxDegree + 1) * (kMaxDegree + 1); t += 64) {
@@ -465,7 +470,8 @@ __device__ __forceinline__ void read_order(const double *c, int n, int q, int pm
 }
 
 template <bool IRREG, bool GRAD>
-__global__ __launch_bounds__(EVAL_THREADS) void k_evaluate(int p, const uint32_t *row_leaf, const uint32_t *row_off,
+__global__ __launch_bounds__(EVAL_THREADS) void k_evaluate(int p, uint32_t rows, const uint32_t *row_leaf,
+                                                           const uint32_t *row_off,
                                                            const uint32_t *row_box, const uint32_t *box_cell,
                                                            const uint32_t *leaf_cell, const uint32_t *tgt_start,
                                                            const double *tx, const double *ty, const double *tz,
@@ -473,7 +479,8 @@ __global__ __launch_bounds__(EVAL_THREADS) void k_evaluate(int p, const uint32_t
   __shared__ double C[kMaxCoef];
   __shared__ double G[3][kMaxCoefGrad];
   const int nc = (p + 1) * (p + 1);
-  const uint32_t row = blockIdx.x;
+  const uint32_t row = xcd_index(rows);
+  if (row >= rows) return;
   const uint32_t lf = row_leaf[row];
   const uint32_t t0 = tgt_start[lf], t1 = tgt_start[lf + 1];
   if (t0 == t1) return;
@@ -610,8 +617,8 @@ void output_pass(hipStream_t s, uint32_t nleaves, const uint32_t *tgt_start, con
                  const uint32_t *order, double side, const double *phi_hat, const double *grad_hat,
                  double *phi, double *grad, int gradients) {
   if (nleaves == 0) return;
-  hipLaunchKernelGGL(k_output, dim3(nleaves), dim3(128), 0, s, tgt_start, leaf_cell, order, side, phi_hat,
-                     grad_hat, phi, grad, gradients);
+  hipLaunchKernelGGL(k_output, dim3(xcd_grid(nleaves)), dim3(128), 0, s, nleaves, tgt_start, leaf_cell, order, side,
+                     phi_hat, grad_hat, phi, grad, gradients);
 }
 
 template <typename T>
@@ -621,11 +628,11 @@ void p2p(hipStream_t s, uint32_t leaf_first, uint32_t count, int gradients, cons
          const double *tx, const double *ty, const double *tz, double *phi, double *grad) {
   if (count == 0) return;
   if (gradients)
-    hipLaunchKernelGGL((k_p2p<T, true>), dim3(count), dim3(64), 0, s, leaf_first, near_off, near_ent, leaf_cell,
-                       src_start, tgt_start, sx, sy, sz, sq, tx, ty, tz, phi, grad);
+    hipLaunchKernelGGL((k_p2p<T, true>), dim3(xcd_grid(count)), dim3(64), 0, s, leaf_first, count, near_off, near_ent,
+                       leaf_cell, src_start, tgt_start, sx, sy, sz, sq, tx, ty, tz, phi, grad);
   else
-    hipLaunchKernelGGL((k_p2p<T, false>), dim3(count), dim3(64), 0, s, leaf_first, near_off, near_ent, leaf_cell,
-                       src_start, tgt_start, sx, sy, sz, sq, tx, ty, tz, phi, grad);
+    hipLaunchKernelGGL((k_p2p<T, false>), dim3(xcd_grid(count)), dim3(64), 0, s, leaf_first, count, near_off,
+                       near_ent, leaf_cell, src_start, tgt_start, sx, sy, sz, sq, tx, ty, tz, phi, grad);
 }
 
 template void p2p<float>(hipStream_t, uint32_t, uint32_t, int, const uint32_t *, const uint32_t *, const uint32_t *,
@@ -642,11 +649,11 @@ void expand(hipStream_t s, int p, int irregular, uint32_t rows, const uint32_t *
             double *coeffs) {
   if (rows == 0) return;
   if (irregular)
-    hipLaunchKernelGGL(k_expand<true>, dim3(rows), dim3(64), 0, s, p, row_box, row_cell, row_off, row_ent, leaf_cell,
-                       src_start, sx, sy, sz, sq, coeffs);
+    hipLaunchKernelGGL(k_expand<true>, dim3(xcd_grid(rows)), dim3(64), 0, s, p, rows, row_box, row_cell, row_off,
+                       row_ent, leaf_cell, src_start, sx, sy, sz, sq, coeffs);
   else
-    hipLaunchKernelGGL(k_expand<false>, dim3(rows), dim3(64), 0, s, p, row_box, row_cell, row_off, row_ent, leaf_cell,
-                       src_start, sx, sy, sz, sq, coeffs);
+    hipLaunchKernelGGL(k_expand<false>, dim3(xcd_grid(rows)), dim3(64), 0, s, p, rows, row_box, row_cell, row_off,
+                       row_ent, leaf_cell, src_start, sx, sy, sz, sq, coeffs);
 }
 
 void evaluate(hipStream_t s, int p, int irregular, int gradients, uint32_t rows, const uint32_t *row_leaf,
@@ -655,8 +662,8 @@ void evaluate(hipStream_t s, int p, int irregular, int gradients, uint32_t rows,
               const double *tz, const double *coeffs, double *phi, double *grad) {
   if (rows == 0) return;
 #define FMMH_EVAL(I, G)                                                                                          \
-  hipLaunchKernelGGL((k_evaluate<I, G>), dim3(rows), dim3(EVAL_THREADS), 0, s, p, row_leaf, row_off, row_box,    \
-                     box_cell, leaf_cell, tgt_start, tx, ty, tz, coeffs, phi, grad)
+  hipLaunchKernelGGL((k_evaluate<I, G>), dim3(xcd_grid(rows)), dim3(EVAL_THREADS), 0, s, p, rows, row_leaf,     \
+                     row_off, row_box, box_cell, leaf_cell, tgt_start, tx, ty, tz, coeffs, phi, grad)
   if (irregular) {
     if (gradients)
       FMMH_EVAL(true, true);

@@ -11,13 +11,29 @@ namespace dev {
 constexpr int kMaxDegree = 15;  // device path: p <= 15, (p+1)^2 <= 256 coefficients
 constexpr int kMaxCoef = (kMaxDegree + 1) * (kMaxDegree + 1);
 constexpr int kMaxCoefGrad = (kMaxDegree + 2) * (kMaxDegree + 2);
-constexpr int P2P_THREADS = 128;
-constexpr int P2P_TILE = 128;
-constexpr int EXP_THREADS = 64;
-constexpr int EXP_TILE = 16;
 constexpr int EVAL_THREADS = 128;
 constexpr int COLS = 128;  // output columns (boxes) per contraction task: 8 waves x 16
 constexpr int KC = 4;      // MFMA k-steps per LDS chunk of a table
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2: block b
+// runs on XCD b % 8.  With a chunk C > 0, xcd_index deals chunks of C consecutive items
+// (leaves, boxes, tasks in Morton order) to the XCDs in turn, so neighbouring items share
+// one L2 while every XCD still sees the whole range (the items' costs vary along it).
+// C = 0 keeps block b on item b.  Launch xcd_grid(n) blocks; an index >= n is idle.
+#ifndef FMMH_XCD_CHUNK
+#define FMMH_XCD_CHUNK 0
+#endif
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kXcdChunk = FMMH_XCD_CHUNK;
+inline uint32_t xcd_grid(uint32_t n) {
+  return kXcdChunk == 0 ? n : (n + kXcds * kXcdChunk - 1) / (kXcds * kXcdChunk) * (kXcds * kXcdChunk);
+}
+__device__ __forceinline__ uint32_t xcd_index(uint32_t n) {
+  (void)n;
+  if (kXcdChunk == 0) return blockIdx.x;
+  const uint32_t x = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
+  return ((k / kXcdChunk) * kXcds + x) * kXcdChunk + k % kXcdChunk;
+}
 
 hipError_t upload_constants();
 
@@ -51,19 +67,22 @@ void evaluate(hipStream_t s, int p, int irregular, int gradients, uint32_t rows,
 
 // Grouped dense contraction on MFMA (contract.hip): for every task (COLS output columns)
 // and every group of the task, Y[out[c]] += T_group X[src[g, c]]; f64, or bf16x3 split
-// with f32 accumulation per group and f64 across groups.
+// with f32 accumulation per group and f64 across groups.  An output id >= first_slot is
+// a scratch slot (pair mode, written by exactly one group): its row out - first_slot of S
+// is stored, not added to (f32 rows for bf16x3: one group's f32 sum).
 // width: columns per task, COLS (wide) or 16 (narrow: the waves split the rows)
 void contract_f64(hipStream_t s, int p, int width, uint32_t ntasks, const uint32_t *task_group_off,
                   const uint16_t *group_table, const uint32_t *group_src, const uint32_t *col_out,
-                  const double *tables_swizzled, const double *X, double *Y);
+                  const double *tables_swizzled, const double *X, double *Y, uint32_t first_slot, double *S);
 void contract_bf16(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off,
                    const uint16_t *group_table, const uint32_t *group_src, const uint32_t *col_out,
-                   const uint16_t *tables_swizzled, const uint16_t *X16, double *Y);
+                   const uint16_t *tables_swizzled, const uint16_t *X16, double *Y, uint32_t first_slot, float *S);
 // X16: three bf16 planes (hi, mid, lo) of split_expansion_size(p) / 3 values per box
 void split_bf16(hipStream_t s, int p, size_t nboxes, const double *X, uint16_t *X16);
-// Y[row_box[r]] += sum of S rows row_ent[row_off[r] .. row_off[r+1]) in order
+// Y[row_box[r]] += sum of the scratch rows row_ent[row_off[r] .. row_off[r+1]) - first_slot
+// in order, from S (f64) or, when S32 is not null, S32 (f32)
 void reduce_slots(hipStream_t s, int p, uint32_t rows, const uint32_t *row_box, const uint32_t *row_off,
-                  const uint32_t *row_ent, const double *S, double *Y);
+                  const uint32_t *row_ent, uint32_t first_slot, const double *S, const float *S32, double *Y);
 
 // Swizzled table sizes (elements) per table at degree p, and the host-side swizzles.
 size_t swizzled_table_size(int p);

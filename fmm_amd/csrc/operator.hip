@@ -176,7 +176,7 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
   q_.alloc(ns_);
   q_in_.alloc(ns_);
 
-  // expansions and outputs (loc_ gains the pair-mode scratch rows in build_schedules)
+  // expansions and outputs (the pair-mode scratch rows are sized in build_schedules)
   mult_.alloc(nbox_total_ * nc_);
   phi_hat_.alloc(nt_);
   grad_hat_.alloc(opt_.gradients ? 3 * nt_ : 0);
@@ -271,7 +271,11 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
     upload_rows(p2l_rows_, id, cell, off, ent);
   }
   build_schedules(plan);
-  loc_.alloc((nbox_total_ + nslots_) * nc_);
+  loc_.alloc(nbox_total_ * nc_);
+  if (opt_.precision == kMixed)
+    slots32_.alloc(nslots_ * nc_);
+  else
+    slots_.alloc(nslots_ * nc_);
   if (opt_.precision == kMixed) mult16_.alloc(nbox_total_ * dev::split_expansion_size(opt_.p));
   hip_check(hipDeviceSynchronize(), "operator build");
 }
@@ -454,13 +458,15 @@ void DeviceOperator::build_schedules(const Plan &plan) {
 }
 
 size_t DeviceOperator::device_bytes() const {
-  return mult_.bytes() + loc_.bytes() + t_m2l_.bytes() + t_m2m_.bytes() + t_l2l_.bytes() + sx_.bytes() * 8;
+  return mult_.bytes() + loc_.bytes() + slots_.bytes() + slots32_.bytes() + t_m2l_.bytes() + t_m2m_.bytes() +
+         t_l2l_.bytes() + sx_.bytes() * 8;
 }
 
 void DeviceOperator::launch_contract(const Schedule &s, uint32_t task_first, uint32_t ntasks, const double *X,
                                      double *Y, const DevBuf<double> &tables) {
   dev::contract_f64(stream_, opt_.p, s.width, ntasks, s.task_group_off.ptr + task_first, s.group_table.ptr,
-                    s.group_src.ptr, s.col_out.ptr + size_t(task_first) * s.width, tables.ptr, X, Y);
+                    s.group_src.ptr, s.col_out.ptr + size_t(task_first) * s.width, tables.ptr, X, Y,
+                    static_cast<uint32_t>(nbox_total_), slots_.ptr);
 }
 
 // M2L of the levels level_first .. level_last: the contraction tasks, then the ordered
@@ -475,7 +481,7 @@ void DeviceOperator::launch_m2l(int level_first, int level_last) {
       timed(kKindM2lContract, [&] {
         dev::contract_bf16(stream_, opt_.p, t1 - t0, m2l_.task_group_off.ptr + t0, m2l_.group_table.ptr,
                            m2l_.group_src.ptr, m2l_.col_out.ptr + size_t(t0) * dev::COLS, t_m2l16_.ptr,
-                           mult16_.ptr, loc_.ptr);
+                           mult16_.ptr, loc_.ptr, static_cast<uint32_t>(nbox_total_), slots32_.ptr);
       });
     } else {
       timed(kKindM2lContract, [&] { launch_contract(m2l_, t0, t1 - t0, mult_.ptr, loc_.ptr, t_m2l_); });
@@ -484,7 +490,7 @@ void DeviceOperator::launch_m2l(int level_first, int level_last) {
   const uint32_t r0 = m2l_red_level_[level_first], r1 = m2l_red_level_[level_last + 1];
   if (r1 > r0)
     dev::reduce_slots(stream_, opt_.p, r1 - r0, m2l_red_.row_id.ptr + r0, m2l_red_.row_off.ptr + r0,
-                      m2l_red_.row_ent.ptr, loc_.ptr, loc_.ptr);
+                      m2l_red_.row_ent.ptr, static_cast<uint32_t>(nbox_total_), slots_.ptr, slots32_.ptr, loc_.ptr);
 }
 
 void DeviceOperator::launch_p2l(int level_first, int level_last) {

@@ -157,7 +157,9 @@ class DeviceOperator {
   DevBuf<double> mult_, loc_, phi_hat_, grad_hat_, phi_, grad_;
   DevBuf<double> t_m2m_, t_l2l_, t_m2l_;
   DevBuf<uint16_t> t_m2l16_, mult16_;  // mixed precision: split tables and multipoles
-  size_t nslots_ = 0;                  // scratch rows after the boxes in loc_ (pair mode)
+  size_t nslots_ = 0;                  // pair-mode scratch rows (slot ids follow the box ids)
+  DevBuf<double> slots_;               // plain f64: scratch rows
+  DevBuf<float> slots32_;              // mixed: scratch rows (one group's f32 sum each)
 
   bool timing_ = false;
   struct Mark {
