@@ -21,6 +21,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -107,7 +108,8 @@ void swizzle_table_bf16(int p, const double *a, uint16_t *out) {
 // (every wave all M-tiles of its 16 columns), WM = 8 for narrow tasks of 16 columns on
 // the coarse levels, where the waves split the rows instead.
 template <int MT, int WM>
-__global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, uint32_t ntasks, const uint32_t *task_group_off,
+__global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, uint32_t ntasks, uint32_t xchunk,
+                                                      const uint32_t *task_group_off,
                                                       const uint16_t *group_table, const uint32_t *group_src,
                                                       const uint32_t *col_out, const double *__restrict__ tsw,
                                                       const double *__restrict__ X, double *__restrict__ Y,
@@ -120,7 +122,7 @@ __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, uint3
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm = wave % WM, col = (wave / WM) * 16 + (lane & 15), g = lane >> 4;
   const int KS = nchunks * KC;
-  const uint32_t task = xcd_index(ntasks);
+  const uint32_t task = xcd_index(ntasks, xchunk);
   if (task >= ntasks) return;  // uniform over the workgroup
   const uint32_t out = col_out[size_t(task) * W + col];
   const uint32_t g0 = task_group_off[task], g1 = task_group_off[task + 1];
@@ -195,6 +197,18 @@ __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, uint3
 
 // ---- bf16x3 contraction ---------------------------------------------------------------------
 
+// 16 bytes per lane global -> LDS without registers (global_load_lds_dwordx4): lane i
+// writes lds + 16 i; lds is the same for the whole wave.  The builtin exists only in the
+// device compilation.
+__device__ __forceinline__ void dma16(const u4 *g, u4 *lds) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_amdgcn_global_load_lds(g, lds, 16, 0, 0);
+#else
+  (void)g;
+  (void)lds;
+#endif
+}
+
 __device__ __forceinline__ bf8 as_bf8(u4 v) {
   bf8 r;
   memcpy(&r, &v, 16);
@@ -204,21 +218,30 @@ __device__ __forceinline__ bf8 as_bf8(u4 v) {
 // Waves tile the task two-dimensionally: wave w takes the M-tiles of half w % 2 (MH of
 // them) for the 32 columns of pair w / 2, so each A fragment read from LDS feeds two
 // N-tiles (half the LDS traffic of one column tile per wave).
+//
+// Pipeline: the table slices of k-steps go global -> LDS by DMA (global_load_lds) into a
+// ring of NST = 3 buffers, issued two k-steps ahead; the source expansions (B operand)
+// go to registers one k-step ahead.  One barrier per k-step: after it every wave's DMA
+// for this step has landed and every wave is done with the buffer the next DMA refills.
 template <int MT>
-__global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, uint32_t ntasks, const uint32_t *task_group_off,
-                                                       const uint16_t *group_table, const uint32_t *group_src,
-                                                       const uint32_t *col_out, const u4 *__restrict__ tsw,
+__global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, uint32_t ntasks, uint32_t xchunk,
+                                                       const uint32_t *__restrict__ task_group_off,
+                                                       const uint16_t *__restrict__ group_table,
+                                                       const uint32_t *__restrict__ group_src,
+                                                       const uint32_t *__restrict__ col_out, const u4 *__restrict__ tsw,
                                                        const u4 *__restrict__ X16, double *__restrict__ Y,
                                                        uint32_t first_slot, float *__restrict__ S) {
-  constexpr int CHUNK = MT * 3 * 64;  // u4 per k-step: [plane][mt][lane]
-  constexpr int PER = (CHUNK + 511) / 512;
+  constexpr int CHUNK = MT * 3 * 64;    // u4 per k-step: [plane][mt][lane]
+  constexpr int SEGS = CHUNK / 64;      // 1 KB segments, one DMA instruction each
+  constexpr int SEGW = (SEGS + 7) / 8;  // DMA instructions per wave per k-step
+  constexpr int NST = 3;
   constexpr int MH = (MT + 1) / 2;
-  __shared__ u4 As[2][CHUNK];
+  __shared__ u4 As[NST][CHUNK];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int half = wave & 1, pair = wave >> 1, g = lane >> 4;
   const int mt0 = half * MH;
   const int kq = ksteps * 4;  // u4 per plane of a split expansion
-  const uint32_t task = xcd_index(ntasks);
+  const uint32_t task = xcd_index(ntasks, xchunk);
   if (task >= ntasks) return;  // uniform over the workgroup
   int col[2];
   uint32_t out[2];
@@ -240,41 +263,80 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, uint3
 #pragma unroll
       for (int r = 0; r < 4; ++r) sum[i][nt][r] = 0.0;
     }
-  u4 pa[PER], pb[2][3], b[2][3];
+  u4 pb[2][3], b[2][3];
   const u4 zero4 = {0u, 0u, 0u, 0u};
-  auto fetch = [&](int step) {
+  // table slice of k-step `step` into ring buffer `buf`: wave w copies the segments
+  // w, w + 8, ...; a wave past the last segment copies the last one again (same bytes,
+  // same place), so every wave issues SEGW DMA instructions
+  // the table index of a group, through a 32-bit load of the pair holding it (uniform:
+  // a scalar load, outside the vector-memory counter)
+  auto table_of = [&](uint32_t gi) -> uint32_t {
+    const uint32_t w = reinterpret_cast<const uint32_t *>(group_table)[gi >> 1];
+    return (gi & 1) ? (w >> 16) : (w & 0xFFFFu);
+  };
+  auto dma = [&](int step, int buf) {
     const uint32_t gi = g0 + uint32_t(step / ksteps);
     const int ks = step % ksteps;
-    const uint32_t t = group_table[gi];
-    const u4 *gsrc = tsw + (size_t(t) * ksteps + ks) * CHUNK;
-    // clamped index: every load is issued unconditionally, so pa stays in registers
+    const u4 *gsrc = tsw + (size_t(table_of(gi)) * ksteps + ks) * CHUNK;
 #pragma unroll
-    for (int i = 0; i < PER; ++i) pa[i] = gsrc[min(tid + i * 512, CHUNK - 1)];
+    for (int i = 0; i < SEGW; ++i) {
+      const int seg = min(wave + 8 * i, SEGS - 1);
+      dma16(gsrc + seg * 64 + lane, &As[buf][seg * 64]);
+    }
+  };
+  // Every lane issues the same loads whether its column has a source or not (the value
+  // is zeroed after the load), so the vector-memory counter advances the same way on
+  // every path and the waits can count on it.  The source indices of a step are loaded
+  // a step before its B loads, which are loaded a step before use.
+  uint32_t src_n[2];
+  auto fetch_src = [&](int step) {
+    const uint32_t gi = g0 + uint32_t(step / ksteps);
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) src_n[nt] = group_src[size_t(gi) * COLS + col[nt]];
+  };
+  auto fetch_b = [&](int step) {
+    const int ks = step % ksteps;
 #pragma unroll
     for (int nt = 0; nt < 2; ++nt) {
-      const uint32_t src = group_src[size_t(gi) * COLS + col[nt]];
+      const uint32_t src = src_n[nt];
+      const size_t s = src != 0xFFFFFFFFu ? src : 0;
 #pragma unroll
-      for (int pl = 0; pl < 3; ++pl)
-        pb[nt][pl] = src != 0xFFFFFFFFu ? X16[(size_t(src) * 3 + pl) * kq + ks * 4 + g] : zero4;
+      for (int pl = 0; pl < 3; ++pl) {
+        const u4 v = X16[(s * 3 + pl) * kq + ks * 4 + g];
+        pb[nt][pl] = src != 0xFFFFFFFFu ? v : zero4;
+      }
     }
   };
-  auto stash = [&](int buf) {
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = tid + i * 512;
-      if (e < CHUNK) As[buf][e] = pa[i];
-    }
+  // s_waitcnt with vmcnt = V (expcnt and lgkmcnt left free), gfx9 encoding
+  auto wait_vm = [](auto n) {
+    constexpr int V = decltype(n)::value;
+    __builtin_amdgcn_s_waitcnt((V & 15) | ((V >> 4) << 14) | (7 << 4) | (15 << 8));
   };
-  fetch(0);
-  stash(0);
+  // Issue order in step s: B loads of s + 1, source indices of s + 2, DMA of s + 2
+  // (steps past the end repeat the last step, into a ring slot nobody reads again).  The
+  // B loads of s + 1 are waited for at the end of step s, which also retires everything
+  // issued before them, the DMA of s + 1 included.
+  const int last = nsteps - 1;
+  fetch_src(0);
+  fetch_b(0);
+  fetch_src(min(1, last));
+  dma(0, 0);
+  dma(min(1, last), 1);
 #pragma unroll
   for (int nt = 0; nt < 2; ++nt)
 #pragma unroll
     for (int pl = 0; pl < 3; ++pl) b[nt][pl] = pb[nt][pl];
-  __syncthreads();
   for (int step = 0; step < nsteps; ++step) {
-    const int buf = step & 1;
-    if (step + 1 < nsteps) fetch(step + 1);
+    const int buf = step % NST;
+    wait_vm(std::integral_constant<int, SEGW>());  // this wave's part of step s landed
+    __builtin_amdgcn_s_barrier();                  // every wave's part; ring slot s - 1 free
+    fetch_b(min(step + 1, last));
+    fetch_src(min(step + 2, last));
+    // keep the issue order: the DMA after the loads (a wait for the B loads must not
+    // retire the DMA with them)
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("" ::: "memory");
+    dma(min(step + 2, last), (step + 2) % NST);
     bf8 bh[2], bm[2], bl[2];
 #pragma unroll
     for (int nt = 0; nt < 2; ++nt) {
@@ -314,13 +376,13 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, uint3
           acc[i][nt] = f4{0.f, 0.f, 0.f, 0.f};
         }
     }
-    if (step + 1 < nsteps) stash(buf ^ 1);
-    __syncthreads();
 #pragma unroll
     for (int nt = 0; nt < 2; ++nt)
 #pragma unroll
       for (int pl = 0; pl < 3; ++pl) b[nt][pl] = pb[nt][pl];
   }
+  // no DMA may land in LDS after the workgroup ends
+  wait_vm(std::integral_constant<int, 0>());
   // accumulator element r of lane (g, j): row 4 g + r of the M-tile (bf16 16x16x32)
 #pragma unroll
   for (int nt = 0; nt < 2; ++nt) {
@@ -387,17 +449,19 @@ __global__ void k_reduce_slots(int nc, uint32_t rows, const uint32_t *row_box, c
 
 void contract_f64(hipStream_t s, int p, int width, uint32_t ntasks, const uint32_t *task_group_off,
                   const uint16_t *group_table, const uint32_t *group_src, const uint32_t *col_out, const double *tsw,
-                  const double *X, double *Y, uint32_t first_slot, double *S) {
+                  const double *X, double *Y, uint32_t first_slot, double *S, uint32_t xchunk) {
   if (ntasks == 0) return;
   const int nc = (p + 1) * (p + 1), nch = k_chunks(p);
 #define FMMH_CONTRACT(MT)                                                                                        \
   case MT:                                                                                                       \
     if (width == 16)                                                                                             \
-      hipLaunchKernelGGL((k_contract_f64<MT, 8>), dim3(xcd_grid(ntasks)), dim3(512), 0, s, nc, nch, ntasks,      \
+      hipLaunchKernelGGL((k_contract_f64<MT, 8>), dim3(xcd_grid(ntasks, xchunk)), dim3(512), 0, s, nc, nch,      \
+                         ntasks, xchunk,                                                                         \
                          task_group_off,                                                                         \
                          group_table, group_src, col_out, tsw, X, Y, first_slot, S);                             \
     else                                                                                                         \
-      hipLaunchKernelGGL((k_contract_f64<MT, 1>), dim3(xcd_grid(ntasks)), dim3(512), 0, s, nc, nch, ntasks,      \
+      hipLaunchKernelGGL((k_contract_f64<MT, 1>), dim3(xcd_grid(ntasks, xchunk)), dim3(512), 0, s, nc, nch,      \
+                         ntasks, xchunk,                                                                         \
                          task_group_off,                                                                         \
                          group_table, group_src, col_out, tsw, X, Y, first_slot, S);                             \
     break;
@@ -426,14 +490,15 @@ void contract_f64(hipStream_t s, int p, int width, uint32_t ntasks, const uint32
 
 void contract_bf16(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off, const uint16_t *group_table,
                    const uint32_t *group_src, const uint32_t *col_out, const uint16_t *tsw16, const uint16_t *X16,
-                   double *Y, uint32_t first_slot, float *S) {
+                   double *Y, uint32_t first_slot, float *S, uint32_t xchunk) {
   if (ntasks == 0) return;
   const int nc = (p + 1) * (p + 1), ks = k_steps16(p);
   const u4 *t4 = reinterpret_cast<const u4 *>(tsw16);
   const u4 *x4 = reinterpret_cast<const u4 *>(X16);
 #define FMMH_CONTRACT(MT)                                                                                        \
   case MT:                                                                                                       \
-    hipLaunchKernelGGL(k_contract_bf16<MT>, dim3(xcd_grid(ntasks)), dim3(512), 0, s, nc, ks, ntasks,             \
+    hipLaunchKernelGGL(k_contract_bf16<MT>, dim3(xcd_grid(ntasks, xchunk)), dim3(512), 0, s, nc, ks, ntasks,     \
+                       xchunk,                                                                                   \
                        task_group_off, group_table,                                                              \
                        group_src, col_out, t4, x4, Y, first_slot, S);                                            \
     break;

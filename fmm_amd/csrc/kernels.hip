@@ -122,8 +122,8 @@ __global__ void k_output(uint32_t nleaves, const uint32_t *tgt_start, const uint
 // f64 in HBM; each is split into hi + lo floats (hi = fl32(u), lo = fl32(u - hi)) and the
 // separation is formed as (t_hi - y_hi) + (t_lo - y_lo), which keeps it accurate to about
 // 2^-24 of |d| even for close pairs (a plain f32 difference loses 2^-24 of |u| >> |d|);
-// r^2, the inverse square root and the sums are f32, added into the f64 output once per
-// target.  Coincident points give d = 0 exactly in both forms.
+// r^2, the inverse square root and the sums of a tile (at most 64 sources) are f32; the
+// tile sums are added in f64.  Coincident points give d = 0 exactly in both forms.
 
 // Lane mapping (one wave per target leaf, targets processed in chunks of 128): a chunk of
 // n <= 64 targets uses np = the next power of two >= n lanes per source phase and
@@ -154,6 +154,9 @@ __global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, uint32_t count,
   using V4 = typename std::conditional<SPLIT, float4, double4>::type;
   __shared__ V4 ys[TILE];
   __shared__ float4 yl[SPLIT ? TILE : 1];
+  __shared__ double fr[TILE][4];       // frame (c, r) of each entry of a chunk of the near row
+  __shared__ uint32_t fstart[TILE];    // first source of each entry
+  __shared__ uint32_t fcum[TILE + 1];  // exclusive prefix sums of the entries' source counts
   const int lane = threadIdx.x;
   const uint32_t li = xcd_index(count);
   if (li >= count) return;
@@ -187,6 +190,21 @@ __global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, uint32_t count,
       lz[a] = T(z - double(uz[a]));
       ap[a] = ax[a] = ay[a] = az[a] = T(0);
     }
+    // f32 sums are per tile (at most 64 terms) and go into f64 after each tile, so long
+    // near rows do not lose digits to one growing f32 sum
+    double cp[2] = {0.0, 0.0}, cx[2] = {0.0, 0.0}, cy[2] = {0.0, 0.0}, cz[2] = {0.0, 0.0};
+    auto flush = [&]() {
+      if (SPLIT) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          cp[a] += double(ap[a]);
+          cx[a] += double(ax[a]);
+          cy[a] += double(ay[a]);
+          cz[a] += double(az[a]);
+          ap[a] = ax[a] = ay[a] = az[a] = T(0);
+        }
+      }
+    };
     // one source j of the tile against the lane's targets
     auto step = [&](int j, bool check) {
       const V4 v = ys[j];
@@ -219,42 +237,14 @@ __global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, uint32_t count,
         }
       }
     };
-    // tile cursor over the near row: entry e, sources sb .. s1 of leaf sl
-    uint32_t e = e0, sl = 0, sb = 0, s1 = 0;
-    auto seek = [&](uint32_t from) {
-      for (e = from; e < e1; ++e) {
-        sl = near_ent[e];
-        sb = src_start[sl];
-        s1 = src_start[sl + 1];
-        if (sb < s1) return;
-      }
-    };
-    double px = 0.0, py = 0.0, pz = 0.0, pq = 0.0;
-    auto prefetch = [&]() {
-      const uint32_t s = sb + uint32_t(lane);
-      if (s < s1) {
-        px = sx[s];
-        py = sy[s];
-        pz = sz[s];
-        pq = sq[s];
-      }
-    };
-    seek(e0);
-    if (e < e1) prefetch();
-    while (e < e1) {
-      const int cnt = int((s1 - sb) < uint32_t(TILE) ? (s1 - sb) : uint32_t(TILE));
-      const int steps = (cnt + phases - 1) / phases;
-      const bool self = sl == lf;
-      double c[3], r;
-      relative_frame<double>(leaf_cell + 4 * size_t(sl), ct, c, r);
+    // stage one tile: lane < cnt holds the source (x, y, z, q) in the target frame;
+    // lanes cnt .. steps * phases - 1 hold zero charges far away
+    auto stage = [&](int cnt, int steps, bool valid, double yx, double yy, double yz, double q) {
       __syncthreads();
       if (lane < steps * phases) {
-        double yx = FAR, yy = FAR, yz = FAR, q = 0.0;
-        if (lane < cnt) {
-          yx = self ? px : fma(r, px, c[0]);
-          yy = self ? py : fma(r, py, c[1]);
-          yz = self ? pz : fma(r, pz, c[2]);
-          q = pq;
+        if (!(valid && lane < cnt)) {
+          yx = yy = yz = FAR;
+          q = 0.0;
         }
         V4 v;
         v.x = T(yx);
@@ -267,41 +257,125 @@ __global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, uint32_t count,
                                  float(yz - double(float(yz))), 0.0f);
       }
       __syncthreads();
-      // advance the cursor and start the next tile's loads before computing this one
-      if (sb + TILE < s1)
-        sb += TILE;
-      else
-        seek(e + 1);
-      if (e < e1) prefetch();
-      if (self) {
-        for (int k = 0; k < steps; ++k) step(phase + k * phases, true);
-      } else {
-        int k = 0;
-        for (; k + 2 <= steps; k += 2) {
-          step(phase + k * phases, false);
-          step(phase + (k + 1) * phases, false);
+    };
+    // The near row is walked in chunks of up to 64 entries.  The own leaf's sources are
+    // computed apart (r^2 == 0 select); the other entries' sources form one stream, cut
+    // into full tiles of 64 whatever the leaf sizes, each lane staging a source of the
+    // entry its stream position falls in (frames and prefix sums in LDS).
+    for (uint32_t cb = e0; cb < e1; cb += TILE) {
+      const int nce = int((e1 - cb) < uint32_t(TILE) ? (e1 - cb) : uint32_t(TILE));
+      uint32_t st = 0, cnt = 0;
+      bool own = false;
+      double c[3] = {0.0, 0.0, 0.0}, r = 1.0;
+      if (lane < nce) {
+        const uint32_t sl = near_ent[cb + lane];
+        st = src_start[sl];
+        cnt = src_start[sl + 1] - st;
+        own = sl == lf;
+        if (own) cnt = 0;
+        relative_frame<double>(leaf_cell + 4 * size_t(sl), ct, c, r);
+      }
+      uint32_t inc = cnt;  // inclusive prefix sum over the lanes
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(inc, off);
+        if (lane >= off) inc += v;
+      }
+      const uint32_t total = __shfl(inc, 63);
+      const bool has_own = __ballot(own) != 0;
+      __syncthreads();  // the previous chunk's tiles are done with the entry table
+      fr[lane][0] = c[0];
+      fr[lane][1] = c[1];
+      fr[lane][2] = c[2];
+      fr[lane][3] = r;
+      fstart[lane] = st;
+      fcum[lane + 1] = inc;
+      if (lane == 0) fcum[0] = 0;
+      if (has_own) {
+        // the own leaf: y = u_s; coincident points add nothing
+        const uint32_t s0 = src_start[lf], s1 = src_start[lf + 1];
+        for (uint32_t sb = s0; sb < s1; sb += TILE) {
+          const int cn = int((s1 - sb) < uint32_t(TILE) ? (s1 - sb) : uint32_t(TILE));
+          const int steps = (cn + phases - 1) / phases;
+          const uint32_t s = sb + uint32_t(lane);
+          const bool valid = s < s1;
+          stage(cn, steps, valid, valid ? sx[s] : 0.0, valid ? sy[s] : 0.0, valid ? sz[s] : 0.0,
+                valid ? sq[s] : 0.0);
+          for (int k2 = 0; k2 < steps; ++k2) step(phase + k2 * phases, true);
+          flush();
         }
-        if (k < steps) step(phase + k * phases, false);
+      } else {
+        __syncthreads();  // the entry table is visible to every lane
+      }
+      // stream position f of lane: entry e with fcum[e] <= f < fcum[e + 1]
+      uint32_t ea = 0;  // first entry of the current tile (uniform)
+      uint32_t pe = 0;
+      bool pvalid = false;
+      double px = 0.0, py = 0.0, pz = 0.0, pq = 0.0;
+      auto prefetch = [&](uint32_t base) {
+        while (fcum[ea + 1] <= base) ++ea;
+        const uint32_t f = base + uint32_t(lane);
+        pvalid = f < total;
+        if (pvalid) {
+          uint32_t e = ea;
+          while (fcum[e + 1] <= f) ++e;
+          pe = e;
+          const uint32_t s = fstart[e] + (f - fcum[e]);
+          px = sx[s];
+          py = sy[s];
+          pz = sz[s];
+          pq = sq[s];
+        }
+      };
+      if (total > 0) prefetch(0);
+      for (uint32_t base = 0; base < total; base += TILE) {
+        const int cn = int((total - base) < uint32_t(TILE) ? (total - base) : uint32_t(TILE));
+        const int steps = (cn + phases - 1) / phases;
+        double yx = 0.0, yy = 0.0, yz = 0.0;
+        if (pvalid) {
+          yx = fma(fr[pe][3], px, fr[pe][0]);
+          yy = fma(fr[pe][3], py, fr[pe][1]);
+          yz = fma(fr[pe][3], pz, fr[pe][2]);
+        }
+        stage(cn, steps, pvalid, yx, yy, yz, pq);
+        // start the next tile's loads before computing this one
+        if (base + TILE < total) prefetch(base + TILE);
+        int k2 = 0;
+        for (; k2 + 2 <= steps; k2 += 2) {
+          step(phase + k2 * phases, false);
+          step(phase + (k2 + 1) * phases, false);
+        }
+        if (k2 < steps) step(phase + k2 * phases, false);
+        flush();
+      }
+    }
+    if (!SPLIT) {
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        cp[a] = double(ap[a]);
+        cx[a] = double(ax[a]);
+        cy[a] = double(ay[a]);
+        cz[a] = double(az[a]);
       }
     }
     // add the phases' partial sums (lanes slot + np f share a target)
     for (int off = np; off < 64; off <<= 1) {
-      ap[0] += __shfl_xor(ap[0], off);
+      cp[0] += __shfl_xor(cp[0], off);
       if (GRAD) {
-        ax[0] += __shfl_xor(ax[0], off);
-        ay[0] += __shfl_xor(ay[0], off);
-        az[0] += __shfl_xor(az[0], off);
+        cx[0] += __shfl_xor(cx[0], off);
+        cy[0] += __shfl_xor(cy[0], off);
+        cz[0] += __shfl_xor(cz[0], off);
       }
     }
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
       const uint32_t i = tb + slot + a * 64;
       if (ok[a] && phase == 0) {
-        phi[i] += double(ap[a]);
+        phi[i] += cp[a];
         if (GRAD) {
-          grad[3 * size_t(i) + 0] += double(ax[a]);
-          grad[3 * size_t(i) + 1] += double(ay[a]);
-          grad[3 * size_t(i) + 2] += double(az[a]);
+          grad[3 * size_t(i) + 0] += cx[a];
+          grad[3 * size_t(i) + 1] += cy[a];
+          grad[3 * size_t(i) + 2] += cz[a];
         }
       }
     }
@@ -323,7 +397,8 @@ __global__ __launch_bounds__(64) void k_expand(int p, uint32_t rows, const uint3
                                                const double *sx, const double *sy, const double *sz,
                                                const double *sq, double *coeffs) {
   constexpr int TP = 32;                       // points per tile
-  constexpr int DW = (kMaxDegree + 1) * TP;    // doubles per diagonal plane
+  constexpr int LD = TP + 1;                   // row stride of a diagonal plane (no bank conflicts)
+  constexpr int DW = (kMaxDegree + 1) * LD;    // doubles per diagonal plane
   __shared__ double pt[3][TP];                 // z, q, r2 (regular) or 1/r2 (irregular)
   __shared__ double diag[2 * DW];              // Re, Im of X_m^m at [m][j]; then the phase sums
   __shared__ double inv_tab[kMaxDegree + 1][kMaxDegree + 1];  // 1 / ((n + m)(n - m)) at [n - m][m]
@@ -378,8 +453,8 @@ __global__ __launch_bounds__(64) void k_expand(int p, uint32_t rows, const uint3
           const double nr = f * (x * dr - y * di), ni = f * (x * di + y * dr);
           dr = nr;
           di = ni;
-          dre[k * TP + lane] = dr;
-          dim[k * TP + lane] = di;
+          dre[k * LD + lane] = dr;
+          dim[k * LD + lane] = di;
         }
         pt[0][lane] = z;
         pt[1][lane] = sq[s];
@@ -389,7 +464,7 @@ __global__ __launch_bounds__(64) void k_expand(int p, uint32_t rows, const uint3
       if (active)
         for (int j = ph; j < cnt; j += phases) {
           const double z = pt[0][j], q = pt[1][j], w = pt[2][j];
-          double pr = 0.0, pi = 0.0, cr = dre[m * TP + j], ci = dim[m * TP + j];
+          double pr = 0.0, pi = 0.0, cr = dre[m * LD + j], ci = dim[m * LD + j];
 #pragma unroll
           for (int k = 0; k <= kMaxDegree; ++k) {
             const int n = m + k;

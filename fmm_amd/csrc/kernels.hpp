@@ -34,6 +34,16 @@ __device__ __forceinline__ uint32_t xcd_index(uint32_t n) {
   const uint32_t x = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
   return ((k / kXcdChunk) * kXcds + x) * kXcdChunk + k % kXcdChunk;
 }
+// The same with a chunk chosen at launch (0: block b on item b).
+inline uint32_t xcd_grid(uint32_t n, uint32_t chunk) {
+  return chunk == 0 ? n : (n + kXcds * chunk - 1) / (kXcds * chunk) * (kXcds * chunk);
+}
+__device__ __forceinline__ uint32_t xcd_index(uint32_t n, uint32_t chunk) {
+  (void)n;
+  if (chunk == 0) return blockIdx.x;
+  const uint32_t x = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
+  return ((k / chunk) * kXcds + x) * chunk + k % chunk;
+}
 
 hipError_t upload_constants();
 
@@ -71,12 +81,16 @@ void evaluate(hipStream_t s, int p, int irregular, int gradients, uint32_t rows,
 // a scratch slot (pair mode, written by exactly one group): its row out - first_slot of S
 // is stored, not added to (f32 rows for bf16x3: one group's f32 sum).
 // width: columns per task, COLS (wide) or 16 (narrow: the waves split the rows)
+// xchunk: tasks dealt to the XCDs in runs of xchunk consecutive tasks (0: block b runs
+// task b).
 void contract_f64(hipStream_t s, int p, int width, uint32_t ntasks, const uint32_t *task_group_off,
                   const uint16_t *group_table, const uint32_t *group_src, const uint32_t *col_out,
-                  const double *tables_swizzled, const double *X, double *Y, uint32_t first_slot, double *S);
+                  const double *tables_swizzled, const double *X, double *Y, uint32_t first_slot, double *S,
+                  uint32_t xchunk);
 void contract_bf16(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off,
                    const uint16_t *group_table, const uint32_t *group_src, const uint32_t *col_out,
-                   const uint16_t *tables_swizzled, const uint16_t *X16, double *Y, uint32_t first_slot, float *S);
+                   const uint16_t *tables_swizzled, const uint16_t *X16, double *Y, uint32_t first_slot, float *S,
+                   uint32_t xchunk);
 // X16: three bf16 planes (hi, mid, lo) of split_expansion_size(p) / 3 values per box
 void split_bf16(hipStream_t s, int p, size_t nboxes, const double *X, uint16_t *X16);
 // Y[row_box[r]] += sum of the scratch rows row_ent[row_off[r] .. row_off[r+1]) - first_slot

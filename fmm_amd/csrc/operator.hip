@@ -30,26 +30,29 @@ struct ScheduleBuilder {
 
   explicit ScheduleBuilder(int w = dev::COLS) : width(w) {}
 
-  void add_task(const std::vector<uint32_t> &cols, const std::vector<std::vector<std::pair<uint16_t, uint32_t>>> &lists) {
+  // Groups are ordered by table, or by rank[table] when a rank is given.
+  void add_task(const std::vector<uint32_t> &cols, const std::vector<std::vector<std::pair<uint16_t, uint32_t>>> &lists,
+                const uint16_t *rank = nullptr) {
     const int C = width;
-    std::map<uint16_t, std::array<uint32_t, dev::COLS>> by_table;
+    std::map<uint16_t, std::pair<uint16_t, std::array<uint32_t, dev::COLS>>> by_key;
     for (int c = 0; c < C; ++c) {
       col_out.push_back(c < int(cols.size()) ? cols[c] : kNone);
       if (c >= int(cols.size())) continue;
       for (const auto &e : lists[c]) {
-        auto it = by_table.find(e.first);
-        if (it == by_table.end()) {
+        const uint16_t key = rank ? rank[e.first] : e.first;
+        auto it = by_key.find(key);
+        if (it == by_key.end()) {
           std::array<uint32_t, dev::COLS> a;
           a.fill(kNone);
-          it = by_table.emplace(e.first, a).first;
+          it = by_key.emplace(key, std::make_pair(e.first, a)).first;
         }
-        it->second[c] = e.second;
+        it->second.second[c] = e.second;
         ++pairs;
       }
     }
-    for (const auto &kv : by_table) {
-      group_table.push_back(kv.first);
-      group_src.insert(group_src.end(), kv.second.begin(), kv.second.begin() + C);
+    for (const auto &kv : by_key) {
+      group_table.push_back(kv.second.first);
+      group_src.insert(group_src.end(), kv.second.second.begin(), kv.second.second.begin() + C);
       slots += C;
     }
     task_group_off.push_back(static_cast<uint32_t>(group_table.size()));
@@ -336,6 +339,22 @@ void DeviceOperator::kind_times(double *ms, int64_t *launches) {
 
 void DeviceOperator::build_schedules(const Plan &plan) {
   auto gid = [&](int level, uint32_t b) { return box_base_[level] + b; };
+  // rank of offset d for targets of octant o: (parent-level neighbour D, source octant o')
+  for (int oc = 0; oc < 8; ++oc) {
+    const int ob[3] = {(oc >> 2) & 1, (oc >> 1) & 1, oc & 1};
+    for (int d = 0; d < 316; ++d) {
+      int dv[3];
+      offset_of_index(d, dv);
+      int D = 0, op = 0;
+      for (int a = 0; a < 3; ++a) {
+        const int s = ob[a] + dv[a];
+        const int bit = ((s % 2) + 2) % 2;
+        D = 3 * D + ((s - bit) / 2 + 1);
+        op = 2 * op + bit;
+      }
+      m2l_rank_[oc][d] = static_cast<uint16_t>(8 * D + op);
+    }
+  }
   const int C = dev::COLS;
   m2m_.resize(nlevels_);
   l2l_.resize(nlevels_);
@@ -399,24 +418,33 @@ void DeviceOperator::build_schedules(const Plan &plan) {
   for (int l = 0; l < nlevels_; ++l) {
     if (l >= 2) {
       const Csr &V = plan.v[l];
+      // Tasks of the eight octant classes interleaved (task k of class 0, of class 1, ...),
+      // and each task's groups ordered by (parent-level neighbour D, source octant o'),
+      // d = 2 D + o' - o: the k-th tasks of the eight classes hold the children of the
+      // same parents, so at the same group position they read the same source boxes,
+      // and the launch keeps those eight tasks on one XCD (one L2).
       ScheduleBuilder tile;
-      for (int o = 0; o < 8; ++o) {
-        std::vector<uint32_t> cols;
-        std::vector<std::vector<std::pair<uint16_t, uint32_t>>> lists;
-        for (uint32_t b = 0; b < plan.nboxes(l); ++b) {
-          if (octant_of(plan.keys[l][b]) != o || V.off[b + 1] == V.off[b]) continue;
-          cols.push_back(gid(l, b));
-          std::vector<std::pair<uint16_t, uint32_t>> li;
-          for (uint32_t e = V.off[b]; e < V.off[b + 1]; ++e) li.push_back({V.aux[e], gid(l, V.ent[e])});
-          lists.push_back(std::move(li));
-          if (int(cols.size()) == C) {
-            tile.add_task(cols, lists);
-            cols.clear();
-            lists.clear();
-          }
-        }
-        if (!cols.empty()) tile.add_task(cols, lists);
+      std::vector<std::vector<uint32_t>> cls_cols(8);
+      std::vector<std::vector<std::vector<std::pair<uint16_t, uint32_t>>>> cls_lists(8);
+      for (uint32_t b = 0; b < plan.nboxes(l); ++b) {
+        if (V.off[b + 1] == V.off[b]) continue;
+        const int o = octant_of(plan.keys[l][b]);
+        cls_cols[o].push_back(gid(l, b));
+        std::vector<std::pair<uint16_t, uint32_t>> li;
+        for (uint32_t e = V.off[b]; e < V.off[b + 1]; ++e) li.push_back({V.aux[e], gid(l, V.ent[e])});
+        cls_lists[o].push_back(std::move(li));
       }
+      size_t most = 0;
+      for (int o = 0; o < 8; ++o) most = std::max(most, cls_cols[o].size());
+      for (size_t at = 0; at < most; at += size_t(C))
+        for (int o = 0; o < 8; ++o) {
+          if (at >= cls_cols[o].size()) continue;
+          const size_t end = std::min(cls_cols[o].size(), at + size_t(C));
+          std::vector<uint32_t> cols(cls_cols[o].begin() + at, cls_cols[o].begin() + end);
+          std::vector<std::vector<std::pair<uint16_t, uint32_t>>> lists(cls_lists[o].begin() + at,
+                                                                          cls_lists[o].begin() + end);
+          tile.add_task(cols, lists, m2l_rank_[o].data());
+        }
       if (tile.pairs == 0 || 2 * tile.pairs >= tile.slots) {
         all.append(tile);
       } else {
@@ -463,10 +491,10 @@ size_t DeviceOperator::device_bytes() const {
 }
 
 void DeviceOperator::launch_contract(const Schedule &s, uint32_t task_first, uint32_t ntasks, const double *X,
-                                     double *Y, const DevBuf<double> &tables) {
+                                     double *Y, const DevBuf<double> &tables, uint32_t xchunk) {
   dev::contract_f64(stream_, opt_.p, s.width, ntasks, s.task_group_off.ptr + task_first, s.group_table.ptr,
                     s.group_src.ptr, s.col_out.ptr + size_t(task_first) * s.width, tables.ptr, X, Y,
-                    static_cast<uint32_t>(nbox_total_), slots_.ptr);
+                    static_cast<uint32_t>(nbox_total_), slots_.ptr, xchunk);
 }
 
 // M2L of the levels level_first .. level_last: the contraction tasks, then the ordered
@@ -481,10 +509,10 @@ void DeviceOperator::launch_m2l(int level_first, int level_last) {
       timed(kKindM2lContract, [&] {
         dev::contract_bf16(stream_, opt_.p, t1 - t0, m2l_.task_group_off.ptr + t0, m2l_.group_table.ptr,
                            m2l_.group_src.ptr, m2l_.col_out.ptr + size_t(t0) * dev::COLS, t_m2l16_.ptr,
-                           mult16_.ptr, loc_.ptr, static_cast<uint32_t>(nbox_total_), slots32_.ptr);
+                           mult16_.ptr, loc_.ptr, static_cast<uint32_t>(nbox_total_), slots32_.ptr, kM2lXcdRun);
       });
     } else {
-      timed(kKindM2lContract, [&] { launch_contract(m2l_, t0, t1 - t0, mult_.ptr, loc_.ptr, t_m2l_); });
+      timed(kKindM2lContract, [&] { launch_contract(m2l_, t0, t1 - t0, mult_.ptr, loc_.ptr, t_m2l_, kM2lXcdRun); });
     }
   }
   const uint32_t r0 = m2l_red_level_[level_first], r1 = m2l_red_level_[level_last + 1];

@@ -10,6 +10,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -20,6 +21,8 @@ namespace fmmh {
 
 constexpr int kNumKinds = 10;        // timing kinds: the operator kinds, other, M2L contraction
 constexpr int kKindM2lContract = 9;  // the M2L contraction kernel alone
+// M2L tasks go to the XCDs in runs of 8: the eight octant classes of one block of parents
+constexpr uint32_t kM2lXcdRun = 8;
 
 enum Precision : int {
   kPlainF64 = 0,  // every operator in f64
@@ -137,7 +140,7 @@ class DeviceOperator {
   void collect_timings();
   void launch_p2p(uint32_t leaf_first, uint32_t count);
   void launch_contract(const Schedule &s, uint32_t task_first, uint32_t ntasks, const double *X, double *Y,
-                       const DevBuf<double> &tables);
+                       const DevBuf<double> &tables, uint32_t xchunk = 0);
   void launch_m2l(int level_first, int level_last);
   void launch_p2l(int level_first, int level_last);
 
@@ -157,6 +160,7 @@ class DeviceOperator {
   DevBuf<double> mult_, loc_, phi_hat_, grad_hat_, phi_, grad_;
   DevBuf<double> t_m2m_, t_l2l_, t_m2l_;
   DevBuf<uint16_t> t_m2l16_, mult16_;  // mixed precision: split tables and multipoles
+  std::array<std::array<uint16_t, 316>, 8> m2l_rank_;  // M2L group order per target octant
   size_t nslots_ = 0;                  // pair-mode scratch rows (slot ids follow the box ids)
   DevBuf<double> slots_;               // plain f64: scratch rows
   DevBuf<float> slots32_;              // mixed: scratch rows (one group's f32 sum each)
