@@ -103,22 +103,50 @@ void swizzle_table_bf16(int p, const double *a, uint16_t *out) {
 
 // ---- f64 contraction ---------------------------------------------------------------------
 
+// 16 bytes per lane global -> LDS without registers (global_load_lds_dwordx4): lane i
+// writes lds + 16 i; lds is the same for the whole wave.  The builtin exists only in the
+// device compilation.
+__device__ __forceinline__ void dma16(const u4 *g, u4 *lds) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_amdgcn_global_load_lds(g, lds, 16, 0, 0);
+#else
+  (void)g;
+  (void)lds;
+#endif
+}
+
+// s_waitcnt with vmcnt = V (expcnt and lgkmcnt left free), gfx9 encoding
+template <int V>
+__device__ __forceinline__ void wait_vm() {
+  static_assert(V >= 0 && V < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((V & 15) | ((V >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
 // Wave w owns the columns of group w / WM (16 each, 8 / WM groups, so a task has
 // 128 / WM columns) and the M-tiles mt = w % WM, w % WM + WM, ...: WM = 1 for wide tasks
 // (every wave all M-tiles of its 16 columns), WM = 8 for narrow tasks of 16 columns on
 // the coarse levels, where the waves split the rows instead.
+//
+// Pipeline as in the bf16x3 kernel below: table chunks global -> LDS by DMA into a ring
+// of 3 buffers two chunks ahead, source coefficients to registers one chunk ahead, one
+// barrier per chunk.
 template <int MT, int WM>
 __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, uint32_t ntasks, uint32_t xchunk,
-                                                      const uint32_t *task_group_off,
-                                                      const uint16_t *group_table, const uint32_t *group_src,
-                                                      const uint32_t *col_out, const double *__restrict__ tsw,
-                                                      const double *__restrict__ X, double *__restrict__ Y,
-                                                      uint32_t first_slot, double *__restrict__ S) {
-  constexpr int CHUNK2 = MT * KC * 32;  // dbl2 per chunk
-  constexpr int PER = (CHUNK2 + 511) / 512;
+                                                      const uint32_t *__restrict__ task_group_off,
+                                                      const uint16_t *__restrict__ group_table,
+                                                      const uint32_t *__restrict__ group_src,
+                                                      const uint32_t *__restrict__ col_out,
+                                                      const double *__restrict__ tsw, const double *__restrict__ X,
+                                                      double *__restrict__ Y, uint32_t first_slot,
+                                                      double *__restrict__ S) {
+  constexpr int CHUNK2 = MT * KC * 32;  // 16-byte units per chunk
+  constexpr int SEGS = CHUNK2 / 64;     // 1 KB segments (MT KC / 2 of them)
+  constexpr int SEGW = (SEGS + 7) / 8;  // DMA instructions per wave per chunk
+  constexpr int NST = 3;
   constexpr int W = 128 / WM;              // columns per task
   constexpr int MTW = (MT + WM - 1) / WM;  // M-tiles per wave
-  __shared__ dbl2 As[2][CHUNK2];
+  static_assert(CHUNK2 % 64 == 0, "whole segments");
+  __shared__ u4 As[NST][CHUNK2];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm = wave % WM, col = (wave / WM) * 16 + (lane & 15), g = lane >> 4;
   const int KS = nchunks * KC;
@@ -131,38 +159,57 @@ __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, uint3
   d4 acc[MTW];
 #pragma unroll
   for (int i = 0; i < MTW; ++i) acc[i] = d4{0.0, 0.0, 0.0, 0.0};
-  dbl2 pa[PER];
   double pb[KC], b[KC];
-  auto fetch = [&](int step) {
+  uint32_t src_n = 0;
+  auto table_of = [&](uint32_t gi) -> uint32_t {
+    const uint32_t w = reinterpret_cast<const uint32_t *>(group_table)[gi >> 1];
+    return (gi & 1) ? (w >> 16) : (w & 0xFFFFu);
+  };
+  auto dma = [&](int step, int buf) {
     const uint32_t gi = g0 + uint32_t(step / nchunks);
     const int c = step % nchunks;
-    const uint32_t t = group_table[gi];
-    const uint32_t src = group_src[size_t(gi) * W + col];
-    const dbl2 *gsrc = reinterpret_cast<const dbl2 *>(tsw + (size_t(t) * nchunks + c) * (CHUNK2 * 2));
-    // clamped index: every load is issued unconditionally, so pa stays in registers
+    const u4 *gsrc = reinterpret_cast<const u4 *>(tsw + (size_t(table_of(gi)) * nchunks + c) * (CHUNK2 * 2));
 #pragma unroll
-    for (int i = 0; i < PER; ++i) pa[i] = gsrc[min(tid + i * 512, CHUNK2 - 1)];
+    for (int i = 0; i < SEGW; ++i) {
+      const int seg = min(wave + 8 * i, SEGS - 1);
+      dma16(gsrc + seg * 64 + lane, &As[buf][seg * 64]);
+    }
+  };
+  auto fetch_src = [&](int step) { src_n = group_src[size_t(g0 + uint32_t(step / nchunks)) * W + col]; };
+  // The loads are unconditional (a column without a source reads row 0, k is clamped;
+  // k >= nc meets zero table entries) and the value is zeroed afterwards for a column
+  // without a source: every path issues the same loads.
+  auto fetch_b = [&](int step) {
+    const int c = step % nchunks;
+    const size_t s = src_n != 0xFFFFFFFFu ? src_n : 0;
 #pragma unroll
     for (int kk = 0; kk < KC; ++kk) {
       const int k = c * KC + kk + g * KS;
-      pb[kk] = (src != 0xFFFFFFFFu && k < nc) ? X[size_t(src) * nc + k] : 0.0;
+      const double v = X[s * nc + min(k, nc - 1)];
+      pb[kk] = src_n != 0xFFFFFFFFu ? v : 0.0;
     }
   };
-  auto stash = [&](int buf) {
+  auto take_b = [&]() {
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = tid + i * 512;
-      if (e < CHUNK2) As[buf][e] = pa[i];
-    }
+    for (int kk = 0; kk < KC; ++kk) b[kk] = pb[kk];
   };
-  fetch(0);
-  stash(0);
-#pragma unroll
-  for (int kk = 0; kk < KC; ++kk) b[kk] = pb[kk];
-  __syncthreads();
+  const int last = nsteps - 1;
+  fetch_src(0);
+  fetch_b(0);
+  fetch_src(min(1, last));
+  dma(0, 0);
+  dma(min(1, last), 1);
   for (int step = 0; step < nsteps; ++step) {
-    const int buf = step & 1;
-    if (step + 1 < nsteps) fetch(step + 1);
+    const int buf = step % NST;
+    wait_vm<SEGW>();               // chunk s and its B loads landed (only the DMA of s + 1 may not)
+    __builtin_amdgcn_s_barrier();  // every wave's part; ring slot s - 1 free
+    __builtin_amdgcn_sched_barrier(0);  // nothing that reads this step's data moves above
+    take_b();
+    fetch_b(min(step + 1, last));
+    fetch_src(min(step + 2, last));
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("" ::: "memory");
+    dma(min(step + 2, last), (step + 2) % NST);
     const double *a = reinterpret_cast<const double *>(As[buf]);
 #pragma unroll
     for (int kk = 0; kk < KC; ++kk)
@@ -172,11 +219,8 @@ __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, uint3
         if (mt < MT)
           acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[(mt * KC + kk) * 64 + lane], b[kk], acc[i], 0, 0, 0);
       }
-    if (step + 1 < nsteps) stash(buf ^ 1);
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < KC; ++kk) b[kk] = pb[kk];
   }
+  wait_vm<0>();  // no DMA may land in LDS after the workgroup ends
   if (out != 0xFFFFFFFFu) {
     // a box row adds; a scratch slot (pair mode: one group, one writer) is stored
     const bool slot = out >= first_slot;
@@ -196,18 +240,6 @@ __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, uint3
 }
 
 // ---- bf16x3 contraction ---------------------------------------------------------------------
-
-// 16 bytes per lane global -> LDS without registers (global_load_lds_dwordx4): lane i
-// writes lds + 16 i; lds is the same for the whole wave.  The builtin exists only in the
-// device compilation.
-__device__ __forceinline__ void dma16(const u4 *g, u4 *lds) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  __builtin_amdgcn_global_load_lds(g, lds, 16, 0, 0);
-#else
-  (void)g;
-  (void)lds;
-#endif
-}
 
 __device__ __forceinline__ bf8 as_bf8(u4 v) {
   bf8 r;
@@ -307,29 +339,25 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, uint3
       }
     }
   };
-  // s_waitcnt with vmcnt = V (expcnt and lgkmcnt left free), gfx9 encoding
-  auto wait_vm = [](auto n) {
-    constexpr int V = decltype(n)::value;
-    __builtin_amdgcn_s_waitcnt((V & 15) | ((V >> 4) << 14) | (7 << 4) | (15 << 8));
-  };
   // Issue order in step s: B loads of s + 1, source indices of s + 2, DMA of s + 2
-  // (steps past the end repeat the last step, into a ring slot nobody reads again).  The
-  // B loads of s + 1 are waited for at the end of step s, which also retires everything
-  // issued before them, the DMA of s + 1 included.
+  // (steps past the end repeat the last step, into a ring slot nobody reads again).  At
+  // the top of step s + 1 only the DMA of s + 2 is newer than the B loads of s + 1 and
+  // the DMA of s + 1, so a wait for all but SEGW operations covers both.
   const int last = nsteps - 1;
   fetch_src(0);
   fetch_b(0);
   fetch_src(min(1, last));
   dma(0, 0);
   dma(min(1, last), 1);
-#pragma unroll
-  for (int nt = 0; nt < 2; ++nt)
-#pragma unroll
-    for (int pl = 0; pl < 3; ++pl) b[nt][pl] = pb[nt][pl];
   for (int step = 0; step < nsteps; ++step) {
     const int buf = step % NST;
-    wait_vm(std::integral_constant<int, SEGW>());  // this wave's part of step s landed
-    __builtin_amdgcn_s_barrier();                  // every wave's part; ring slot s - 1 free
+    wait_vm<SEGW>();               // step s and its B loads landed (only the DMA of s + 1 may not)
+    __builtin_amdgcn_s_barrier();  // every wave's part; ring slot s - 1 free
+    __builtin_amdgcn_sched_barrier(0);  // nothing that reads this step's data moves above
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+      for (int pl = 0; pl < 3; ++pl) b[nt][pl] = pb[nt][pl];
     fetch_b(min(step + 1, last));
     fetch_src(min(step + 2, last));
     // keep the issue order: the DMA after the loads (a wait for the B loads must not
@@ -376,13 +404,9 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, uint3
           acc[i][nt] = f4{0.f, 0.f, 0.f, 0.f};
         }
     }
-#pragma unroll
-    for (int nt = 0; nt < 2; ++nt)
-#pragma unroll
-      for (int pl = 0; pl < 3; ++pl) b[nt][pl] = pb[nt][pl];
   }
   // no DMA may land in LDS after the workgroup ends
-  wait_vm(std::integral_constant<int, 0>());
+  wait_vm<0>();
   // accumulator element r of lane (g, j): row 4 g + r of the M-tile (bf16 16x16x32)
 #pragma unroll
   for (int nt = 0; nt < 2; ++nt) {

@@ -126,3 +126,57 @@ def test_leaf_kinds_alone(setup):
     # the far field is formed here as a difference of two totals whose rounding (~1e-14
     # of the near field) is measured against the smaller far part: 1e-11 is that bar
     assert e1 < 1e-11 and e2 < 1e-11
+
+
+@pytest.fixture(scope="module")
+def mixed_pair():
+    # p = 12 (the 1e-6 setting) on a cube plus a cluster: dense levels run as tiles,
+    # the refined cluster levels as pairs into f32 slots
+    rng = O.Rng(77)
+    pts = np.vstack([rng.cube(20000), rng.plummer(10000, (0.3, 0.6, 0.5), 0.04)])
+    q = rng.normals(len(pts))
+    plain = fmm_amd.FmmBuilder(12).max_points_per_leaf(32).build(pts)
+    mixed = fmm_amd.FmmBuilder(12).precision("mixed").max_points_per_leaf(32).build(pts)
+    return pts, q, plain, mixed
+
+
+def test_mixed_m2l_within_split_precision(mixed_pair):
+    # M2L alone in bf16x3 (six split products, f32 per offset, f64 across offsets)
+    # against M2L in f64 on the same multipoles: the split keeps 24 bits of each factor,
+    # so the locals agree to the f32 contract of 1e-6 (laplace-fmm-plan.md:1696)
+    pts, q, plain, mixed = mixed_pair
+    L = plain.plan.nlevels - 1
+    st = mixed.operator.stats()
+    _log(f"  m2l pairs {st['m2l_pairs']} slots {st['m2l_slots']} levels {L + 1}")
+    locs = []
+    for f in (plain, mixed):
+        op = f.operator
+        op.load_charges(q)
+        op.reset()
+        for level in range(L, -1, -1):
+            op.call("p2m", level)
+        for level in range(L - 1, -1, -1):
+            op.call("m2m", level)
+        for level in range(2, L + 1):
+            op.call("m2l", level)
+        locs.append([op.expansions(True, level) for level in range(L + 1)])
+    for level in range(2, L + 1):
+        e = _rel(locs[1][level], locs[0][level])
+        _log(f"mixed m2l level {level}: rel {e:.3e}")
+        assert e < 1e-6, level
+
+
+def test_mixed_p2p_within_f32(mixed_pair):
+    # P2P alone in f32 with double-float separations and f64 tile sums, against the
+    # f64 near-field direct sums
+    pts, q, plain, mixed = mixed_pair
+    op = mixed.operator
+    op.load_charges(q)
+    op.reset()
+    for level in range(mixed.plan.nlevels):
+        op.call("p2p", level)
+    out = op.read_output()
+    phi, grad = _near_field(mixed.plan, pts, q, mixed)
+    e1, e2 = _rel(out.potential, phi), _rel(out.gradient, grad)
+    _log(f"mixed p2p alone: rel phi {e1:.3e} grad {e2:.3e}")
+    assert e1 < 1e-6 and e2 < 1e-6
