@@ -37,6 +37,29 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_laplace4M_p12_mixed.txt")
+
+
+def hbm_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of the default workload (scaled by the copy-probe factors)."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    scale, rows = {}, {}
+    for line in open(TRAFFIC_FILE):
+        f = line.split()
+        if not f or f[0].startswith("#"):
+            continue
+        if f[0] in ("fetch_scale", "write_scale"):
+            scale[f[0]] = float(f[1])
+        else:
+            rows[f[0]] = (float(f[1]), float(f[2]))
+    if kernel not in rows:
+        return None
+    fk, wk = rows[kernel]
+    return fk * 1024 * scale.get("fetch_scale", 1.0) + wk * 1024 * scale.get("write_scale", 1.0)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,7 +168,9 @@ def main():
     mixed = run("mixed")
     plain = None if args.no_plain else run("f64")
 
-    # roofline of the dominant kernel of the headline (mixed) path
+    # roofline of the dominant kernel of the headline (mixed) path; `traffic` (HBM bytes
+    # per launch) comes from the committed counter runs of the default workload
+    default_workload = (args.n, args.p, args.leaf, args.near) == (4_000_000, 12, 128, "device")
     stats = mixed["stats"]
     kinds = mixed["kinds"]
     n = (args.p + 1) ** 2
@@ -159,7 +184,8 @@ def main():
         avg_ms = kinds["m2l_contract"][0] / kinds["m2l_contract"][1]
         achieved = flops / (avg_ms * 1e-3) / 1e12
         roof = {"kernel": "k_contract_bf16 (M2L, bf16x3 split)", "bound": "mfma", "achieved": achieved,
-                "peak": PEAK_BF16, "unit": "TFLOP/s", "frac": achieved / PEAK_BF16, "traffic": None,
+                "peak": PEAK_BF16, "unit": "TFLOP/s", "frac": achieved / PEAK_BF16,
+                "traffic": hbm_traffic("k_contract_bf16") if default_workload else None,
                 "algorithmic": "2 (p+1)^4 flops per V-list pair x m2l_pairs per launch; the bf16x3 split "
                                "issues 6 bf16 products per algorithmic product",
                 "executed_bf16_tflops": 6 * achieved, "executed_frac": 6 * achieved / PEAK_BF16,
@@ -169,9 +195,19 @@ def main():
         avg_ms = kinds["p2p"][0] / kinds["p2p"][1]
         achieved = 30.0 * pairs / (avg_ms * 1e-3) / 1e12
         roof = {"kernel": "k_p2p<float> (P2P)", "bound": "mfma", "achieved": achieved, "peak": PEAK_F32,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_F32, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_F32,
+                "traffic": hbm_traffic("k_p2p") if default_workload else None,
                 "algorithmic": "15 lane-ops (30 flops) per source-target pair x p2p_pairs per launch, FP32 peak",
+                "compute_unit": "FP32 vector ALU (no matrix cores: the bound is arithmetic, not HBM)",
                 "flops_per_launch": 30.0 * pairs}
+    # the other large kernel beside it
+    c_ms = kinds["m2l_contract"][0] / max(1, kinds["m2l_contract"][1]) if "m2l_contract" in kinds else 0.0
+    if c_ms > 0:
+        c_flops = 2.0 * n * n * stats["m2l_pairs"]
+        c_tf = c_flops / (c_ms * 1e-3) / 1e12
+        roof["m2l_contract"] = {"avg_launch_ms": c_ms, "achieved": c_tf, "executed_bf16_tflops": 6 * c_tf,
+                                "executed_frac_bf16": 6 * c_tf / PEAK_BF16,
+                                "traffic": hbm_traffic("k_contract_bf16") if default_workload else None}
     roof["avg_launch_ms"] = avg_ms
 
     cpu = None
