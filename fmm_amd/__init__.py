@@ -189,6 +189,26 @@ class Operator:
     def sync(self):
         _lib.check(_lib.lib().fmmh_sync(self._h))
 
+    def vjp(self, charges, cotangent, field=None):
+        """dL/dq and (with the forward gradient `field`) dL/dpoints for a plan whose
+        targets are its sources; numpy arrays in caller order."""
+        q = np.ascontiguousarray(charges, dtype=np.float64)
+        g = np.ascontiguousarray(cotangent, dtype=np.float64)
+        gq = np.zeros(len(q))
+        f = gp = None
+        if field is not None:
+            f = np.ascontiguousarray(field, dtype=np.float64).reshape(-1, 3)
+            gp = np.zeros((len(q), 3))
+        _lib.check(_lib.lib().fmmh_vjp(self._h, _lib.ptr(q), _lib.ptr(g), _lib.ptr(f), 0, _lib.ptr(gq),
+                                       _lib.ptr(gp)))
+        return gq, gp
+
+    def vjp_device(self, charges, cotangent, field, grad_charges, grad_points=None):
+        """The same on device buffers (e.g. torch tensors on cuda:0, float64, contiguous)."""
+        vp = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+        _lib.check(_lib.lib().fmmh_vjp(self._h, vp(charges), vp(cotangent), vp(field), 1, vp(grad_charges),
+                                       vp(grad_points)))
+
     def expansions(self, locals_: bool, level: int) -> np.ndarray:
         n = (self.p + 1) ** 2
         out = np.zeros((len(self.plan.box_keys(level)), n))
@@ -237,6 +257,11 @@ class Fmm:
             for k in ("l2p", "m2p", "p2p"):
                 op.call(k, level)
         return op.read_output()
+
+    def vjp(self, charges, cotangent, field=None):
+        """Adjoint of evaluate's potentials (sources = targets): (dL/dq, dL/dpoints) for the
+        cotangent dL/dphi; `field` is evaluate's gradient (needed for dL/dpoints)."""
+        return self.operator.vjp(charges, cotangent, field)
 
 
 class FmmBuilder:

@@ -35,7 +35,7 @@ EXPORTS = [
     "fmmh_device_count", "fmmh_operator_create", "fmmh_operator_free", "fmmh_operator_stats",
     "fmmh_load_charges", "fmmh_reset", "fmmh_p2m", "fmmh_m2m", "fmmh_m2l", "fmmh_p2l", "fmmh_l2l",
     "fmmh_l2p", "fmmh_m2p", "fmmh_p2p", "fmmh_evaluate", "fmmh_read_output", "fmmh_sync",
-    "fmmh_kind_timing", "fmmh_kind_times", "fmmh_read_expansions",
+    "fmmh_kind_timing", "fmmh_kind_times", "fmmh_read_expansions", "fmmh_vjp",
 ]
 
 
@@ -78,6 +78,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise FmmError(1, f"{LIB_PATH} is not built (run fmm_amd.build())")
+    # One HIP runtime per process: when PyTorch is installed, its runtime is loaded first
+    # and the library binds to it (same soname), so torch tensors and the library share
+    # one runtime whichever is used first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp = C.c_void_p
     i32, i64, dbl = C.c_int32, C.c_int64, C.c_double
@@ -116,6 +123,7 @@ def lib():
     L.fmmh_kind_timing.argtypes = [vp, i32]
     L.fmmh_read_expansions.argtypes = [vp, i32, i32, vp]
     L.fmmh_kind_times.argtypes = [vp, vp, vp]
+    L.fmmh_vjp.argtypes = [vp, vp, vp, vp, i32, vp, vp]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is C.c_int:  # default

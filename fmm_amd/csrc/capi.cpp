@@ -241,6 +241,15 @@ int32_t fmmh_read_output(fmmh_operator *op, double *phi, double *grad, int32_t o
   });
 }
 
+int32_t fmmh_vjp(fmmh_operator *op, const double *charges, const double *cotangent, const double *field,
+                 int32_t on_device, double *grad_charges, double *grad_points) {
+  return guard([&] {
+    need(op && charges && cotangent && grad_charges, "fmmh_vjp: NULL argument");
+    need(grad_points == nullptr || field != nullptr, "fmmh_vjp: grad_points needs the forward field");
+    op->op->vjp(charges, cotangent, field, on_device != 0, grad_charges, grad_points);
+  });
+}
+
 int32_t fmmh_sync(fmmh_operator *op) {
   return guard([&] {
     need(op != nullptr, "fmmh_sync: NULL operator");

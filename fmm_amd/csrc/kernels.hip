@@ -112,6 +112,18 @@ __global__ void k_output(uint32_t nleaves, const uint32_t *tgt_start, const uint
   }
 }
 
+// Adjoint of a potential-based loss (sources = targets, caller order): with psi and
+// grad psi the field of the charges g = dL/dphi, dL/dq_j = psi_j and
+// dL/dz_j = q_j grad psi_j + g_j grad phi_j (the second term from moving z_j as a target).
+__global__ void k_vjp_combine(size_t n, const double *q, const double *g, const double *psi,
+                              const double *dpsi, const double *field, double *grad_q, double *grad_pos) {
+  for (size_t j = blockIdx.x * size_t(blockDim.x) + threadIdx.x; j < n; j += size_t(gridDim.x) * blockDim.x) {
+    grad_q[j] = psi[j];
+    if (grad_pos)
+      for (int k = 0; k < 3; ++k) grad_pos[3 * j + k] = q[j] * dpsi[3 * j + k] + g[j] * field[3 * j + k];
+  }
+}
+
 // ---- P2P (CONVENTIONS §3.13 "Device kernels") ---------------------------------------
 // One wave per target leaf, targets in registers; every source leaf of the near row is
 // staged through LDS in tiles of 64 points, with
@@ -681,6 +693,13 @@ static inline unsigned grid_for(size_t n, int threads) {
 void fill(hipStream_t s, double *p, size_t n, double v) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
+}
+
+void vjp_combine(hipStream_t s, size_t n, const double *q, const double *g, const double *psi, const double *dpsi,
+                 const double *field, double *grad_q, double *grad_pos) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_vjp_combine, dim3(grid_for(n, 256)), dim3(256), 0, s, n, q, g, psi, dpsi, field, grad_q,
+                     grad_pos);
 }
 
 void gather_charges(hipStream_t s, const double *q, const uint32_t *order, size_t n, double *q64, float *q32) {

@@ -49,6 +49,9 @@ hipError_t upload_constants();
 
 void fill(hipStream_t s, double *p, size_t n, double v);
 void gather_charges(hipStream_t s, const double *q, const uint32_t *order, size_t n, double *q64, float *q32);
+// grad_q = psi, grad_pos = q dpsi + g field (n points, caller order; grad_pos may be null)
+void vjp_combine(hipStream_t s, size_t n, const double *q, const double *g, const double *psi, const double *dpsi,
+                 const double *field, double *grad_q, double *grad_pos);
 void output_pass(hipStream_t s, uint32_t nleaves, const uint32_t *tgt_start, const uint32_t *leaf_cell,
                  const uint32_t *order, double side, const double *phi_hat, const double *grad_hat,
                  double *phi, double *grad, int gradients);

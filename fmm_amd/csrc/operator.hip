@@ -107,6 +107,7 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
   side_ = plan.side;
   ns_ = plan.ns;
   nt_ = plan.nt;
+  same_points_ = plan.same_points;
   nleaves_ = plan.nleaves();
   leaf_off_ = plan.leaf_off;
   box_base_.assign(nlevels_ + 1, 0);
@@ -660,6 +661,44 @@ void DeviceOperator::read_output(double *phi, double *grad, bool on_device) {
     if (opt_.gradients && grad)
       hip_check(hipMemcpyAsync(grad, grad_.ptr, 3 * nt_ * sizeof(double), hipMemcpyDeviceToHost, stream_),
                 "download grad");
+  }
+  sync();
+}
+
+void DeviceOperator::vjp(const double *q, const double *g, const double *field, bool on_device, double *grad_q,
+                         double *grad_pos) {
+  require(same_points_, "vjp: the adjoint is implemented for sources = targets", kUnsupportedLayout);
+  require(opt_.gradients, "vjp: the operator must compute gradients", kUnsupportedLayout);
+  const size_t n = size_t(nt_);
+  if (n == 0) return;
+  const double *dq = q, *dg = g, *df = field;
+  double *oq = grad_q, *op = grad_pos;
+  if (!on_device) {
+    if (vjp_in_.count == 0) vjp_in_.alloc(5 * n);
+    if (vjp_out_.count == 0) vjp_out_.alloc(4 * n);
+    hip_check(hipMemcpyAsync(vjp_in_.ptr, q, n * sizeof(double), hipMemcpyHostToDevice, stream_), "vjp q");
+    hip_check(hipMemcpyAsync(vjp_in_.ptr + n, g, n * sizeof(double), hipMemcpyHostToDevice, stream_), "vjp g");
+    if (grad_pos)
+      hip_check(hipMemcpyAsync(vjp_in_.ptr + 2 * n, field, 3 * n * sizeof(double), hipMemcpyHostToDevice, stream_),
+                "vjp field");
+    dq = vjp_in_.ptr;
+    dg = vjp_in_.ptr + n;
+    df = vjp_in_.ptr + 2 * n;
+    oq = vjp_out_.ptr;
+    op = grad_pos ? vjp_out_.ptr + n : nullptr;
+  }
+  // the adjoint field: charges g at the same points, potential and gradient
+  load_charges(dg, true);
+  reset();
+  evaluate();
+  dev::output_pass(stream_, nleaves_, tgt_start_.ptr, leaf_cell_.ptr, tgt_order_.ptr, side_, phi_hat_.ptr,
+                   grad_hat_.ptr, phi_.ptr, grad_.ptr, 1);
+  dev::vjp_combine(stream_, n, dq, dg, phi_.ptr, grad_.ptr, df, oq, op);
+  if (!on_device) {
+    hip_check(hipMemcpyAsync(grad_q, oq, n * sizeof(double), hipMemcpyDeviceToHost, stream_), "vjp grad_q");
+    if (grad_pos)
+      hip_check(hipMemcpyAsync(grad_pos, op, 3 * n * sizeof(double), hipMemcpyDeviceToHost, stream_),
+                "vjp grad_pos");
   }
   sync();
 }

@@ -114,6 +114,14 @@ class DeviceOperator {
   void evaluate();
   // output pass + download in caller order (phi[nt], grad[3 nt]); the one sync
   void read_output(double *phi, double *grad, bool on_device);
+  // Vector-Jacobian product of the potentials (sources = targets): given the charges q,
+  // the cotangent g = dL/dphi and the forward gradient field (grad phi, 3 per point), all
+  // in caller order, writes grad_q = dL/dq and, when grad_pos is not null, grad_pos =
+  // dL/dpositions.  Runs one more evaluation with charges g (the adjoint field), so it
+  // overwrites the operator's charges and expansions.  Buffers for host arrays are
+  // allocated on the first call.
+  void vjp(const double *q, const double *g, const double *field, bool on_device, double *grad_q,
+           double *grad_pos);
   void sync();
   // copy the multipoles (locals = false) or locals of one level to host, box-major
   void read_expansions(bool locals, int level, double *out);
@@ -158,6 +166,8 @@ class DeviceOperator {
   DevBuf<uint32_t> near_off_, near_ent_;
   DevBuf<double> sx_, sy_, sz_, tx_, ty_, tz_, q_, q_in_;
   DevBuf<double> mult_, loc_, phi_hat_, grad_hat_, phi_, grad_;
+  DevBuf<double> vjp_in_, vjp_out_;  // host-array staging of vjp (5 n and 4 n doubles)
+  bool same_points_ = false;
   DevBuf<double> t_m2m_, t_l2l_, t_m2l_;
   DevBuf<uint16_t> t_m2l16_, mult16_;  // mixed precision: split tables and multipoles
   std::array<std::array<uint16_t, 316>, 8> m2l_rank_;  // M2L group order per target octant

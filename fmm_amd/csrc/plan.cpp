@@ -281,6 +281,7 @@ Plan build_plan(const double *sources, int64_t ns, const double *targets, int64_
   plan.options = options;
   plan.ns = ns;
   plan.nt = nt;
+  plan.same_points = sources == targets && ns == nt;
   const int64_t n = ns + nt;
   auto point = [&](int64_t i) { return i < ns ? sources + 3 * i : targets + 3 * (i - ns); };
 

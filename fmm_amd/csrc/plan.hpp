@@ -68,6 +68,7 @@ struct Plan {
   // src_order/tgt_order: caller index of the point at each leaf-order position;
   // us/ut: leaf-scaled coordinates (3 per point) in leaf order, f64.
   int64_t ns = 0, nt = 0;
+  bool same_points = false;  // targets are the sources (one array given for both)
   std::vector<uint32_t> src_start, tgt_start, src_order, tgt_order;
   std::vector<double> us, ut;
 

@@ -143,6 +143,17 @@ int32_t fmmh_evaluate(fmmh_operator *op);
 /* Output pass and download: phi[nt], grad[3 nt] (may be NULL) in caller order, to the
  * host (on_device = 0) or to device buffers.  Synchronises. */
 int32_t fmmh_read_output(fmmh_operator *op, double *phi, double *grad, int32_t on_device);
+/* Adjoint (vector-Jacobian product) of the potentials, for a plan built with one array
+ * for sources and targets.  Inputs in caller order: charges q[n], the cotangent
+ * g[n] = dL/dphi (e.g. 2 phi / n for L = sum phi^2 / n) and the forward gradient
+ * field[3 n] = grad phi.  Outputs: grad_charges[n] = dL/dq_j = psi(z_j) and, when
+ * grad_points is not NULL, grad_points[3 n] = dL/dz_j = q_j grad psi(z_j) + g_j grad phi(z_j),
+ * psi the potential of the charges g.  One more evaluation (the operator's charges and
+ * expansions are overwritten); the operator must compute gradients.  No counterpart in
+ * the reference's Rust API (its north-star addition: the adjoint of Fmm::evaluate,
+ * fmm-exec/src/fmm.rs:3691).  Synchronises. */
+int32_t fmmh_vjp(fmmh_operator *op, const double *charges, const double *cotangent, const double *field,
+                 int32_t on_device, double *grad_charges, double *grad_points);
 int32_t fmmh_sync(fmmh_operator *op);
 /* Multipoles (locals = 0) or locals of one level, box-major (p+1)^2 doubles per box in
  * the box index's order: the values an operator hands to the host store at the "send"
