@@ -151,6 +151,21 @@ def main():
             times.append(time.perf_counter() - s)
         kinds = op.kind_times()
         op.kind_timing(False)
+        # the same steps with reset + evaluate replayed as one hipGraph launch
+        def step_graph():
+            op.load_charges(q_dev)
+            op.evaluate_graph()
+            op.read_output_device(phi_dev, grad_dev)
+
+        step_graph()
+        torch.cuda.synchronize()
+        gtimes = []
+        for _ in range(args.steps):
+            torch.cuda.synchronize()
+            s = time.perf_counter()
+            step_graph()
+            torch.cuda.synchronize()
+            gtimes.append(time.perf_counter() - s)
         phi = phi_dev.cpu().numpy()
         grad = grad_dev.cpu().numpy().reshape(-1, 3)
         e_phi = O.error_norms(phi[sample], exact_phi)[0]
@@ -160,8 +175,10 @@ def main():
             f"err phi {e_phi:.3e} grad {e_grad:.3e}")
         log(f"[{precision}] kinds (ms, launches): " +
             ", ".join(f"{k} {v[0] / max(v[1], 1):.3f}x{v[1]}" for k, v in kinds.items() if v[1]))
+        gmed = float(np.median(gtimes))
+        log(f"[{precision}] hipGraph replay: median {gmed * 1e3:.2f} ms ({args.n / gmed / 1e6:.1f} Mpts/s)")
         res = dict(ms=med * 1e3, times=times, mpts=args.n / med / 1e6, err_phi=e_phi, err_grad=e_grad,
-                   kinds=kinds, stats=stats)
+                   kinds=kinds, stats=stats, graph_ms=gmed * 1e3)
         del op
         return res
 
@@ -242,6 +259,7 @@ def main():
                    "charges": "normal", "gradients": True, "near_rule": near, "sampled_targets": len(sample),
                    "leaves": int(plan.nleaves), "levels": int(plan.nlevels), "leaf_levels": hist},
         "errors": {"mixed": {"phi": mixed["err_phi"], "grad": mixed["err_grad"]}},
+        "graph_ms_per_step": mixed["graph_ms"],
         "kind_ms": {k: round(v, 4) for k, v in per_step.items()},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -57,6 +57,25 @@ class Plan:
         _lib.check(_lib.lib().fmmh_plan_build(_lib.ptr(src), len(src), _lib.ptr(tgt), len(tgt), C.byref(o), C.byref(h)))
         self._h = h
 
+    @classmethod
+    def forest(cls, clouds, leaf_points=64, max_level=16, near_rule=None):
+        """Independent clouds (a list of (n_c, 3) arrays; sources = targets in each) as one
+        plan: each cloud's own tree, every cloud evaluated by the same launches."""
+        arrays = [np.ascontiguousarray(c, dtype=np.float64).reshape(-1, 3) for c in clouds]
+        pts = np.ascontiguousarray(np.vstack(arrays))
+        counts = np.array([len(a) for a in arrays], dtype=np.int64)
+        o = _lib.PlanOptions(int(leaf_points), int(max_level), -1.0, -1.0)
+        if near_rule is not None:
+            o.near_w, o.near_x = float(near_rule[0]), float(near_rule[1])
+        self = cls.__new__(cls)
+        self.sources = self.targets = pts
+        self.counts = counts
+        h = C.c_void_p()
+        _lib.check(_lib.lib().fmmh_forest_build(_lib.ptr(pts), _lib.ptr(counts), len(counts), C.byref(o),
+                                                C.byref(h)))
+        self._h = h
+        return self
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
@@ -174,6 +193,10 @@ class Operator:
 
     def evaluate(self):
         _lib.check(_lib.lib().fmmh_evaluate(self._h))
+
+    def evaluate_graph(self):
+        """reset() + evaluate() as one replayed hipGraph launch."""
+        _lib.check(_lib.lib().fmmh_evaluate_graph(self._h))
 
     def read_output(self) -> Output:
         phi = np.zeros(self.ntargets)
@@ -317,5 +340,12 @@ class FmmBuilder:
 
     def build(self, sources, targets=None) -> Fmm:
         plan = self.build_plan(sources, targets)
+        op = Operator(plan, self._p, self._precision, self._gradients, self._device)
+        return Fmm(plan, op)
+
+    def build_batched(self, clouds) -> Fmm:
+        """One Fmm over many independent clouds (sources = targets in each); charges and
+        outputs are the clouds' points in order."""
+        plan = Plan.forest(clouds, self._leaf_points, self._max_level, self._near)
         op = Operator(plan, self._p, self._precision, self._gradients, self._device)
         return Fmm(plan, op)

@@ -35,7 +35,8 @@ EXPORTS = [
     "fmmh_device_count", "fmmh_operator_create", "fmmh_operator_free", "fmmh_operator_stats",
     "fmmh_load_charges", "fmmh_reset", "fmmh_p2m", "fmmh_m2m", "fmmh_m2l", "fmmh_p2l", "fmmh_l2l",
     "fmmh_l2p", "fmmh_m2p", "fmmh_p2p", "fmmh_evaluate", "fmmh_read_output", "fmmh_sync",
-    "fmmh_kind_timing", "fmmh_kind_times", "fmmh_read_expansions", "fmmh_vjp",
+    "fmmh_kind_timing", "fmmh_kind_times", "fmmh_read_expansions", "fmmh_vjp", "fmmh_forest_build",
+    "fmmh_evaluate_graph",
 ]
 
 
@@ -118,12 +119,14 @@ def lib():
     for name in ("p2m", "m2m", "m2l", "p2l", "l2l", "l2p", "m2p", "p2p"):
         getattr(L, "fmmh_" + name).argtypes = [vp, i32]
     L.fmmh_evaluate.argtypes = [vp]
+    L.fmmh_evaluate_graph.argtypes = [vp]
     L.fmmh_read_output.argtypes = [vp, vp, vp, i32]
     L.fmmh_sync.argtypes = [vp]
     L.fmmh_kind_timing.argtypes = [vp, i32]
     L.fmmh_read_expansions.argtypes = [vp, i32, i32, vp]
     L.fmmh_kind_times.argtypes = [vp, vp, vp]
     L.fmmh_vjp.argtypes = [vp, vp, vp, vp, i32, vp, vp]
+    L.fmmh_forest_build.argtypes = [vp, vp, i32, C.POINTER(PlanOptions), C.POINTER(vp)]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is C.c_int:  # default

@@ -72,6 +72,25 @@ int32_t fmmh_plan_build(const double *sources, int64_t nsources, const double *t
   });
 }
 
+int32_t fmmh_forest_build(const double *points, const int64_t *counts, int32_t nclouds,
+                          const fmmh_plan_options *options, fmmh_plan **out) {
+  return guard([&] {
+    need(out != nullptr, "fmmh_forest_build: out is NULL");
+    need(points != nullptr && counts != nullptr && nclouds >= 1, "fmmh_forest_build: no clouds");
+    fmmh::PlanOptions o;
+    if (options) {
+      o.leaf_points = options->leaf_points;
+      o.max_level = options->max_level;
+      o.near_w = options->near_w;
+      o.near_x = options->near_x;
+    }
+    need(o.max_level >= 0 && o.max_level <= fmmh::kDeepest, "fmmh_forest_build: max_level must lie in 0..16");
+    auto p = std::make_unique<fmmh_plan>();
+    p->plan = fmmh::build_forest(points, counts, nclouds, o);
+    *out = p.release();
+  });
+}
+
 void fmmh_plan_free(fmmh_plan *plan) { delete plan; }
 
 int32_t fmmh_plan_nlevels(const fmmh_plan *plan) { return plan ? plan->plan.nlevels : -1; }
@@ -231,6 +250,13 @@ int32_t fmmh_evaluate(fmmh_operator *op) {
   return guard([&] {
     need(op != nullptr, "fmmh_evaluate: NULL operator");
     op->op->evaluate();
+  });
+}
+
+int32_t fmmh_evaluate_graph(fmmh_operator *op) {
+  return guard([&] {
+    need(op != nullptr, "fmmh_evaluate_graph: NULL operator");
+    op->op->evaluate_graph();
   });
 }
 

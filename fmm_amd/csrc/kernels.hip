@@ -92,14 +92,15 @@ __global__ void k_gather_charges(const double *q, const uint32_t *order, size_t 
 }
 
 // Output pass (fmm-exec/src/fmm.rs:117-145; CONVENTIONS §3.13 "Output"): per leaf,
-// phi = phi^ / (4 pi r_t) and grad = g^ / (4 pi r_t^2) in f64, into caller order.
-__global__ void k_output(uint32_t nleaves, const uint32_t *tgt_start, const uint32_t *leaf_cell,
-                         const uint32_t *order, double side, const double *phi_hat, const double *grad_hat,
+// phi = phi^ / (4 pi r_t) and grad = g^ / (4 pi r_t^2) in f64, into caller order; r_t is
+// the leaf's radius (its own cloud's in a forest).
+__global__ void k_output(uint32_t nleaves, const uint32_t *tgt_start, const double *leaf_r,
+                         const uint32_t *order, const double *phi_hat, const double *grad_hat,
                          double *phi, double *grad, int gradients) {
   const uint32_t lf = xcd_index(nleaves);
   if (lf >= nleaves) return;
   const uint32_t t0 = tgt_start[lf], t1 = tgt_start[lf + 1];
-  const double rt = ldexp(side, -int(leaf_cell[4 * lf]) - 1);
+  const double rt = leaf_r[lf];
   const double s1 = 4.0 * 3.141592653589793 * rt, s2 = s1 * rt;
   for (uint32_t i = t0 + threadIdx.x; i < t1; i += blockDim.x) {
     const uint32_t c = order[i];
@@ -707,12 +708,12 @@ void gather_charges(hipStream_t s, const double *q, const uint32_t *order, size_
   hipLaunchKernelGGL(k_gather_charges, dim3(grid_for(n, 256)), dim3(256), 0, s, q, order, n, q64, q32);
 }
 
-void output_pass(hipStream_t s, uint32_t nleaves, const uint32_t *tgt_start, const uint32_t *leaf_cell,
-                 const uint32_t *order, double side, const double *phi_hat, const double *grad_hat,
-                 double *phi, double *grad, int gradients) {
+void output_pass(hipStream_t s, uint32_t nleaves, const uint32_t *tgt_start, const double *leaf_r,
+                 const uint32_t *order, const double *phi_hat, const double *grad_hat, double *phi, double *grad,
+                 int gradients) {
   if (nleaves == 0) return;
-  hipLaunchKernelGGL(k_output, dim3(xcd_grid(nleaves)), dim3(128), 0, s, nleaves, tgt_start, leaf_cell, order, side,
-                     phi_hat, grad_hat, phi, grad, gradients);
+  hipLaunchKernelGGL(k_output, dim3(xcd_grid(nleaves)), dim3(128), 0, s, nleaves, tgt_start, leaf_r, order, phi_hat,
+                     grad_hat, phi, grad, gradients);
 }
 
 template <typename T>

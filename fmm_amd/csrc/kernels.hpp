@@ -52,9 +52,9 @@ void gather_charges(hipStream_t s, const double *q, const uint32_t *order, size_
 // grad_q = psi, grad_pos = q dpsi + g field (n points, caller order; grad_pos may be null)
 void vjp_combine(hipStream_t s, size_t n, const double *q, const double *g, const double *psi, const double *dpsi,
                  const double *field, double *grad_q, double *grad_pos);
-void output_pass(hipStream_t s, uint32_t nleaves, const uint32_t *tgt_start, const uint32_t *leaf_cell,
-                 const uint32_t *order, double side, const double *phi_hat, const double *grad_hat,
-                 double *phi, double *grad, int gradients);
+void output_pass(hipStream_t s, uint32_t nleaves, const uint32_t *tgt_start, const double *leaf_r,
+                 const uint32_t *order, const double *phi_hat, const double *grad_hat, double *phi, double *grad,
+                 int gradients);
 
 // P2P over the leaves leaf_first .. leaf_first + count (global leaf indices), computed in
 // T (double, or float with double-float separations); f64 coordinates and charges.

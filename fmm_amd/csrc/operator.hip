@@ -104,7 +104,6 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
 
   nlevels_ = plan.nlevels;
   nc_ = ncoef(opt_.p);
-  side_ = plan.side;
   ns_ = plan.ns;
   nt_ = plan.nt;
   same_points_ = plan.same_points;
@@ -135,6 +134,7 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
     }
   leaf_cell_.upload(leaf_cell);
   box_cell_.upload(box_cell);
+  leaf_r_.upload(plan.leaf_radius);
   src_start_.upload(plan.src_start);
   tgt_start_.upload(plan.tgt_start);
   src_order_.upload(plan.src_order);
@@ -285,6 +285,8 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
 }
 
 DeviceOperator::~DeviceOperator() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
   for (hipEvent_t e : events_) (void)hipEventDestroy(e);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -652,8 +654,22 @@ void DeviceOperator::evaluate() {
   timed(7, [&] { launch_p2p(0, nleaves_); });
 }
 
+void DeviceOperator::evaluate_graph() {
+  require(!timing_, "evaluate_graph: kind timing must be off (events are not captured)", kUnsupportedLayout);
+  if (!graph_exec_) {
+    // every buffer is fixed at build and every clear is a kernel, so the captured
+    // launches stay valid for the operator's lifetime
+    hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    reset();
+    evaluate();
+    hip_check(hipStreamEndCapture(stream_, &graph_), "hipStreamEndCapture");
+    hip_check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
+  }
+  hip_check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
+}
+
 void DeviceOperator::read_output(double *phi, double *grad, bool on_device) {
-  dev::output_pass(stream_, nleaves_, tgt_start_.ptr, leaf_cell_.ptr, tgt_order_.ptr, side_, phi_hat_.ptr,
+  dev::output_pass(stream_, nleaves_, tgt_start_.ptr, leaf_r_.ptr, tgt_order_.ptr, phi_hat_.ptr,
                    grad_hat_.ptr, on_device ? phi : phi_.ptr, on_device ? grad : grad_.ptr,
                    opt_.gradients && grad != nullptr);
   if (!on_device) {
@@ -691,7 +707,7 @@ void DeviceOperator::vjp(const double *q, const double *g, const double *field, 
   load_charges(dg, true);
   reset();
   evaluate();
-  dev::output_pass(stream_, nleaves_, tgt_start_.ptr, leaf_cell_.ptr, tgt_order_.ptr, side_, phi_hat_.ptr,
+  dev::output_pass(stream_, nleaves_, tgt_start_.ptr, leaf_r_.ptr, tgt_order_.ptr, phi_hat_.ptr,
                    grad_hat_.ptr, phi_.ptr, grad_.ptr, 1);
   dev::vjp_combine(stream_, n, dq, dg, phi_.ptr, grad_.ptr, df, oq, op);
   if (!on_device) {

@@ -112,6 +112,9 @@ class DeviceOperator {
   void p2p(int level);
   // the evaluator's fixed order (fmm-plan/src/operator.rs:408-425), all levels
   void evaluate();
+  // reset() + evaluate() as one hipGraph launch: captured on the first call (timing must
+  // be off), replayed afterwards; the charges are loaded and the output read as usual.
+  void evaluate_graph();
   // output pass + download in caller order (phi[nt], grad[3 nt]); the one sync
   void read_output(double *phi, double *grad, bool on_device);
   // Vector-Jacobian product of the potentials (sources = targets): given the charges q,
@@ -155,7 +158,6 @@ class DeviceOperator {
   OperatorOptions opt_;
   int nlevels_ = 0, nc_ = 0;
   hipStream_t stream_ = nullptr;
-  double side_ = 0.0;
   int64_t ns_ = 0, nt_ = 0;
   uint32_t nleaves_ = 0;
   size_t nbox_total_ = 0;
@@ -165,6 +167,9 @@ class DeviceOperator {
   DevBuf<uint32_t> leaf_cell_, box_cell_, src_start_, tgt_start_, src_order_, tgt_order_;
   DevBuf<uint32_t> near_off_, near_ent_;
   DevBuf<double> sx_, sy_, sz_, tx_, ty_, tz_, q_, q_in_;
+  DevBuf<double> leaf_r_;  // leaf radius in domain units (output scaling)
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
   DevBuf<double> mult_, loc_, phi_hat_, grad_hat_, phi_, grad_;
   DevBuf<double> vjp_in_, vjp_out_;  // host-array staging of vjp (5 n and 4 n doubles)
   bool same_points_ = false;

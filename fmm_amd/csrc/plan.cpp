@@ -561,6 +561,9 @@ Plan build_plan(const double *sources, int64_t ns, const double *targets, int64_
   assign_points(sources, ns, plan, morton, morton_to_leaf, plan.src_start, plan.src_order, plan.us);
   assign_points(targets, nt, plan, morton, morton_to_leaf, plan.tgt_start, plan.tgt_order, plan.ut);
   apply_near_rule(plan);
+  plan.leaf_radius.resize(plan.nleaves());
+  for (int l = 0; l < L; ++l)
+    for (uint32_t i = plan.leaf_off[l]; i < plan.leaf_off[l + 1]; ++i) plan.leaf_radius[i] = plan.radius(l);
   return plan;
 }
 

@@ -72,6 +72,9 @@ struct Plan {
   std::vector<uint32_t> src_start, tgt_start, src_order, tgt_order;
   std::vector<double> us, ut;
 
+  // Radius of each leaf in domain units (side / 2^(level + 1); per cloud in a forest).
+  std::vector<double> leaf_radius;
+
   uint32_t nleaves() const { return static_cast<uint32_t>(leaf_key.size()); }
   uint32_t nboxes(int level) const { return static_cast<uint32_t>(keys[level].size()); }
   double radius(int level) const;  // r_l = w / 2^(l+1)
@@ -80,6 +83,11 @@ struct Plan {
 // Builds the plan of sources and targets (3 doubles per point).
 Plan build_plan(const double *sources, int64_t ns, const double *targets, int64_t nt,
                 const PlanOptions &options);
+
+// A forest: independent clouds (each its own domain and tree, sources = targets) as one
+// plan whose levels hold every cloud's boxes and leaves, cloud after cloud; lists never
+// cross clouds and caller indices run over the clouds in order (counts[c] points each).
+Plan build_forest(const double *points, const int64_t *counts, int nclouds, const PlanOptions &options);
 
 // Tables (CONVENTIONS §3.12): column-major (p+1)^2 x (p+1)^2 matrices, entry (i, k) at
 // i + k n; M2M and L2L by child octant (8 each), M2L by offset index (316).

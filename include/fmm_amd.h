@@ -99,6 +99,15 @@ int32_t fmmh_abi_version(void);
 /* sources/targets: 3 doubles per point (x, y, z); targets may equal sources. */
 int32_t fmmh_plan_build(const double *sources, int64_t nsources, const double *targets, int64_t ntargets,
                         const fmmh_plan_options *options, fmmh_plan **out);
+/* Batched independent clouds as one plan (a forest): nclouds clouds of counts[c] points,
+ * concatenated in points (3 doubles each), sources = targets within each cloud.  Each
+ * cloud is refined and listed as if alone (its own domain); one operator on the forest
+ * evaluates every cloud in the same launches; outputs and charges follow the points'
+ * order.  fmmh_plan_domain reports the first cloud's domain.  No counterpart in the
+ * reference (its batched-clouds configuration is a north-star addition); it stands where
+ * FmmBuilder::build (fmm-exec/src/fmm.rs:1853-1935) would be called once per cloud. */
+int32_t fmmh_forest_build(const double *points, const int64_t *counts, int32_t nclouds,
+                          const fmmh_plan_options *options, fmmh_plan **out);
 void fmmh_plan_free(fmmh_plan *plan);
 int32_t fmmh_plan_nlevels(const fmmh_plan *plan);
 int64_t fmmh_plan_nboxes(const fmmh_plan *plan, int32_t level);
@@ -140,6 +149,11 @@ int32_t fmmh_m2p(fmmh_operator *op, int32_t level);
 int32_t fmmh_p2p(fmmh_operator *op, int32_t level);
 /* Every level call in the evaluator's order (fmm-plan/src/evaluator.rs:873-881). */
 int32_t fmmh_evaluate(fmmh_operator *op);
+/* fmmh_reset + fmmh_evaluate as one hipGraph launch, captured on the first call and
+ * replayed after (kind timing must be off).  No counterpart in the reference: it replaces
+ * the per-launch host work of an evaluation (DeviceOperator's launches,
+ * fmm-exec/src/device.rs:5212-5446) by one graph launch. */
+int32_t fmmh_evaluate_graph(fmmh_operator *op);
 /* Output pass and download: phi[nt], grad[3 nt] (may be NULL) in caller order, to the
  * host (on_device = 0) or to device buffers.  Synchronises. */
 int32_t fmmh_read_output(fmmh_operator *op, double *phi, double *grad, int32_t on_device);

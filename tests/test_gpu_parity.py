@@ -180,3 +180,24 @@ def test_published_error_reproduced_on_gpu():
     e_grad = _rel(out.gradient[sample], egrad / (4 * np.pi), True)
     assert abs(e_phi - row["phi"]) <= 0.005 * row["phi"], e_phi
     assert abs(e_grad - row["grad"]) <= 0.005 * row["grad"], e_grad
+
+
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+def test_graph_replays_equal_eager(precision):
+    # reset + evaluate captured once in a hipGraph and replayed, with eager evaluations
+    # and synchronisations in between: every replay gives the eager bits
+    _log(f"case graph replays {precision}")
+    pts = _points("mixed", 9000, 61)
+    q = O.Rng(62).normals(len(pts))
+    fmm = fmm_amd.FmmBuilder(8).precision(precision).max_points_per_leaf(32).build(pts)
+    eager = fmm.evaluate(q)
+    op = fmm.operator
+    q2 = O.Rng(63).normals(len(pts))
+    for rep in range(3):
+        op.load_charges(q)
+        op.evaluate_graph()
+        out = op.read_output()
+        assert np.array_equal(out.potential, eager.potential), rep
+        assert np.array_equal(out.gradient, eager.gradient), rep
+        other = fmm.evaluate(q2)  # eager work between replays
+        assert np.all(np.isfinite(other.potential))
