@@ -75,16 +75,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy 256 MiB device to device (a known-size kernel for counter scaling)")
+    ap.add_argument("--workload", default="cube", choices=("cube", "batched"),
+                    help="cube: 4M uniform points (the headline); batched: 256 independent Gaussian clouds, "
+                         "p = 6, one forest evaluation replayed as a hipGraph, plus its adjoint")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each box has one GPU)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched":
+        return batched(args)
 
     import torch
 
     import fmm_amd
-    from oracle import oracle as O
+    from fmm_amd import points as P
 
     dev = torch.device("cuda:0")
     if args.copy_probe:
@@ -95,10 +100,7 @@ def main():
         log("copy probe: 268435456 bytes read, 268435456 bytes written")
         del a, b
     t0 = time.time()
-    rng = O.Rng(O.BENCH_SEED)
-    pts = rng.cube(args.n)
-    q = rng.normals(args.n)
-    sample = rng.sample(args.n, args.sampled)
+    pts, q, sample = P.bench_cube(args.n, args.sampled)
     log(f"problem: N={args.n} drawn in {time.time() - t0:.1f}s")
 
     if args.near == "off":
@@ -119,10 +121,14 @@ def main():
     grad_dev = torch.zeros(3 * args.n, dtype=torch.float64, device=dev)
 
     t0 = time.time()
-    exact_phi, exact_grad = O.direct_sum(pts, q, pts[sample])
+    exact_phi, exact_grad = fmm_amd.direct_sum(pts, q, pts[sample])  # compensated f64, on the GPU
     exact_phi /= 4 * np.pi
     exact_grad /= 4 * np.pi
     log(f"direct sum at {len(sample)} targets in {time.time() - t0:.1f}s")
+
+    def error_norm(a, b):
+        """Relative L2 error (fmm-validate/src/metrics.rs:51-170)."""
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
     def run(precision):
         t0 = time.time()
@@ -168,8 +174,8 @@ def main():
             gtimes.append(time.perf_counter() - s)
         phi = phi_dev.cpu().numpy()
         grad = grad_dev.cpu().numpy().reshape(-1, 3)
-        e_phi = O.error_norms(phi[sample], exact_phi)[0]
-        e_grad = O.error_norms(grad[sample], exact_grad, True)[0]
+        e_phi = error_norm(phi[sample], exact_phi)
+        e_grad = error_norm(grad[sample], exact_grad)
         med = float(np.median(times))
         log(f"[{precision}] median {med * 1e3:.2f} ms ({args.n / med / 1e6:.1f} Mpts/s), "
             f"err phi {e_phi:.3e} grad {e_grad:.3e}")
@@ -229,6 +235,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        from oracle import oracle as O  # the CPU baseline only
+
         m = min(args.cpu_sample, args.n)
         t0 = time.time()
         f = O.Fmm(pts[:m], p=args.p, leaf_points=args.leaf, near_rule=near)
@@ -272,6 +280,84 @@ def main():
         line["errors"]["plain_f64"] = {"phi": plain["err_phi"], "grad": plain["err_grad"]}
     line["p2p_pairs"] = stats["p2p_pairs"]
     line["m2l_pairs"] = stats["m2l_pairs"]
+    print(json.dumps(line), flush=True)
+
+
+def batched(args):
+    """256 independent clouds (sizes log-uniform in [2k, 256k], one Gaussian blob each,
+    charges N(0,1)), p = 6, f32 P2P + bf16x3 M2L, <= 64 points per leaf: one forest
+    operator, reset + evaluate replayed as one hipGraph launch; then the adjoint of
+    L = sum(phi^2)/N (one more evaluation and the combine kernel)."""
+    import torch
+
+    import fmm_amd
+    from fmm_amd import points as P
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    clouds, charges = P.batched_clouds()
+    n = int(sum(len(c) for c in clouds))
+    log(f"batched: {len(clouds)} clouds, {n} points, drawn in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    fmm = fmm_amd.FmmBuilder(6).precision("mixed").max_points_per_leaf(64).build_batched(clouds)
+    log(f"forest plan + operator in {time.time() - t0:.1f}s: {fmm.operator.stats()}")
+    op = fmm.operator
+    q_dev = torch.from_numpy(np.concatenate(charges)).to(dev)
+    phi = torch.zeros(n, dtype=torch.float64, device=dev)
+    field = torch.zeros(n, 3, dtype=torch.float64, device=dev)
+
+    def step():
+        op.load_charges(q_dev)
+        op.evaluate_graph()
+        op.read_output_device(phi, field)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        s = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - s)
+    med = float(np.median(times))
+    # adjoint of L = sum(phi^2) / N on the device
+    g = (2.0 / n) * phi
+    gq = torch.empty_like(phi)
+    gz = torch.empty_like(field)
+    vt = []
+    for _ in range(max(2, min(args.steps, 5))):
+        torch.cuda.synchronize()
+        s = time.perf_counter()
+        op.vjp_device(q_dev, g, field, gq, gz)
+        torch.cuda.synchronize()
+        vt.append(time.perf_counter() - s)
+    step()
+    # errors: 32 sampled targets per cloud against the compensated f64 direct sum
+    ph = phi.cpu().numpy()
+    gr = field.cpu().numpy()
+    first = np.concatenate([[0], np.cumsum([len(c) for c in clouds])])
+    num = den = numg = deng = 0.0
+    for c, (pts, q) in enumerate(zip(clouds, charges)):
+        idx = P.SplitMix64(c + 1).sample(len(pts), 32)
+        e, eg = fmm_amd.direct_sum(pts, q, pts[idx])
+        e /= 4 * np.pi
+        eg /= 4 * np.pi
+        num += float(np.sum((ph[first[c] + idx] - e) ** 2))
+        den += float(np.sum(e ** 2))
+        numg += float(np.sum((gr[first[c] + idx] - eg) ** 2))
+        deng += float(np.sum(eg ** 2))
+    line = {
+        "metric": "batched Laplace FMM Mpts/s (256 clouds, one forest, hipGraph replay)",
+        "value": n / med / 1e6, "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": med * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 P2P (double-float separations) + bf16x3 M2L + f64 other", "data": "synthetic",
+        "config": {"workload": "batched-gaussian-clouds", "clouds": len(clouds), "n": n, "p": 6, "leaf_points": 64,
+                   "sizes": [int(len(c)) for c in clouds][:8] + ["..."]},
+        "errors": {"phi": (num / den) ** 0.5, "grad": (numg / deng) ** 0.5, "sampled_per_cloud": 32},
+        "adjoint_ms": float(np.median(vt)) * 1e3,
+    }
     print(json.dumps(line), flush=True)
 
 

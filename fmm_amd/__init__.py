@@ -248,6 +248,19 @@ class Operator:
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KINDS)}
 
 
+def direct_sum(sources, charges, targets, gradients=True, device=0):
+    """Compensated f64 direct sum on the GPU (unscaled: sum q / r, -sum q d / r^3), as
+    nd_fmm_ref::p2p::direct_sum (fmm-ref/src/p2p.rs:147-205)."""
+    src = np.ascontiguousarray(sources, dtype=np.float64).reshape(-1, 3)
+    q = np.ascontiguousarray(charges, dtype=np.float64)
+    tgt = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 3)
+    phi = np.zeros(len(tgt))
+    grad = np.zeros((len(tgt), 3)) if gradients else None
+    _lib.check(_lib.lib().fmmh_direct_sum(int(device), _lib.ptr(src), _lib.ptr(q), len(src), _lib.ptr(tgt), len(tgt),
+                                          _lib.ptr(phi), _lib.ptr(grad), 0))
+    return phi, grad
+
+
 class Fmm:
     """An FMM built for fixed sources and targets (fmm-exec's Fmm)."""
 

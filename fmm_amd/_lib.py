@@ -36,7 +36,7 @@ EXPORTS = [
     "fmmh_load_charges", "fmmh_reset", "fmmh_p2m", "fmmh_m2m", "fmmh_m2l", "fmmh_p2l", "fmmh_l2l",
     "fmmh_l2p", "fmmh_m2p", "fmmh_p2p", "fmmh_evaluate", "fmmh_read_output", "fmmh_sync",
     "fmmh_kind_timing", "fmmh_kind_times", "fmmh_read_expansions", "fmmh_vjp", "fmmh_forest_build",
-    "fmmh_evaluate_graph",
+    "fmmh_evaluate_graph", "fmmh_direct_sum",
 ]
 
 
@@ -120,6 +120,7 @@ def lib():
         getattr(L, "fmmh_" + name).argtypes = [vp, i32]
     L.fmmh_evaluate.argtypes = [vp]
     L.fmmh_evaluate_graph.argtypes = [vp]
+    L.fmmh_direct_sum.argtypes = [i32, vp, vp, i64, vp, i64, vp, vp, i32]
     L.fmmh_read_output.argtypes = [vp, vp, vp, i32]
     L.fmmh_sync.argtypes = [vp]
     L.fmmh_kind_timing.argtypes = [vp, i32]

@@ -276,6 +276,15 @@ int32_t fmmh_vjp(fmmh_operator *op, const double *charges, const double *cotange
   });
 }
 
+int32_t fmmh_direct_sum(int32_t device, const double *sources, const double *charges, int64_t nsources,
+                        const double *targets, int64_t ntargets, double *phi, double *grad, int32_t on_device) {
+  return guard([&] {
+    need(nsources == 0 || (sources && charges), "fmmh_direct_sum: NULL sources");
+    need(ntargets == 0 || (targets && phi), "fmmh_direct_sum: NULL targets or output");
+    fmmh::direct_sum(device, sources, charges, nsources, targets, ntargets, phi, grad, on_device != 0);
+  });
+}
+
 int32_t fmmh_sync(fmmh_operator *op) {
   return guard([&] {
     need(op != nullptr, "fmmh_sync: NULL operator");

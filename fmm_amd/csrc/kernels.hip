@@ -113,6 +113,87 @@ __global__ void k_output(uint32_t nleaves, const uint32_t *tgt_start, const doub
   }
 }
 
+// ---- direct sum (fmm-ref/src/p2p.rs:147-205, term :370-403, Neumaier :407-431) --------
+// Compensated f64 sums per target over all sources, unscaled (sum q / r and its
+// gradient -sum q d / r^3, d = x - y); a source equal to the target is skipped.  Thread
+// = target; blockIdx.y = a chunk of sources whose compensated partials (sum, comp for phi
+// and the three gradient components) go to part[chunk][target]; a second kernel adds the
+// chunks in order.
+
+struct Neumaier {
+  double s = 0.0, c = 0.0;
+  __device__ void add(double t) {
+    const double u = s + t;
+    c += fabs(s) >= fabs(t) ? (s - u) + t : (t - u) + s;
+    s = u;
+  }
+};
+
+__global__ __launch_bounds__(256) void k_direct_partial(const double *src, const double *q, int64_t ns,
+                                                        const double *tgt, int64_t nt, int64_t chunk,
+                                                        int gradients, double *part) {
+  __shared__ double ys[256][4];
+  const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t s0 = int64_t(blockIdx.y) * chunk, s1 = s0 + chunk < ns ? s0 + chunk : ns;
+  double x[3] = {0.0, 0.0, 0.0};
+  if (t < nt)
+    for (int k = 0; k < 3; ++k) x[k] = tgt[3 * t + k];
+  Neumaier p, g[3];
+  for (int64_t sb = s0; sb < s1; sb += 256) {
+    const int cnt = int((s1 - sb) < 256 ? (s1 - sb) : 256);
+    __syncthreads();
+    if (int(threadIdx.x) < cnt) {
+      const int64_t s = sb + threadIdx.x;
+      ys[threadIdx.x][0] = src[3 * s];
+      ys[threadIdx.x][1] = src[3 * s + 1];
+      ys[threadIdx.x][2] = src[3 * s + 2];
+      ys[threadIdx.x][3] = q[s];
+    }
+    __syncthreads();
+    if (t < nt)
+      for (int j = 0; j < cnt; ++j) {
+        const double y0 = ys[j][0], y1 = ys[j][1], y2 = ys[j][2], qj = ys[j][3];
+        if (x[0] == y0 && x[1] == y1 && x[2] == y2) continue;
+        const double d0 = x[0] - y0, d1 = x[1] - y1, d2 = x[2] - y2;
+        const double r2 = (d0 * d0 + d1 * d1) + d2 * d2;
+        const double r = sqrt(r2);
+        p.add(qj / r);
+        if (gradients) {
+          const double r3 = r2 * r;
+          g[0].add(-(qj * d0 / r3));
+          g[1].add(-(qj * d1 / r3));
+          g[2].add(-(qj * d2 / r3));
+        }
+      }
+  }
+  if (t < nt) {
+    double *o = part + (size_t(blockIdx.y) * nt + t) * 8;
+    o[0] = p.s;
+    o[1] = p.c;
+    for (int k = 0; k < 3; ++k) {
+      o[2 + 2 * k] = g[k].s;
+      o[3 + 2 * k] = g[k].c;
+    }
+  }
+}
+
+__global__ void k_direct_combine(int64_t nt, int nchunks, int gradients, const double *part, double *phi,
+                                 double *grad) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  Neumaier acc[4];
+  for (int c = 0; c < nchunks; ++c) {
+    const double *o = part + (size_t(c) * nt + t) * 8;
+    for (int k = 0; k < 4; ++k) {
+      acc[k].add(o[2 * k]);
+      acc[k].add(o[2 * k + 1]);
+    }
+  }
+  phi[t] = acc[0].s + acc[0].c;
+  if (gradients)
+    for (int k = 0; k < 3; ++k) grad[3 * t + k] = acc[1 + k].s + acc[1 + k].c;
+}
+
 // Adjoint of a potential-based loss (sources = targets, caller order): with psi and
 // grad psi the field of the charges g = dL/dphi, dL/dq_j = psi_j and
 // dL/dz_j = q_j grad psi_j + g_j grad phi_j (the second term from moving z_j as a target).
@@ -694,6 +775,26 @@ static inline unsigned grid_for(size_t n, int threads) {
 void fill(hipStream_t s, double *p, size_t n, double v) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
+}
+
+int direct_chunks(int64_t ns, int64_t nt) {
+  const int64_t tblocks = (nt + 255) / 256;
+  int64_t nch = (1024 + tblocks - 1) / tblocks;
+  const int64_t most = (ns + 1023) / 1024;  // at least 1024 sources per chunk
+  if (nch > most) nch = most;
+  if (nch < 1) nch = 1;
+  if (nch > 65535) nch = 65535;
+  return int(nch);
+}
+
+void direct_sum(hipStream_t s, const double *src, const double *q, int64_t ns, const double *tgt, int64_t nt,
+                int gradients, double *part, int nchunks, double *phi, double *grad) {
+  if (nt == 0) return;
+  const int64_t chunk = (ns + nchunks - 1) / nchunks;
+  hipLaunchKernelGGL(k_direct_partial, dim3(unsigned((nt + 255) / 256), unsigned(nchunks)), dim3(256), 0, s, src, q,
+                     ns, tgt, nt, chunk, gradients, part);
+  hipLaunchKernelGGL(k_direct_combine, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, s, nt, nchunks, gradients,
+                     part, phi, grad);
 }
 
 void vjp_combine(hipStream_t s, size_t n, const double *q, const double *g, const double *psi, const double *dpsi,

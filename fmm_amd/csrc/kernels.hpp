@@ -49,6 +49,11 @@ hipError_t upload_constants();
 
 void fill(hipStream_t s, double *p, size_t n, double v);
 void gather_charges(hipStream_t s, const double *q, const uint32_t *order, size_t n, double *q64, float *q32);
+// Compensated f64 direct sum (unscaled): chunks of sources per block, partials in
+// part[nchunks][nt][8], then added per target in chunk order.
+int direct_chunks(int64_t ns, int64_t nt);
+void direct_sum(hipStream_t s, const double *src, const double *q, int64_t ns, const double *tgt, int64_t nt,
+                int gradients, double *part, int nchunks, double *phi, double *grad);
 // grad_q = psi, grad_pos = q dpsi + g field (n points, caller order; grad_pos may be null)
 void vjp_combine(hipStream_t s, size_t n, const double *q, const double *g, const double *psi, const double *dpsi,
                  const double *field, double *grad_q, double *grad_pos);

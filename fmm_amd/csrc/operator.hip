@@ -733,4 +733,38 @@ void DeviceOperator::sync() {
   if (!marks_.empty()) collect_timings();
 }
 
+void direct_sum(int device, const double *sources, const double *charges, int64_t ns, const double *targets,
+                int64_t nt, double *phi, double *grad, bool on_device) {
+  require(ns >= 0 && nt >= 0, "direct_sum: negative size");
+  if (nt == 0) return;
+  hip_check(hipSetDevice(device), "hipSetDevice");
+  hipStream_t s = nullptr;
+  hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+  const int nch = dev::direct_chunks(ns, nt);
+  DevBuf<double> part, in, out;
+  part.alloc(size_t(nch) * nt * 8);
+  const double *src = sources, *q = charges, *tgt = targets;
+  double *ph = phi, *gr = grad;
+  if (!on_device) {
+    in.alloc(4 * size_t(ns) + 3 * size_t(nt));
+    out.alloc(4 * size_t(nt));
+    hip_check(hipMemcpyAsync(in.ptr, sources, 3 * ns * sizeof(double), hipMemcpyHostToDevice, s), "sources");
+    hip_check(hipMemcpyAsync(in.ptr + 3 * ns, charges, ns * sizeof(double), hipMemcpyHostToDevice, s), "charges");
+    hip_check(hipMemcpyAsync(in.ptr + 4 * ns, targets, 3 * nt * sizeof(double), hipMemcpyHostToDevice, s), "targets");
+    src = in.ptr;
+    q = in.ptr + 3 * ns;
+    tgt = in.ptr + 4 * ns;
+    ph = out.ptr;
+    gr = out.ptr + nt;
+  }
+  dev::direct_sum(s, src, q, ns, tgt, nt, grad != nullptr, part.ptr, nch, ph, grad ? gr : nullptr);
+  if (!on_device) {
+    hip_check(hipMemcpyAsync(phi, ph, nt * sizeof(double), hipMemcpyDeviceToHost, s), "phi");
+    if (grad) hip_check(hipMemcpyAsync(grad, gr, 3 * nt * sizeof(double), hipMemcpyDeviceToHost, s), "grad");
+  }
+  const hipError_t e = hipStreamSynchronize(s);
+  (void)hipStreamDestroy(s);
+  hip_check(e, "direct_sum");
+}
+
 }  // namespace fmmh

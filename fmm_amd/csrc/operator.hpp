@@ -202,4 +202,9 @@ class DeviceOperator {
   std::vector<uint32_t> m2l_red_level_;
 };
 
+// Compensated f64 direct sum on the device (unscaled sums q / r and their gradient):
+// inputs and outputs on the host (on_device = false) or device pointers; synchronous.
+void direct_sum(int device, const double *sources, const double *charges, int64_t ns, const double *targets,
+                int64_t nt, double *phi, double *grad, bool on_device);
+
 }  // namespace fmmh

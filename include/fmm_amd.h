@@ -169,6 +169,13 @@ int32_t fmmh_read_output(fmmh_operator *op, double *phi, double *grad, int32_t o
 int32_t fmmh_vjp(fmmh_operator *op, const double *charges, const double *cotangent, const double *field,
                  int32_t on_device, double *grad_charges, double *grad_points);
 int32_t fmmh_sync(fmmh_operator *op);
+/* The compensated f64 direct sum of nd_fmm_ref::p2p::direct_sum (fmm-ref/src/p2p.rs:147-205):
+ * per target phi = sum q / r and grad = -sum q d / r^3 (d = target - source, no 1/(4 pi)),
+ * Neumaier-compensated (:407-431), a source equal to the target skipped.  Points 3
+ * doubles each; grad may be NULL; host arrays (on_device = 0) or device pointers on
+ * `device`.  Synchronous; allocates its own scratch. */
+int32_t fmmh_direct_sum(int32_t device, const double *sources, const double *charges, int64_t nsources,
+                        const double *targets, int64_t ntargets, double *phi, double *grad, int32_t on_device);
 /* Multipoles (locals = 0) or locals of one level, box-major (p+1)^2 doubles per box in
  * the box index's order: the values an operator hands to the host store at the "send"
  * events of HostData (fmm-plan/src/operator.rs:344-404).  Synchronises. */
