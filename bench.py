@@ -75,6 +75,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy 256 MiB device to device (a known-size kernel for counter scaling)")
+    ap.add_argument("--m2l", default="dense", choices=("dense", "rotation"),
+                    help="M2L formulation of both legs (rotation: f32 in the mixed leg)")
     ap.add_argument("--workload", default="cube", choices=("cube", "batched"),
                     help="cube: 4M uniform points (the headline); batched: 256 independent Gaussian clouds, "
                          "p = 6, one forest evaluation replayed as a hipGraph, plus its adjoint")
@@ -132,7 +134,7 @@ def main():
 
     def run(precision):
         t0 = time.time()
-        op = fmm_amd.Operator(plan, args.p, precision, True, 0)
+        op = fmm_amd.Operator(plan, args.p, precision, True, 0, args.m2l)
         stats = op.stats()
         log(f"[{precision}] operator built in {time.time() - t0:.1f}s: {stats}")
 
@@ -264,6 +266,7 @@ def main():
         "dtype": "f32 P2P (double-float separations) + bf16x3 M2L (f32 MFMA accumulation) + f64 other",
         "data": "synthetic: SplitMix64(0x00be9c06) uniform cube points, N(0,1) charges",
         "config": {"workload": "laplace-fmm-uniform-cube", "n": args.n, "p": args.p, "leaf_points": args.leaf,
+                   "m2l": args.m2l,
                    "charges": "normal", "gradients": True, "near_rule": near, "sampled_targets": len(sample),
                    "leaves": int(plan.nleaves), "levels": int(plan.nlevels), "leaf_levels": hist},
         "errors": {"mixed": {"phi": mixed["err_phi"], "grad": mixed["err_grad"]}},

@@ -24,6 +24,7 @@ from ._lib import FmmError, build  # noqa: F401
 PLAIN_F64 = 0
 MIXED = 1
 PRECISIONS = {"f64": PLAIN_F64, "plain": PLAIN_F64, "mixed": MIXED}
+M2L_METHODS = {"dense": 0, "rotation": 1}
 KINDS = ("p2m", "m2m", "l2l", "m2l", "p2l", "l2p", "m2p", "p2p", "other", "m2l_contract")
 LISTS = {"near": 0, "v": 1, "w": 2, "x": 3}
 
@@ -153,12 +154,13 @@ class Operator:
     """The device operator: the level calls of FmmOperator (fmm-plan/src/operator.rs)
     on the GPU, plus the evaluator's whole order in one call."""
 
-    def __init__(self, plan: Plan, p: int, precision="f64", gradients=True, device=0):
+    def __init__(self, plan: Plan, p: int, precision="f64", gradients=True, device=0, m2l="dense"):
         self.plan = plan
         self.p = p
         self.gradients = gradients
         self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
-        o = _lib.OperatorOptions(int(p), self.precision, int(bool(gradients)), int(device))
+        self.m2l = M2L_METHODS[m2l] if isinstance(m2l, str) else int(m2l)
+        o = _lib.OperatorOptions(int(p), self.precision, int(bool(gradients)), int(device), self.m2l)
         h = C.c_void_p()
         _lib.check(_lib.lib().fmmh_operator_create(plan.handle, C.byref(o), C.byref(h)))
         self._h = h
@@ -311,6 +313,7 @@ class FmmBuilder:
         self._precision = "f64"
         self._device = 0
         self._near = None
+        self._m2l = "dense"
 
     def gradients(self, on: bool):
         self._gradients = bool(on)
@@ -348,17 +351,25 @@ class FmmBuilder:
             self._near = (float(rule[0]), float(rule[1]))
         return self
 
+    def m2l(self, method: str):
+        """The M2L formulation (fmm-exec's M2lMethod): "dense" (tables on MFMA) or "rotation"
+        (rotate, coaxial shift, rotate back; f32 in the mixed precision)."""
+        if method not in M2L_METHODS:
+            raise ValueError(f"unknown M2L method {method!r}")
+        self._m2l = method
+        return self
+
     def build_plan(self, sources, targets=None) -> Plan:
         return Plan(sources, targets, self._leaf_points, self._max_level, self._near)
 
     def build(self, sources, targets=None) -> Fmm:
         plan = self.build_plan(sources, targets)
-        op = Operator(plan, self._p, self._precision, self._gradients, self._device)
+        op = Operator(plan, self._p, self._precision, self._gradients, self._device, self._m2l)
         return Fmm(plan, op)
 
     def build_batched(self, clouds) -> Fmm:
         """One Fmm over many independent clouds (sources = targets in each); charges and
         outputs are the clouds' points in order."""
         plan = Plan.forest(clouds, self._leaf_points, self._max_level, self._near)
-        op = Operator(plan, self._p, self._precision, self._gradients, self._device)
+        op = Operator(plan, self._p, self._precision, self._gradients, self._device, self._m2l)
         return Fmm(plan, op)

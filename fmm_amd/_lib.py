@@ -36,7 +36,7 @@ EXPORTS = [
     "fmmh_load_charges", "fmmh_reset", "fmmh_p2m", "fmmh_m2m", "fmmh_m2l", "fmmh_p2l", "fmmh_l2l",
     "fmmh_l2p", "fmmh_m2p", "fmmh_p2p", "fmmh_evaluate", "fmmh_read_output", "fmmh_sync",
     "fmmh_kind_timing", "fmmh_kind_times", "fmmh_read_expansions", "fmmh_vjp", "fmmh_forest_build",
-    "fmmh_evaluate_graph", "fmmh_direct_sum",
+    "fmmh_evaluate_graph", "fmmh_direct_sum", "fmmh_m2l_rotation_table",
 ]
 
 
@@ -55,7 +55,8 @@ class PlanOptions(C.Structure):
 
 
 class OperatorOptions(C.Structure):
-    _fields_ = [("p", C.c_int32), ("precision", C.c_int32), ("gradients", C.c_int32), ("device", C.c_int32)]
+    _fields_ = [("p", C.c_int32), ("precision", C.c_int32), ("gradients", C.c_int32), ("device", C.c_int32),
+                ("m2l", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -109,6 +110,7 @@ def lib():
     L.fmmh_plan_point_leaves.argtypes = [vp, i32, vp]
     L.fmmh_plan_leaf_coordinates.argtypes = [vp, i32, vp, vp]
     L.fmmh_tables.argtypes = [i32, vp, vp, vp]
+    L.fmmh_m2l_rotation_table.argtypes = [i32, i32, vp]
     L.fmmh_device_count.argtypes = [C.POINTER(i32)]
     L.fmmh_operator_create.argtypes = [vp, C.POINTER(OperatorOptions), C.POINTER(vp)]
     L.fmmh_operator_free.argtypes = [vp]

@@ -1,4 +1,5 @@
 // capi.cpp -- the extern "C" boundary declared in include/fmm_amd.h.
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -50,7 +51,7 @@ extern "C" {
 
 const char *fmmh_last_error(void) { return g_last_error.c_str(); }
 
-int32_t fmmh_abi_version(void) { return 2; }
+int32_t fmmh_abi_version(void) { return 3; }
 
 int32_t fmmh_plan_build(const double *sources, int64_t nsources, const double *targets, int64_t ntargets,
                         const fmmh_plan_options *options, fmmh_plan **out) {
@@ -162,6 +163,53 @@ int32_t fmmh_plan_leaf_coordinates(const fmmh_plan *plan, int32_t targets, doubl
   });
 }
 
+int32_t fmmh_m2l_rotation_table(int32_t p, int32_t offset, double *table) {
+  return guard([&] {
+    need(table != nullptr && offset >= 0 && offset < 316, "fmmh_m2l_rotation_table: bad argument");
+    std::vector<double> fwd, bwd, coax, dist2;
+    std::vector<uint16_t> dist_of;
+    fmmh::build_rotation_tables(p, fwd, bwd, coax, dist_of, dist2);
+    const int nc = (p + 1) * (p + 1), RB = fmmh::rotation_block_size(p), CB = fmmh::coaxial_block_size(p);
+    // column c of G A F: F e_c, then A, then G
+    std::vector<double> x(nc), y(nc);
+    for (int c = 0; c < nc; ++c) {
+      std::fill(x.begin(), x.end(), 0.0);
+      x[c] = 1.0;
+      const int n0 = int(std::sqrt(double(c)));
+      // F: degree n0 block
+      std::fill(y.begin(), y.end(), 0.0);
+      size_t bo = 0;
+      for (int n = 0; n < n0; ++n) bo += size_t(2 * n + 1) * (2 * n + 1);
+      const int k = 2 * n0 + 1, cc = c - n0 * n0;
+      for (int r = 0; r < k; ++r) y[n0 * n0 + r] = fwd[size_t(offset) * RB + bo + r + size_t(cc) * k];
+      // A: per order
+      std::fill(x.begin(), x.end(), 0.0);
+      size_t ao = size_t(dist_of[offset]) * CB;
+      for (int i = 0; i <= p; ++i) {
+        const int w = p + 1 - i;
+        for (int j = i; j <= p; ++j)
+          for (int n = i; n <= p; ++n) {
+            const double a = coax[ao + (j - i) + size_t(n - i) * w];
+            x[j * j + j + i] += a * y[n * n + n + i];
+            if (i > 0) x[j * j + j - i] += a * y[n * n + n - i];
+          }
+        ao += size_t(w) * w;
+      }
+      // G: per degree
+      bo = 0;
+      for (int n = 0; n <= p; ++n) {
+        const int kk = 2 * n + 1;
+        for (int r = 0; r < kk; ++r) {
+          double s = 0.0;
+          for (int q = 0; q < kk; ++q) s += bwd[size_t(offset) * RB + bo + r + size_t(q) * kk] * x[n * n + q];
+          table[(n * n + r) + size_t(c) * nc] = s;
+        }
+        bo += size_t(kk) * kk;
+      }
+    }
+  });
+}
+
 int32_t fmmh_tables(int32_t p, double *m2m, double *l2l, double *m2l) {
   return guard([&] {
     std::vector<double> a, b, c;
@@ -190,6 +238,7 @@ int32_t fmmh_operator_create(const fmmh_plan *plan, const fmmh_operator_options 
       o.precision = options->precision;
       o.gradients = options->gradients;
       o.device = options->device;
+      o.m2l = options->m2l;
     }
     auto h = std::make_unique<fmmh_operator>();
     h->op = std::make_unique<fmmh::DeviceOperator>(plan->plan, o);

@@ -49,6 +49,13 @@ hipError_t upload_constants();
 
 void fill(hipStream_t s, double *p, size_t n, double v);
 void gather_charges(hipStream_t s, const double *q, const uint32_t *order, size_t n, double *q64, float *q32);
+// Rotation M2L (rotation.hip): rows of target boxes, entries (source box, offset index);
+// tables per offset (fwd, bwd: rb values) and per distance (coax: cb values).
+template <typename T>
+void m2l_rotation(hipStream_t s, int p, uint32_t rows, const uint32_t *row_box, const uint32_t *row_off,
+                  const uint32_t *ent_src, const uint16_t *ent_offset, const uint16_t *dist_of, const T *fwd,
+                  const T *bwd, const T *coax, int rb, int cb, const double *M, double *L);
+
 // Compensated f64 direct sum (unscaled): chunks of sources per block, partials in
 // part[nchunks][nt][8], then added per target in chunk order.
 int direct_chunks(int64_t ns, int64_t nt);

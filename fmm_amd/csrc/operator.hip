@@ -95,6 +95,7 @@ void upload_rows(Rows &r, const std::vector<uint32_t> &id, const std::vector<uin
 DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options) : opt_(options) {
   require(opt_.p >= 0 && opt_.p <= dev::kMaxDegree, "the device path supports 0 <= p <= 15", kUnsupportedLayout);
   require(opt_.precision == kPlainF64 || opt_.precision == kMixed, "unknown precision", kUnsupportedPrecision);
+  require(opt_.m2l == kM2lDense || opt_.m2l == kM2lRotation, "unknown M2L method", kUnsupportedLayout);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(kNoDevice, "no HIP device");
   require(opt_.device >= 0 && opt_.device < ndev, "device ordinal out of range", kNoDevice);
@@ -199,7 +200,7 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
     };
     swz(m2m, 8, t_m2m_);
     swz(l2l, 8, t_l2l_);
-    if (nlevels_ > 2) {
+    if (nlevels_ > 2 && opt_.m2l == kM2lDense) {
       if (opt_.precision == kMixed) {
         const size_t sw16 = dev::swizzled_table_size_bf16(opt_.p);
         std::vector<uint16_t> out(sw16 * 316);
@@ -275,6 +276,7 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
     upload_rows(p2l_rows_, id, cell, off, ent);
   }
   build_schedules(plan);
+  if (opt_.m2l == kM2lRotation) build_rotation(plan);
   loc_.alloc(nbox_total_ * nc_);
   if (opt_.precision == kMixed)
     slots32_.alloc(nslots_ * nc_);
@@ -419,7 +421,10 @@ void DeviceOperator::build_schedules(const Plan &plan) {
   m2l_red_level_.assign(nlevels_ + 1, 0);
   nslots_ = 0;
   for (int l = 0; l < nlevels_; ++l) {
-    if (l >= 2) {
+    if (l >= 2 && opt_.m2l == kM2lRotation) {
+      // the rotation M2L has its own rows (build_rotation); count the pairs only
+      all.pairs += plan.v[l].size();
+    } else if (l >= 2) {
       const Csr &V = plan.v[l];
       // Tasks of the eight octant classes interleaved (task k of class 0, of class 1, ...),
       // and each task's groups ordered by (parent-level neighbour D, source octant o'),
@@ -488,6 +493,45 @@ void DeviceOperator::build_schedules(const Plan &plan) {
   upload_rows(m2l_red_, red_id, red_cell, red_off, red_ent);
 }
 
+void DeviceOperator::build_rotation(const Plan &plan) {
+  std::vector<double> fwd, bwd, coax, dist2;
+  std::vector<uint16_t> dist_of;
+  build_rotation_tables(opt_.p, fwd, bwd, coax, dist_of, dist2);
+  rot_rb_ = rotation_block_size(opt_.p);
+  rot_cb_ = coaxial_block_size(opt_.p);
+  if (opt_.precision == kMixed) {
+    rot_fwd32_.upload(std::vector<float>(fwd.begin(), fwd.end()));
+    rot_bwd32_.upload(std::vector<float>(bwd.begin(), bwd.end()));
+    rot_coax32_.upload(std::vector<float>(coax.begin(), coax.end()));
+  } else {
+    rot_fwd_.upload(fwd);
+    rot_bwd_.upload(bwd);
+    rot_coax_.upload(coax);
+  }
+  rot_dist_.upload(dist_of);
+  // rows: target boxes with V entries, level by level; entries in row order
+  std::vector<uint32_t> box, off{0}, src;
+  std::vector<uint16_t> ofs;
+  rot_level_.assign(nlevels_ + 1, 0);
+  for (int l = 0; l < nlevels_; ++l) {
+    const Csr &V = plan.v[l];
+    for (uint32_t b = 0; l >= 2 && b < plan.nboxes(l); ++b) {
+      if (V.off[b + 1] == V.off[b]) continue;
+      box.push_back(static_cast<uint32_t>(box_base_[l] + b));
+      for (uint32_t e = V.off[b]; e < V.off[b + 1]; ++e) {
+        src.push_back(static_cast<uint32_t>(box_base_[l] + V.ent[e]));
+        ofs.push_back(V.aux[e]);
+      }
+      off.push_back(static_cast<uint32_t>(src.size()));
+    }
+    rot_level_[l + 1] = static_cast<uint32_t>(box.size());
+  }
+  rot_box_.upload(box);
+  rot_off_.upload(off);
+  rot_src_.upload(src);
+  rot_offset_.upload(ofs);
+}
+
 size_t DeviceOperator::device_bytes() const {
   return mult_.bytes() + loc_.bytes() + slots_.bytes() + slots32_.bytes() + t_m2l_.bytes() + t_m2m_.bytes() +
          t_l2l_.bytes() + sx_.bytes() * 8;
@@ -503,6 +547,20 @@ void DeviceOperator::launch_contract(const Schedule &s, uint32_t task_first, uin
 // M2L of the levels level_first .. level_last: the contraction tasks, then the ordered
 // reduction of the pair-mode levels.
 void DeviceOperator::launch_m2l(int level_first, int level_last) {
+  if (opt_.m2l == kM2lRotation) {
+    const uint32_t r0 = rot_level_[level_first], r1 = rot_level_[level_last + 1];
+    timed(kKindM2lContract, [&] {
+      if (opt_.precision == kMixed)
+        dev::m2l_rotation<float>(stream_, opt_.p, r1 - r0, rot_box_.ptr + r0, rot_off_.ptr + r0, rot_src_.ptr,
+                                 rot_offset_.ptr, rot_dist_.ptr, rot_fwd32_.ptr, rot_bwd32_.ptr, rot_coax32_.ptr,
+                                 rot_rb_, rot_cb_, mult_.ptr, loc_.ptr);
+      else
+        dev::m2l_rotation<double>(stream_, opt_.p, r1 - r0, rot_box_.ptr + r0, rot_off_.ptr + r0, rot_src_.ptr,
+                                  rot_offset_.ptr, rot_dist_.ptr, rot_fwd_.ptr, rot_bwd_.ptr, rot_coax_.ptr, rot_rb_,
+                                  rot_cb_, mult_.ptr, loc_.ptr);
+    });
+    return;
+  }
   const uint32_t t0 = m2l_level_task_[level_first], t1 = m2l_level_task_[level_last + 1];
   if (t1 > t0) {
     if (opt_.precision == kMixed) {

@@ -86,11 +86,17 @@ struct Rows {
   DevBuf<uint32_t> row_id, row_cell, row_off, row_ent;
 };
 
+enum M2lMethod : int {
+  kM2lDense = 0,     // dense tables, grouped contraction on MFMA
+  kM2lRotation = 1,  // rotation, coaxial shift, rotation back (f32 in the mixed precision)
+};
+
 struct OperatorOptions {
   int p = 6;
   int precision = kPlainF64;
   int gradients = 1;
   int device = 0;
+  int m2l = kM2lDense;
 };
 
 class DeviceOperator {
@@ -176,6 +182,14 @@ class DeviceOperator {
   DevBuf<double> t_m2m_, t_l2l_, t_m2l_;
   DevBuf<uint16_t> t_m2l16_, mult16_;  // mixed precision: split tables and multipoles
   std::array<std::array<uint16_t, 316>, 8> m2l_rank_;  // M2L group order per target octant
+  // rotation M2L: rows of target boxes with V entries (by level), entry (source, offset)
+  DevBuf<uint32_t> rot_box_, rot_off_, rot_src_;
+  DevBuf<uint16_t> rot_offset_, rot_dist_;
+  DevBuf<double> rot_fwd_, rot_bwd_, rot_coax_;
+  DevBuf<float> rot_fwd32_, rot_bwd32_, rot_coax32_;
+  std::vector<uint32_t> rot_level_;
+  int rot_rb_ = 0, rot_cb_ = 0;
+  void build_rotation(const Plan &plan);
   size_t nslots_ = 0;                  // pair-mode scratch rows (slot ids follow the box ids)
   DevBuf<double> slots_;               // plain f64: scratch rows
   DevBuf<float> slots32_;              // mixed: scratch rows (one group's f32 sum each)

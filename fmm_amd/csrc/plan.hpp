@@ -91,6 +91,16 @@ Plan build_forest(const double *points, const int64_t *counts, int nclouds, cons
 
 // Tables (CONVENTIONS §3.12): column-major (p+1)^2 x (p+1)^2 matrices, entry (i, k) at
 // i + k n; M2M and L2L by child octant (8 each), M2L by offset index (316).
+// Rotation M2L tables: per offset index (316) the forward multipole rotation F(Q) and the
+// backward local rotation G(Q^T), per degree n a column-major (2n+1)^2 block (degrees in
+// order, rotation_block_size(p) values per offset); per distinct |d|^2 (dist2, indexed by
+// dist_of[offset]) the coaxial M2L, per order i a column-major (p+1-i)^2 block [j-i][n-i]
+// (coaxial_block_size(p) values per distance).  T(d) = G A F equals the dense M2L table.
+int rotation_block_size(int p);
+int coaxial_block_size(int p);
+void build_rotation_tables(int p, std::vector<double> &fwd, std::vector<double> &bwd, std::vector<double> &coax,
+                           std::vector<uint16_t> &dist_of, std::vector<double> &dist2);
+
 void build_tables(int p, std::vector<double> &m2m, std::vector<double> &l2l,
                   std::vector<double> &m2l);
 

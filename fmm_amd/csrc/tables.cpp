@@ -12,7 +12,9 @@
 // B: O(1) per entry, O(n^2) per table.  Layout: column-major (p+1)^2 x (p+1)^2, entry
 // (i, k) at i + k n; M2M/L2L by octant o = 4x + 2y + z, M2L by offset index
 // (fmm-tables/src/m2l.rs:67-160, octant.rs).
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <vector>
 
 #include "plan.hpp"
@@ -160,7 +162,184 @@ void m2l_table(int p, const double b[3], double sigma, double *T) {
       }
 }
 
+// ---- rotation M2L (CONVENTIONS §3.11 "Rotation of coefficients", "Coaxial translations")
+//
+// T(d) = G(Q^T) A(|d|) F(Q), Q the rotation with Q d = |d| e_z.  F(Q) maps the multipole
+// coefficients of a configuration to those of the configuration rotated by Q, G(R) the
+// same for local coefficients; both act on each degree alone, a (2n+1)^2 block.  A(s) is
+// the M2L table of the shift s e_z, which keeps the order: per order i a (p+1-i)^2 block
+// from input degree n to output degree j, the same for slots +i and -i.
+//
+// The blocks are built from their definition: for sample points y, the coefficient vector
+// of a unit charge at Q y is the block times the vector of the charge at y (multipoles:
+// conj R_n(y), locals: conj I_n(y), in real storage); the block is the least-squares
+// solution over 2 (2n+1) points, after scaling every coefficient to unit size so the
+// factorials of the scaled harmonics do not spoil the solve.
+
+// Real-storage coefficient vector of degree n of a unit charge at y: conj of the
+// harmonic (slot +m: Re, slot -m: -Im), as P2M and P2L form them.
+void unit_coefficients(const Harm &h, int n, double *out) {
+  for (int m = -n; m <= n; ++m) {
+    double a, b;
+    h.get(n, m < 0 ? -m : m, a, b);
+    out[m + n] = m < 0 ? -b : a;
+  }
+}
+
+// Solves X S = S' in least squares for X (k x k), S and S' k x K column-major (K >= k),
+// by Householder QR of S^T.
+void least_squares_block(int k, int K, const std::vector<double> &S, const std::vector<double> &Sp,
+                         std::vector<double> &X) {
+  // A = S^T (K x k), B = Sp^T (K x k); minimise |A X^T - B|
+  std::vector<double> A(size_t(K) * k), B(size_t(K) * k);
+  for (int r = 0; r < K; ++r)
+    for (int c = 0; c < k; ++c) {
+      A[r + size_t(c) * K] = S[c + size_t(r) * k];
+      B[r + size_t(c) * K] = Sp[c + size_t(r) * k];
+    }
+  for (int c = 0; c < k; ++c) {
+    double norm = 0.0;
+    for (int r = c; r < K; ++r) norm += A[r + size_t(c) * K] * A[r + size_t(c) * K];
+    norm = std::sqrt(norm);
+    const double a0 = A[c + size_t(c) * K];
+    const double alpha = a0 > 0 ? -norm : norm;
+    std::vector<double> v(K, 0.0);
+    for (int r = c; r < K; ++r) v[r] = A[r + size_t(c) * K];
+    v[c] -= alpha;
+    double vv = 0.0;
+    for (int r = c; r < K; ++r) vv += v[r] * v[r];
+    if (vv == 0.0) continue;
+    auto reflect = [&](std::vector<double> &M, int cols, int from) {
+      for (int cc = from; cc < cols; ++cc) {
+        double s = 0.0;
+        for (int r = c; r < K; ++r) s += v[r] * M[r + size_t(cc) * K];
+        s *= 2.0 / vv;
+        for (int r = c; r < K; ++r) M[r + size_t(cc) * K] -= s * v[r];
+      }
+    };
+    reflect(A, k, c);
+    reflect(B, k, 0);
+  }
+  // back substitution: R X^T = (Q^T B)[0:k]
+  X.assign(size_t(k) * k, 0.0);
+  for (int cc = 0; cc < k; ++cc)
+    for (int r = k - 1; r >= 0; --r) {
+      double s = B[r + size_t(cc) * K];
+      for (int q = r + 1; q < k; ++q) s -= A[r + size_t(q) * K] * X[cc + size_t(q) * k];
+      X[cc + size_t(r) * k] = s / A[r + size_t(r) * K];
+    }
+}
+
+// Rotation block of degree n (column-major, (2n+1)^2) for rotation matrix Rm (row-major
+// 3 x 3): multipoles (local = false) or locals (local = true).
+void rotation_block(int n, const double Rm[9], bool local, double *blk) {
+  const int k = 2 * n + 1, K = 2 * k;
+  std::vector<double> S(size_t(k) * K), Sp(size_t(k) * K);
+  // sample points: a fixed spread on the unit sphere (golden-angle spiral, tilted)
+  for (int s = 0; s < K; ++s) {
+    const double z = 1.0 - (2.0 * s + 1.0) / K;
+    const double rad = std::sqrt(1.0 - z * z), phi = 2.399963229728653 * s + 0.3;
+    const double y[3] = {rad * std::cos(phi), rad * std::sin(phi), z};
+    const double qy[3] = {Rm[0] * y[0] + Rm[1] * y[1] + Rm[2] * y[2], Rm[3] * y[0] + Rm[4] * y[1] + Rm[5] * y[2],
+                          Rm[6] * y[0] + Rm[7] * y[1] + Rm[8] * y[2]};
+    const Harm a = local ? irregular(n, y) : regular(n, y);
+    const Harm b = local ? irregular(n, qy) : regular(n, qy);
+    unit_coefficients(a, n, &S[size_t(s) * k]);
+    unit_coefficients(b, n, &Sp[size_t(s) * k]);
+  }
+  // scale each coefficient to unit size over the samples (the same scale on both sides)
+  std::vector<double> w(k);
+  for (int c = 0; c < k; ++c) {
+    double ss = 0.0;
+    for (int s = 0; s < K; ++s) ss += S[c + size_t(s) * k] * S[c + size_t(s) * k];
+    w[c] = ss > 0 ? 1.0 / std::sqrt(ss / K) : 1.0;
+    for (int s = 0; s < K; ++s) {
+      S[c + size_t(s) * k] *= w[c];
+      Sp[c + size_t(s) * k] *= w[c];
+    }
+  }
+  std::vector<double> X;
+  least_squares_block(k, K, S, Sp, X);
+  // unscale: block = W^-1 X W
+  for (int c = 0; c < k; ++c)
+    for (int r = 0; r < k; ++r) blk[r + size_t(c) * k] = X[r + size_t(c) * k] / w[r] * w[c];
+}
+
 }  // namespace
+
+int rotation_block_size(int p) {
+  int s = 0;
+  for (int n = 0; n <= p; ++n) s += (2 * n + 1) * (2 * n + 1);
+  return s;
+}
+
+int coaxial_block_size(int p) {
+  int s = 0;
+  for (int i = 0; i <= p; ++i) s += (p + 1 - i) * (p + 1 - i);
+  return s;
+}
+
+void build_rotation_tables(int p, std::vector<double> &fwd, std::vector<double> &bwd, std::vector<double> &coax,
+                           std::vector<uint16_t> &dist_of, std::vector<double> &dist2) {
+  require(p >= 0 && p <= 30, "build_rotation_tables: p must lie in 0..30");
+  const int RB = rotation_block_size(p), CB = coaxial_block_size(p), nc = ncoef(p);
+  fwd.assign(size_t(316) * RB, 0.0);
+  bwd.assign(size_t(316) * RB, 0.0);
+  dist_of.assign(316, 0);
+  dist2.clear();
+  for (int t = 0; t < 316; ++t) {
+    int d[3];
+    offset_of_index(t, d);
+    const double s2 = double(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    size_t at = 0;
+    while (at < dist2.size() && dist2[at] != s2) ++at;
+    if (at == dist2.size()) dist2.push_back(s2);
+    dist_of[t] = static_cast<uint16_t>(at);
+  }
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int t = 0; t < 316; ++t) {
+    int d[3];
+    offset_of_index(t, d);
+    const double len = std::sqrt(double(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
+    const double ct = d[2] / len, st = std::sqrt(std::max(0.0, 1.0 - ct * ct));
+    const double ph = std::atan2(double(d[1]), double(d[0]));
+    const double cp = std::cos(ph), sp = std::sin(ph);
+    // Q = R_y(-theta) R_z(-phi): Q d = |d| e_z
+    const double Rz[9] = {cp, sp, 0, -sp, cp, 0, 0, 0, 1};
+    const double Ry[9] = {ct, 0, -st, 0, 1, 0, st, 0, ct};
+    double Q[9], Qt[9];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        double s = 0.0;
+        for (int q = 0; q < 3; ++q) s += Ry[3 * r + q] * Rz[3 * q + c];
+        Q[3 * r + c] = s;
+      }
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) Qt[3 * r + c] = Q[3 * c + r];
+    size_t bo = 0;
+    for (int n = 0; n <= p; ++n) {
+      rotation_block(n, Q, false, &fwd[size_t(t) * RB + bo]);
+      rotation_block(n, Qt, true, &bwd[size_t(t) * RB + bo]);
+      bo += size_t(2 * n + 1) * (2 * n + 1);
+    }
+  }
+  // coaxial blocks: the M2L table of the shift (0, 0, 2 |d|) (box radius units), order by
+  // order
+  coax.assign(dist2.size() * CB, 0.0);
+  std::vector<double> T(size_t(nc) * nc);
+  for (size_t q = 0; q < dist2.size(); ++q) {
+    std::fill(T.begin(), T.end(), 0.0);
+    const double b[3] = {0.0, 0.0, 2.0 * std::sqrt(dist2[q])};
+    m2l_table(p, b, 1.0, T.data());
+    size_t bo = 0;
+    for (int i = 0; i <= p; ++i) {
+      const int w = p + 1 - i;
+      for (int n = i; n <= p; ++n)
+        for (int j = i; j <= p; ++j) coax[q * CB + bo + (j - i) + size_t(n - i) * w] = T[idx(j, i) + size_t(idx(n, i)) * nc];
+      bo += size_t(w) * w;
+    }
+  }
+}
 
 void build_tables(int p, std::vector<double> &m2m, std::vector<double> &l2l,
                   std::vector<double> &m2l) {

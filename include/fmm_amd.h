@@ -47,6 +47,11 @@ typedef enum {
   FMMH_MIXED = 1      /* f32 P2P (double-float separations), bf16x3 M2L on MFMA; f64 elsewhere */
 } fmmh_precision;
 
+/* M2L formulation (fmm-exec's M2lMethod): dense tables on MFMA (fmm-kernels/src/translate.rs:1703)
+ * or rotation, coaxial shift and rotation back (fmm-kernels/src/rotation.rs:441); in the
+ * mixed precision the rotation runs in f32. */
+typedef enum { FMMH_M2L_DENSE = 0, FMMH_M2L_ROTATION = 1 } fmmh_m2l_method;
+
 typedef enum { FMMH_LIST_NEAR = 0, FMMH_LIST_V = 1, FMMH_LIST_W = 2, FMMH_LIST_X = 3 } fmmh_list_kind;
 
 /* Operator kinds for timings: the order of fmm-bench's kind table. */
@@ -80,6 +85,7 @@ typedef struct {
   int32_t precision; /* fmmh_precision */
   int32_t gradients; /* 1: potential and gradient, 0: potential only */
   int32_t device;    /* HIP device ordinal */
+  int32_t m2l;       /* fmmh_m2l_method */
 } fmmh_operator_options;
 
 typedef struct {
@@ -128,6 +134,10 @@ int32_t fmmh_plan_leaf_coordinates(const fmmh_plan *plan, int32_t targets, doubl
 
 /* Operator tables of degree p (column-major, 8 / 8 / 316 matrices of (p+1)^4 doubles). */
 int32_t fmmh_tables(int32_t p, double *m2m, double *l2l, double *m2l);
+/* The M2L of offset index `offset` composed from the rotation-M2L tables, G(Q^T) A(|d|)
+ * F(Q) (fmm-tables/src/rotation.rs:307 ShiftTables::apply), as a dense column-major
+ * (p+1)^2 x (p+1)^2 matrix: equal to the dense table to rounding. */
+int32_t fmmh_m2l_rotation_table(int32_t p, int32_t offset, double *table);
 
 /* ---- device operator -------------------------------------------------------------- */
 int32_t fmmh_device_count(int32_t *count);
