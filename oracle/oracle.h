@@ -120,4 +120,30 @@ void orc_fmm_leaf_coordinates(const orc_fmm *f, int targets, double *u, int64_t 
 #ifdef __cplusplus
 }
 #endif
+/* Helmholtz: compensated f64 direct sum of e^{ikr}/r (no 1/(4 pi)); complex values
+ * interleaved (re, im): q[2 ns], pot[2 nt], grad[6 nt] (x re, x im, y re, ...) or NULL. */
+void orc_helmholtz_direct_sum(double kre, double kim, const double *src, const double *q, int64_t ns,
+                              const double *tgt, int64_t nt, double *pot, double *grad);
+
+/* Helmholtz expansions (hexpansion.c): level units, stored form; complex expansions and
+ * tables as two real planes (real, then imaginary). */
+void orc_h_solid(int p, const double x[3], double *out);
+void orc_h_solid_grad(int p, const double x[3], double *gx, double *gy, double *gz);
+void orc_h_radial(int P, double zre, double zim, double *j, double *h);
+void orc_h_p2m(int p, double kre, double kim, const double c[3], double r, const double *src, const double *q,
+               int64_t n, double *out);
+void orc_h_l2p(int p, double kre, double kim, const double c[3], double r, const double *lc, const double *tgt,
+               int64_t n, double *phi, double *grad);
+void orc_h_p2l(int p, double kre, double kim, const double c[3], double r, const double *src, const double *q,
+               int64_t n, double *out);
+void orc_h_m2p(int p, double kre, double kim, const double c[3], double r, const double *mc, const double *tgt,
+               int64_t n, double *phi, double *grad);
+void orc_h_rotation_blocks(int p, const double Q[9], double *out);
+void orc_h_coaxial(int p, int singular_start, double kre, double kim, double t, double *out);
+void orc_h_m2m(int p, double kre, double kim, int o, double *out);
+void orc_h_l2l(int p, double kre, double kim, int o, double *out);
+void orc_h_m2l(int p, double kre, double kim, const int d[3], double *out);
+void orc_h_apply(int p, const double *T, const double *x, double *y);
+void orc_hfmm_evaluate(orc_fmm *f, double kre, double kim, const double *q, double *phi, double *grad);
+
 #endif

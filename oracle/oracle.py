@@ -49,6 +49,8 @@ def lib():
         pd = C.c_void_p
         L.orc_p2p.argtypes = [_f64p, _f64p, C.c_int64, _f64p, C.c_int64, _f64p, pd]
         L.orc_direct_sum.argtypes = [_f64p, _f64p, C.c_int64, _f64p, C.c_int64, _f64p, pd]
+        L.orc_helmholtz_direct_sum.argtypes = [C.c_double, C.c_double, _f64p, _f64p, C.c_int64, _f64p, C.c_int64,
+                                               _f64p, pd]
         L.orc_neumaier_sum.argtypes = [_f64p, C.c_int64]
         L.orc_neumaier_sum.restype = C.c_double
         L.orc_error_norms.argtypes = [_f64p, _f64p, C.c_int64, C.c_int, _f64p]
@@ -97,6 +99,22 @@ def lib():
         L.orc_fmm_point_leaves.argtypes = [C.c_void_p, C.c_int, _i64p]
         L.orc_fmm_leaf_coordinates.argtypes = [C.c_void_p, C.c_int, _f64p, _i64p]
         L.orc_fmm_expansions.argtypes = [C.c_void_p, C.c_int, C.c_int, _f64p]
+        L.orc_h_solid.argtypes = [C.c_int, _f64p, _f64p]
+        L.orc_h_solid_grad.argtypes = [C.c_int, _f64p, _f64p, _f64p, _f64p]
+        L.orc_h_radial.argtypes = [C.c_int, C.c_double, C.c_double, _f64p, _f64p]
+        for name in ("orc_h_p2m", "orc_h_p2l"):
+            getattr(L, name).argtypes = [C.c_int, C.c_double, C.c_double, _f64p, C.c_double, _f64p, _f64p,
+                                         C.c_int64, _f64p]
+        for name in ("orc_h_l2p", "orc_h_m2p"):
+            getattr(L, name).argtypes = [C.c_int, C.c_double, C.c_double, _f64p, C.c_double, _f64p, _f64p,
+                                         C.c_int64, _f64p, pd]
+        L.orc_h_rotation_blocks.argtypes = [C.c_int, _f64p, _f64p]
+        L.orc_h_coaxial.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _f64p]
+        for name in ("orc_h_m2m", "orc_h_l2l"):
+            getattr(L, name).argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, _f64p]
+        L.orc_h_m2l.argtypes = [C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int), _f64p]
+        L.orc_h_apply.argtypes = [C.c_int, _f64p, _f64p, _f64p]
+        L.orc_hfmm_evaluate.argtypes = [C.c_void_p, C.c_double, C.c_double, _f64p, _f64p, pd]
         _lib = L
     return _lib
 
@@ -177,6 +195,125 @@ def direct_sum(src, q, tgt, gradients=True):
     grad = np.zeros((len(tgt), 3)) if gradients else None
     lib().orc_direct_sum(src, q, len(src), tgt, len(tgt), phi, _ptr(grad))
     return phi, grad
+
+
+def helmholtz_direct_sum(k, src, q, tgt, gradients=True):
+    """Compensated f64 direct sum of q e^{ikr}/r and its gradient (complex k and q; no
+    1/(4 pi)); returns complex arrays (nt,) and (nt, 3)."""
+    src = np.ascontiguousarray(src, np.float64)
+    tgt = np.ascontiguousarray(tgt, np.float64)
+    qc = np.ascontiguousarray(np.asarray(q, np.complex128).view(np.float64))
+    pot = np.zeros(2 * len(tgt))
+    grad = np.zeros(6 * len(tgt)) if gradients else None
+    lib().orc_helmholtz_direct_sum(float(np.real(k)), float(np.imag(k)), src, qc, len(src), tgt, len(tgt), pot,
+                                   _ptr(grad))
+    g = grad.view(np.complex128).reshape(-1, 3) if gradients else None
+    return pot.view(np.complex128), g
+
+
+def h_solid(p, x):
+    out = np.zeros(nc(p))
+    lib().orc_h_solid(p, np.ascontiguousarray(x, np.float64), out)
+    return out
+
+
+def h_solid_grad(p, x):
+    g = np.zeros((3, nc(p)))
+    lib().orc_h_solid_grad(p, np.ascontiguousarray(x, np.float64), g[0], g[1], g[2])
+    return g
+
+
+def h_radial(P, z):
+    """Spherical j_n(z) and h_n^(1)(z), n = 0..P, for complex z."""
+    j, h = np.zeros(2 * (P + 1)), np.zeros(2 * (P + 1))
+    lib().orc_h_radial(P, float(np.real(z)), float(np.imag(z)), j, h)
+    return j.view(np.complex128), h.view(np.complex128)
+
+
+def _cplx_planes(x):
+    x = np.asarray(x, np.complex128)
+    return np.ascontiguousarray(np.concatenate([x.real, x.imag]))
+
+
+def _from_planes(v):
+    n = len(v) // 2
+    return v[:n] + 1j * v[n:]
+
+
+def h_p2m(p, k, c, r, src, q):
+    out = np.zeros(2 * nc(p))
+    qc = np.ascontiguousarray(np.asarray(q, np.complex128).view(np.float64))
+    lib().orc_h_p2m(p, float(np.real(k)), float(np.imag(k)), np.asarray(c, np.float64), float(r),
+                    np.ascontiguousarray(src, np.float64), qc, len(src), out)
+    return _from_planes(out)
+
+
+def h_p2l(p, k, c, r, src, q):
+    out = np.zeros(2 * nc(p))
+    qc = np.ascontiguousarray(np.asarray(q, np.complex128).view(np.float64))
+    lib().orc_h_p2l(p, float(np.real(k)), float(np.imag(k)), np.asarray(c, np.float64), float(r),
+                    np.ascontiguousarray(src, np.float64), qc, len(src), out)
+    return _from_planes(out)
+
+
+def _h_eval(name, p, k, c, r, coeffs, tgt, gradients):
+    tgt = np.ascontiguousarray(tgt, np.float64)
+    phi = np.zeros(2 * len(tgt))
+    grad = np.zeros(6 * len(tgt)) if gradients else None
+    getattr(lib(), name)(p, float(np.real(k)), float(np.imag(k)), np.asarray(c, np.float64), float(r),
+                         _cplx_planes(coeffs), tgt, len(tgt), phi, _ptr(grad))
+    return phi.view(np.complex128), (grad.view(np.complex128).reshape(-1, 3) if gradients else None)
+
+
+def h_l2p(p, k, c, r, local, tgt, gradients=True):
+    return _h_eval("orc_h_l2p", p, k, c, r, local, tgt, gradients)
+
+
+def h_m2p(p, k, c, r, mult, tgt, gradients=True):
+    return _h_eval("orc_h_m2p", p, k, c, r, mult, tgt, gradients)
+
+
+def h_rotation_blocks(p, Q):
+    total = sum((2 * n + 1) ** 2 for n in range(p + 1))
+    out = np.zeros(total)
+    lib().orc_h_rotation_blocks(p, np.ascontiguousarray(Q, np.float64).ravel(), out)
+    blocks, off = [], 0
+    for n in range(p + 1):
+        w = 2 * n + 1
+        blocks.append(out[off:off + w * w].reshape(w, w))
+        off += w * w
+    return blocks
+
+
+def h_coaxial(p, singular_start, k, t):
+    """E^m_{n'n}(t) as an array [m, n', n] (complex)."""
+    out = np.zeros(2 * (p + 1) ** 3)
+    lib().orc_h_coaxial(p, int(singular_start), float(np.real(k)), float(np.imag(k)), float(t), out)
+    return out.view(np.complex128).reshape(p + 1, p + 1, p + 1)
+
+
+def _h_table(out, p):
+    n = nc(p)
+    return (out[: n * n] + 1j * out[n * n:]).reshape(n, n).T  # row = target slot
+
+
+def h_m2m(p, k, o):
+    out = np.zeros(2 * nc(p) ** 2)
+    lib().orc_h_m2m(p, float(np.real(k)), float(np.imag(k)), int(o), out)
+    return _h_table(out, p)
+
+
+def h_l2l(p, k, o):
+    out = np.zeros(2 * nc(p) ** 2)
+    lib().orc_h_l2l(p, float(np.real(k)), float(np.imag(k)), int(o), out)
+    return _h_table(out, p)
+
+
+def h_m2l(p, k, d):
+    out = np.zeros(2 * nc(p) ** 2)
+    dv = (C.c_int * 3)(*[int(v) for v in d])
+    lib().orc_h_m2l(p, float(np.real(k)), float(np.imag(k)), dv, out)
+    return _h_table(out, p)
 
 
 def p2p(src, q, tgt, gradients=True):
@@ -335,6 +472,16 @@ class Fmm:
         grad = np.zeros((len(self.targets), 3)) if self.gradients else None
         lib().orc_fmm_evaluate(self._h, q, phi, _ptr(grad))
         return phi, grad
+
+    def evaluate_helmholtz(self, k, charges):
+        """Helmholtz FMM on the same tree (wavenumber k, complex charges): complex phi (nt,)
+        and grad (nt, 3)."""
+        q = np.ascontiguousarray(np.asarray(charges, np.complex128).view(np.float64))
+        phi = np.zeros(2 * len(self.targets))
+        grad = np.zeros(6 * len(self.targets)) if self.gradients else None
+        lib().orc_hfmm_evaluate(self._h, float(np.real(k)), float(np.imag(k)), q, phi, _ptr(grad))
+        g = grad.view(np.complex128).reshape(-1, 3) if self.gradients else None
+        return phi.view(np.complex128), g
 
     @property
     def nlevels(self):

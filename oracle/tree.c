@@ -852,6 +852,146 @@ void orc_fmm_evaluate(orc_fmm *f, const double *q_caller, double *phi, double *g
   free(gr);
 }
 
+/* Helmholtz FMM over the same tree and lists (CONVENTIONS §3.15; nd_fmm_exec::helmholtz):
+ * complex charges q[2 ns] (re, im interleaved, caller order), outputs phi[2 nt] and
+ * grad[6 nt].  Level l works in its own units, wavenumber kh_l = k r_l; stored expansions
+ * as hexpansion.c; P2P sums e^{i kh r}/r in the target leaf's units; the output pass divides
+ * by 4 pi r_t (potential) and 4 pi r_t^2 (gradient) as for Laplace. */
+void orc_hfmm_evaluate(orc_fmm *f, double kre, double kim, const double *q_caller, double *phi, double *grad) {
+  const int p = f->p, n = nc(p), L = f->nlevels - 1;
+  const size_t n2 = (size_t)n * n, w = 2 * (size_t)n;
+  double **mult = calloc(f->nlevels, sizeof(double *)), **loc = calloc(f->nlevels, sizeof(double *));
+  double *kh = malloc(sizeof(double) * 2 * f->nlevels);
+  for (int l = 0; l <= L; ++l) {
+    mult[l] = calloc((size_t)f->nbox[l] * w + 1, sizeof(double));
+    loc[l] = calloc((size_t)f->nbox[l] * w + 1, sizeof(double));
+    const double r = f->side / ldexp(1.0, l + 1);
+    kh[2 * l] = kre * r;
+    kh[2 * l + 1] = kim * r;
+  }
+  double *q = malloc(sizeof(double) * 2 * (f->ns + 1));
+  for (int64_t i = 0; i < f->ns; ++i) {
+    q[2 * i] = q_caller[2 * f->src_order[i]];
+    q[2 * i + 1] = q_caller[2 * f->src_order[i] + 1];
+  }
+  double *ph = calloc(2 * f->nt + 2, sizeof(double)), *gr = calloc(6 * f->nt + 6, sizeof(double));
+  double *gp = f->gradients ? gr : NULL;
+  double *T = malloc(sizeof(double) * 2 * n2);
+  const double zero3[3] = {0, 0, 0};
+  /* upward */
+  for (int l = L; l >= 0; --l) {
+    for (int64_t lf = f->leaf_off[l]; lf < f->leaf_off[l + 1]; ++lf) {
+      int64_t b = f->leaf_box[lf], s0 = f->src_start[lf], s1 = f->src_start[lf + 1];
+      orc_h_p2m(p, kh[2 * l], kh[2 * l + 1], zero3, 1.0, f->us + 3 * s0, q + 2 * s0, s1 - s0, mult[l] + b * w);
+    }
+    if (l >= 1)
+      for (int o = 0; o < 8; ++o) {
+        orc_h_m2m(p, kh[2 * (l - 1)], kh[2 * (l - 1) + 1], o, T);
+        for (int64_t b = 0; b < f->nbox[l - 1]; ++b) {
+          int64_t c0 = f->first_child[l - 1][b];
+          if (c0 >= 0) orc_h_apply(p, T, mult[l] + (c0 + o) * w, mult[l - 1] + b * w);
+        }
+      }
+  }
+  /* downward */
+  char *have = malloc(316);
+  double *m2l = malloc(sizeof(double) * 2 * n2 * 316);
+  for (int l = 1; l <= L; ++l) {
+    for (int o = 0; o < 8; ++o) {
+      orc_h_l2l(p, kh[2 * (l - 1)], kh[2 * (l - 1) + 1], o, T);
+      for (int64_t b = 0; b < f->nbox[l]; ++b) {
+        int64_t d[4];
+        orc_decode(f->keys[l][b], d);
+        if ((int)(((d[1] & 1) << 2) | ((d[2] & 1) << 1) | (d[3] & 1)) != o) continue;
+        int64_t pb = find(f->keys[l - 1], f->nbox[l - 1], orc_parent(f->keys[l][b]));
+        orc_h_apply(p, T, loc[l - 1] + pb * w, loc[l] + b * w);
+      }
+    }
+    memset(have, 0, 316);
+    for (int64_t b = 0; b < f->nbox[l]; ++b)
+      for (int64_t e = f->v[l].off[b]; e < f->v[l].off[b + 1]; ++e) {
+        int t = (int)f->v[l].aux[e];
+        if (!have[t]) {
+          int dv[3];
+          orc_offset_of_index(t, dv);
+          orc_h_m2l(p, kh[2 * l], kh[2 * l + 1], dv, m2l + 2 * n2 * t);
+          have[t] = 1;
+        }
+        orc_h_apply(p, m2l + 2 * n2 * t, mult[l] + f->v[l].ent[e] * w, loc[l] + b * w);
+      }
+    for (int64_t b = 0; b < f->nbox[l]; ++b)
+      for (int64_t e = f->x[l].off[b]; e < f->x[l].off[b + 1]; ++e) {
+        int64_t sl = f->x[l].ent[e], s0 = f->src_start[sl], s1 = f->src_start[sl + 1];
+        double c[3], r;
+        relative_frame(f->keys[l][b], f->leaf_key[sl], c, &r); /* ĉ(t|s), r̂(t|s) */
+        orc_h_p2l(p, kh[2 * l], kh[2 * l + 1], c, r, f->us + 3 * s0, q + 2 * s0, s1 - s0, loc[l] + b * w);
+      }
+  }
+  free(have);
+  free(m2l);
+  /* leaves */
+  for (int l = 0; l <= L; ++l) {
+    for (int64_t lf = f->leaf_off[l]; lf < f->leaf_off[l + 1]; ++lf) {
+      int64_t t0 = f->tgt_start[lf], t1 = f->tgt_start[lf + 1], b = f->leaf_box[lf];
+      orc_h_l2p(p, kh[2 * l], kh[2 * l + 1], zero3, 1.0, loc[l] + b * w, f->ut + 3 * t0, t1 - t0, ph + 2 * t0,
+                gp ? gp + 6 * t0 : NULL);
+    }
+    for (int64_t lf = f->leaf_off[l]; lf < f->leaf_off[l + 1]; ++lf) {
+      int64_t r = lf - f->leaf_off[l], t0 = f->tgt_start[lf], t1 = f->tgt_start[lf + 1];
+      for (int64_t e = f->w[l].off[r]; e < f->w[l].off[r + 1]; ++e) {
+        int64_t sb = f->w[l].ent[e];
+        int sl = (int)f->w[l].aux[e];
+        double c[3], rr;
+        relative_frame(f->keys[sl][sb], f->leaf_key[lf], c, &rr); /* ĉ(s|t), r̂(s|t) */
+        orc_h_m2p(p, kh[2 * sl], kh[2 * sl + 1], c, rr, mult[sl] + sb * w, f->ut + 3 * t0, t1 - t0, ph + 2 * t0,
+                  gp ? gp + 6 * t0 : NULL);
+      }
+    }
+    for (int64_t lf = f->leaf_off[l]; lf < f->leaf_off[l + 1]; ++lf) {
+      int64_t r = lf - f->leaf_off[l], t0 = f->tgt_start[lf], t1 = f->tgt_start[lf + 1];
+      for (int64_t e = f->u[l].off[r]; e < f->u[l].off[r + 1]; ++e) {
+        int64_t sl = f->u[l].ent[e], s0 = f->src_start[sl], s1 = f->src_start[sl + 1];
+        double *y = malloc(sizeof(double) * 3 * (s1 - s0 + 1));
+        if (sl == lf) {
+          memcpy(y, f->us + 3 * s0, sizeof(double) * 3 * (s1 - s0));
+        } else {
+          double c[3], rr;
+          relative_frame(f->leaf_key[sl], f->leaf_key[lf], c, &rr); /* ĉ(s|t), r̂(s|t) */
+          for (int64_t j = 0; j < s1 - s0; ++j)
+            for (int k = 0; k < 3; ++k) y[3 * j + k] = c[k] + rr * f->us[3 * (s0 + j) + k];
+        }
+        orc_helmholtz_direct_sum(kh[2 * l], kh[2 * l + 1], y, q + 2 * s0, s1 - s0, f->ut + 3 * t0, t1 - t0,
+                                 ph + 2 * t0, gp ? gp + 6 * t0 : NULL);
+        free(y);
+      }
+    }
+  }
+  /* output pass */
+  for (int64_t lf = 0; lf < f->nleaves; ++lf) {
+    int lvl = key_level(f->leaf_key[lf]);
+    double rt = f->side / ldexp(1.0, lvl + 1);
+    double s1 = 4.0 * 3.141592653589793 * rt, s2 = 4.0 * 3.141592653589793 * rt * rt;
+    for (int64_t i = f->tgt_start[lf]; i < f->tgt_start[lf + 1]; ++i) {
+      int64_t c = f->tgt_order[i];
+      phi[2 * c] = ph[2 * i] / s1;
+      phi[2 * c + 1] = ph[2 * i + 1] / s1;
+      if (grad && f->gradients)
+        for (int k = 0; k < 6; ++k) grad[6 * c + k] = gr[6 * i + k] / s2;
+    }
+  }
+  for (int l = 0; l <= L; ++l) {
+    free(mult[l]);
+    free(loc[l]);
+  }
+  free(mult);
+  free(loc);
+  free(kh);
+  free(q);
+  free(ph);
+  free(gr);
+  free(T);
+}
+
 /* ---- inspection --------------------------------------------------------------------- */
 
 int orc_fmm_nlevels(const orc_fmm *f) { return f->nlevels; }
