@@ -77,9 +77,11 @@ def main():
                     help="first copy 256 MiB device to device (a known-size kernel for counter scaling)")
     ap.add_argument("--m2l", default="dense", choices=("dense", "rotation"),
                     help="M2L formulation of both legs (rotation: f32 in the mixed leg)")
-    ap.add_argument("--workload", default="cube", choices=("cube", "batched"),
+    ap.add_argument("--workload", default="cube", choices=("cube", "batched", "helmholtz"),
                     help="cube: 4M uniform points (the headline); batched: 256 independent Gaussian clouds, "
-                         "p = 6, one forest evaluation replayed as a hipGraph, plus its adjoint")
+                         "p = 6, one forest evaluation replayed as a hipGraph, plus its adjoint; helmholtz: "
+                         "1M uniform points, k = 25, complex charges, degree by the rule at 1e-6")
+    ap.add_argument("--k", type=float, default=25.0, help="helmholtz: the wavenumber (real)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each box has one GPU)",
@@ -87,6 +89,8 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         return batched(args)
+    if args.workload == "helmholtz":
+        return helmholtz(args)
 
     import torch
 
@@ -360,6 +364,105 @@ def batched(args):
                    "sizes": [int(len(c)) for c in clouds][:8] + ["..."]},
         "errors": {"phi": (num / den) ** 0.5, "grad": (numg / deng) ** 0.5, "sampled_per_cloud": 32},
         "adjoint_ms": float(np.median(vt)) * 1e3,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def helmholtz(args):
+    """The Helmholtz configuration: N = 1,000,000 uniform in the unit cube, k = 25 (about
+    four wavelengths across), complex charges with N(0,1) parts, degree by the rule at
+    1e-6 (fmm-exec/src/helmholtz/degree.rs), at most 256 points per leaf (the reference's
+    default at 1e-6), potential and gradient.  Plain f64 and mixed (complex64 P2P with
+    complex128 sums, bf16x3 M2L on MFMA) both timed; errors against the compensated f64
+    direct sum (on the GPU) at 1,000 sampled targets; the oracle's serial Helmholtz FMM on
+    a bounded sample is the CPU baseline."""
+    import torch
+
+    import fmm_amd
+    from fmm_amd import points as P
+
+    n = args.n if args.n != 4_000_000 else 1_000_000
+    leaf = args.leaf if args.leaf != 128 else 256
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    pts, q, sample = P.helmholtz_cube(n, 1000)
+    log(f"helmholtz: N={n} k={args.k} drawn in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    b = fmm_amd.HelmholtzFmmBuilder(args.k, tolerance=1e-6)
+    p = b.degree_for(fmm_amd.Plan(pts, leaf_points=leaf))
+    # W and X pairs to P2P at this path's crossovers, w = x = (p+1)^2 (WxNear::Crossovers)
+    near = None if args.near == "off" else (float((p + 1) ** 2),) * 2
+    plan = fmm_amd.Plan(pts, leaf_points=leaf, near_rule=near)
+    lk = plan.leaf_keys()
+    hist = {int(l): int(c) for l, c in zip(*np.unique(lk & np.uint64(0x7FFF), return_counts=True))}
+    log(f"plan: {plan.nlevels} levels, leaves {hist}, p = {p}, built in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    ep, eg = fmm_amd.helmholtz_direct_sum(args.k, pts, q, pts[sample])
+    ep, eg = ep / (4 * np.pi), eg / (4 * np.pi)
+    log(f"direct sum at {len(sample)} targets in {time.time() - t0:.1f}s")
+    q_dev = torch.from_numpy(np.ascontiguousarray(q.view(np.float64))).to(dev)
+
+    def run(precision):
+        t0 = time.time()
+        op = fmm_amd.HelmholtzOperator(plan, p, args.k, precision, True, 0)
+        log(f"{precision}: operator built in {time.time() - t0:.1f}s: {op.stats()}")
+
+        def step():
+            op.load_charges(q_dev)
+            op.reset()
+            op.evaluate()
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(args.steps):
+            torch.cuda.synchronize()
+            s = time.perf_counter()
+            step()
+            torch.cuda.synchronize()  # the device, so the operator's own stream too
+            times.append(time.perf_counter() - s)
+        op.kind_timing(True)
+        step()
+        kinds = op.kind_times()
+        op.kind_timing(False)
+        out = op.read_output()
+        err = {"phi": float(np.linalg.norm(out.potential[sample] - ep) / np.linalg.norm(ep)),
+               "grad": float(np.linalg.norm(out.gradient[sample] - eg) / np.linalg.norm(eg))}
+        med = float(np.median(times))
+        log(f"{precision}: {med * 1e3:.2f} ms/step, errors {err}, kinds "
+            + ", ".join(f"{k} {v[0]:.2f}" for k, v in kinds.items() if v[1]))
+        return med, err, kinds, op.stats()
+
+    mixed = run("mixed")
+    plain = run("f64")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        m = 6000
+        t0 = time.time()
+        f = O.Fmm(pts[:m], p=p, leaf_points=leaf)
+        f.evaluate_helmholtz(args.k, q[:m])
+        dt = time.time() - t0
+        cpu = {"value": m / dt / 1e6, "unit": "Mpts/s", "cores": 1, "kind": "port",
+               "sample": f"oracle serial Helmholtz FMM (tree, tables and evaluation) on the first {m} points, "
+                         f"p = {p}, leaf {leaf}"}
+    med, err, kinds, stats = mixed
+    p2p_ms = kinds["p2p"][0]
+    line = {
+        "metric": "Helmholtz FMM Mpts/s (1M uniform, k = 25, 1e-6 degree rule, complex charges)",
+        "value": n / med / 1e6, "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": med * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "complex64 P2P (complex128 sums) + bf16x3 M2L + f64 other", "data": "synthetic",
+        "config": {"workload": "helmholtz-uniform-cube", "n": n, "k": args.k, "p": p, "leaf_points": leaf,
+                   "near_wx": near},
+        "errors": err,
+        "plain_f64": {"ms_per_step": plain[0] * 1e3, "value": n / plain[0] / 1e6, "errors": plain[1]},
+        "kinds_ms": {k: v[0] for k, v in kinds.items() if v[1]},
+        "interactions_per_s": stats["p2p_pairs"] / (p2p_ms * 1e-3) if p2p_ms > 0 else None,
+        "m2l_pairs": stats["m2l_pairs"],
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 

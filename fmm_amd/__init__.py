@@ -373,3 +373,211 @@ class FmmBuilder:
         plan = Plan.forest(clouds, self._leaf_points, self._max_level, self._near)
         op = Operator(plan, self._p, self._precision, self._gradients, self._device, self._m2l)
         return Fmm(plan, op)
+
+
+# ---- Helmholtz --------------------------------------------------------------------------
+
+def helmholtz_degree(k, radius: float, tolerance: float) -> int:
+    """The degree rule of fmm-exec/src/helmholtz/degree.rs:8-15 for boxes of half-width
+    `radius`: max(ceil(0.30 L^2 + 0.25 L + 2.60), ceil(x + 0.80 (ln 1/eps)^(2/3) x^(1/3))),
+    x = Re(k) sqrt(3) radius, L = log10(1/eps)."""
+    p = _lib.lib().fmmh_helmholtz_degree(float(np.real(k)), float(radius), float(tolerance))
+    if p < 0:
+        raise FmmError(9, "helmholtz_degree: 0 < tolerance < 1, radius > 0, Re k >= 0")
+    return int(p)
+
+
+def helmholtz_table(p: int, kh, kind: str, index: int) -> np.ndarray:
+    """One translation table of a level whose wavenumber in its own units is kh: "m2m" from
+    the child of octant `index`, "l2l" into it, "m2l" of offset index `index`; a complex
+    matrix A with stored output = A @ stored input."""
+    n = (p + 1) ** 2
+    out = np.zeros(2 * n * n)
+    k = {"m2m": 0, "l2l": 1, "m2l": 2}[kind]
+    _lib.check(_lib.lib().fmmh_helmholtz_table(int(p), float(np.real(kh)), float(np.imag(kh)), k, int(index),
+                                               _lib.ptr(out)))
+    return (out[: n * n] + 1j * out[n * n:]).reshape(n, n).T.copy()
+
+
+def helmholtz_direct_sum(k, sources, charges, targets, gradients=True, device=0):
+    """Compensated f64 direct sum on the GPU of q e^{ikr}/r and its gradient (no 1/(4 pi)),
+    nd_fmm_ref::helmholtz::p2p::direct_sum (fmm-ref/src/helmholtz/p2p.rs:155); complex."""
+    src = np.ascontiguousarray(sources, dtype=np.float64).reshape(-1, 3)
+    q = np.ascontiguousarray(np.asarray(charges, dtype=np.complex128).view(np.float64))
+    tgt = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 3)
+    phi = np.zeros(2 * len(tgt))
+    grad = np.zeros(6 * len(tgt)) if gradients else None
+    _lib.check(_lib.lib().fmmh_helmholtz_direct_sum(int(device), float(np.real(k)), float(np.imag(k)),
+                                                    _lib.ptr(src), _lib.ptr(q), len(src), _lib.ptr(tgt), len(tgt),
+                                                    _lib.ptr(phi), _lib.ptr(grad), 0))
+    return phi.view(np.complex128), (grad.view(np.complex128).reshape(-1, 3) if gradients else None)
+
+
+class HelmholtzOperator:
+    """The Helmholtz device operator over a plan (complex charges, kernel e^{ikr}/(4 pi r))."""
+
+    def __init__(self, plan: Plan, p: int, k, precision="f64", gradients=True, device=0):
+        self.plan = plan
+        self.p = p
+        self.k = complex(k)
+        self.gradients = gradients
+        self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        o = _lib.HelmholtzOptions(int(p), self.k.real, self.k.imag, self.precision, int(bool(gradients)),
+                                  int(device))
+        h = C.c_void_p()
+        _lib.check(_lib.lib().fmmh_helmholtz_create(plan.handle, C.byref(o), C.byref(h)))
+        self._h = h
+        self.ntargets = len(plan.targets)
+        self.nsources = len(plan.sources)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.lib().fmmh_helmholtz_free(h)
+            self._h = None
+
+    def stats(self) -> dict:
+        s = _lib.Stats()
+        _lib.check(_lib.lib().fmmh_helmholtz_stats(self._h, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in _lib.Stats._fields_}
+
+    def load_charges(self, charges):
+        if hasattr(charges, "data_ptr") and getattr(charges, "is_cuda", False):
+            _lib.check(_lib.lib().fmmh_helmholtz_load_charges(self._h, C.c_void_p(charges.data_ptr()), 1))
+            return
+        q = np.ascontiguousarray(np.asarray(charges, dtype=np.complex128).reshape(-1))
+        if len(q) != self.nsources:
+            raise FmmError(9, f"expected {self.nsources} charges, got {len(q)}")
+        _lib.check(_lib.lib().fmmh_helmholtz_load_charges(self._h, _lib.ptr(q.view(np.float64)), 0))
+
+    def reset(self):
+        _lib.check(_lib.lib().fmmh_helmholtz_reset(self._h))
+
+    def evaluate(self):
+        _lib.check(_lib.lib().fmmh_helmholtz_evaluate(self._h))
+
+    def read_output(self) -> Output:
+        phi = np.zeros(2 * self.ntargets)
+        grad = np.zeros(6 * self.ntargets) if self.gradients else None
+        _lib.check(_lib.lib().fmmh_helmholtz_read_output(self._h, _lib.ptr(phi), _lib.ptr(grad), 0))
+        return Output(phi.view(np.complex128),
+                      grad.view(np.complex128).reshape(-1, 3) if self.gradients else None)
+
+    def expansions(self, locals_: bool, level: int) -> np.ndarray:
+        n = (self.p + 1) ** 2
+        nb = len(self.plan.box_keys(level))
+        out = np.zeros((nb, 2, n))
+        _lib.check(_lib.lib().fmmh_helmholtz_read_expansions(self._h, int(locals_), int(level), _lib.ptr(out)))
+        return out[:, 0] + 1j * out[:, 1]
+
+    def kind_timing(self, enable=True):
+        _lib.check(_lib.lib().fmmh_helmholtz_kind_timing(self._h, int(enable)))
+
+    def kind_times(self):
+        ms = np.zeros(len(KINDS))
+        n = np.zeros(len(KINDS), np.int64)
+        _lib.check(_lib.lib().fmmh_helmholtz_kind_times(self._h, _lib.ptr(ms), _lib.ptr(n)))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KINDS)}
+
+
+class HelmholtzFmm:
+    """A Helmholtz FMM built for fixed sources and targets (fmm-exec's HelmholtzFmm,
+    fmm-exec/src/helmholtz/fmm.rs:2202-2300)."""
+
+    def __init__(self, plan: Plan, operator: HelmholtzOperator):
+        self.plan = plan
+        self.operator = operator
+
+    @property
+    def p(self) -> int:
+        return self.operator.p
+
+    def evaluate(self, charges) -> Output:
+        """Complex potential and gradient at the targets (1/(4 pi) included), caller order."""
+        op = self.operator
+        op.load_charges(charges)
+        op.reset()
+        op.evaluate()
+        return op.read_output()
+
+
+class HelmholtzFmmBuilder:
+    """Settings of a Helmholtz FMM (fmm-exec/src/helmholtz/fmm.rs:419-1017): the wavenumber
+    (Wavenumber::new: k != 0, Re k >= 0, Im k >= 0) and the degrees, a tolerance for the
+    degree rule (Degrees::Tolerance) or one degree for every level."""
+
+    def __init__(self, k, tolerance: Optional[float] = 1e-6, degree: Optional[int] = None):
+        k = complex(k)
+        if k == 0 or k.real < 0 or k.imag < 0 or not np.isfinite(k):
+            raise FmmError(9, "the wavenumber must be nonzero with Re k >= 0 and Im k >= 0")
+        if degree is not None:
+            tolerance = None  # a named degree replaces the rule
+        elif tolerance is None or not 0.0 < tolerance < 1.0:
+            raise FmmError(9, "give a tolerance in (0, 1) or a degree")
+        self._k = k
+        self._tol = tolerance
+        self._degree = degree
+        self._gradients = True
+        self._max_level = 16
+        self._leaf_points = 256
+        self._precision = "f64"
+        self._device = 0
+        self._near = None
+
+    def gradients(self, on: bool):
+        self._gradients = bool(on)
+        return self
+
+    def max_level(self, level: int):
+        self._max_level = int(level)
+        return self
+
+    def max_points_per_leaf(self, n: int):
+        self._leaf_points = int(n)
+        return self
+
+    def precision(self, precision: str):
+        if precision not in PRECISIONS:
+            raise FmmError(3, f"unknown precision {precision!r}")
+        self._precision = precision
+        return self
+
+    def device(self, ordinal: int):
+        self._device = int(ordinal)
+        return self
+
+    def near_wx(self, rule):
+        """W and X pairs to P2P (WxNear, fmm-exec/src/helmholtz/fmm.rs:370-400): None or
+        "off"; (w, x) crossovers in points; or "device", this GPU path's cost model: an M2P
+        (P2L) costs about (p+1)^2 pair interactions per point, so w = x = (p+1)^2 at the
+        FMM's degree (resolved at build)."""
+        if rule is None or rule == "off":
+            self._near = None
+        elif rule == "device":
+            self._near = "device"
+        else:
+            self._near = (float(rule[0]), float(rule[1]))
+        return self
+
+    def degree_for(self, plan: Plan) -> int:
+        """One degree for every level: the rule's largest over the levels that translate
+        (2 and deeper; levels 0 and 1 get degree 0 under the rule, degree.rs:55-58)."""
+        if self._degree is not None:
+            return int(self._degree)
+        _, side = plan.domain()
+        degs = [helmholtz_degree(self._k, side / 2 ** (l + 1), self._tol) for l in range(2, max(plan.nlevels, 3))]
+        return max(degs)
+
+    def build(self, sources, targets=None) -> HelmholtzFmm:
+        near = self._near
+        if near == "device":
+            # the degree depends only on the domain and the levels, which the near rule
+            # leaves unchanged: resolve it on the plan without the rule
+            p = self.degree_for(Plan(sources, targets, self._leaf_points, self._max_level, None))
+            near = (float((p + 1) ** 2),) * 2
+            plan = Plan(sources, targets, self._leaf_points, self._max_level, near)
+        else:
+            plan = Plan(sources, targets, self._leaf_points, self._max_level, near)
+            p = self.degree_for(plan)
+        op = HelmholtzOperator(plan, p, self._k, self._precision, self._gradients, self._device)
+        return HelmholtzFmm(plan, op)

@@ -37,6 +37,10 @@ EXPORTS = [
     "fmmh_l2p", "fmmh_m2p", "fmmh_p2p", "fmmh_evaluate", "fmmh_read_output", "fmmh_sync",
     "fmmh_kind_timing", "fmmh_kind_times", "fmmh_read_expansions", "fmmh_vjp", "fmmh_forest_build",
     "fmmh_evaluate_graph", "fmmh_direct_sum", "fmmh_m2l_rotation_table",
+    "fmmh_helmholtz_degree", "fmmh_helmholtz_create", "fmmh_helmholtz_free", "fmmh_helmholtz_stats",
+    "fmmh_helmholtz_load_charges", "fmmh_helmholtz_reset", "fmmh_helmholtz_evaluate",
+    "fmmh_helmholtz_read_output", "fmmh_helmholtz_read_expansions", "fmmh_helmholtz_kind_timing",
+    "fmmh_helmholtz_kind_times", "fmmh_helmholtz_table", "fmmh_helmholtz_direct_sum",
 ]
 
 
@@ -57,6 +61,11 @@ class PlanOptions(C.Structure):
 class OperatorOptions(C.Structure):
     _fields_ = [("p", C.c_int32), ("precision", C.c_int32), ("gradients", C.c_int32), ("device", C.c_int32),
                 ("m2l", C.c_int32)]
+
+
+class HelmholtzOptions(C.Structure):
+    _fields_ = [("p", C.c_int32), ("k_re", C.c_double), ("k_im", C.c_double), ("precision", C.c_int32),
+                ("gradients", C.c_int32), ("device", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -130,6 +139,20 @@ def lib():
     L.fmmh_kind_times.argtypes = [vp, vp, vp]
     L.fmmh_vjp.argtypes = [vp, vp, vp, vp, i32, vp, vp]
     L.fmmh_forest_build.argtypes = [vp, vp, i32, C.POINTER(PlanOptions), C.POINTER(vp)]
+    L.fmmh_helmholtz_degree.argtypes = [dbl, dbl, dbl]
+    L.fmmh_helmholtz_create.argtypes = [vp, C.POINTER(HelmholtzOptions), C.POINTER(vp)]
+    L.fmmh_helmholtz_free.argtypes = [vp]
+    L.fmmh_helmholtz_free.restype = None
+    L.fmmh_helmholtz_stats.argtypes = [vp, C.POINTER(Stats)]
+    L.fmmh_helmholtz_load_charges.argtypes = [vp, vp, i32]
+    L.fmmh_helmholtz_reset.argtypes = [vp]
+    L.fmmh_helmholtz_evaluate.argtypes = [vp]
+    L.fmmh_helmholtz_read_output.argtypes = [vp, vp, vp, i32]
+    L.fmmh_helmholtz_read_expansions.argtypes = [vp, i32, i32, vp]
+    L.fmmh_helmholtz_kind_timing.argtypes = [vp, i32]
+    L.fmmh_helmholtz_kind_times.argtypes = [vp, vp, vp]
+    L.fmmh_helmholtz_table.argtypes = [i32, dbl, dbl, i32, i32, vp]
+    L.fmmh_helmholtz_direct_sum.argtypes = [i32, dbl, dbl, vp, vp, i64, vp, i64, vp, vp, i32]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is C.c_int:  # default

@@ -4,8 +4,11 @@
 #include <exception>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "../../include/fmm_amd.h"
+#include "hoperator.hpp"
+#include "htables.hpp"
 #include "operator.hpp"
 #include "plan.hpp"
 
@@ -17,6 +20,13 @@ struct fmmh_operator {
   std::unique_ptr<fmmh::DeviceOperator> op;
   int64_t nlevels = 0;
   int64_t nleaves = 0;
+};
+
+struct fmmh_helmholtz {
+  std::unique_ptr<fmmh::HelmholtzOperator> op;
+  int64_t nlevels = 0;
+  int64_t nleaves = 0;
+  int64_t nboxes = 0;
 };
 
 namespace {
@@ -51,7 +61,7 @@ extern "C" {
 
 const char *fmmh_last_error(void) { return g_last_error.c_str(); }
 
-int32_t fmmh_abi_version(void) { return 3; }
+int32_t fmmh_abi_version(void) { return 4; }
 
 int32_t fmmh_plan_build(const double *sources, int64_t nsources, const double *targets, int64_t ntargets,
                         const fmmh_plan_options *options, fmmh_plan **out) {
@@ -360,6 +370,132 @@ int32_t fmmh_kind_times(fmmh_operator *op, double *ms, int64_t *launches) {
   return guard([&] {
     need(op != nullptr, "fmmh_kind_times: NULL operator");
     op->op->kind_times(ms, launches);
+  });
+}
+
+int32_t fmmh_helmholtz_degree(double k_re, double radius, double eps) {
+  if (!(eps > 0.0 && eps < 1.0) || !(radius > 0.0) || !(k_re >= 0.0)) return -1;
+  const double L = std::log10(1.0 / eps), x = k_re * std::sqrt(3.0) * radius;
+  const double low = std::ceil(0.30 * L * L + 0.25 * L + 2.60);
+  const double high = std::ceil(x + 0.80 * std::pow(std::log(1.0 / eps), 2.0 / 3.0) * std::cbrt(x));
+  return static_cast<int32_t>(low > high ? low : high);
+}
+
+int32_t fmmh_helmholtz_create(const fmmh_plan *plan, const fmmh_helmholtz_options *options, fmmh_helmholtz **out) {
+  return guard([&] {
+    need(plan && options && out, "fmmh_helmholtz_create: NULL argument");
+    fmmh::HelmholtzOptions o;
+    o.p = options->p;
+    o.kre = options->k_re;
+    o.kim = options->k_im;
+    o.precision = options->precision;
+    o.gradients = options->gradients;
+    o.device = options->device;
+    auto h = std::make_unique<fmmh_helmholtz>();
+    h->op = std::make_unique<fmmh::HelmholtzOperator>(plan->plan, o);
+    h->nlevels = plan->plan.nlevels;
+    h->nleaves = plan->plan.nleaves();
+    for (int l = 0; l < plan->plan.nlevels; ++l) h->nboxes += plan->plan.nboxes(l);
+    *out = h.release();
+  });
+}
+
+void fmmh_helmholtz_free(fmmh_helmholtz *op) { delete op; }
+
+int32_t fmmh_helmholtz_stats(const fmmh_helmholtz *op, fmmh_stats *out) {
+  return guard([&] {
+    need(op && out, "fmmh_helmholtz_stats: NULL argument");
+    out->nlevels = op->op->nlevels();
+    out->nleaves = op->nleaves;
+    out->nboxes = op->nboxes;
+    out->p2p_pairs = op->op->p2p_pairs();
+    out->m2l_pairs = op->op->m2l_pairs();
+    out->m2l_slots = 0;
+    out->device_bytes = op->op->device_bytes();
+  });
+}
+
+int32_t fmmh_helmholtz_load_charges(fmmh_helmholtz *op, const double *charges, int32_t on_device) {
+  return guard([&] {
+    need(op && charges, "fmmh_helmholtz_load_charges: NULL argument");
+    op->op->load_charges(charges, on_device != 0);
+  });
+}
+
+int32_t fmmh_helmholtz_reset(fmmh_helmholtz *op) {
+  return guard([&] {
+    need(op != nullptr, "fmmh_helmholtz_reset: NULL operator");
+    op->op->reset();
+  });
+}
+
+int32_t fmmh_helmholtz_evaluate(fmmh_helmholtz *op) {
+  return guard([&] {
+    need(op != nullptr, "fmmh_helmholtz_evaluate: NULL operator");
+    op->op->evaluate();
+  });
+}
+
+int32_t fmmh_helmholtz_read_output(fmmh_helmholtz *op, double *phi, double *grad, int32_t on_device) {
+  return guard([&] {
+    need(op && phi, "fmmh_helmholtz_read_output: NULL argument");
+    op->op->read_output(phi, grad, on_device != 0);
+  });
+}
+
+int32_t fmmh_helmholtz_read_expansions(fmmh_helmholtz *op, int32_t locals, int32_t level, double *out) {
+  return guard([&] {
+    need(op && out, "fmmh_helmholtz_read_expansions: NULL argument");
+    need(level >= 0 && level < op->nlevels, "fmmh_helmholtz_read_expansions: level out of range");
+    op->op->read_expansions(locals != 0, level, out);
+  });
+}
+
+int32_t fmmh_helmholtz_kind_timing(fmmh_helmholtz *op, int32_t enable) {
+  return guard([&] {
+    need(op != nullptr, "fmmh_helmholtz_kind_timing: NULL operator");
+    op->op->set_timing(enable != 0);
+  });
+}
+
+int32_t fmmh_helmholtz_kind_times(fmmh_helmholtz *op, double *ms, int64_t *launches) {
+  return guard([&] {
+    need(op != nullptr, "fmmh_helmholtz_kind_times: NULL operator");
+    op->op->kind_times(ms, launches);
+  });
+}
+
+int32_t fmmh_helmholtz_table(int32_t p, double kh_re, double kh_im, int32_t kind, int32_t index, double *out) {
+  return guard([&] {
+    need(out != nullptr, "fmmh_helmholtz_table: NULL out");
+    need(p >= 0 && p <= 30, "fmmh_helmholtz_table: p must lie in 0..30");
+    need(kh_re != 0.0 || kh_im != 0.0, "fmmh_helmholtz_table: kh must be nonzero");
+    const fmmh::cplx kh(kh_re, kh_im);
+    std::vector<double> t;
+    const size_t sz = 2 * size_t(fmmh::ncoef(p)) * fmmh::ncoef(p);
+    if (kind == 0 || kind == 1) {
+      need(index >= 0 && index < 8, "fmmh_helmholtz_table: octant must lie in 0..7");
+      if (kind == 0)
+        fmmh::h_m2m_tables(p, kh, t);
+      else
+        fmmh::h_l2l_tables(p, kh, t);
+      std::memcpy(out, t.data() + size_t(index) * sz, sz * sizeof(double));
+    } else {
+      need(kind == 2 && index >= 0 && index < 316, "fmmh_helmholtz_table: kind 0..2, offset index 0..315");
+      fmmh::h_m2l_tables(p, kh, {uint16_t(index)}, t);
+      std::memcpy(out, t.data(), sz * sizeof(double));
+    }
+  });
+}
+
+int32_t fmmh_helmholtz_direct_sum(int32_t device, double k_re, double k_im, const double *sources,
+                                  const double *charges, int64_t nsources, const double *targets, int64_t ntargets,
+                                  double *phi, double *grad, int32_t on_device) {
+  return guard([&] {
+    need(nsources == 0 || (sources && charges), "fmmh_helmholtz_direct_sum: NULL sources");
+    need(ntargets == 0 || (targets && phi), "fmmh_helmholtz_direct_sum: NULL targets or output");
+    fmmh::helmholtz_direct_sum(device, k_re, k_im, sources, charges, nsources, targets, ntargets, phi, grad,
+                               on_device != 0);
   });
 }
 

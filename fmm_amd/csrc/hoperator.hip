@@ -1,0 +1,616 @@
+// hoperator.hip -- the Helmholtz device operator (see hoperator.hpp).
+//
+// Box b holds three coefficient planes per expansion (vector ids 3b, 3b + 1, 3b + 2: real
+// part, imaginary part, negated imaginary part).  A complex table T = Tr + i Ti is stored as
+// the real tables Tr and Ti, and T x becomes, per output box, two columns of the grouped
+// contraction with two groups each:
+//   real column:  Tr <- plane 0 of the source,  Ti <- plane 2 (-Im)
+//   imag column:  Tr <- plane 1 (Im),           Ti <- plane 0
+// so every column of every group holds a source (no sign per column, no empty slots).  The
+// negated planes are written by a small kernel once a level's expansions are complete.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <map>
+
+#include "hkernels.hpp"
+#include "hoperator.hpp"
+#include "htables.hpp"
+#include "kernels.hpp"
+#include "schedule.hpp"
+
+namespace fmmh {
+
+namespace {
+
+// rank of offset d among the M2L groups of a target of octant o: (parent-level neighbour,
+// source octant), as the Laplace operator orders them for L2 reuse
+uint16_t m2l_rank(int oc, int d) {
+  const int ob[3] = {(oc >> 2) & 1, (oc >> 1) & 1, oc & 1};
+  int dv[3];
+  offset_of_index(d, dv);
+  int D = 0, op = 0;
+  for (int a = 0; a < 3; ++a) {
+    const int s = ob[a] + dv[a];
+    const int bit = ((s % 2) + 2) % 2;
+    D = 3 * D + ((s - bit) / 2 + 1);
+    op = 2 * op + bit;
+  }
+  return static_cast<uint16_t>(8 * D + op);
+}
+
+bool all_zero(const double *v, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (v[i] != 0.0) return false;
+  return true;
+}
+
+}  // namespace
+
+HelmholtzOperator::HelmholtzOperator(const Plan &plan, const HelmholtzOptions &options) : opt_(options) {
+  require(opt_.p >= 0 && opt_.p <= dev::kMaxDegree, "the device path supports 0 <= p <= 15", kUnsupportedLayout);
+  require(opt_.precision == kPlainF64 || opt_.precision == kMixed, "unknown precision", kUnsupportedPrecision);
+  require(std::isfinite(opt_.kre) && std::isfinite(opt_.kim) && (opt_.kre != 0.0 || opt_.kim != 0.0),
+          "Helmholtz: the wavenumber must be finite and nonzero");
+  require(opt_.kre >= 0.0 && opt_.kim >= 0.0, "Helmholtz: Re k >= 0 and Im k >= 0 (outgoing, not growing)");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(kNoDevice, "no HIP device");
+  require(opt_.device >= 0 && opt_.device < ndev, "device ordinal out of range", kNoDevice);
+  hip_check(hipSetDevice(opt_.device), "hipSetDevice");
+  hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+
+  const int p = opt_.p;
+  nlevels_ = plan.nlevels;
+  nc_ = ncoef(p);
+  ns_ = plan.ns;
+  nt_ = plan.nt;
+  nleaves_ = plan.nleaves();
+  box_base_.assign(nlevels_ + 1, 0);
+  for (int l = 0; l < nlevels_; ++l) box_base_[l + 1] = box_base_[l] + plan.nboxes(l);
+  nbox_total_ = box_base_[nlevels_];
+  require(3 * nbox_total_ < size_t(kNone), "Helmholtz: too many boxes", kTooLarge);
+  auto gid = [&](int level, uint32_t b) { return box_base_[level] + b; };
+  // one domain: every leaf's radius is its level's (a forest's clouds are not one domain)
+  for (int l = 0; l < nlevels_; ++l)
+    for (uint32_t lf = plan.leaf_off[l]; lf < plan.leaf_off[l + 1]; ++lf)
+      require(plan.leaf_radius[lf] == plan.radius(l), "Helmholtz: a plan of one domain is required",
+              kUnsupportedLayout);
+
+  // level constants in each level's units
+  const cplx k(opt_.kre, opt_.kim);
+  std::vector<cplx> kh(nlevels_);
+  std::vector<double> hlev(4 * size_t(std::max(nlevels_, 1)), 0.0);
+  for (int l = 0; l < nlevels_; ++l) {
+    kh[l] = k * plan.radius(l);
+    hlev[4 * l] = kh[l].real();
+    hlev[4 * l + 1] = kh[l].imag();
+    hlev[4 * l + 2] = h_scale(kh[l]);
+    hlev[4 * l + 3] = h_beta(kh[l]);
+  }
+  hlev_.upload(hlev);
+  std::vector<double> norm;
+  h_norm_factors(p, norm);
+  norm_.upload(norm);
+
+  // cells, points, near rows (as the Laplace operator)
+  std::vector<uint32_t> leaf_cell(4 * size_t(nleaves_)), box_cell(4 * nbox_total_);
+  for (uint32_t i = 0; i < nleaves_; ++i) {
+    const Cell c = decode(plan.leaf_key[i]);
+    leaf_cell[4 * i] = c.level;
+    leaf_cell[4 * i + 1] = uint32_t(c.i);
+    leaf_cell[4 * i + 2] = uint32_t(c.j);
+    leaf_cell[4 * i + 3] = uint32_t(c.k);
+  }
+  for (int l = 0; l < nlevels_; ++l)
+    for (uint32_t b = 0; b < plan.nboxes(l); ++b) {
+      const Cell c = decode(plan.keys[l][b]);
+      const size_t g = gid(l, b);
+      box_cell[4 * g] = c.level;
+      box_cell[4 * g + 1] = uint32_t(c.i);
+      box_cell[4 * g + 2] = uint32_t(c.j);
+      box_cell[4 * g + 3] = uint32_t(c.k);
+    }
+  leaf_cell_.upload(leaf_cell);
+  box_cell_.upload(box_cell);
+  leaf_r_.upload(plan.leaf_radius);
+  src_start_.upload(plan.src_start);
+  tgt_start_.upload(plan.tgt_start);
+  src_order_.upload(plan.src_order);
+  tgt_order_.upload(plan.tgt_order);
+  std::vector<uint32_t> near_off(nleaves_ + 1, 0), near_ent;
+  for (int l = 0; l < nlevels_; ++l) {
+    const Csr &N = plan.near[l];
+    for (uint32_t r = 0; r < N.rows(); ++r) {
+      const uint32_t lf = plan.leaf_off[l] + r;
+      near_off[lf + 1] = N.off[r + 1] - N.off[r];
+      for (uint32_t e = N.off[r]; e < N.off[r + 1]; ++e) {
+        near_ent.push_back(N.ent[e]);
+        const uint32_t sl = N.ent[e];
+        p2p_pairs_ += uint64_t(plan.src_start[sl + 1] - plan.src_start[sl]) *
+                      (plan.tgt_start[lf + 1] - plan.tgt_start[lf]);
+      }
+    }
+  }
+  for (uint32_t i = 0; i < nleaves_; ++i) near_off[i + 1] += near_off[i];
+  near_off_.upload(near_off);
+  near_ent_.upload(near_ent);
+  {
+    std::vector<double> a(ns_), b(ns_), c(ns_);
+    for (int64_t i = 0; i < ns_; ++i) {
+      a[i] = plan.us[3 * i];
+      b[i] = plan.us[3 * i + 1];
+      c[i] = plan.us[3 * i + 2];
+    }
+    sx_.upload(a);
+    sy_.upload(b);
+    sz_.upload(c);
+    a.resize(nt_);
+    b.resize(nt_);
+    c.resize(nt_);
+    for (int64_t i = 0; i < nt_; ++i) {
+      a[i] = plan.ut[3 * i];
+      b[i] = plan.ut[3 * i + 1];
+      c[i] = plan.ut[3 * i + 2];
+    }
+    tx_.upload(a);
+    ty_.upload(b);
+    tz_.upload(c);
+  }
+  qr_.alloc(ns_);
+  qi_.alloc(ns_);
+  q_in_.alloc(2 * ns_);
+  mult_.alloc(3 * nbox_total_ * nc_);
+  loc_.alloc(3 * nbox_total_ * nc_);
+  phi_hat_.alloc(2 * nt_);
+  grad_hat_.alloc(opt_.gradients ? 6 * nt_ : 0);
+  phi_.alloc(2 * nt_);
+  grad_.alloc(opt_.gradients ? 6 * nt_ : 0);
+
+  // leaf-kernel rows
+  {
+    std::vector<uint32_t> id, cell, off{0}, ent;
+    for (int l = 0; l < nlevels_; ++l)
+      for (uint32_t lf = plan.leaf_off[l]; lf < plan.leaf_off[l + 1]; ++lf) {
+        if (plan.src_start[lf + 1] == plan.src_start[lf]) continue;
+        id.push_back(gid(l, plan.leaf_box[lf]));
+        cell.insert(cell.end(), {leaf_cell[4 * lf], leaf_cell[4 * lf + 1], leaf_cell[4 * lf + 2], leaf_cell[4 * lf + 3]});
+        ent.push_back(lf);
+        off.push_back(static_cast<uint32_t>(ent.size()));
+      }
+    upload_rows(p2m_rows_, id, cell, off, ent);
+  }
+  {
+    std::vector<uint32_t> id, cell, off{0}, ent;
+    for (int l = 0; l < nlevels_; ++l)
+      for (uint32_t lf = plan.leaf_off[l]; lf < plan.leaf_off[l + 1]; ++lf) {
+        if (plan.tgt_start[lf + 1] == plan.tgt_start[lf]) continue;
+        id.push_back(lf);
+        ent.push_back(gid(l, plan.leaf_box[lf]));
+        off.push_back(static_cast<uint32_t>(ent.size()));
+      }
+    upload_rows(l2p_rows_, id, cell, off, ent);
+  }
+  {
+    std::vector<uint32_t> id, cell, off{0}, ent;
+    for (int l = 0; l < nlevels_; ++l) {
+      const Csr &W = plan.w[l];
+      for (uint32_t r = 0; r < W.rows(); ++r) {
+        const uint32_t lf = plan.leaf_off[l] + r;
+        if (W.off[r + 1] == W.off[r] || plan.tgt_start[lf + 1] == plan.tgt_start[lf]) continue;
+        id.push_back(lf);
+        for (uint32_t e = W.off[r]; e < W.off[r + 1]; ++e) ent.push_back(gid(l + 1, W.ent[e]));
+        off.push_back(static_cast<uint32_t>(ent.size()));
+      }
+    }
+    upload_rows(m2p_rows_, id, cell, off, ent);
+  }
+  {
+    std::vector<uint32_t> id, cell, off{0}, ent;
+    for (int l = 1; l < nlevels_; ++l) {
+      const Csr &X = plan.x[l];
+      for (uint32_t b = 0; b < X.rows(); ++b) {
+        if (X.off[b + 1] == X.off[b]) continue;
+        const size_t g = gid(l, b);
+        id.push_back(uint32_t(g));
+        cell.insert(cell.end(), {box_cell[4 * g], box_cell[4 * g + 1], box_cell[4 * g + 2], box_cell[4 * g + 3]});
+        for (uint32_t e = X.off[b]; e < X.off[b + 1]; ++e) ent.push_back(X.ent[e]);
+        off.push_back(static_cast<uint32_t>(ent.size()));
+      }
+    }
+    upload_rows(p2l_rows_, id, cell, off, ent);
+  }
+
+  // tables and schedules
+  const size_t n2 = size_t(nc_) * nc_, sw = dev::swizzled_table_size(p);
+  auto swizzle_planes = [&](const std::vector<double> &t, size_t count, std::vector<double> &out) {
+    const size_t base = out.size();
+    out.resize(base + 2 * count * sw);
+    for (size_t i = 0; i < 2 * count; ++i) dev::swizzle_table(p, t.data() + i * n2, out.data() + base + i * sw);
+  };
+  // M2M and L2L: per parent level l, 8 octants x (re, im), ids 16 l + 2 o + plane
+  {
+    std::vector<double> m2m_sw, l2l_sw;
+    m2m_.resize(nlevels_);
+    l2l_.resize(nlevels_);
+    m2m_complex_ = false;
+    for (int l = 0; l + 1 < nlevels_; ++l) {
+      std::vector<double> tm, tl;
+      h_m2m_tables(p, kh[l], tm);
+      h_l2l_tables(p, kh[l], tl);
+      for (int o = 0; o < 8; ++o)
+        if (!all_zero(tm.data() + (2 * o + 1) * n2, n2) || !all_zero(tl.data() + (2 * o + 1) * n2, n2))
+          m2m_complex_ = true;
+      swizzle_planes(tm, 8, m2m_sw);
+      swizzle_planes(tl, 8, l2l_sw);
+    }
+    if (m2m_sw.empty()) m2m_sw.assign(sw, 0.0);
+    if (l2l_sw.empty()) l2l_sw.assign(sw, 0.0);
+    t_m2m_.upload(m2m_sw);
+    t_l2l_.upload(l2l_sw);
+    for (int l = 0; l + 1 < nlevels_; ++l) {
+      const int width = 2 * plan.nboxes(l) < 64u * dev::COLS ? 16 : dev::COLS;
+      // M2M into the parents of level l
+      ScheduleBuilder sb(width);
+      std::vector<uint32_t> cols;
+      std::vector<std::vector<std::pair<uint16_t, uint32_t>>> lists;
+      for (uint32_t b = 0; b < plan.nboxes(l); ++b) {
+        const uint32_t fc = plan.first_child[l][b];
+        if (fc == kNone) continue;
+        std::vector<std::pair<uint16_t, uint32_t>> re, im;
+        for (int o = 0; o < 8; ++o) {
+          const uint32_t c = gid(l + 1, fc + o);
+          const uint16_t tr = uint16_t(16 * l + 2 * o), ti = uint16_t(tr + 1);
+          re.push_back({tr, 3 * c});
+          im.push_back({tr, 3 * c + 1});
+          if (m2m_complex_) {
+            re.push_back({ti, 3 * c + 2});
+            im.push_back({ti, 3 * c});
+          }
+        }
+        cols.push_back(3 * gid(l, b));
+        lists.push_back(std::move(re));
+        cols.push_back(3 * gid(l, b) + 1);
+        lists.push_back(std::move(im));
+        if (int(cols.size()) == width) {
+          sb.add_task(cols, lists);
+          cols.clear();
+          lists.clear();
+        }
+      }
+      if (!cols.empty()) sb.add_task(cols, lists);
+      sb.finish(m2m_[l]);
+    }
+    for (int l = 1; l < nlevels_; ++l) {
+      // L2L into the boxes of level l, by octant (a task shares its tables)
+      const int width = 2 * plan.nboxes(l) < 64u * dev::COLS ? 16 : dev::COLS;
+      ScheduleBuilder sb(width);
+      for (int o = 0; o < 8; ++o) {
+        std::vector<uint32_t> cols;
+        std::vector<std::vector<std::pair<uint16_t, uint32_t>>> lists;
+        const uint16_t tr = uint16_t(16 * (l - 1) + 2 * o), ti = uint16_t(tr + 1);
+        for (uint32_t b = 0; b < plan.nboxes(l); ++b) {
+          if (octant_of(plan.keys[l][b]) != o) continue;
+          const uint32_t P = gid(l - 1, plan.parent[l][b]);
+          std::vector<std::pair<uint16_t, uint32_t>> re{{tr, 3 * P}}, im{{tr, 3 * P + 1}};
+          if (m2m_complex_) {
+            re.push_back({ti, 3 * P + 2});
+            im.push_back({ti, 3 * P});
+          }
+          cols.push_back(3 * gid(l, b));
+          lists.push_back(std::move(re));
+          cols.push_back(3 * gid(l, b) + 1);
+          lists.push_back(std::move(im));
+          if (int(cols.size()) == width) {
+            sb.add_task(cols, lists);
+            cols.clear();
+            lists.clear();
+          }
+        }
+        if (!cols.empty()) sb.add_task(cols, lists);
+      }
+      sb.finish(l2l_[l]);
+    }
+  }
+  // M2L: per level the offsets present, ids base_l + 2 j + plane; tiles of 64 targets of
+  // one octant class, classes interleaved, groups by (parent-level neighbour, octant)
+  {
+    std::vector<double> m2l_sw;
+    std::vector<uint16_t> m2l16;
+    const size_t sw16 = dev::swizzled_table_size_bf16(p);
+    ScheduleBuilder all;
+    uint32_t id_base = 0;
+    for (int l = 2; l < nlevels_; ++l) {
+      const Csr &V = plan.v[l];
+      if (V.size() == 0) continue;
+      m2l_pairs_ += V.size();
+      std::vector<uint16_t> used(V.aux.begin(), V.aux.end());
+      std::sort(used.begin(), used.end());
+      used.erase(std::unique(used.begin(), used.end()), used.end());
+      require(id_base + 2 * used.size() < 65536u, "Helmholtz: too many M2L tables", kTooLarge);
+      std::vector<int> slot(316, -1);
+      for (size_t j = 0; j < used.size(); ++j) slot[used[j]] = int(j);
+      std::vector<double> t;
+      h_m2l_tables(p, kh[l], used, t);
+      if (opt_.precision == kMixed) {
+        const size_t base = m2l16.size();
+        m2l16.resize(base + 2 * used.size() * sw16);
+        for (size_t i = 0; i < 2 * used.size(); ++i)
+          dev::swizzle_table_bf16(p, t.data() + i * n2, m2l16.data() + base + i * sw16);
+      } else {
+        swizzle_planes(t, used.size(), m2l_sw);
+      }
+      // ranks of this level's table ids per target octant
+      const uint32_t nid = id_base + 2 * uint32_t(used.size());
+      std::vector<std::vector<uint16_t>> rank(8, std::vector<uint16_t>(nid, 0));
+      for (int o = 0; o < 8; ++o)
+        for (size_t j = 0; j < used.size(); ++j) {
+          rank[o][id_base + 2 * j] = uint16_t(2 * m2l_rank(o, used[j]));
+          rank[o][id_base + 2 * j + 1] = uint16_t(2 * m2l_rank(o, used[j]) + 1);
+        }
+      std::vector<std::vector<uint32_t>> cls_cols(8);
+      std::vector<std::vector<std::vector<std::pair<uint16_t, uint32_t>>>> cls_lists(8);
+      for (uint32_t b = 0; b < plan.nboxes(l); ++b) {
+        if (V.off[b + 1] == V.off[b]) continue;
+        const int o = octant_of(plan.keys[l][b]);
+        std::vector<std::pair<uint16_t, uint32_t>> re, im;
+        for (uint32_t e = V.off[b]; e < V.off[b + 1]; ++e) {
+          const uint32_t s = gid(l, V.ent[e]);
+          const uint16_t tr = uint16_t(id_base + 2 * slot[V.aux[e]]), ti = uint16_t(tr + 1);
+          re.push_back({tr, 3 * s});
+          re.push_back({ti, 3 * s + 2});
+          im.push_back({tr, 3 * s + 1});
+          im.push_back({ti, 3 * s});
+        }
+        cls_cols[o].push_back(3 * gid(l, b));
+        cls_lists[o].push_back(std::move(re));
+        cls_cols[o].push_back(3 * gid(l, b) + 1);
+        cls_lists[o].push_back(std::move(im));
+      }
+      const size_t C = dev::COLS;
+      size_t most = 0;
+      for (int o = 0; o < 8; ++o) most = std::max(most, cls_cols[o].size());
+      for (size_t at = 0; at < most; at += C)
+        for (int o = 0; o < 8; ++o) {
+          if (at >= cls_cols[o].size()) continue;
+          const size_t end = std::min(cls_cols[o].size(), at + C);
+          std::vector<uint32_t> cols(cls_cols[o].begin() + at, cls_cols[o].begin() + end);
+          std::vector<std::vector<std::pair<uint16_t, uint32_t>>> lists(cls_lists[o].begin() + at,
+                                                                          cls_lists[o].begin() + end);
+          all.add_task(cols, lists, rank[o].data());
+        }
+      id_base = nid;
+    }
+    all.finish(m2l_);
+    if (opt_.precision == kMixed) {
+      if (m2l16.empty()) m2l16.assign(sw16, 0);
+      t_m2l16_.upload(m2l16);
+      mult16_.alloc(3 * nbox_total_ * dev::split_expansion_size(p));
+    } else {
+      if (m2l_sw.empty()) m2l_sw.assign(sw, 0.0);
+      t_m2l_.upload(m2l_sw);
+    }
+  }
+  hip_check(hipDeviceSynchronize(), "Helmholtz operator build");
+}
+
+HelmholtzOperator::~HelmholtzOperator() {
+  for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+size_t HelmholtzOperator::device_bytes() const {
+  return mult_.bytes() + loc_.bytes() + t_m2m_.bytes() + t_l2l_.bytes() + t_m2l_.bytes() + t_m2l16_.bytes() +
+         mult16_.bytes() + sx_.bytes() * 8;
+}
+
+void HelmholtzOperator::set_timing(bool enable) {
+  sync();
+  timing_ = enable;
+  if (enable && events_.empty()) {
+    events_.resize(512);
+    for (auto &e : events_) hip_check(hipEventCreate(&e), "hipEventCreate");
+  }
+  marks_.clear();
+  events_used_ = 0;
+  for (int k = 0; k < 16; ++k) {
+    kind_ms_[k] = 0.0;
+    kind_launches_[k] = 0;
+  }
+}
+
+template <class F>
+void HelmholtzOperator::timed(int kind, F &&f) {
+  if (!timing_ || events_used_ + 2 > events_.size()) {
+    f();
+    return;
+  }
+  Mark m{kind, events_[events_used_], events_[events_used_ + 1]};
+  events_used_ += 2;
+  hip_check(hipEventRecord(m.a, stream_), "hipEventRecord");
+  f();
+  hip_check(hipEventRecord(m.b, stream_), "hipEventRecord");
+  marks_.push_back(m);
+}
+
+void HelmholtzOperator::collect_timings() {
+  for (const Mark &m : marks_) {
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, m.a, m.b), "hipEventElapsedTime");
+    kind_ms_[m.kind] += ms;
+    kind_launches_[m.kind] += 1;
+  }
+  marks_.clear();
+  events_used_ = 0;
+}
+
+void HelmholtzOperator::kind_times(double *ms, int64_t *launches) {
+  sync();
+  for (int k = 0; k < kNumKinds; ++k) {
+    if (ms) ms[k] = kind_ms_[k];
+    if (launches) launches[k] = kind_launches_[k];
+  }
+}
+
+void HelmholtzOperator::load_charges(const double *charges, bool on_device) {
+  if (ns_ == 0) return;
+  const double *src = charges;
+  if (!on_device) {
+    hip_check(hipMemcpyAsync(q_in_.ptr, charges, 2 * ns_ * sizeof(double), hipMemcpyHostToDevice, stream_),
+              "charges");
+    src = q_in_.ptr;
+  }
+  dev::h_gather(stream_, src, src_order_.ptr, ns_, qr_.ptr, qi_.ptr);
+}
+
+void HelmholtzOperator::reset() {
+  dev::fill(stream_, mult_.ptr, mult_.count, 0.0);
+  dev::fill(stream_, loc_.ptr, loc_.count, 0.0);
+  dev::fill(stream_, phi_hat_.ptr, phi_hat_.count, 0.0);
+  dev::fill(stream_, grad_hat_.ptr, grad_hat_.count, 0.0);
+}
+
+void HelmholtzOperator::launch_p2p() {
+  const int damped = opt_.kim != 0.0;
+  if (opt_.precision == kMixed)
+    dev::h_p2p<float>(stream_, 0, nleaves_, opt_.gradients, damped, near_off_.ptr, near_ent_.ptr, leaf_cell_.ptr,
+                      src_start_.ptr, tgt_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, qr_.ptr, qi_.ptr, tx_.ptr, ty_.ptr,
+                      tz_.ptr, hlev_.ptr, phi_hat_.ptr, grad_hat_.ptr);
+  else
+    dev::h_p2p<double>(stream_, 0, nleaves_, opt_.gradients, damped, near_off_.ptr, near_ent_.ptr, leaf_cell_.ptr,
+                       src_start_.ptr, tgt_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, qr_.ptr, qi_.ptr, tx_.ptr,
+                       ty_.ptr, tz_.ptr, hlev_.ptr, phi_hat_.ptr, grad_hat_.ptr);
+}
+
+void HelmholtzOperator::evaluate() {
+  const int L = nlevels_ - 1, p = opt_.p;
+  const uint32_t none = kNone;
+  // upward: P2M of every leaf, then per level the negated planes and M2M into the parents
+  timed(0, [&] {
+    dev::h_expand(stream_, p, 0, p2m_rows_.rows, p2m_rows_.row_id.ptr, p2m_rows_.row_cell.ptr, p2m_rows_.row_off.ptr,
+                  p2m_rows_.row_ent.ptr, leaf_cell_.ptr, src_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, qr_.ptr, qi_.ptr,
+                  hlev_.ptr, norm_.ptr, mult_.ptr);
+  });
+  for (int l = L - 1; l >= 0; --l)
+    timed(1, [&] {
+      dev::h_negate(stream_, p, box_base_[l + 1], box_base_[l + 2], mult_.ptr);
+      const Schedule &s = m2m_[l];
+      if (s.ntasks)
+        dev::contract_f64(stream_, p, s.width, s.ntasks, s.task_group_off.ptr, s.group_table.ptr, s.group_src.ptr,
+                          s.col_out.ptr, t_m2m_.ptr, mult_.ptr, mult_.ptr, none, nullptr, 0);
+    });
+  // M2L of every level in one launch
+  if (m2l_.ntasks)
+    timed(3, [&] {
+      if (opt_.precision == kMixed) {
+        const size_t v0 = 3 * size_t(box_base_[2]), v1 = 3 * size_t(box_base_[L + 1]);
+        const size_t sz = dev::split_expansion_size(p);
+        dev::split_bf16(stream_, p, v1 - v0, mult_.ptr + v0 * nc_, mult16_.ptr + v0 * sz);
+        timed(kKindM2lContract, [&] {
+          dev::contract_bf16(stream_, p, m2l_.ntasks, m2l_.task_group_off.ptr, m2l_.group_table.ptr,
+                             m2l_.group_src.ptr, m2l_.col_out.ptr, t_m2l16_.ptr, mult16_.ptr, loc_.ptr, none,
+                             nullptr, kM2lXcdRun);
+        });
+      } else {
+        timed(kKindM2lContract, [&] {
+          dev::contract_f64(stream_, p, m2l_.width, m2l_.ntasks, m2l_.task_group_off.ptr, m2l_.group_table.ptr,
+                            m2l_.group_src.ptr, m2l_.col_out.ptr, t_m2l_.ptr, mult_.ptr, loc_.ptr, none, nullptr,
+                            kM2lXcdRun);
+        });
+      }
+    });
+  if (p2l_rows_.rows)
+    timed(4, [&] {
+      dev::h_expand(stream_, p, 1, p2l_rows_.rows, p2l_rows_.row_id.ptr, p2l_rows_.row_cell.ptr,
+                    p2l_rows_.row_off.ptr, p2l_rows_.row_ent.ptr, leaf_cell_.ptr, src_start_.ptr, sx_.ptr, sy_.ptr,
+                    sz_.ptr, qr_.ptr, qi_.ptr, hlev_.ptr, norm_.ptr, loc_.ptr);
+    });
+  // downward: L2L from the root
+  for (int l = 1; l <= L; ++l)
+    timed(2, [&] {
+      if (m2m_complex_) dev::h_negate(stream_, p, box_base_[l - 1], box_base_[l], loc_.ptr);
+      const Schedule &s = l2l_[l];
+      if (s.ntasks)
+        dev::contract_f64(stream_, p, s.width, s.ntasks, s.task_group_off.ptr, s.group_table.ptr, s.group_src.ptr,
+                          s.col_out.ptr, t_l2l_.ptr, loc_.ptr, loc_.ptr, none, nullptr, 0);
+    });
+  // leaves: L2P, M2P, P2P
+  timed(5, [&] {
+    dev::h_evaluate(stream_, p, 0, opt_.gradients, l2p_rows_.rows, l2p_rows_.row_id.ptr, l2p_rows_.row_off.ptr,
+                    l2p_rows_.row_ent.ptr, box_cell_.ptr, leaf_cell_.ptr, tgt_start_.ptr, tx_.ptr, ty_.ptr, tz_.ptr,
+                    hlev_.ptr, norm_.ptr, loc_.ptr, phi_hat_.ptr, grad_hat_.ptr);
+  });
+  if (m2p_rows_.rows)
+    timed(6, [&] {
+      dev::h_evaluate(stream_, p, 1, opt_.gradients, m2p_rows_.rows, m2p_rows_.row_id.ptr, m2p_rows_.row_off.ptr,
+                      m2p_rows_.row_ent.ptr, box_cell_.ptr, leaf_cell_.ptr, tgt_start_.ptr, tx_.ptr, ty_.ptr, tz_.ptr,
+                      hlev_.ptr, norm_.ptr, mult_.ptr, phi_hat_.ptr, grad_hat_.ptr);
+    });
+  timed(7, [&] { launch_p2p(); });
+}
+
+void HelmholtzOperator::read_output(double *phi, double *grad, bool on_device) {
+  const bool g = opt_.gradients && grad != nullptr;
+  dev::h_output(stream_, nleaves_, tgt_start_.ptr, leaf_r_.ptr, tgt_order_.ptr, phi_hat_.ptr, grad_hat_.ptr,
+                on_device ? phi : phi_.ptr, on_device ? grad : grad_.ptr, g);
+  if (!on_device) {
+    hip_check(hipMemcpyAsync(phi, phi_.ptr, 2 * nt_ * sizeof(double), hipMemcpyDeviceToHost, stream_), "phi");
+    if (g) hip_check(hipMemcpyAsync(grad, grad_.ptr, 6 * nt_ * sizeof(double), hipMemcpyDeviceToHost, stream_), "grad");
+  }
+  sync();
+}
+
+void HelmholtzOperator::read_expansions(bool locals, int level, double *out) {
+  require(level >= 0 && level < nlevels_, "read_expansions: level out of range");
+  sync();
+  const double *src = locals ? loc_.ptr : mult_.ptr;
+  for (uint32_t b = box_base_[level]; b < box_base_[level + 1]; ++b)
+    hip_check(hipMemcpy(out + size_t(b - box_base_[level]) * 2 * nc_, src + 3 * size_t(b) * nc_,
+                        2 * nc_ * sizeof(double), hipMemcpyDeviceToHost),
+              "read_expansions");
+}
+
+void HelmholtzOperator::sync() {
+  hip_check(hipStreamSynchronize(stream_), "evaluation");
+  hip_check(hipGetLastError(), "launch");
+  if (!marks_.empty()) collect_timings();
+}
+
+void helmholtz_direct_sum(int device, double kre, double kim, const double *sources, const double *charges,
+                          int64_t ns, const double *targets, int64_t nt, double *phi, double *grad, bool on_device) {
+  require(ns >= 0 && nt >= 0, "helmholtz_direct_sum: negative size");
+  if (nt == 0) return;
+  hip_check(hipSetDevice(device), "hipSetDevice");
+  hipStream_t s = nullptr;
+  hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+  const int nch = dev::h_direct_chunks(ns, nt);
+  DevBuf<double> part, in, out;
+  part.alloc(size_t(nch) * nt * 16);
+  const double *src = sources, *q = charges, *tgt = targets;
+  double *ph = phi, *gr = grad;
+  if (!on_device) {
+    in.alloc(5 * size_t(ns) + 3 * size_t(nt));
+    out.alloc(8 * size_t(nt));
+    hip_check(hipMemcpyAsync(in.ptr, sources, 3 * ns * sizeof(double), hipMemcpyHostToDevice, s), "sources");
+    hip_check(hipMemcpyAsync(in.ptr + 3 * ns, charges, 2 * ns * sizeof(double), hipMemcpyHostToDevice, s),
+              "charges");
+    hip_check(hipMemcpyAsync(in.ptr + 5 * ns, targets, 3 * nt * sizeof(double), hipMemcpyHostToDevice, s),
+              "targets");
+    src = in.ptr;
+    q = in.ptr + 3 * ns;
+    tgt = in.ptr + 5 * ns;
+    ph = out.ptr;
+    gr = out.ptr + 2 * nt;
+  }
+  dev::h_direct_sum(s, kre, kim, src, q, ns, tgt, nt, grad != nullptr, part.ptr, nch, ph, grad ? gr : nullptr);
+  if (!on_device) {
+    hip_check(hipMemcpyAsync(phi, ph, 2 * nt * sizeof(double), hipMemcpyDeviceToHost, s), "phi");
+    if (grad) hip_check(hipMemcpyAsync(grad, gr, 6 * nt * sizeof(double), hipMemcpyDeviceToHost, s), "grad");
+  }
+  const hipError_t e = hipStreamSynchronize(s);
+  (void)hipStreamDestroy(s);
+  hip_check(e, "helmholtz_direct_sum");
+}
+
+}  // namespace fmmh

@@ -1,0 +1,98 @@
+// hoperator.hpp -- the Helmholtz device operator: an FMM of e^{ikr}/(4 pi r) with complex
+// charges on one MI355X, over the same plan (tree, lists, leaf-scaled points) as the
+// Laplace operator.  The reference evaluates its Helmholtz FMM on the host only
+// (nd_fmm_exec::helmholtz, fmm-exec/src/helmholtz/fmm.rs:71; operator
+// fmm-exec/src/helmholtz/operator.rs:448-480); this operator keeps its evaluator's order
+// (P2M, M2M, M2L, P2L, L2L, L2P, M2P, P2P) and its stored forms, and runs every kind on the
+// GPU: leaf kernels in f64, the translations as dense complex tables per level on the MFMA
+// contraction, P2P in f64 (plain) or f32 with f64 sums (mixed, where M2L also runs in
+// bf16x3 on MFMA).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <vector>
+
+#include "operator.hpp"
+#include "plan.hpp"
+
+namespace fmmh {
+
+struct HelmholtzOptions {
+  int p = 8;
+  double kre = 1.0, kim = 0.0;  // wavenumber k in the caller's units: k != 0, Re k >= 0, Im k >= 0
+  int precision = kPlainF64;
+  int gradients = 1;
+  int device = 0;
+};
+
+class HelmholtzOperator {
+ public:
+  HelmholtzOperator(const Plan &plan, const HelmholtzOptions &options);
+  ~HelmholtzOperator();
+
+  // complex charges (re, im) per source in caller order, host or device pointer
+  void load_charges(const double *charges, bool on_device);
+  void reset();
+  void evaluate();
+  // phi[2 nt], grad[6 nt] (x re, x im, y re, ...) in caller order; the one sync
+  void read_output(double *phi, double *grad, bool on_device);
+  void sync();
+  // the stored coefficients of one level, box-major, real plane then imaginary plane
+  void read_expansions(bool locals, int level, double *out);
+  void set_timing(bool enable);
+  void kind_times(double *ms, int64_t *launches);
+
+  int nlevels() const { return nlevels_; }
+  uint64_t p2p_pairs() const { return p2p_pairs_; }
+  uint64_t m2l_pairs() const { return m2l_pairs_; }
+  size_t device_bytes() const;
+
+ private:
+  template <class F>
+  void timed(int kind, F &&f);
+  void collect_timings();
+  void launch_p2p();
+
+  HelmholtzOptions opt_;
+  int nlevels_ = 0, nc_ = 0;
+  bool m2m_complex_ = true;  // M2M / L2L tables have an imaginary part (not for real k)
+  hipStream_t stream_ = nullptr;
+  int64_t ns_ = 0, nt_ = 0;
+  uint32_t nleaves_ = 0;
+  size_t nbox_total_ = 0;
+  std::vector<uint32_t> box_base_;
+  uint64_t p2p_pairs_ = 0, m2l_pairs_ = 0;
+
+  DevBuf<uint32_t> leaf_cell_, box_cell_, src_start_, tgt_start_, src_order_, tgt_order_;
+  DevBuf<uint32_t> near_off_, near_ent_;
+  DevBuf<double> sx_, sy_, sz_, tx_, ty_, tz_, qr_, qi_, q_in_;
+  DevBuf<double> leaf_r_, hlev_, norm_;
+  DevBuf<double> mult_, loc_, phi_hat_, grad_hat_, phi_, grad_;
+  DevBuf<double> t_m2m_, t_l2l_, t_m2l_;
+  DevBuf<uint16_t> t_m2l16_, mult16_;
+
+  Rows p2m_rows_, l2p_rows_, m2p_rows_, p2l_rows_;
+  std::vector<Schedule> m2m_, l2l_;  // per level (M2M into the parents of level l, L2L into level l)
+  Schedule m2l_;
+
+  bool timing_ = false;
+  struct Mark {
+    int kind;
+    hipEvent_t a, b;
+  };
+  std::vector<hipEvent_t> events_;
+  std::vector<Mark> marks_;
+  size_t events_used_ = 0;
+  double kind_ms_[16] = {0};
+  int64_t kind_launches_[16] = {0};
+};
+
+// Compensated f64 direct sum of q e^{ikr}/r and its gradient (no 1/(4 pi)); complex
+// charges and outputs interleaved, host or device pointers; synchronous.
+void helmholtz_direct_sum(int device, double kre, double kim, const double *sources, const double *charges,
+                          int64_t ns, const double *targets, int64_t nt, double *phi, double *grad, bool on_device);
+
+}  // namespace fmmh

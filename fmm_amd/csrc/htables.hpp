@@ -1,0 +1,36 @@
+// htables.hpp -- Helmholtz translation tables of one level (htables.cpp).
+//
+// A table is a dense complex (p+1)^2 x (p+1)^2 matrix from stored source coefficients to
+// stored target coefficients, as two column-major real planes (real part, then imaginary
+// part; entry (i, k) at i + k (p+1)^2).  kh is a level's wavenumber in its own units
+// (k r_l); the stored form and the level units are those of the Helmholtz kernels.
+#pragma once
+
+#include <complex>
+#include <cstdint>
+#include <vector>
+
+namespace fmmh {
+
+using cplx = std::complex<double>;
+
+// Orthonormal real solid harmonics S_n^m(x) = |x|^n Y_n^m(x^) in slots n^2 + n + m, and the
+// factor norm(n, m) that takes the Racah harmonic's real (m >= 0) or imaginary (m < 0)
+// part to them.
+void h_norm_factors(int p, std::vector<double> &norm);
+
+// M2M from the child of octant o into its parent (kh: the parent's level), L2L from the
+// parent into that child, 8 tables each (octant order); M2L of the listed offset indices
+// (d = target cell - source cell).  Each table 2 (p+1)^4 values, appended to out.
+void h_m2m_tables(int p, cplx kh, std::vector<double> &out);
+void h_l2l_tables(int p, cplx kh, std::vector<double> &out);
+void h_m2l_tables(int p, cplx kh, const std::vector<uint16_t> &offsets, std::vector<double> &out);
+
+// Level constants in a level's units: s = min(|kh| sqrt 3, 1), beta = Im(kh) sqrt 3.
+inline double h_scale(cplx kh) {
+  const double v = std::abs(kh) * 1.7320508075688772;
+  return v < 1.0 ? v : 1.0;
+}
+inline double h_beta(cplx kh) { return kh.imag() * 1.7320508075688772; }
+
+}  // namespace fmmh

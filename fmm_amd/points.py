@@ -101,3 +101,15 @@ def batched_clouds(nclouds: int = 256, nmin: int = 2000, nmax: int = 256000, see
         clouds.append(rng.gaussian_blob(int(n), centre, 0.1))
         charges.append(rng.normals(int(n)))
     return clouds, charges
+
+
+def helmholtz_cube(n: int, sampled: int = 1000, seed: int = BENCH_SEED):
+    """The Helmholtz benchmark problem: n points uniform in the unit cube (the generator of
+    bench_cube), then complex charges with real and imaginary parts N(0,1) from the same
+    stream (all real parts, then all imaginary parts), then the sampled targets."""
+    rng = SplitMix64(seed)
+    pts = rng.cube(n)
+    re = rng.normals(n)
+    im = rng.normals(n)
+    sample = rng.sample(n, sampled)
+    return pts, re + 1j * im, sample

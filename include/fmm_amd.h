@@ -195,6 +195,56 @@ int32_t fmmh_read_expansions(fmmh_operator *op, int32_t locals, int32_t level, d
 int32_t fmmh_kind_timing(fmmh_operator *op, int32_t enable);
 int32_t fmmh_kind_times(fmmh_operator *op, double *ms, int64_t *launches);
 
+/* ---- Helmholtz (kernel e^{ikr} / (4 pi r), complex charges) ---------------------- */
+/* The reference's Helmholtz FMM (nd_fmm_exec::helmholtz) runs on the host only
+ * (fmm-exec/src/helmholtz/fmm.rs:71); these entry points are its device path over the same
+ * plan, keeping its stored forms (docs/CONVENTIONS.md:1435-1700) and its evaluator's order.
+ * Complex values are interleaved (re, im): charges[2 ns], phi[2 nt], grad[6 nt] (x re,
+ * x im, y re, y im, z re, z im).  The wavenumber is in the units of the points. */
+typedef struct fmmh_helmholtz fmmh_helmholtz;
+
+typedef struct {
+  int32_t p;         /* expansion degree of every level, 0..15 */
+  double k_re;       /* wavenumber k: k != 0, Re k >= 0, Im k >= 0 (Wavenumber::new, */
+  double k_im;       /*   fmm-exec/src/helmholtz/mod.rs) */
+  int32_t precision; /* fmmh_precision: plain f64, or f32 P2P and bf16x3 M2L */
+  int32_t gradients;
+  int32_t device;
+} fmmh_helmholtz_options;
+
+/* The degree rule at tolerance eps for boxes of half-width radius (fmm-exec/src/helmholtz/
+ * degree.rs:8-15): max(ceil(0.30 L^2 + 0.25 L + 2.60), ceil(x + 0.80 (ln 1/eps)^(2/3) x^(1/3))),
+ * x = Re(k) sqrt(3) radius, L = log10(1/eps). */
+int32_t fmmh_helmholtz_degree(double k_re, double radius, double eps);
+/* HelmholtzFmm built over a plan (fmm-exec/src/helmholtz/fmm.rs; operator.rs:448-480):
+ * per-level translation tables (dense complex, rotation-coaxial-rotation) uploaded, every
+ * buffer allocated.  A plan of one domain (not a forest). */
+int32_t fmmh_helmholtz_create(const fmmh_plan *plan, const fmmh_helmholtz_options *options, fmmh_helmholtz **out);
+void fmmh_helmholtz_free(fmmh_helmholtz *op);
+int32_t fmmh_helmholtz_stats(const fmmh_helmholtz *op, fmmh_stats *out);
+int32_t fmmh_helmholtz_load_charges(fmmh_helmholtz *op, const double *charges, int32_t on_device);
+int32_t fmmh_helmholtz_reset(fmmh_helmholtz *op);
+/* P2M, M2M, M2L, P2L, L2L, L2P, M2P, P2P over every level (the order of
+ * fmm-exec/src/helmholtz/fmm.rs's evaluation). */
+int32_t fmmh_helmholtz_evaluate(fmmh_helmholtz *op);
+int32_t fmmh_helmholtz_read_output(fmmh_helmholtz *op, double *phi, double *grad, int32_t on_device);
+/* Stored multipoles or locals of one level: per box the real plane then the imaginary
+ * plane of (p+1)^2 values. */
+int32_t fmmh_helmholtz_read_expansions(fmmh_helmholtz *op, int32_t locals, int32_t level, double *out);
+int32_t fmmh_helmholtz_kind_timing(fmmh_helmholtz *op, int32_t enable);
+int32_t fmmh_helmholtz_kind_times(fmmh_helmholtz *op, double *ms, int64_t *launches);
+/* One translation table of a level whose wavenumber in its own units is kh (host):
+ * kind 0 M2M from the child of octant `index` (kh of the parent level), 1 L2L into that
+ * child, 2 M2L of offset index `index`; out[2 (p+1)^4] real plane then imaginary plane,
+ * column-major (the operators of fmm-tables/src/helmholtz/dense.rs). */
+int32_t fmmh_helmholtz_table(int32_t p, double kh_re, double kh_im, int32_t kind, int32_t index, double *out);
+/* Compensated f64 direct sum of q e^{ikr}/r and its gradient (ik r - 1) e^{ikr} d / r^3
+ * (no 1/(4 pi)), nd_fmm_ref::helmholtz::p2p::direct_sum (fmm-ref/src/helmholtz/p2p.rs:155);
+ * a source equal to the target skipped.  Host or device pointers; synchronous. */
+int32_t fmmh_helmholtz_direct_sum(int32_t device, double k_re, double k_im, const double *sources,
+                                  const double *charges, int64_t nsources, const double *targets, int64_t ntargets,
+                                  double *phi, double *grad, int32_t on_device);
+
 #ifdef __cplusplus
 }
 #endif

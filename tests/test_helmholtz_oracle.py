@@ -172,3 +172,38 @@ def test_oracle_helmholtz_fmm_matches_direct_sum(layout):
         ep, eg = _direct(k, pts, q, pts[s])
         assert rel(ph[s], ep / (4 * np.pi)) < 3e-5
         assert rel(gr[s], eg / (4 * np.pi)) < 1e-4
+
+
+# ---- the product's host side against the oracle (no GPU) -----------------------------------
+
+def test_product_tables_match_oracle_tables():
+    import fmm_amd
+
+    for p, kh in [(6, 0.8), (8, 1.3 + 0.2j), (10, 0.05), (12, 3.1), (8, 2.0j)]:
+        for kind, i in [("m2m", 3), ("l2l", 6), ("m2m", 0), ("l2l", 7), ("m2l", 0), ("m2l", 157), ("m2l", 300)]:
+            A = fmm_amd.helmholtz_table(p, kh, kind, i)
+            if kind == "m2m":
+                B = O.h_m2m(p, kh, i)
+            elif kind == "l2l":
+                B = O.h_l2l(p, kh, i)
+            else:
+                B = O.h_m2l(p, kh, O.offset_of_index(i))
+            assert np.abs(A - B).max() <= 1e-13 * np.abs(B).max(), (p, kh, kind, i)
+
+
+def test_degree_rule_matches_the_reference_examples():
+    """fmm-exec/src/helmholtz/degree.rs:63-76: the plateau 15 at 1e-6 and kD = 100 on the
+    unit cube giving 35 and 22 on levels 2 and 3."""
+    import fmm_amd
+
+    assert fmm_amd.helmholtz_degree(1.0, 0.01, 1e-6) == 15
+    assert fmm_amd.helmholtz_degree(100.0, 0.125, 1e-6) == 35
+    assert fmm_amd.helmholtz_degree(100.0, 0.0625, 1e-6) == 22
+
+
+def test_wavenumber_rules():
+    import fmm_amd
+
+    for k in (0.0, -1.0, 1.0 - 0.5j):
+        with pytest.raises(fmm_amd.FmmError):
+            fmm_amd.HelmholtzFmmBuilder(k)
