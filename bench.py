@@ -75,8 +75,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy 256 MiB device to device (a known-size kernel for counter scaling)")
-    ap.add_argument("--m2l", default="dense", choices=("dense", "rotation"),
-                    help="M2L formulation of both legs (rotation: f32 in the mixed leg)")
+    ap.add_argument("--m2l", default="dense", choices=("dense", "rotation", "plane_wave"),
+                    help="M2L formulation of both legs (rotation: f32 in the mixed leg; plane_wave: the plain "
+                         "leg only, the mixed leg keeps the dense bf16x3 M2L)")
     ap.add_argument("--workload", default="cube", choices=("cube", "batched", "helmholtz"),
                     help="cube: 4M uniform points (the headline); batched: 256 independent Gaussian clouds, "
                          "p = 6, one forest evaluation replayed as a hipGraph, plus its adjoint; helmholtz: "
@@ -138,7 +139,8 @@ def main():
 
     def run(precision):
         t0 = time.time()
-        op = fmm_amd.Operator(plan, args.p, precision, True, 0, args.m2l)
+        m2l = "dense" if (args.m2l == "plane_wave" and precision == "mixed") else args.m2l
+        op = fmm_amd.Operator(plan, args.p, precision, True, 0, m2l)
         stats = op.stats()
         log(f"[{precision}] operator built in {time.time() - t0:.1f}s: {stats}")
 

@@ -24,7 +24,7 @@ from ._lib import FmmError, build  # noqa: F401
 PLAIN_F64 = 0
 MIXED = 1
 PRECISIONS = {"f64": PLAIN_F64, "plain": PLAIN_F64, "mixed": MIXED}
-M2L_METHODS = {"dense": 0, "rotation": 1}
+M2L_METHODS = {"dense": 0, "rotation": 1, "plane_wave": 2}
 KINDS = ("p2m", "m2m", "l2l", "m2l", "p2l", "l2p", "m2p", "p2p", "other", "m2l_contract")
 LISTS = {"near": 0, "v": 1, "w": 2, "x": 3}
 
@@ -352,8 +352,9 @@ class FmmBuilder:
         return self
 
     def m2l(self, method: str):
-        """The M2L formulation (fmm-exec's M2lMethod): "dense" (tables on MFMA) or "rotation"
-        (rotate, coaxial shift, rotate back; f32 in the mixed precision)."""
+        """The M2L formulation (fmm-exec's M2lMethod): "dense" (tables on MFMA), "rotation"
+        (rotate, coaxial shift, rotate back; f32 in the mixed precision) or "plane_wave"
+        (waves in six directions, f64, to a quadrature tolerance of about 10^-(p/2 + 2))."""
         if method not in M2L_METHODS:
             raise ValueError(f"unknown M2L method {method!r}")
         self._m2l = method
@@ -373,6 +374,24 @@ class FmmBuilder:
         plan = Plan.forest(clouds, self._leaf_points, self._max_level, self._near)
         op = Operator(plan, self._p, self._precision, self._gradients, self._device, self._m2l)
         return Fmm(plan, op)
+
+
+def plane_wave_quadrature(p: int, eps: float):
+    """(lambda, weight, alpha counts, H) of the plane-wave quadrature for degree p."""
+    n, h = C.c_int32(0), C.c_int32(0)
+    lam, w, na = np.zeros(400), np.zeros(400), np.zeros(400, np.int32)
+    _lib.check(_lib.lib().fmmh_plane_wave_quadrature(int(p), float(eps), C.byref(n), C.byref(h), _lib.ptr(lam),
+                                                     _lib.ptr(w), _lib.ptr(na)))
+    k = n.value
+    return lam[:k].copy(), w[:k].copy(), na[:k].copy(), h.value
+
+
+def plane_wave_m2l(p: int, eps: float, offset: int, multipole) -> np.ndarray:
+    """One V pair's plane-wave M2L on the host (scaled storage): the local it adds."""
+    x = np.ascontiguousarray(multipole, dtype=np.float64)
+    out = np.zeros((p + 1) ** 2)
+    _lib.check(_lib.lib().fmmh_plane_wave_m2l(int(p), float(eps), int(offset), _lib.ptr(x), _lib.ptr(out)))
+    return out
 
 
 # ---- Helmholtz --------------------------------------------------------------------------

@@ -3,6 +3,7 @@
 #include <cstring>
 #include <exception>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -11,6 +12,7 @@
 #include "htables.hpp"
 #include "operator.hpp"
 #include "plan.hpp"
+#include "pwtables.hpp"
 
 struct fmmh_plan {
   fmmh::Plan plan;
@@ -496,6 +498,37 @@ int32_t fmmh_helmholtz_direct_sum(int32_t device, double k_re, double k_im, cons
     need(ntargets == 0 || (targets && phi), "fmmh_helmholtz_direct_sum: NULL targets or output");
     fmmh::helmholtz_direct_sum(device, k_re, k_im, sources, charges, nsources, targets, ntargets, phi, grad,
                                on_device != 0);
+  });
+}
+
+int32_t fmmh_plane_wave_quadrature(int32_t p, double eps, int32_t *nodes, int32_t *half_nodes, double *lambda,
+                                   double *weight, int32_t *nalpha) {
+  return guard([&] {
+    need(nodes != nullptr, "fmmh_plane_wave_quadrature: NULL nodes");
+    const fmmh::PlaneWave pw = fmmh::build_plane_wave(p, eps);
+    *nodes = pw.nodes;
+    if (half_nodes) *half_nodes = pw.H;
+    for (int k = 0; k < pw.nodes; ++k) {
+      if (lambda) lambda[k] = pw.lambda[k];
+      if (weight) weight[k] = pw.weight[k];
+      if (nalpha) nalpha[k] = pw.nalpha[k];
+    }
+  });
+}
+
+int32_t fmmh_plane_wave_m2l(int32_t p, double eps, int32_t offset, const double *in, double *out) {
+  return guard([&] {
+    need(in && out, "fmmh_plane_wave_m2l: NULL argument");
+    need(offset >= 0 && offset < 316, "fmmh_plane_wave_m2l: offset index 0..315");
+    // the last quadrature is kept for the next call of the same (p, eps)
+    static std::mutex mu;
+    static std::unique_ptr<fmmh::PlaneWave> last;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!last || last->p != p || last->eps != eps) last = std::make_unique<fmmh::PlaneWave>(fmmh::build_plane_wave(p, eps));
+    const fmmh::PlaneWave &pw = *last;
+    int d[3];
+    fmmh::offset_of_index(offset, d);
+    fmmh::plane_wave_m2l_host(pw, d, in, out);
   });
 }
 

@@ -2,12 +2,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <array>
 #include <cstring>
 #include <map>
 
 #include "kernels.hpp"
 #include "operator.hpp"
+#include "pwtables.hpp"
 #include "schedule.hpp"
 
 namespace fmmh {
@@ -21,7 +23,10 @@ void hip_check(hipError_t e, const char *what) {
 DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options) : opt_(options) {
   require(opt_.p >= 0 && opt_.p <= dev::kMaxDegree, "the device path supports 0 <= p <= 15", kUnsupportedLayout);
   require(opt_.precision == kPlainF64 || opt_.precision == kMixed, "unknown precision", kUnsupportedPrecision);
-  require(opt_.m2l == kM2lDense || opt_.m2l == kM2lRotation, "unknown M2L method", kUnsupportedLayout);
+  require(opt_.m2l == kM2lDense || opt_.m2l == kM2lRotation || opt_.m2l == kM2lPlaneWave, "unknown M2L method",
+          kUnsupportedLayout);
+  require(opt_.m2l != kM2lPlaneWave || opt_.precision == kPlainF64, "the plane-wave M2L runs in f64",
+          kUnsupportedPrecision);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(kNoDevice, "no HIP device");
   require(opt_.device >= 0 && opt_.device < ndev, "device ordinal out of range", kNoDevice);
@@ -202,7 +207,8 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
     upload_rows(p2l_rows_, id, cell, off, ent);
   }
   build_schedules(plan);
-  if (opt_.m2l == kM2lRotation) build_rotation(plan);
+  if (opt_.m2l == kM2lRotation || opt_.m2l == kM2lPlaneWave) build_rotation(plan);
+  if (opt_.m2l == kM2lPlaneWave) build_plane_wave(plan);
   loc_.alloc(nbox_total_ * nc_);
   if (opt_.precision == kMixed)
     slots32_.alloc(nslots_ * nc_);
@@ -347,8 +353,8 @@ void DeviceOperator::build_schedules(const Plan &plan) {
   m2l_red_level_.assign(nlevels_ + 1, 0);
   nslots_ = 0;
   for (int l = 0; l < nlevels_; ++l) {
-    if (l >= 2 && opt_.m2l == kM2lRotation) {
-      // the rotation M2L has its own rows (build_rotation); count the pairs only
+    if (l >= 2 && opt_.m2l != kM2lDense) {
+      // the rotation and plane-wave M2L have their own rows (build_rotation); count the pairs only
       all.pairs += plan.v[l].size();
     } else if (l >= 2) {
       const Csr &V = plan.v[l];
@@ -419,7 +425,55 @@ void DeviceOperator::build_schedules(const Plan &plan) {
   upload_rows(m2l_red_, red_id, red_cell, red_off, red_ent);
 }
 
+// The plane-wave quadrature's tolerance at degree p: a tenth of the M2L truncation, about
+// 10^-(p/2 + 2) (the reference picks its node count per degree by measurement,
+// docs/CONVENTIONS.md:1335-1345).
+static double plane_wave_tolerance(int p) { return std::pow(10.0, -(0.5 * p + 2.0)); }
+
+void DeviceOperator::build_plane_wave(const Plan &plan) {
+  const PlaneWave pw = ::fmmh::build_plane_wave(opt_.p, plane_wave_tolerance(opt_.p));
+  require(pw.nodes <= dev::kPwMaxNodes, "plane wave: too many quadrature nodes", kUnsupportedLayout);
+  const size_t lds = dev::pw_incoming_lds(opt_.p, pw.H);
+  require(lds <= 160 * 1024, "plane wave: the quadrature does not fit in LDS", kUnsupportedLayout);
+  hip_check(dev::pw_allow_lds(lds), "plane wave LDS");
+  pw_lambda_.upload(pw.lambda);
+  pw_first_.upload(pw.first);
+  pw_nalpha_.upload(pw.nalpha);
+  pw_hw_.upload(pw.hw);
+  pw_hc_.upload(pw.hc);
+  pw_hs_.upload(pw.hs);
+  pw_hk_.upload(pw.hk);
+  pw_fwd_.upload(pw.fwd);
+  pw_bwd_.upload(pw.bwd);
+  // per offset: its direction and the shift T(d') of every half node
+  std::vector<int> dir(316);
+  std::vector<double> shift(size_t(316) * 2 * pw.H);
+  for (int t = 0; t < 316; ++t) {
+    int d[3], f, up;
+    double dp[3];
+    offset_of_index(t, d);
+    plane_wave_direction(d, f, up, dp);
+    dir[t] = 2 * f + (up > 0 ? 0 : 1);
+    for (int h = 0; h < pw.H; ++h) {
+      const double lam = pw.hl[h], amp = std::exp(-lam * std::fabs(dp[2]));
+      const double ph = lam * (dp[0] * pw.hc[h] + dp[1] * pw.hs[h]);
+      shift[size_t(t) * 2 * pw.H + h] = amp * std::cos(ph);
+      shift[size_t(t) * 2 * pw.H + pw.H + h] = amp * std::sin(ph);
+    }
+  }
+  pw_dir_.upload(dir);
+  pw_shift_.upload(shift);
+  pw_box0_ = nlevels_ > 2 ? box_base_[2] : 0;
+  pw_nbox_ = nlevels_ > 2 ? static_cast<uint32_t>(box_base_[nlevels_] - box_base_[2]) : 0;
+  pw_waves_.alloc(size_t(pw_nbox_) * 12 * pw.H);
+  pw_ = dev::PwArgs{opt_.p,        pw.nodes,      pw.H,          rotation_block_size(opt_.p), pw_lambda_.ptr,
+                    pw_first_.ptr, pw_nalpha_.ptr, pw_hw_.ptr,   pw_hc_.ptr,                  pw_hs_.ptr,
+                    pw_hk_.ptr,    pw_fwd_.ptr,   pw_bwd_.ptr,   pw_shift_.ptr,               pw_dir_.ptr};
+  (void)plan;
+}
+
 void DeviceOperator::build_rotation(const Plan &plan) {
+  if (opt_.m2l == kM2lRotation) {
   std::vector<double> fwd, bwd, coax, dist2;
   std::vector<uint16_t> dist_of;
   build_rotation_tables(opt_.p, fwd, bwd, coax, dist_of, dist2);
@@ -435,6 +489,7 @@ void DeviceOperator::build_rotation(const Plan &plan) {
     rot_coax_.upload(coax);
   }
   rot_dist_.upload(dist_of);
+  }
   // rows: target boxes with V entries, level by level; entries in row order
   std::vector<uint32_t> box, off{0}, src;
   std::vector<uint16_t> ofs;
@@ -473,6 +528,18 @@ void DeviceOperator::launch_contract(const Schedule &s, uint32_t task_first, uin
 // M2L of the levels level_first .. level_last: the contraction tasks, then the ordered
 // reduction of the pair-mode levels.
 void DeviceOperator::launch_m2l(int level_first, int level_last) {
+  if (opt_.m2l == kM2lPlaneWave) {
+    const uint32_t r0 = rot_level_[level_first], r1 = rot_level_[level_last + 1];
+    if (r1 == r0) return;
+    // the waves of the source boxes of these levels, then the pairs of the target rows
+    const uint32_t b0 = box_base_[level_first], b1 = box_base_[level_last + 1];
+    dev::pw_outgoing(stream_, pw_, b1 - b0, b0, mult_.ptr, pw_waves_.ptr + size_t(b0 - pw_box0_) * 12 * pw_.H);
+    timed(kKindM2lContract, [&] {
+      dev::pw_incoming(stream_, pw_, r1 - r0, rot_box_.ptr + r0, rot_off_.ptr + r0, rot_src_.ptr, rot_offset_.ptr,
+                       pw_box0_, pw_waves_.ptr, loc_.ptr);
+    });
+    return;
+  }
   if (opt_.m2l == kM2lRotation) {
     const uint32_t r0 = rot_level_[level_first], r1 = rot_level_[level_last + 1];
     timed(kKindM2lContract, [&] {

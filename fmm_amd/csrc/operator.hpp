@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "plan.hpp"
+#include "pwkernels.hpp"
 
 namespace fmmh {
 
@@ -89,6 +90,7 @@ struct Rows {
 enum M2lMethod : int {
   kM2lDense = 0,     // dense tables, grouped contraction on MFMA
   kM2lRotation = 1,  // rotation, coaxial shift, rotation back (f32 in the mixed precision)
+  kM2lPlaneWave = 2, // plane waves in six directions (f64; approximate to the quadrature's tolerance)
 };
 
 struct OperatorOptions {
@@ -190,6 +192,12 @@ class DeviceOperator {
   std::vector<uint32_t> rot_level_;
   int rot_rb_ = 0, rot_cb_ = 0;
   void build_rotation(const Plan &plan);
+  void build_plane_wave(const Plan &plan);
+  // plane-wave M2L: quadrature tables, the waves of the boxes of levels >= 2
+  dev::PwArgs pw_{};
+  DevBuf<double> pw_lambda_, pw_hw_, pw_hc_, pw_hs_, pw_fwd_, pw_bwd_, pw_shift_, pw_waves_;
+  DevBuf<int> pw_first_, pw_nalpha_, pw_hk_, pw_dir_;
+  uint32_t pw_box0_ = 0, pw_nbox_ = 0;
   size_t nslots_ = 0;                  // pair-mode scratch rows (slot ids follow the box ids)
   DevBuf<double> slots_;               // plain f64: scratch rows
   DevBuf<float> slots32_;              // mixed: scratch rows (one group's f32 sum each)

@@ -97,6 +97,9 @@ Plan build_forest(const double *points, const int64_t *counts, int nclouds, cons
 // dist_of[offset]) the coaxial M2L, per order i a column-major (p+1-i)^2 block [j-i][n-i]
 // (coaxial_block_size(p) values per distance).  T(d) = G A F equals the dense M2L table.
 int rotation_block_size(int p);
+// Every degree's block (column-major, degrees in order) of the coefficient rotation for the
+// rotation matrix R (row-major): multipoles (local = false) or locals (local = true).
+void rotation_blocks_of(int p, const double R[9], bool local, double *out);
 int coaxial_block_size(int p);
 void build_rotation_tables(int p, std::vector<double> &fwd, std::vector<double> &bwd, std::vector<double> &coax,
                            std::vector<uint16_t> &dist_of, std::vector<double> &dist2);

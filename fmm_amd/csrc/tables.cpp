@@ -267,6 +267,14 @@ void rotation_block(int n, const double Rm[9], bool local, double *blk) {
 
 }  // namespace
 
+void rotation_blocks_of(int p, const double R[9], bool local, double *out) {
+  size_t bo = 0;
+  for (int n = 0; n <= p; ++n) {
+    rotation_block(n, R, local, out + bo);
+    bo += size_t(2 * n + 1) * (2 * n + 1);
+  }
+}
+
 int rotation_block_size(int p) {
   int s = 0;
   for (int n = 0; n <= p; ++n) s += (2 * n + 1) * (2 * n + 1);

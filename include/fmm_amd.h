@@ -47,10 +47,11 @@ typedef enum {
   FMMH_MIXED = 1      /* f32 P2P (double-float separations), bf16x3 M2L on MFMA; f64 elsewhere */
 } fmmh_precision;
 
-/* M2L formulation (fmm-exec's M2lMethod): dense tables on MFMA (fmm-kernels/src/translate.rs:1703)
- * or rotation, coaxial shift and rotation back (fmm-kernels/src/rotation.rs:441); in the
- * mixed precision the rotation runs in f32. */
-typedef enum { FMMH_M2L_DENSE = 0, FMMH_M2L_ROTATION = 1 } fmmh_m2l_method;
+/* The M2L formulation (M2lMethod): dense tables (fmm-kernels/src/translate.rs:1703),
+ * rotation, coaxial shift and rotation back (fmm-kernels/src/rotation.rs:441; in the mixed
+ * precision in f32), or plane waves in six directions (fmm-kernels/src/plane_wave.rs:1159;
+ * f64, approximate to a quadrature tolerance of about 10^-(p/2 + 2)). */
+typedef enum { FMMH_M2L_DENSE = 0, FMMH_M2L_ROTATION = 1, FMMH_M2L_PLANE_WAVE = 2 } fmmh_m2l_method;
 
 typedef enum { FMMH_LIST_NEAR = 0, FMMH_LIST_V = 1, FMMH_LIST_W = 2, FMMH_LIST_X = 3 } fmmh_list_kind;
 
@@ -194,6 +195,15 @@ int32_t fmmh_read_expansions(fmmh_operator *op, int32_t locals, int32_t level, d
  * call with enable: ms[FMMH_NKINDS]. */
 int32_t fmmh_kind_timing(fmmh_operator *op, int32_t enable);
 int32_t fmmh_kind_times(fmmh_operator *op, double *ms, int64_t *launches);
+
+/* The plane-wave quadrature built for degree p and tolerance eps (fmm-tables/src/plane_wave/
+ * mod.rs:111 uses FMM3D's tables; this build compresses a Gauss-Legendre rule): node count,
+ * half-node count H, and per node lambda, weight and alpha count (arrays of >= 400). */
+int32_t fmmh_plane_wave_quadrature(int32_t p, double eps, int32_t *nodes, int32_t *half_nodes, double *lambda,
+                                   double *weight, int32_t *nalpha);
+/* One V pair's plane-wave M2L on the host (the device kernels' steps; plane_wave::m2l,
+ * fmm-kernels/src/plane_wave.rs:1159): out += M2L_pw(offset) in, scaled storage. */
+int32_t fmmh_plane_wave_m2l(int32_t p, double eps, int32_t offset, const double *in, double *out);
 
 /* ---- Helmholtz (kernel e^{ikr} / (4 pi r), complex charges) ---------------------- */
 /* The reference's Helmholtz FMM (nd_fmm_exec::helmholtz) runs on the host only
