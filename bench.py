@@ -78,10 +78,11 @@ def main():
     ap.add_argument("--m2l", default="dense", choices=("dense", "rotation", "plane_wave"),
                     help="M2L formulation of both legs (rotation: f32 in the mixed leg; plane_wave: the plain "
                          "leg only, the mixed leg keeps the dense bf16x3 M2L)")
-    ap.add_argument("--workload", default="cube", choices=("cube", "batched", "helmholtz"),
+    ap.add_argument("--workload", default="cube", choices=("cube", "batched", "helmholtz", "nonuniform"),
                     help="cube: 4M uniform points (the headline); batched: 256 independent Gaussian clouds, "
                          "p = 6, one forest evaluation replayed as a hipGraph, plus its adjoint; helmholtz: "
-                         "1M uniform points, k = 25, complex charges, degree by the rule at 1e-6")
+                         "1M uniform points, k = 25, complex charges, degree by the rule at 1e-6; nonuniform: 2M "
+                         "sphere-surface + 1M Plummer points, p = 10, rotation M2L in f32")
     ap.add_argument("--k", type=float, default=25.0, help="helmholtz: the wavenumber (real)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -92,6 +93,8 @@ def main():
         return batched(args)
     if args.workload == "helmholtz":
         return helmholtz(args)
+    if args.workload == "nonuniform":
+        return nonuniform(args)
 
     import torch
 
@@ -366,6 +369,87 @@ def batched(args):
                    "sizes": [int(len(c)) for c in clouds][:8] + ["..."]},
         "errors": {"phi": (num / den) ** 0.5, "grad": (numg / deng) ** 0.5, "sampled_per_cloud": 32},
         "adjoint_ms": float(np.median(vt)) * 1e3,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def nonuniform(args):
+    """The non-uniform configuration: 2,000,000 points on the sphere surface + 1,000,000
+    Plummer points (a = 0.1, truncated at 10a) in the unit cube, charges uniform in [0, 1),
+    p = 10, deep adaptive octree (max level 16), at most 128 points per leaf, near rule at
+    the device crossovers; rotation M2L in f32 with f64 M2M/L2L and leaf operators (the
+    configuration's method), beside the dense bf16x3 and the plain f64 paths.  Reports the
+    leaf-level histogram and the W/X pair counts; errors against the f64 direct sum at 4,096
+    sampled targets."""
+    import torch
+
+    import fmm_amd
+    from fmm_amd import points as P
+
+    p, leaf = (args.p if args.p != 12 else 10), args.leaf
+    t0 = time.time()
+    pts, q = P.nonuniform()
+    n = len(pts)
+    sample = P.SplitMix64(P.BENCH_SEED + 3).sample(n, args.sampled)
+    log(f"nonuniform: N={n} drawn in {time.time() - t0:.1f}s")
+    near = (0.5 * (p + 1) ** 2,) * 2
+    t0 = time.time()
+    plan = fmm_amd.Plan(pts, leaf_points=leaf, near_rule=near)
+    lk = plan.leaf_keys()
+    hist = {int(l): int(c) for l, c in zip(*np.unique(lk & np.uint64(0x7FFF), return_counts=True))}
+    wx = {"w": int(sum(len(plan.list("w", l)[1]) for l in range(plan.nlevels))),
+          "x": int(sum(len(plan.list("x", l)[1]) for l in range(plan.nlevels)))}
+    log(f"plan: {plan.nlevels} levels, leaves {hist}, W/X pairs {wx}, built in {time.time() - t0:.1f}s")
+    ep, eg = fmm_amd.direct_sum(pts, q, pts[sample])
+    ep, eg = ep / (4 * np.pi), eg / (4 * np.pi)
+    dev = torch.device("cuda:0")
+    q_dev = torch.from_numpy(q).to(dev)
+    phi = torch.zeros(n, dtype=torch.float64, device=dev)
+    grad = torch.zeros(3 * n, dtype=torch.float64, device=dev)
+
+    def run(precision, m2l):
+        op = fmm_amd.Operator(plan, p, precision, True, 0, m2l)
+
+        def step():
+            op.load_charges(q_dev)
+            op.reset()
+            op.evaluate()
+            op.read_output_device(phi, grad)
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        op.kind_timing(True)
+        times = []
+        for _ in range(args.steps):
+            torch.cuda.synchronize()
+            s = time.perf_counter()
+            step()
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - s)
+        kinds = op.kind_times()
+        op.kind_timing(False)
+        ph = phi.cpu().numpy()
+        gr = grad.cpu().numpy().reshape(-1, 3)
+        err = {"phi": float(np.linalg.norm(ph[sample] - ep) / np.linalg.norm(ep)),
+               "grad": float(np.linalg.norm(gr[sample] - eg) / np.linalg.norm(eg))}
+        med = float(np.median(times))
+        per = {k: round(v[0] / args.steps, 3) for k, v in kinds.items() if v[1]}
+        log(f"[{precision}, {m2l}] {med * 1e3:.2f} ms ({n / med / 1e6:.1f} Mpts/s), errors {err}, kinds {per}")
+        return {"ms_per_step": med * 1e3, "value": n / med / 1e6, "errors": err, "kind_ms": per}
+
+    rot = run("mixed", "rotation")
+    dense = run("mixed", "dense")
+    plain = run("f64", "dense")
+    line = {
+        "metric": "non-uniform Laplace FMM Mpts/s (2M sphere + 1M Plummer, p = 10, rotation M2L f32)",
+        "value": rot["value"], "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": rot["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 P2P + f32 rotation M2L + f64 M2M/L2L and leaf operators", "data": "synthetic",
+        "config": {"workload": "nonuniform-sphere-plummer", "n": n, "p": p, "leaf_points": leaf, "max_level": 16,
+                   "near_rule": near, "leaf_levels": hist, "wx_pairs": wx, "levels": int(plan.nlevels)},
+        "errors": rot["errors"], "kind_ms": rot["kind_ms"],
+        "mixed_dense_bf16x3": dense, "plain_f64_dense": plain,
     }
     print(json.dumps(line), flush=True)
 

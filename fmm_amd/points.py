@@ -8,6 +8,8 @@ fmm-bench's partial Fisher-Yates (fmm-bench/src/measure.rs:52-57).
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 BENCH_SEED = 0x00BE9C06
@@ -78,6 +80,79 @@ class SplitMix64:
         return np.asarray(centre, np.float64) + width * out
 
 
+def _next_accepted(ok: np.ndarray) -> np.ndarray:
+    """nxt[j] = the first j' >= j with j' = j (mod 3) and ok[j'] (len(ok) where none)."""
+    n = len(ok)
+    nxt = np.full(n + 3, n, dtype=np.int64)
+    for c in range(3):
+        idx = np.arange(c, n, 3)
+        cand = np.where(ok[idx], idx, n)
+        nxt[idx] = np.minimum.accumulate(cand[::-1])[::-1]
+    return nxt
+
+
+def sphere(rng: SplitMix64, n: int, centre=(0.5, 0.5, 0.5), radius=0.5) -> np.ndarray:
+    """points::sphere (fmm-validate/src/points.rs:48-57): in_unit_ball with norm >= 1e-3,
+    normalised, one triple of draws per try."""
+    out = np.empty((0, 3))
+    while len(out) < n:
+        want = n - len(out)
+        before = rng.state
+        u = rng.uniform(3 * (want + want // 2 + 64))
+        t = u.reshape(-1, 3)
+        v = -1.0 + 2.0 * t
+        s = (v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]) + v[:, 2] * v[:, 2]
+        ok = np.nonzero((s <= 1.0) & (s >= 1e-3 * 1e-3))[0][:want]
+        with np.errstate(over="ignore"):
+            rng.state = before + np.uint64(3 * (int(ok[-1]) + 1 if len(ok) == want else len(t))) * _GAMMA
+        vv = v[ok]
+        out = np.vstack([out, vv / np.sqrt(s[ok])[:, None]])
+    return np.asarray(centre, np.float64) + radius * out
+
+
+def plummer(rng: SplitMix64, n: int, centre=(0.5, 0.5, 0.5), scale=0.1) -> np.ndarray:
+    """points::plummer (fmm-validate/src/points.rs:60-100): per point one draw u for the
+    radius, r = a / sqrt((m u)^(-2/3) - 1) with the mass m inside 10a, then a direction by
+    in_unit_ball (norm >= 1e-3)."""
+    x = 10.0
+    mass = x ** 3 / (1.0 + x * x) ** 1.5
+    pts = np.empty((n, 3))
+    done = 0
+    while done < n:
+        want = n - done
+        before = rng.state
+        count = 8 * want + 256
+        u = rng.uniform(count)
+        v = -1.0 + 2.0 * u
+        s = np.full(count, 2.0)
+        s[: count - 2] = (v[:-2] * v[:-2] + v[1:-1] * v[1:-1]) + v[2:] * v[2:]
+        ok = (s <= 1.0) & (s >= 1e-3 * 1e-3)
+        nxt = _next_accepted(ok)
+        starts = []
+        pos = 0
+        while len(starts) < want:
+            if pos + 1 >= count:
+                break
+            j = int(nxt[pos + 1])
+            if j + 3 > count:
+                break
+            starts.append((pos, j))
+            pos = j + 3
+        st = np.array(starts, dtype=np.int64).reshape(-1, 2)
+        k = len(st)
+        um = mass * u[st[:, 0]]
+        # libm pow per value (numpy's vector pow may round differently)
+        pw = np.array([math.pow(x, -2.0 / 3.0) if x > 0.0 else 2.0 for x in um.tolist()])
+        r = np.where(um > 0.0, scale / np.sqrt(pw - 1.0), 0.0)
+        d = np.stack([v[st[:, 1]], v[st[:, 1] + 1], v[st[:, 1] + 2]], axis=1)
+        d /= np.sqrt(s[st[:, 1]])[:, None]
+        pts[done:done + k] = np.asarray(centre, np.float64) + r[:, None] * d
+        done += k
+        with np.errstate(over="ignore"):
+            rng.state = before + np.uint64(pos) * _GAMMA
+    return pts
+
+
 def bench_cube(n: int, sampled: int = 4096, charges: str = "normal", seed: int = BENCH_SEED):
     """The benchmark problem of fmm-bench/src/measure.rs:35-67: n points uniform in the
     unit cube, then charges from the same stream (N(0,1) by Box-Muller, or uniform in
@@ -101,6 +176,17 @@ def batched_clouds(nclouds: int = 256, nmin: int = 2000, nmax: int = 256000, see
         clouds.append(rng.gaussian_blob(int(n), centre, 0.1))
         charges.append(rng.normals(int(n)))
     return clouds, charges
+
+
+def nonuniform(n_sphere: int = 2_000_000, n_plummer: int = 1_000_000, seed: int = BENCH_SEED):
+    """The non-uniform configuration: n_sphere points on the surface of the sphere inscribed
+    in the unit cube, then n_plummer Plummer points (a = 0.1, truncated at 10a) about its
+    centre, then charges uniform in [0, 1), all from one stream."""
+    rng = SplitMix64(seed)
+    a = sphere(rng, n_sphere, (0.5, 0.5, 0.5), 0.5)
+    b = plummer(rng, n_plummer, (0.5, 0.5, 0.5), 0.1)
+    q = rng.uniform(n_sphere + n_plummer)
+    return np.vstack([a, b]), q
 
 
 def helmholtz_cube(n: int, sampled: int = 1000, seed: int = BENCH_SEED):

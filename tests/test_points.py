@@ -60,3 +60,16 @@ def test_gpu_direct_sum_matches_oracle():
                                 np.array([[0.0, 0, 0], [3, 4, 0]]))
     assert np.allclose(p2, [-1 / 5, 2 / 5], rtol=1e-15)
     assert np.allclose(g2[1], [-6 / 125, -8 / 125, 0], rtol=1e-15, atol=1e-18)
+
+
+def test_sphere_and_plummer_match_the_oracle_bit_for_bit():
+    """points::sphere and points::plummer (fmm-validate/src/points.rs:48-100): the package's
+    vectorised parse of the stream against the oracle's point-by-point draws, and the
+    stream left at the same state."""
+    from fmm_amd import points as P
+
+    for seed in (1, 99, 0x00BE9C06):
+        r1, r2 = P.SplitMix64(seed), O.Rng(seed)
+        assert np.array_equal(P.sphere(r1, 3000, (0.5, 0.5, 0.5), 0.5), r2.sphere(3000, (0.5, 0.5, 0.5), 0.5))
+        assert np.array_equal(P.plummer(r1, 4000, (0.5, 0.5, 0.5), 0.1), r2.plummer(4000, (0.5, 0.5, 0.5), 0.1))
+        assert np.array_equal(r1.uniform(5), np.array([r2.uniform() for _ in range(5)]))
