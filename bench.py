@@ -409,6 +409,7 @@ def nonuniform(args):
 
     def run(precision, m2l):
         op = fmm_amd.Operator(plan, p, precision, True, 0, m2l)
+        log(f"[{precision}, {m2l}] operator: {op.stats()}")
 
         def step():
             op.load_charges(q_dev)
