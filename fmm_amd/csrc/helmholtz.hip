@@ -504,16 +504,49 @@ __global__ __launch_bounds__(64) void k_h_evaluate(int p, uint32_t rows, const u
 // f64.  A pair with r^2 == 0 adds nothing (selects, no branch).  T = float: separations
 // from double-float coordinates (hi + lo), the pair arithmetic in f32, each source leaf's
 // f32 sums added to f64 accumulators.
+// Pair kernel pieces per precision.  f64: sqrt, division and sincos of the math library.
+// f32: 1/r from v_rsq_f32 (1 ulp) and r = r^2 / r; sine and cosine by quadrant reduction,
+// x = q pi/2 + y with pi/2 split in two f32 parts (q exact while |x| < 2^17), and the Taylor
+// polynomials of sin y to y^9 and cos y to y^8 on |y| <= pi/4 (truncation below 3e-9).
 template <typename T>
-__device__ __forceinline__ void sincos_t(T x, T *s, T *c);
+struct PairMath;
+
 template <>
-__device__ __forceinline__ void sincos_t<float>(float x, float *s, float *c) {
-  sincosf(x, s, c);
-}
+struct PairMath<double> {
+  __device__ static __forceinline__ void rinv(double r2, double &r, double &ir) {
+    r = sqrt(r2);
+    ir = r2 == 0.0 ? 0.0 : 1.0 / r;
+  }
+  __device__ static __forceinline__ void sincos(double x, double *s, double *c) { ::sincos(x, s, c); }
+};
+
 template <>
-__device__ __forceinline__ void sincos_t<double>(double x, double *s, double *c) {
-  sincos(x, s, c);
-}
+struct PairMath<float> {
+  __device__ static __forceinline__ void rinv(float r2, float &r, float &ir) {
+    const float y = __builtin_amdgcn_rsqf(r2);
+    ir = r2 == 0.0f ? 0.0f : y;
+    r = r2 * ir;
+  }
+  __device__ static __forceinline__ void sincos(float x, float *s, float *c) {
+    const float q = rintf(x * 0.636619772367581343f);  // 2/pi
+    // pi/2 = 1.57079637050628662109375 - 4.37113900018624283e-8 (two f32 parts)
+    float y = fmaf(-q, 1.57079637050628662109375f, x);
+    y = fmaf(q, 4.37113900018624283e-8f, y);
+    const float y2 = y * y;
+    float ps = fmaf(y2, 2.75573192e-6f, -1.98412698e-4f);  // 1/9!, -1/7!
+    ps = fmaf(y2, ps, 8.33333333e-3f);                     // 1/5!
+    ps = fmaf(y2, ps, -1.66666667e-1f);                    // -1/3!
+    const float sy = fmaf(y * y2, ps, y);
+    float pc = fmaf(y2, 2.48015873e-5f, -1.38888889e-3f);  // 1/8!, -1/6!
+    pc = fmaf(y2, pc, 4.16666667e-2f);                     // 1/4!
+    pc = fmaf(y2, pc, -0.5f);
+    const float cy = fmaf(y2, pc, 1.0f);
+    const int qi = int(q) & 3;
+    const float ss = (qi & 1) ? cy : sy, cc = (qi & 1) ? sy : cy;
+    *s = (qi & 2) ? -ss : ss;
+    *c = ((qi + 1) & 2) ? -cc : cc;
+  }
+};
 
 template <typename T, bool GRAD, bool DAMP>
 __global__ __launch_bounds__(64) void k_h_p2p(uint32_t leaf_first, uint32_t count,
@@ -587,10 +620,10 @@ __global__ __launch_bounds__(64) void k_h_p2p(uint32_t leaf_first, uint32_t coun
               dz += zl - ys[i][5];
             }
             const T r2 = dx * dx + dy * dy + dz * dz;
-            const T rr = sqrt(r2);
-            const T ir = r2 == T(0) ? T(0) : T(1) / rr;
+            T rr, ir;
+            PairMath<T>::rinv(r2, rr, ir);
             T sn, cs;
-            sincos_t<T>(kre * rr, &sn, &cs);
+            PairMath<T>::sincos(kre * rr, &sn, &cs);
             const T dmp = DAMP ? T(exp(-kim * rr)) : T(1);
             const T er = dmp * cs * ir, ei = dmp * sn * ir;
             const T q0 = ys[i][6], q1 = ys[i][7];
