@@ -372,13 +372,22 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, uint3
       bm[nt] = as_bf8(b[nt][1]);
       bl[nt] = as_bf8(b[nt][2]);
     }
+    // the A fragments of M-tile i + 1 are read from LDS while M-tile i is multiplied
+    u4 af[2][3];
+    auto read_a = [&](int i, u4 *f) {
+      const int mt = min(mt0 + i, MT - 1);
+#pragma unroll
+      for (int pl = 0; pl < 3; ++pl) f[pl] = As[buf][(pl * MT + mt) * 64 + lane];
+    };
+    read_a(0, af[0]);
 #pragma unroll
     for (int i = 0; i < MH; ++i) {
       const int mt = mt0 + i;
+      if (i + 1 < MH) read_a(i + 1, af[(i + 1) & 1]);
       if (mt < MT) {
-        const bf8 ah = as_bf8(As[buf][(0 * MT + mt) * 64 + lane]);
-        const bf8 am = as_bf8(As[buf][(1 * MT + mt) * 64 + lane]);
-        const bf8 al = as_bf8(As[buf][(2 * MT + mt) * 64 + lane]);
+        const bf8 ah = as_bf8(af[i & 1][0]);
+        const bf8 am = as_bf8(af[i & 1][1]);
+        const bf8 al = as_bf8(af[i & 1][2]);
 #pragma unroll
         for (int nt = 0; nt < 2; ++nt) {
           // smallest terms first

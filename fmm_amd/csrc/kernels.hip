@@ -83,11 +83,11 @@ __global__ void k_fill(double *p, size_t n, double v) {
     p[i] = v;
 }
 
-__global__ void k_gather_charges(const double *q, const uint32_t *order, size_t n, double *q64, float *q32) {
+__global__ void k_gather_charges(const double *q, const uint32_t *order, size_t n, double *q64, float *rec) {
   for (size_t j = blockIdx.x * size_t(blockDim.x) + threadIdx.x; j < n; j += size_t(gridDim.x) * blockDim.x) {
     const double v = q[order[j]];
     q64[j] = v;
-    if (q32) q32[j] = float(v);
+    if (rec) rec[8 * j + 3] = float(v);
   }
 }
 
@@ -232,13 +232,146 @@ __global__ void k_vjp_combine(size_t n, const double *q, const double *g, const 
 // leaves): that tile selects 0 for r^2 == 0; other tiles clamp r^2 below at the smallest
 // normal number, which changes nothing for distinct points (r^2 >= 1e-22 in leaf units
 // at the deepest level) and keeps a separation rounded to zero finite.
+// Chunks of more than 32 targets in the mixed path: each lane holds one target (TWO:
+// two, the second 64 further on) and at every step all lanes take the same source, read
+// from its record (u_hi, q, u_lo: f32, the source's own leaf frame) by uniform loads, so
+// no LDS staging and no lanes split over phases.  Per near entry the targets are moved
+// into the source leaf's frame once, x' = c'' + rho u_t in f64 (c'' = (C_t - C_s) / r_s,
+// rho = r_t / r_s), and split into hi + lo floats; the separation d' = x' - u_s is formed
+// as (x'_hi - u_hi) + (x'_lo - u_lo) as in the staged path.  In target units q / |d| =
+// rho q / |d'| and q d / |d|^3 = rho^2 q d' / |d'|^3: the f32 sums of an entry (at most
+// 128 sources) are scaled by these powers of two as they go into the f64 sums.
+template <bool GRAD, bool TWO>
+__device__ __forceinline__ void p2p_uniform(uint32_t lf, uint32_t tb, uint32_t n, uint32_t e0, uint32_t e1,
+                                            const uint32_t *__restrict__ near_ent,
+                                            const uint32_t *__restrict__ leaf_cell,
+                                            const uint32_t *__restrict__ src_start, const float4 *__restrict__ rec,
+                                            const double *__restrict__ tx, const double *__restrict__ ty,
+                                            const double *__restrict__ tz, double *__restrict__ phi,
+                                            double *__restrict__ grad) {
+  constexpr int A = TWO ? 2 : 1;
+  constexpr float TINY = 1.17549435e-38f;
+  const int lane = threadIdx.x;
+  double u[A][3];
+  bool ok[A];
+#pragma unroll
+  for (int a = 0; a < A; ++a) {
+    const uint32_t i = tb + uint32_t(lane + 64 * a);
+    ok[a] = uint32_t(lane + 64 * a) < n;
+    u[a][0] = ok[a] ? tx[i] : 0.0;
+    u[a][1] = ok[a] ? ty[i] : 0.0;
+    u[a][2] = ok[a] ? tz[i] : 0.0;
+  }
+  double cp[A], cx[A], cy[A], cz[A];
+#pragma unroll
+  for (int a = 0; a < A; ++a) cp[a] = cx[a] = cy[a] = cz[a] = 0.0;
+  const uint32_t *ct = leaf_cell + 4 * size_t(lf);
+  for (uint32_t e = e0; e < e1; ++e) {
+    const uint32_t sl = near_ent[e];
+    const uint32_t s0 = src_start[sl], s1 = src_start[sl + 1];
+    if (s0 == s1) continue;
+    double c[3], rho;
+    relative_frame<double>(ct, leaf_cell + 4 * size_t(sl), c, rho);
+    float xh[A][3], xl[A][3];
+#pragma unroll
+    for (int a = 0; a < A; ++a)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double x = fma(rho, u[a][k], c[k]);
+        xh[a][k] = float(x);
+        xl[a][k] = float(x - double(xh[a][k]));
+      }
+    const bool own = sl == lf;
+    for (uint32_t jb = s0; jb < s1; jb += 128) {
+      const uint32_t je = (s1 - jb) < 128u ? s1 : jb + 128;
+      float ap[A], ax[A], ay[A], az[A];
+#pragma unroll
+      for (int a = 0; a < A; ++a) ap[a] = ax[a] = ay[a] = az[a] = 0.0f;
+      // the records of the next 4 sources are loaded while the current 4 are computed
+      auto run = [&](auto check) {
+        auto body = [&](const float4 h, const float4 l) {
+#pragma unroll
+          for (int a = 0; a < A; ++a) {
+            const float dx = (xh[a][0] - h.x) + (xl[a][0] - l.x);
+            const float dy = (xh[a][1] - h.y) + (xl[a][1] - l.y);
+            const float dz = (xh[a][2] - h.z) + (xl[a][2] - l.z);
+            // off the own leaf the smallest normal number enters the first product: it
+            // keeps a separation rounded to zero finite and changes no r^2 above 2^-102
+            const float r2 = decltype(check)::value ? dx * dx + dy * dy + dz * dz
+                                                    : fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, TINY)));
+            const float inv = decltype(check)::value ? (r2 == 0.0f ? 0.0f : __builtin_amdgcn_rsqf(r2))
+                                                     : __builtin_amdgcn_rsqf(r2);
+            const float qi = h.w * inv;
+            ap[a] += qi;
+            if (GRAD) {
+              const float qi3 = qi * (inv * inv);
+              ax[a] -= qi3 * dx;
+              ay[a] -= qi3 * dy;
+              az[a] -= qi3 * dz;
+            }
+          }
+        };
+        uint32_t j = jb;
+        if (je - j >= 4) {
+          float4 c[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) c[k] = rec[2 * size_t(j) + k];
+          for (; j + 8 <= je; j += 4) {
+            float4 nx[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) nx[k] = rec[2 * size_t(j + 4) + k];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) body(c[2 * k], c[2 * k + 1]);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) c[k] = nx[k];
+          }
+#pragma unroll
+          for (int k = 0; k < 4; ++k) body(c[2 * k], c[2 * k + 1]);
+          j += 4;
+        }
+        for (; j < je; ++j) body(rec[2 * size_t(j)], rec[2 * size_t(j) + 1]);
+      };
+      if (own)
+        run(std::true_type{});
+      else
+        run(std::false_type{});
+      const double s2 = rho * rho;
+#pragma unroll
+      for (int a = 0; a < A; ++a) {
+        cp[a] += double(ap[a]) * rho;
+        if (GRAD) {
+          cx[a] += double(ax[a]) * s2;
+          cy[a] += double(ay[a]) * s2;
+          cz[a] += double(az[a]) * s2;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < A; ++a) {
+    const uint32_t i = tb + uint32_t(lane + 64 * a);
+    if (ok[a]) {
+      phi[i] += cp[a];
+      if (GRAD) {
+        grad[3 * size_t(i) + 0] += cx[a];
+        grad[3 * size_t(i) + 1] += cy[a];
+        grad[3 * size_t(i) + 2] += cz[a];
+      }
+    }
+  }
+}
+
 template <typename T, bool GRAD>
-__global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, uint32_t count, const uint32_t *near_off,
-                                            const uint32_t *near_ent,
-                                            const uint32_t *leaf_cell, const uint32_t *src_start,
-                                            const uint32_t *tgt_start, const double *sx, const double *sy,
-                                            const double *sz, const double *sq, const double *tx, const double *ty,
-                                            const double *tz, double *phi, double *grad) {
+__global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, uint32_t count, const uint32_t *__restrict__ near_off,
+                                            const uint32_t *__restrict__ near_ent,
+                                            const uint32_t *__restrict__ leaf_cell,
+                                            const uint32_t *__restrict__ src_start,
+                                            const uint32_t *__restrict__ tgt_start, const double *__restrict__ sx,
+                                            const double *__restrict__ sy, const double *__restrict__ sz,
+                                            const double *__restrict__ sq, const float4 *__restrict__ rec,
+                                            const double *__restrict__ tx, const double *__restrict__ ty,
+                                            const double *__restrict__ tz, double *__restrict__ phi,
+                                            double *__restrict__ grad) {
   constexpr bool SPLIT = sizeof(T) == 4;
   constexpr int TILE = 64;
   constexpr T TINY = sizeof(T) == 4 ? T(1.17549435e-38f) : T(2.2250738585072014e-308);
@@ -261,6 +394,13 @@ __global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, uint32_t count,
   const uint32_t e0 = near_off[lf], e1 = near_off[lf + 1];
   for (uint32_t tb = t0; tb < t1; tb += 128) {
     const uint32_t n = (t1 - tb) < 128u ? (t1 - tb) : 128u;
+    if (SPLIT && n > 32) {
+      if (n > 64)
+        p2p_uniform<GRAD, true>(lf, tb, n, e0, e1, near_ent, leaf_cell, src_start, rec, tx, ty, tz, phi, grad);
+      else
+        p2p_uniform<GRAD, false>(lf, tb, n, e0, e1, near_ent, leaf_cell, src_start, rec, tx, ty, tz, phi, grad);
+      continue;
+    }
     int np = 64, phases = 1;
     if (n <= 64) {
       np = 1;
@@ -804,9 +944,22 @@ void vjp_combine(hipStream_t s, size_t n, const double *q, const double *g, cons
                      grad_pos);
 }
 
-void gather_charges(hipStream_t s, const double *q, const uint32_t *order, size_t n, double *q64, float *q32) {
+void build_source_records(const double *sx, const double *sy, const double *sz, size_t n, float *host_rec) {
+  for (size_t j = 0; j < n; ++j) {
+    const double u[3] = {sx[j], sy[j], sz[j]};
+    float *r = host_rec + 8 * j;
+    for (int k = 0; k < 3; ++k) {
+      r[k] = float(u[k]);
+      r[4 + k] = float(u[k] - double(r[k]));
+    }
+    r[3] = 0.0f;
+    r[7] = 0.0f;
+  }
+}
+
+void gather_charges(hipStream_t s, const double *q, const uint32_t *order, size_t n, double *q64, float *rec) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_gather_charges, dim3(grid_for(n, 256)), dim3(256), 0, s, q, order, n, q64, q32);
+  hipLaunchKernelGGL(k_gather_charges, dim3(grid_for(n, 256)), dim3(256), 0, s, q, order, n, q64, rec);
 }
 
 void output_pass(hipStream_t s, uint32_t nleaves, const uint32_t *tgt_start, const double *leaf_r,
@@ -821,23 +974,25 @@ template <typename T>
 void p2p(hipStream_t s, uint32_t leaf_first, uint32_t count, int gradients, const uint32_t *near_off,
          const uint32_t *near_ent, const uint32_t *leaf_cell, const uint32_t *src_start,
          const uint32_t *tgt_start, const double *sx, const double *sy, const double *sz, const double *sq,
-         const double *tx, const double *ty, const double *tz, double *phi, double *grad) {
+         const float *rec, const double *tx, const double *ty, const double *tz, double *phi, double *grad) {
   if (count == 0) return;
+  const float4 *r4 = reinterpret_cast<const float4 *>(rec);
   if (gradients)
     hipLaunchKernelGGL((k_p2p<T, true>), dim3(xcd_grid(count)), dim3(64), 0, s, leaf_first, count, near_off, near_ent,
-                       leaf_cell, src_start, tgt_start, sx, sy, sz, sq, tx, ty, tz, phi, grad);
+                       leaf_cell, src_start, tgt_start, sx, sy, sz, sq, r4, tx, ty, tz, phi, grad);
   else
     hipLaunchKernelGGL((k_p2p<T, false>), dim3(xcd_grid(count)), dim3(64), 0, s, leaf_first, count, near_off,
-                       near_ent, leaf_cell, src_start, tgt_start, sx, sy, sz, sq, tx, ty, tz, phi, grad);
+                       near_ent, leaf_cell, src_start, tgt_start, sx, sy, sz, sq, r4, tx, ty, tz, phi, grad);
 }
 
 template void p2p<float>(hipStream_t, uint32_t, uint32_t, int, const uint32_t *, const uint32_t *, const uint32_t *,
                          const uint32_t *, const uint32_t *, const double *, const double *, const double *,
-                         const double *, const double *, const double *, const double *, double *, double *);
+                         const double *, const float *, const double *, const double *, const double *, double *,
+                         double *);
 template void p2p<double>(hipStream_t, uint32_t, uint32_t, int, const uint32_t *, const uint32_t *,
                           const uint32_t *, const uint32_t *, const uint32_t *, const double *, const double *,
-                          const double *, const double *, const double *, const double *, const double *,
-                          double *, double *);
+                          const double *, const double *, const float *, const double *, const double *,
+                          const double *, double *, double *);
 
 void expand(hipStream_t s, int p, int irregular, uint32_t rows, const uint32_t *row_box, const uint32_t *row_cell,
             const uint32_t *row_off, const uint32_t *row_ent, const uint32_t *leaf_cell,

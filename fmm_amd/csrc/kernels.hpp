@@ -48,7 +48,13 @@ __device__ __forceinline__ uint32_t xcd_index(uint32_t n, uint32_t chunk) {
 hipError_t upload_constants();
 
 void fill(hipStream_t s, double *p, size_t n, double v);
-void gather_charges(hipStream_t s, const double *q, const uint32_t *order, size_t n, double *q64, float *q32);
+// q64[j] = q[order[j]]; with rec set, also rec[8 j + 3] = f32 of it (the charge slot of
+// the mixed-precision P2P source records)
+void gather_charges(hipStream_t s, const double *q, const uint32_t *order, size_t n, double *q64, float *rec);
+// Mixed-precision P2P source records, 8 floats per source in leaf order: (x_hi, y_hi,
+// z_hi, q, x_lo, y_lo, z_lo, 0) of the source's own-leaf coordinates u = hi + lo.  The
+// charge slot is written by gather_charges.
+void build_source_records(const double *sx, const double *sy, const double *sz, size_t n, float *host_rec);
 // Rotation M2L (rotation.hip): rows of target boxes, entries (source box, offset index);
 // tables per offset (fwd, bwd: rb values) and per distance (coax: cb values).
 template <typename T>
@@ -69,12 +75,14 @@ void output_pass(hipStream_t s, uint32_t nleaves, const uint32_t *tgt_start, con
                  int gradients);
 
 // P2P over the leaves leaf_first .. leaf_first + count (global leaf indices), computed in
-// T (double, or float with double-float separations); f64 coordinates and charges.
+// T (double, or float with double-float separations); f64 coordinates and charges.  T =
+// float also reads the source records of build_source_records (rec is unused for T =
+// double and may be null).
 template <typename T>
 void p2p(hipStream_t s, uint32_t leaf_first, uint32_t count, int gradients, const uint32_t *near_off,
          const uint32_t *near_ent, const uint32_t *leaf_cell, const uint32_t *src_start,
          const uint32_t *tgt_start, const double *sx, const double *sy, const double *sz, const double *sq,
-         const double *tx, const double *ty, const double *tz, double *phi, double *grad);
+         const float *rec, const double *tx, const double *ty, const double *tz, double *phi, double *grad);
 
 // P2M (irregular = 0, row_ent the leaf of each row, unit frame) or P2L (irregular = 1,
 // row_ent the X-list leaves).  rows: output box gid and cell per row, CSR of entries.

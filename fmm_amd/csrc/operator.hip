@@ -111,6 +111,11 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
   tz_.upload(tz);
   q_.alloc(ns_);
   q_in_.alloc(ns_);
+  if (opt_.precision == kMixed && ns_ > 0) {
+    std::vector<float> rec(8 * size_t(ns_));
+    dev::build_source_records(sx.data(), sy.data(), sz.data(), size_t(ns_), rec.data());
+    src_rec_.upload(rec);
+  }
 
   // expansions and outputs (the pair-mode scratch rows are sized in build_schedules)
   mult_.alloc(nbox_total_ * nc_);
@@ -515,7 +520,7 @@ void DeviceOperator::build_rotation(const Plan &plan) {
 
 size_t DeviceOperator::device_bytes() const {
   return mult_.bytes() + loc_.bytes() + slots_.bytes() + slots32_.bytes() + t_m2l_.bytes() + t_m2m_.bytes() +
-         t_l2l_.bytes() + sx_.bytes() * 8;
+         t_l2l_.bytes() + sx_.bytes() * 8 + src_rec_.bytes();
 }
 
 void DeviceOperator::launch_contract(const Schedule &s, uint32_t task_first, uint32_t ntasks, const double *X,
@@ -589,7 +594,7 @@ void DeviceOperator::load_charges(const double *charges, bool on_device) {
     hip_check(hipMemcpyAsync(q_in_.ptr, charges, ns_ * sizeof(double), hipMemcpyHostToDevice, stream_), "charges");
     src = q_in_.ptr;
   }
-  dev::gather_charges(stream_, src, src_order_.ptr, ns_, q_.ptr, nullptr);
+  dev::gather_charges(stream_, src, src_order_.ptr, ns_, q_.ptr, src_rec_.count ? src_rec_.ptr : nullptr);
 }
 
 void DeviceOperator::reset() {
@@ -658,12 +663,12 @@ void DeviceOperator::m2p(int level) {
 void DeviceOperator::launch_p2p(uint32_t leaf_first, uint32_t count) {
   if (opt_.precision == kMixed)
     dev::p2p<float>(stream_, leaf_first, count, opt_.gradients, near_off_.ptr, near_ent_.ptr, leaf_cell_.ptr,
-                    src_start_.ptr, tgt_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, q_.ptr, tx_.ptr, ty_.ptr, tz_.ptr,
-                    phi_hat_.ptr, grad_hat_.ptr);
+                    src_start_.ptr, tgt_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, q_.ptr, src_rec_.ptr, tx_.ptr,
+                    ty_.ptr, tz_.ptr, phi_hat_.ptr, grad_hat_.ptr);
   else
     dev::p2p<double>(stream_, leaf_first, count, opt_.gradients, near_off_.ptr, near_ent_.ptr, leaf_cell_.ptr,
-                     src_start_.ptr, tgt_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, q_.ptr, tx_.ptr, ty_.ptr, tz_.ptr,
-                     phi_hat_.ptr, grad_hat_.ptr);
+                     src_start_.ptr, tgt_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, q_.ptr, nullptr, tx_.ptr, ty_.ptr,
+                     tz_.ptr, phi_hat_.ptr, grad_hat_.ptr);
 }
 
 void DeviceOperator::p2p(int level) {

@@ -201,6 +201,7 @@ class DeviceOperator {
   size_t nslots_ = 0;                  // pair-mode scratch rows (slot ids follow the box ids)
   DevBuf<double> slots_;               // plain f64: scratch rows
   DevBuf<float> slots32_;              // mixed: scratch rows (one group's f32 sum each)
+  DevBuf<float> src_rec_;              // mixed: P2P source records, 8 floats per source
 
   bool timing_ = false;
   struct Mark {

@@ -180,3 +180,30 @@ def test_mixed_p2p_within_f32(mixed_pair):
     e1, e2 = _rel(out.potential, phi), _rel(out.gradient, grad)
     _log(f"mixed p2p alone: rel phi {e1:.3e} grad {e2:.3e}")
     assert e1 < 1e-6 and e2 < 1e-6
+
+
+@pytest.mark.parametrize("leaf", [48, 128])
+def test_mixed_p2p_uniform_chunks(leaf):
+    # Leaves of more than 32 targets take the mixed P2P's uniform-source path (one
+    # target per lane up to 64, two above); coincident copies and pairs 1e-9 apart
+    # check the r^2 == 0 select and the double-float separations against f64
+    _log(f"case mixed p2p uniform chunks leaf={leaf}")
+    rng = O.Rng(4242)
+    base = rng.cube(16000 if leaf <= 64 else 50000)
+    near = base[:300] + 1e-9 * rng.cube(300)
+    pts = np.vstack([base, base[300:500], near])
+    q = rng.normals(len(pts))
+    mixed = fmm_amd.FmmBuilder(8).precision("mixed").max_points_per_leaf(leaf).build(pts)
+    sizes = np.bincount(mixed.plan.point_leaves())
+    assert sizes.max() > (64 if leaf > 64 else 32)
+    op = mixed.operator
+    op.load_charges(q)
+    op.reset()
+    for level in range(mixed.plan.nlevels):
+        op.call("p2p", level)
+    out = op.read_output()
+    phi, grad = _near_field(mixed.plan, pts, q, mixed)
+    assert np.all(np.isfinite(out.potential)) and np.all(np.isfinite(out.gradient))
+    e1, e2 = _rel(out.potential, phi), _rel(out.gradient, grad)
+    _log(f"  largest leaf {sizes.max()}: rel phi {e1:.3e} grad {e2:.3e}")
+    assert e1 < 1e-6 and e2 < 1e-6
