@@ -421,7 +421,7 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, uint3
   for (int nt = 0; nt < 2; ++nt) {
     if (out[nt] == 0xFFFFFFFFu) continue;
     if (out[nt] >= first_slot) {
-      // scratch slot of pair mode: the task's one group, whose f32 sum is exact in f32
+      // scratch slot (written by this task only): its f64 sum rounded to f32
       float *s = S + size_t(out[nt] - first_slot) * nc;
 #pragma unroll
       for (int i = 0; i < MH; ++i) {

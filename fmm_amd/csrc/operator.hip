@@ -281,7 +281,10 @@ void DeviceOperator::kind_times(double *ms, int64_t *launches) {
 
 void DeviceOperator::build_schedules(const Plan &plan) {
   auto gid = [&](int level, uint32_t b) { return box_base_[level] + b; };
-  // rank of offset d for targets of octant o: (parent-level neighbour D, source octant o')
+  // rank of offset d for targets of octant o: (parent-level neighbour D, source octant o').
+  // The plan's V offsets are d = target - source, so the source child sits at ob - d
+  // (child units, from the target's parent's lower corner): D = floor((ob - d) / 2) + 1
+  // per axis in 0..2 and o' = (ob - d) mod 2, for every offset a V list can hold.
   for (int oc = 0; oc < 8; ++oc) {
     const int ob[3] = {(oc >> 2) & 1, (oc >> 1) & 1, oc & 1};
     for (int d = 0; d < 316; ++d) {
@@ -289,7 +292,7 @@ void DeviceOperator::build_schedules(const Plan &plan) {
       offset_of_index(d, dv);
       int D = 0, op = 0;
       for (int a = 0; a < 3; ++a) {
-        const int s = ob[a] + dv[a];
+        const int s = ob[a] - dv[a];
         const int bit = ((s % 2) + 2) % 2;
         D = 3 * D + ((s - bit) / 2 + 1);
         op = 2 * op + bit;
@@ -393,34 +396,55 @@ void DeviceOperator::build_schedules(const Plan &plan) {
       if (tile.pairs == 0 || 2 * tile.pairs >= tile.slots) {
         all.append(tile);
       } else {
-        // pair mode: slot of every V pair, by offset then target
-        std::vector<std::vector<std::pair<uint32_t, uint32_t>>> by_d(316);  // (V entry, target box)
-        for (uint32_t b = 0; b < plan.nboxes(l); ++b)
-          for (uint32_t e = V.off[b]; e < V.off[b + 1]; ++e) by_d[V.aux[e]].push_back({e, b});
-        std::vector<uint32_t> slot_of(V.ent.size());
-        ScheduleBuilder pairs;
-        for (int d = 0; d < 316; ++d) {
-          const auto &list = by_d[d];
-          for (size_t at = 0; at < list.size(); at += C) {
-            std::vector<uint32_t> cols;
-            std::vector<std::vector<std::pair<uint16_t, uint32_t>>> lists;
-            for (size_t k = at; k < list.size() && k < at + C; ++k) {
-              const uint32_t e = list[k].first;
-              slot_of[e] = static_cast<uint32_t>(nbox_total_ + nslots_++);
-              cols.push_back(slot_of[e]);
-              lists.push_back({{uint16_t(d), gid(l, V.ent[e])}});
-            }
-            pairs.add_task(cols, lists);
-          }
-        }
-        pairs.slots = pairs.ntasks() * uint64_t(C);
-        all.append(pairs);
+        // Sparse level (few of a target's possible offsets present): clustered mode.
+        // A target's V entries fall into the children of its parent's neighbours D;
+        // each (target, D) gets a scratch slot, and a task holds the slots of one
+        // (target octant o, D) class, whose columns share the same offsets d = 2 D + o'
+        // - o (at most 8 groups, one per source octant o').  The slots of a target are
+        // then added in ascending D, so every local adds its V terms in rank order.
+        // Classes of the same D and chunk position are adjacent for all eight o (the
+        // children of the same parents, reading the sources of the same neighbour cells
+        // on one XCD).
+        struct Member {
+          uint32_t slot;
+          std::vector<std::pair<uint16_t, uint32_t>> list;
+        };
+        std::vector<std::vector<Member>> cls(8 * 27);  // [D * 8 + o]
         for (uint32_t b = 0; b < plan.nboxes(l); ++b) {
           if (V.off[b + 1] == V.off[b]) continue;
+          const int o = octant_of(plan.keys[l][b]);
+          std::map<int, std::vector<std::pair<uint16_t, uint32_t>>> by_D;
+          for (uint32_t e = V.off[b]; e < V.off[b + 1]; ++e) {
+            const int D = m2l_rank_[o][V.aux[e]] / 8;
+            require(D >= 0 && D < 27, "M2L schedule: V offset outside the parent's neighbours", kUnsupportedLayout);
+            by_D[D].push_back({V.aux[e], gid(l, V.ent[e])});
+          }
           red_id.push_back(gid(l, b));
-          for (uint32_t e = V.off[b]; e < V.off[b + 1]; ++e) red_ent.push_back(slot_of[e]);
+          for (auto &kv : by_D) {
+            const uint32_t slot = static_cast<uint32_t>(nbox_total_ + nslots_++);
+            red_ent.push_back(slot);
+            cls[size_t(kv.first) * 8 + o].push_back({slot, std::move(kv.second)});
+          }
           red_off.push_back(static_cast<uint32_t>(red_ent.size()));
         }
+        ScheduleBuilder clustered;
+        for (int D = 0; D < 27; ++D) {
+          size_t most = 0;
+          for (int o = 0; o < 8; ++o) most = std::max(most, cls[size_t(D) * 8 + o].size());
+          for (size_t at = 0; at < most; at += size_t(C))
+            for (int o = 0; o < 8; ++o) {
+              const auto &mem = cls[size_t(D) * 8 + o];
+              if (at >= mem.size()) continue;
+              std::vector<uint32_t> cols;
+              std::vector<std::vector<std::pair<uint16_t, uint32_t>>> lists;
+              for (size_t k = at; k < mem.size() && k < at + size_t(C); ++k) {
+                cols.push_back(mem[k].slot);
+                lists.push_back(mem[k].list);
+              }
+              clustered.add_task(cols, lists, m2l_rank_[o].data());
+            }
+        }
+        all.append(clustered);
       }
     }
     m2l_level_task_[l + 1] = all.ntasks();
