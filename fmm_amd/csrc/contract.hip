@@ -239,6 +239,84 @@ __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, uint3
   }
 }
 
+// ---- M2M with the groups split over the waves ------------------------------------------------
+// Tasks of 16 parents whose up to 8 groups are the child octants: wave w forms the
+// product of group w (its table read straight from L2 in the swizzled layout, its 16
+// children's multipoles as B) over all M-tiles, with no LDS staging and no barrier per
+// chunk, so a task is 11 chunks long instead of 8 x 11.  The eight partial sums then
+// meet in LDS in a fixed tree, ((g0 + g4) + (g2 + g6)) + ((g1 + g5) + (g3 + g7)), and
+// wave 0 adds the result to the parents' rows.
+template <int MT>
+__global__ __launch_bounds__(512) void k_m2m_split(int nc, int nchunks, uint32_t ntasks,
+                                                   const uint32_t *__restrict__ task_group_off,
+                                                   const uint16_t *__restrict__ group_table,
+                                                   const uint32_t *__restrict__ group_src,
+                                                   const uint32_t *__restrict__ col_out,
+                                                   const double *__restrict__ tsw, const double *__restrict__ X,
+                                                   double *__restrict__ Y) {
+  __shared__ d4 red[4][MT][64];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int col = lane & 15, g = lane >> 4;
+  const uint32_t task = blockIdx.x;
+  if (task >= ntasks) return;  // uniform over the workgroup
+  const uint32_t g0 = task_group_off[task], g1 = task_group_off[task + 1];
+  const int KS = nchunks * KC;
+  const size_t tsize = size_t(nchunks) * MT * KC * 64;  // one swizzled table
+  d4 acc[MT];
+#pragma unroll
+  for (int i = 0; i < MT; ++i) acc[i] = d4{0.0, 0.0, 0.0, 0.0};
+  const uint32_t gi = g0 + uint32_t(wave);
+  if (gi < g1) {
+    const uint32_t w2 = reinterpret_cast<const uint32_t *>(group_table)[gi >> 1];
+    const uint32_t table = (gi & 1) ? (w2 >> 16) : (w2 & 0xFFFFu);
+    const double *a = tsw + size_t(table) * tsize;
+    const uint32_t src = group_src[size_t(gi) * 16 + col];
+    const bool has = src != 0xFFFFFFFFu;
+    const double *x = X + size_t(has ? src : 0) * nc;
+    for (int c = 0; c < nchunks; ++c) {
+      double b[KC];
+#pragma unroll
+      for (int kk = 0; kk < KC; ++kk) {
+        const int k = c * KC + kk + g * KS;
+        const double v = x[min(k, nc - 1)];
+        b[kk] = (has && k < nc) ? v : 0.0;
+      }
+#pragma unroll
+      for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+          acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[((size_t(c) * MT + mt) * KC + kk) * 64 + lane], b[kk],
+                                                         acc[mt], 0, 0, 0);
+    }
+  }
+  // fixed-order tree over the waves' partial sums
+#pragma unroll
+  for (int half = 4; half >= 1; half >>= 1) {
+    __syncthreads();
+    if (wave >= half && wave < 2 * half)
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) red[wave - half][mt][lane] = acc[mt];
+    __syncthreads();
+    if (wave < half)
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) acc[mt] += red[wave][mt][lane];
+  }
+  if (wave == 0) {
+    const uint32_t out = col_out[size_t(task) * 16 + col];
+    if (out != 0xFFFFFFFFu) {
+      double *y = Y + size_t(out) * nc;
+      // accumulator element r of lane (g, j): row 4 r + g of the M-tile (f64 16x16x4)
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = mt * 16 + 4 * r + g;
+          if (row < nc) y[row] += acc[mt][r];
+        }
+    }
+  }
+}
+
 // ---- bf16x3 contraction ---------------------------------------------------------------------
 
 __device__ __forceinline__ bf8 as_bf8(u4 v) {
@@ -519,6 +597,38 @@ void contract_f64(hipStream_t s, int p, int width, uint32_t ntasks, const uint32
       break;
   }
 #undef FMMH_CONTRACT
+}
+
+void m2m_split(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off, const uint16_t *group_table,
+               const uint32_t *group_src, const uint32_t *col_out, const double *tsw, const double *X, double *Y) {
+  if (ntasks == 0) return;
+  const int nc = (p + 1) * (p + 1), nch = k_chunks(p);
+#define FMMH_M2M(MT)                                                                                             \
+  case MT:                                                                                                       \
+    hipLaunchKernelGGL((k_m2m_split<MT>), dim3(ntasks), dim3(512), 0, s, nc, nch, ntasks, task_group_off,       \
+                       group_table, group_src, col_out, tsw, X, Y);                                              \
+    break;
+  switch (m_tiles(p)) {
+    FMMH_M2M(1)
+    FMMH_M2M(2)
+    FMMH_M2M(3)
+    FMMH_M2M(4)
+    FMMH_M2M(5)
+    FMMH_M2M(6)
+    FMMH_M2M(7)
+    FMMH_M2M(8)
+    FMMH_M2M(9)
+    FMMH_M2M(10)
+    FMMH_M2M(11)
+    FMMH_M2M(12)
+    FMMH_M2M(13)
+    FMMH_M2M(14)
+    FMMH_M2M(15)
+    FMMH_M2M(16)
+    default:
+      break;
+  }
+#undef FMMH_M2M
 }
 
 void contract_bf16(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off, const uint16_t *group_table,

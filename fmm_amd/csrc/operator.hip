@@ -306,8 +306,8 @@ void DeviceOperator::build_schedules(const Plan &plan) {
   for (int l = 0; l < nlevels_; ++l) {
     // M2M into the parents of level l, by octant (fmm-plan/src/operator.rs:256-262)
     if (l + 1 < nlevels_) {
-      // narrow tasks (waves split the rows) while wide ones would leave the GPU idle
-      const int width = plan.nboxes(l) < 64u * dev::COLS ? 16 : dev::COLS;
+      // tasks of 16 parents, one wave per child octant (dev::m2m_split)
+      const int width = 16;
       ScheduleBuilder sb(width);
       const int C = width;
       std::vector<uint32_t> cols;
@@ -641,7 +641,9 @@ void DeviceOperator::p2m(int level) {
 void DeviceOperator::m2m(int level) {
   if (level < 0 || level + 1 >= nlevels_) return;
   timed(1, [&] {
-    launch_contract(m2m_[level], 0, m2m_[level].ntasks, mult_.ptr, mult_.ptr, t_m2m_);
+    const Schedule &sc = m2m_[level];
+    dev::m2m_split(stream_, opt_.p, sc.ntasks, sc.task_group_off.ptr, sc.group_table.ptr, sc.group_src.ptr,
+                   sc.col_out.ptr, t_m2m_.ptr, mult_.ptr, mult_.ptr);
   });
 }
 
