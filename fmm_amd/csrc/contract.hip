@@ -240,12 +240,13 @@ __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, uint3
 }
 
 // ---- M2M with the groups split over the waves ------------------------------------------------
-// Tasks of 16 parents whose up to 8 groups are the child octants: wave w forms the
-// product of group w (its table read straight from L2 in the swizzled layout, its 16
-// children's multipoles as B) over all M-tiles, with no LDS staging and no barrier per
-// chunk, so a task is 11 chunks long instead of 8 x 11.  The eight partial sums then
-// meet in LDS in a fixed tree, ((g0 + g4) + (g2 + g6)) + ((g1 + g5) + (g3 + g7)), and
-// wave 0 adds the result to the parents' rows.
+// Tasks of 16 parent columns whose groups are the child octants (8, or 16 when each
+// octant has a real and an imaginary table): wave w forms the products of groups w,
+// w + 8, ... (its tables read straight from L2 in the swizzled layout, the children's
+// expansions as B) over all M-tiles, with no LDS staging and no barrier per chunk, so a
+// task is 11 chunks long (p = 12) instead of 8 x 11.  The eight waves' sums then meet in
+// LDS in a fixed tree, ((w0 + w4) + (w2 + w6)) + ((w1 + w5) + (w3 + w7)), and wave 0
+// adds the result to the parents' rows.
 template <int MT>
 __global__ __launch_bounds__(512) void k_m2m_split(int nc, int nchunks, uint32_t ntasks,
                                                    const uint32_t *__restrict__ task_group_off,
@@ -265,8 +266,7 @@ __global__ __launch_bounds__(512) void k_m2m_split(int nc, int nchunks, uint32_t
   d4 acc[MT];
 #pragma unroll
   for (int i = 0; i < MT; ++i) acc[i] = d4{0.0, 0.0, 0.0, 0.0};
-  const uint32_t gi = g0 + uint32_t(wave);
-  if (gi < g1) {
+  for (uint32_t gi = g0 + uint32_t(wave); gi < g1; gi += 8) {
     const uint32_t w2 = reinterpret_cast<const uint32_t *>(group_table)[gi >> 1];
     const uint32_t table = (gi & 1) ? (w2 >> 16) : (w2 & 0xFFFFu);
     const double *a = tsw + size_t(table) * tsize;

@@ -26,14 +26,15 @@ namespace fmmh {
 namespace {
 
 // rank of offset d among the M2L groups of a target of octant o: (parent-level neighbour,
-// source octant), as the Laplace operator orders them for L2 reuse
+// source octant), as the Laplace operator orders them for L2 reuse; the plan's offsets
+// are target minus source, so the source child sits at ob - d from the parent's corner
 uint16_t m2l_rank(int oc, int d) {
   const int ob[3] = {(oc >> 2) & 1, (oc >> 1) & 1, oc & 1};
   int dv[3];
   offset_of_index(d, dv);
   int D = 0, op = 0;
   for (int a = 0; a < 3; ++a) {
-    const int s = ob[a] + dv[a];
+    const int s = ob[a] - dv[a];
     const int bit = ((s % 2) + 2) % 2;
     D = 3 * D + ((s - bit) / 2 + 1);
     op = 2 * op + bit;

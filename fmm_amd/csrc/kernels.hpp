@@ -110,8 +110,9 @@ void contract_f64(hipStream_t s, int p, int width, uint32_t ntasks, const uint32
                   const uint16_t *group_table, const uint32_t *group_src, const uint32_t *col_out,
                   const double *tables_swizzled, const double *X, double *Y, uint32_t first_slot, double *S,
                   uint32_t xchunk);
-// M2M tasks of 16 parents whose groups are the child octants (at most 8), one wave per
-// group, the partial sums added in a fixed tree (contract.hip k_m2m_split).
+// M2M tasks of 16 parent columns whose groups are the child octants' tables: wave w
+// takes groups w, w + 8, ..., the eight waves' sums added in a fixed tree
+// (contract.hip k_m2m_split).
 void m2m_split(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off, const uint16_t *group_table,
                const uint32_t *group_src, const uint32_t *col_out, const double *tsw, const double *X, double *Y);
 void contract_bf16(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off,
