@@ -321,6 +321,7 @@ HelmholtzOperator::HelmholtzOperator(const Plan &plan, const HelmholtzOptions &o
     std::vector<uint16_t> m2l16;
     const size_t sw16 = dev::swizzled_table_size_bf16(p);
     ScheduleBuilder all;
+    std::vector<uint32_t> red_id, red_cell, red_off{0}, red_ent;
     uint32_t id_base = 0;
     for (int l = 2; l < nlevels_; ++l) {
       const Csr &V = plan.v[l];
@@ -350,20 +351,23 @@ HelmholtzOperator::HelmholtzOperator(const Plan &plan, const HelmholtzOptions &o
           rank[o][id_base + 2 * j] = uint16_t(2 * m2l_rank(o, used[j]));
           rank[o][id_base + 2 * j + 1] = uint16_t(2 * m2l_rank(o, used[j]) + 1);
         }
+      // a V entry's (re, im) column terms: Re L += Tr Re M + Ti (-Im M), Im L += Tr Im M + Ti Re M
+      auto terms = [&](uint32_t e, std::vector<std::pair<uint16_t, uint32_t>> &re,
+                       std::vector<std::pair<uint16_t, uint32_t>> &im) {
+        const uint32_t s = gid(l, V.ent[e]);
+        const uint16_t tr = uint16_t(id_base + 2 * slot[V.aux[e]]), ti = uint16_t(tr + 1);
+        re.push_back({tr, 3 * s});
+        re.push_back({ti, 3 * s + 2});
+        im.push_back({tr, 3 * s + 1});
+        im.push_back({ti, 3 * s});
+      };
       std::vector<std::vector<uint32_t>> cls_cols(8);
       std::vector<std::vector<std::vector<std::pair<uint16_t, uint32_t>>>> cls_lists(8);
       for (uint32_t b = 0; b < plan.nboxes(l); ++b) {
         if (V.off[b + 1] == V.off[b]) continue;
         const int o = octant_of(plan.keys[l][b]);
         std::vector<std::pair<uint16_t, uint32_t>> re, im;
-        for (uint32_t e = V.off[b]; e < V.off[b + 1]; ++e) {
-          const uint32_t s = gid(l, V.ent[e]);
-          const uint16_t tr = uint16_t(id_base + 2 * slot[V.aux[e]]), ti = uint16_t(tr + 1);
-          re.push_back({tr, 3 * s});
-          re.push_back({ti, 3 * s + 2});
-          im.push_back({tr, 3 * s + 1});
-          im.push_back({ti, 3 * s});
-        }
+        for (uint32_t e = V.off[b]; e < V.off[b + 1]; ++e) terms(e, re, im);
         cls_cols[o].push_back(3 * gid(l, b));
         cls_lists[o].push_back(std::move(re));
         cls_cols[o].push_back(3 * gid(l, b) + 1);
@@ -372,6 +376,7 @@ HelmholtzOperator::HelmholtzOperator(const Plan &plan, const HelmholtzOptions &o
       const size_t C = dev::COLS;
       size_t most = 0;
       for (int o = 0; o < 8; ++o) most = std::max(most, cls_cols[o].size());
+      ScheduleBuilder tile;
       for (size_t at = 0; at < most; at += C)
         for (int o = 0; o < 8; ++o) {
           if (at >= cls_cols[o].size()) continue;
@@ -379,11 +384,77 @@ HelmholtzOperator::HelmholtzOperator(const Plan &plan, const HelmholtzOptions &o
           std::vector<uint32_t> cols(cls_cols[o].begin() + at, cls_cols[o].begin() + end);
           std::vector<std::vector<std::pair<uint16_t, uint32_t>>> lists(cls_lists[o].begin() + at,
                                                                           cls_lists[o].begin() + end);
-          all.add_task(cols, lists, rank[o].data());
+          tile.add_task(cols, lists, rank[o].data());
         }
+      // Box mode needs both a dense level and enough tasks to fill the GPU: a task walks
+      // all of its ~2 x 189 groups in sequence, so a level of a few dozen tasks (level 4
+      // of a 1M-point tree holds 64) would be one long chain per CU.
+      constexpr uint32_t kMinBoxTasks = 256;
+      if (tile.pairs == 0 || (2 * tile.pairs >= tile.slots && tile.ntasks() >= kMinBoxTasks)) {
+        all.append(tile);
+      } else {
+        // Sparse or small level: clustered mode as in the Laplace operator.  Each
+        // (target, parent-level neighbour D) gets two scratch rows (re, im); a task holds
+        // the rows of one (target octant, D) class, whose columns share their offsets
+        // (at most 2 x 8 groups); a target plane's rows are added in ascending D after.
+        struct Member {
+          uint32_t row;
+          std::vector<std::pair<uint16_t, uint32_t>> list;
+        };
+        std::vector<std::vector<Member>> cls(8 * 27);
+        for (uint32_t b = 0; b < plan.nboxes(l); ++b) {
+          if (V.off[b + 1] == V.off[b]) continue;
+          const int o = octant_of(plan.keys[l][b]);
+          std::map<int, std::vector<uint32_t>> by_D;
+          for (uint32_t e = V.off[b]; e < V.off[b + 1]; ++e) {
+            const int D = m2l_rank(o, V.aux[e]) / 8;
+            require(D >= 0 && D < 27, "Helmholtz M2L schedule: V offset outside the parent's neighbours",
+                    kUnsupportedLayout);
+            by_D[D].push_back(e);
+          }
+          std::vector<uint32_t> rows_re, rows_im;
+          for (const auto &kv : by_D) {
+            std::vector<std::pair<uint16_t, uint32_t>> re, im;
+            for (uint32_t e : kv.second) terms(e, re, im);
+            const uint32_t r0 = static_cast<uint32_t>(3 * nbox_total_ + nslots_), r1 = r0 + 1;
+            nslots_ += 2;
+            rows_re.push_back(r0);
+            rows_im.push_back(r1);
+            cls[size_t(kv.first) * 8 + o].push_back({r0, std::move(re)});
+            cls[size_t(kv.first) * 8 + o].push_back({r1, std::move(im)});
+          }
+          red_id.push_back(3 * gid(l, b));
+          red_ent.insert(red_ent.end(), rows_re.begin(), rows_re.end());
+          red_off.push_back(static_cast<uint32_t>(red_ent.size()));
+          red_id.push_back(3 * gid(l, b) + 1);
+          red_ent.insert(red_ent.end(), rows_im.begin(), rows_im.end());
+          red_off.push_back(static_cast<uint32_t>(red_ent.size()));
+        }
+        for (int D = 0; D < 27; ++D) {
+          size_t most_d = 0;
+          for (int o = 0; o < 8; ++o) most_d = std::max(most_d, cls[size_t(D) * 8 + o].size());
+          for (size_t at = 0; at < most_d; at += C)
+            for (int o = 0; o < 8; ++o) {
+              const auto &mem = cls[size_t(D) * 8 + o];
+              if (at >= mem.size()) continue;
+              std::vector<uint32_t> cols;
+              std::vector<std::vector<std::pair<uint16_t, uint32_t>>> lists;
+              for (size_t k = at; k < mem.size() && k < at + C; ++k) {
+                cols.push_back(mem[k].row);
+                lists.push_back(mem[k].list);
+              }
+              all.add_task(cols, lists, rank[o].data());
+            }
+        }
+      }
       id_base = nid;
     }
     all.finish(m2l_);
+    upload_rows(m2l_red_, red_id, red_cell, red_off, red_ent);
+    if (opt_.precision == kMixed)
+      slots32_.alloc(nslots_ * nc_);
+    else
+      slots_.alloc(nslots_ * nc_);
     if (opt_.precision == kMixed) {
       if (m2l16.empty()) m2l16.assign(sw16, 0);
       t_m2l16_.upload(m2l16);
@@ -487,6 +558,8 @@ void HelmholtzOperator::launch_p2p() {
 void HelmholtzOperator::evaluate() {
   const int L = nlevels_ - 1, p = opt_.p;
   const uint32_t none = kNone;
+  // scratch rows of the sparse M2L levels follow the 3 planes of every box
+  const uint32_t first_slot = static_cast<uint32_t>(3 * nbox_total_);
   // upward: P2M of every leaf, then per level the negated planes and M2M into the parents
   timed(0, [&] {
     dev::h_expand(stream_, p, 0, p2m_rows_.rows, p2m_rows_.row_id.ptr, p2m_rows_.row_cell.ptr, p2m_rows_.row_off.ptr,
@@ -510,16 +583,19 @@ void HelmholtzOperator::evaluate() {
         dev::split_bf16(stream_, p, v1 - v0, mult_.ptr + v0 * nc_, mult16_.ptr + v0 * sz);
         timed(kKindM2lContract, [&] {
           dev::contract_bf16(stream_, p, m2l_.ntasks, m2l_.task_group_off.ptr, m2l_.group_table.ptr,
-                             m2l_.group_src.ptr, m2l_.col_out.ptr, t_m2l16_.ptr, mult16_.ptr, loc_.ptr, none,
-                             nullptr, kM2lXcdRun);
+                             m2l_.group_src.ptr, m2l_.col_out.ptr, t_m2l16_.ptr, mult16_.ptr, loc_.ptr, first_slot,
+                             slots32_.ptr, kM2lXcdRun);
         });
       } else {
         timed(kKindM2lContract, [&] {
           dev::contract_f64(stream_, p, m2l_.width, m2l_.ntasks, m2l_.task_group_off.ptr, m2l_.group_table.ptr,
-                            m2l_.group_src.ptr, m2l_.col_out.ptr, t_m2l_.ptr, mult_.ptr, loc_.ptr, none, nullptr,
-                            kM2lXcdRun);
+                            m2l_.group_src.ptr, m2l_.col_out.ptr, t_m2l_.ptr, mult_.ptr, loc_.ptr, first_slot,
+                            slots_.ptr, kM2lXcdRun);
         });
       }
+      if (m2l_red_.rows)
+        dev::reduce_slots(stream_, p, m2l_red_.rows, m2l_red_.row_id.ptr, m2l_red_.row_off.ptr, m2l_red_.row_ent.ptr,
+                          first_slot, slots_.ptr, opt_.precision == kMixed ? slots32_.ptr : nullptr, loc_.ptr);
     });
   if (p2l_rows_.rows)
     timed(4, [&] {

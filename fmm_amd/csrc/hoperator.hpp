@@ -77,6 +77,12 @@ class HelmholtzOperator {
   Rows p2m_rows_, l2p_rows_, m2p_rows_, p2l_rows_;
   std::vector<Schedule> m2m_, l2l_;  // per level (M2M into the parents of level l, L2L into level l)
   Schedule m2l_;
+  // sparse M2L levels: scratch rows (ids from 3 nbox_total_ on) of one (target plane,
+  // parent-level neighbour) each, and the ordered reduction of a target plane's rows
+  size_t nslots_ = 0;
+  DevBuf<double> slots_;
+  DevBuf<float> slots32_;
+  Rows m2l_red_;
 
   bool timing_ = false;
   struct Mark {
