@@ -393,10 +393,14 @@ void DeviceOperator::build_schedules(const Plan &plan) {
                                                                           cls_lists[o].begin() + end);
           tile.add_task(cols, lists, m2l_rank_[o].data());
         }
-      if (tile.pairs == 0 || 2 * tile.pairs >= tile.slots) {
+      // Box mode needs both a dense level and enough tasks to fill the GPU: a task walks
+      // all of its ~189 groups in sequence, so a level of a few dozen tasks would be one
+      // long chain per CU.
+      constexpr uint32_t kMinBoxTasks = 256;
+      if (tile.pairs == 0 || (2 * tile.pairs >= tile.slots && tile.ntasks() >= kMinBoxTasks)) {
         all.append(tile);
       } else {
-        // Sparse level (few of a target's possible offsets present): clustered mode.
+        // Sparse level (few of a target's possible offsets present), or a small one: clustered mode.
         // A target's V entries fall into the children of its parent's neighbours D;
         // each (target, D) gets a scratch slot, and a task holds the slots of one
         // (target octant o, D) class, whose columns share the same offsets d = 2 D + o'
