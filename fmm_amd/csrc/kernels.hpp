@@ -55,12 +55,18 @@ void gather_charges(hipStream_t s, const double *q, const uint32_t *order, size_
 // z_hi, q, x_lo, y_lo, z_lo, 0) of the source's own-leaf coordinates u = hi + lo.  The
 // charge slot is written by gather_charges.
 void build_source_records(const double *sx, const double *sy, const double *sz, size_t n, float *host_rec);
-// Rotation M2L (rotation.hip): rows of target boxes, entries (source box, offset index);
-// tables per offset (fwd, bwd: rb values) and per distance (coax: cb values).
+// Rotation M2L (rotation.hip): blocks of at most rotation_pairs_per_block pairs of one
+// offset (blk_offset), pairs blk_first[b] .. blk_first[b + 1] with source boxes pair_src;
+// tables per offset (fwd, bwd: rb values) and per distance (coax: cb values); pair q's
+// result goes to scratch row q of S (nc values), added to the locals by reduce_slots.
 template <typename T>
-void m2l_rotation(hipStream_t s, int p, uint32_t rows, const uint32_t *row_box, const uint32_t *row_off,
-                  const uint32_t *ent_src, const uint16_t *ent_offset, const uint16_t *dist_of, const T *fwd,
-                  const T *bwd, const T *coax, int rb, int cb, const double *M, double *L);
+int rotation_pairs_per_block(int p, int rb, int cb);
+template <typename T>
+hipError_t rotation_allow_lds(int p, int rb, int cb);
+template <typename T>
+void m2l_rotation(hipStream_t s, int p, uint32_t nblocks, const uint32_t *blk_first, const uint16_t *blk_offset,
+                  const uint32_t *pair_src, const uint16_t *dist_of, const T *fwd, const T *bwd, const T *coax, int rb,
+                  int cb, const double *M, T *S);
 
 // Compensated f64 direct sum (unscaled): chunks of sources per block, partials in
 // part[nchunks][nt][8], then added per target in chunk order.

@@ -544,6 +544,43 @@ void DeviceOperator::build_rotation(const Plan &plan) {
   rot_off_.upload(off);
   rot_src_.upload(src);
   rot_offset_.upload(ofs);
+  if (opt_.m2l != kM2lRotation) return;
+  // pairs by level, then offset, then row: blocks of at most pb pairs of one offset
+  const bool mixed = opt_.precision == kMixed;
+  const int pb = mixed ? dev::rotation_pairs_per_block<float>(opt_.p, rot_rb_, rot_cb_)
+                       : dev::rotation_pairs_per_block<double>(opt_.p, rot_rb_, rot_cb_);
+  hip_check(mixed ? dev::rotation_allow_lds<float>(opt_.p, rot_rb_, rot_cb_)
+                  : dev::rotation_allow_lds<double>(opt_.p, rot_rb_, rot_cb_),
+            "rotation M2L LDS");
+  std::vector<uint32_t> blk_first, pair_src, pair_of(src.size());
+  std::vector<uint16_t> blk_offset;
+  rot_blk_level_.assign(nlevels_ + 1, 0);
+  uint32_t npairs = 0;
+  for (int l = 0; l < nlevels_; ++l) {
+    std::vector<std::vector<uint32_t>> by_t(316);
+    for (uint32_t r = rot_level_[l]; r < rot_level_[l + 1]; ++r)
+      for (uint32_t e = off[r]; e < off[r + 1]; ++e) by_t[ofs[e]].push_back(e);
+    for (int t = 0; t < 316; ++t)
+      for (size_t k = 0; k < by_t[t].size(); ++k) {
+        if (k % size_t(pb) == 0) {
+          blk_first.push_back(npairs);
+          blk_offset.push_back(static_cast<uint16_t>(t));
+        }
+        const uint32_t e = by_t[t][k];
+        pair_src.push_back(src[e]);
+        pair_of[e] = npairs++;
+      }
+    rot_blk_level_[l + 1] = static_cast<uint32_t>(blk_offset.size());
+  }
+  blk_first.push_back(npairs);
+  rot_blk_first_.upload(blk_first);
+  rot_blk_offset_.upload(blk_offset);
+  rot_pair_src_.upload(pair_src);
+  rot_pair_of_.upload(pair_of);
+  if (mixed)
+    rot_slots32_.alloc(size_t(npairs) * nc_);
+  else
+    rot_slots_.alloc(size_t(npairs) * nc_);
 }
 
 size_t DeviceOperator::device_bytes() const {
@@ -574,17 +611,22 @@ void DeviceOperator::launch_m2l(int level_first, int level_last) {
     return;
   }
   if (opt_.m2l == kM2lRotation) {
+    const uint32_t b0 = rot_blk_level_[level_first], b1 = rot_blk_level_[level_last + 1];
     const uint32_t r0 = rot_level_[level_first], r1 = rot_level_[level_last + 1];
     timed(kKindM2lContract, [&] {
       if (opt_.precision == kMixed)
-        dev::m2l_rotation<float>(stream_, opt_.p, r1 - r0, rot_box_.ptr + r0, rot_off_.ptr + r0, rot_src_.ptr,
-                                 rot_offset_.ptr, rot_dist_.ptr, rot_fwd32_.ptr, rot_bwd32_.ptr, rot_coax32_.ptr,
-                                 rot_rb_, rot_cb_, mult_.ptr, loc_.ptr);
+        dev::m2l_rotation<float>(stream_, opt_.p, b1 - b0, rot_blk_first_.ptr + b0, rot_blk_offset_.ptr + b0,
+                                 rot_pair_src_.ptr, rot_dist_.ptr, rot_fwd32_.ptr, rot_bwd32_.ptr, rot_coax32_.ptr,
+                                 rot_rb_, rot_cb_, mult_.ptr, rot_slots32_.ptr);
       else
-        dev::m2l_rotation<double>(stream_, opt_.p, r1 - r0, rot_box_.ptr + r0, rot_off_.ptr + r0, rot_src_.ptr,
-                                  rot_offset_.ptr, rot_dist_.ptr, rot_fwd_.ptr, rot_bwd_.ptr, rot_coax_.ptr, rot_rb_,
-                                  rot_cb_, mult_.ptr, loc_.ptr);
+        dev::m2l_rotation<double>(stream_, opt_.p, b1 - b0, rot_blk_first_.ptr + b0, rot_blk_offset_.ptr + b0,
+                                  rot_pair_src_.ptr, rot_dist_.ptr, rot_fwd_.ptr, rot_bwd_.ptr, rot_coax_.ptr, rot_rb_,
+                                  rot_cb_, mult_.ptr, rot_slots_.ptr);
     });
+    // each target adds its pairs' rows in the order of its V list
+    if (r1 > r0)
+      dev::reduce_slots(stream_, opt_.p, r1 - r0, rot_box_.ptr + r0, rot_off_.ptr + r0, rot_pair_of_.ptr, 0,
+                        rot_slots_.ptr, opt_.precision == kMixed ? rot_slots32_.ptr : nullptr, loc_.ptr);
     return;
   }
   const uint32_t t0 = m2l_level_task_[level_first], t1 = m2l_level_task_[level_last + 1];

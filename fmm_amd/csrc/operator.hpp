@@ -191,6 +191,13 @@ class DeviceOperator {
   DevBuf<float> rot_fwd32_, rot_bwd32_, rot_coax32_;
   std::vector<uint32_t> rot_level_;
   int rot_rb_ = 0, rot_cb_ = 0;
+  // rotation M2L: the pairs grouped by (level, offset) in blocks of one offset; pair_of
+  // maps each V entry (rot rows' order) to its pair's scratch row
+  DevBuf<uint32_t> rot_blk_first_, rot_pair_src_, rot_pair_of_;
+  DevBuf<uint16_t> rot_blk_offset_;
+  std::vector<uint32_t> rot_blk_level_;
+  DevBuf<double> rot_slots_;
+  DevBuf<float> rot_slots32_;
   void build_rotation(const Plan &plan);
   void build_plane_wave(const Plan &plan);
   // plane-wave M2L: quadrature tables, the waves of the boxes of levels >= 2

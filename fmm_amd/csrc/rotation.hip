@@ -7,13 +7,15 @@
 // slots +i and -i), G rotates the local back.  7,319 multiply-adds per pair at p = 12
 // against 28,561 for the dense table, and 23 KB of tables per offset instead of 228 KB.
 //
-// One workgroup per target box with V entries (all levels in one launch); thread k owns
-// coefficient slot k = n^2 + n + m through the three steps of every entry, in the row's
-// order (offsets ascending), and keeps the target's sum in a register; the step vectors
-// live in LDS between barriers.  T = double (plain path) or float (mixed path: the
-// multipole is rounded to f32, the steps and the row sum are f32, the row sum is added
-// into the f64 local).  Tables are column-major so that the threads of one degree read
-// consecutive entries.
+// Pairs are grouped by offset: a workgroup takes up to PB pairs of one offset, loads that
+// offset's F and G blocks and its distance's A blocks into LDS once, stages the PB source
+// multipoles, and applies the three steps to all of them, each thread forming one output
+// slot k = n^2 + n + m for 8 pairs at a time (every table entry read from LDS serves 8
+// pairs; the pairs' coefficients of one degree are read by all of that degree's threads
+// at once).  Each pair's result goes to its own scratch row; a second pass adds a
+// target's rows into its f64 local in the order of its V list.  T = double (plain path)
+// or float (mixed: multipoles rounded to f32, f32 steps and rows).  Tables are
+// column-major per block (entry (row r, column c) of a degree-n block at r + c (2n+1)).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -23,79 +25,153 @@
 namespace fmmh {
 namespace dev {
 
+constexpr int kRotThreads = 256;
+constexpr int kRotJ = 8;  // pairs per thread item
+
+// LDS of one workgroup: F and G blocks (rb each), A blocks (cb), two staging buffers of
+// PB rows of nc coefficients (row stride nc + 1)
 template <typename T>
-__global__ __launch_bounds__(256) void k_m2l_rotation(int p, uint32_t rows, const uint32_t *__restrict__ row_box,
-                                                      const uint32_t *__restrict__ row_off,
-                                                      const uint32_t *__restrict__ ent_src,
-                                                      const uint16_t *__restrict__ ent_offset,
-                                                      const uint16_t *__restrict__ dist_of, const T *__restrict__ fwd,
-                                                      const T *__restrict__ bwd, const T *__restrict__ coax, int rb,
-                                                      int cb, const double *__restrict__ M, double *__restrict__ L) {
-  __shared__ T x[kMaxCoef], y[kMaxCoef];
-  const uint32_t row = xcd_index(rows);
-  if (row >= rows) return;
+static size_t rot_lds(int p, int rb, int cb, int pb) {
   const int nc = (p + 1) * (p + 1);
-  const int k = threadIdx.x;
-  const bool active = k < nc;
-  // slot k = n^2 + n + m
-  int n = 0;
-  while ((n + 1) * (n + 1) <= k) ++n;
-  const int r = k - n * n, m = r - n, w = 2 * n + 1;
-  const int bo = n * (2 * n - 1) * (2 * n + 1) / 3;  // sum of (2n'+1)^2 over n' < n
-  const int i = m < 0 ? -m : m;
-  int co = 0;  // sum of (p+1-i')^2 over i' < i
-  for (int q = 0; q < i; ++q) co += (p + 1 - q) * (p + 1 - q);
-  const int cw = p + 1 - i;
-  const uint32_t box = row_box[row];
-  T acc = T(0);
-  for (uint32_t e = row_off[row]; e < row_off[row + 1]; ++e) {
-    const uint32_t s = ent_src[e];
-    const uint32_t t = ent_offset[e];
-    __syncthreads();
-    if (active) x[k] = T(M[size_t(s) * nc + k]);
-    __syncthreads();
-    // 1. rotate the multipole: y_n = F_n x_n
-    if (active) {
-      const T *f = fwd + size_t(t) * rb + bo + r;
-      T v = T(0);
-      for (int c = 0; c < w; ++c) v = fma(f[c * w], x[n * n + c], v);
-      y[k] = v;
+  return sizeof(T) * (size_t(2) * rb + cb + size_t(2) * pb * (nc + 1));
+}
+
+template <typename T>
+__global__ __launch_bounds__(kRotThreads) void k_m2l_rot_pairs(int p, int pb, uint32_t nblocks,
+                                                               const uint32_t *__restrict__ blk_first,
+                                                               const uint16_t *__restrict__ blk_offset,
+                                                               const uint32_t *__restrict__ pair_src,
+                                                               const uint16_t *__restrict__ dist_of,
+                                                               const T *__restrict__ fwd, const T *__restrict__ bwd,
+                                                               const T *__restrict__ coax, int rb, int cb,
+                                                               const double *__restrict__ M, T *__restrict__ S) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t blk = xcd_index(nblocks);
+  if (blk >= nblocks) return;  // uniform over the workgroup
+  const int nc = (p + 1) * (p + 1), ld = nc + 1;
+  T *tF = reinterpret_cast<T *>(smem);
+  T *tG = tF + rb;
+  T *tA = tG + rb;
+  T *X = tA + cb;
+  T *Y = X + size_t(pb) * ld;
+  const int tid = threadIdx.x;
+  const uint32_t t = blk_offset[blk];
+  const uint32_t q0 = blk_first[blk], np = blk_first[blk + 1] - q0;  // np <= pb
+  const uint32_t dist = dist_of[t];
+  for (int i = tid; i < rb; i += kRotThreads) {
+    tF[i] = fwd[size_t(t) * rb + i];
+    tG[i] = bwd[size_t(t) * rb + i];
+  }
+  for (int i = tid; i < cb; i += kRotThreads) tA[i] = coax[size_t(dist) * cb + i];
+  for (int i = tid; i < pb * nc; i += kRotThreads) {
+    const int j = i / nc, k = i - j * nc;
+    X[j * ld + k] = uint32_t(j) < np ? T(M[size_t(pair_src[q0 + j]) * nc + k]) : T(0);
+  }
+  __syncthreads();
+  const int items = nc * (pb / kRotJ);
+  // 1. rotate: Y_j = F X_j, per degree block
+  for (int it = tid; it < items; it += kRotThreads) {
+    const int jg = it / nc, k = it - jg * nc;
+    int n = 0;
+    while ((n + 1) * (n + 1) <= k) ++n;
+    const int r = k - n * n, w = 2 * n + 1, bo = n * (2 * n - 1) * (2 * n + 1) / 3;
+    const T *x = X + size_t(jg) * kRotJ * ld + n * n;
+    T y[kRotJ];
+#pragma unroll
+    for (int jj = 0; jj < kRotJ; ++jj) y[jj] = T(0);
+    for (int c = 0; c < w; ++c) {
+      const T f = tF[bo + r + c * w];
+#pragma unroll
+      for (int jj = 0; jj < kRotJ; ++jj) y[jj] = fma(f, x[jj * ld + c], y[jj]);
     }
-    __syncthreads();
-    // 2. coaxial M2L, order i: x_(j,m) = sum_n A_i[j][n] y_(n,m)
-    if (active) {
-      const T *a = coax + size_t(dist_of[t]) * cb + co + (n - i);
-      T v = T(0);
-      for (int q = i; q <= p; ++q) v = fma(a[(q - i) * cw], y[q * q + q + m], v);
-      x[k] = v;
+    T *yo = Y + size_t(jg) * kRotJ * ld + k;
+#pragma unroll
+    for (int jj = 0; jj < kRotJ; ++jj) yo[jj * ld] = y[jj];
+  }
+  __syncthreads();
+  // 2. coaxial M2L, order i = |m|: X_j(n, m) = sum_q A_i[n - i][q - i] Y_j(q, m)
+  for (int it = tid; it < items; it += kRotThreads) {
+    const int jg = it / nc, k = it - jg * nc;
+    int n = 0;
+    while ((n + 1) * (n + 1) <= k) ++n;
+    const int m = k - n * n - n, i = m < 0 ? -m : m, cw = p + 1 - i;
+    int co = 0;  // sum of (p+1-i')^2 over i' < i
+    for (int q = 0; q < i; ++q) co += (p + 1 - q) * (p + 1 - q);
+    const T *a = tA + co + (n - i);
+    const T *yv = Y + size_t(jg) * kRotJ * ld;
+    T z[kRotJ];
+#pragma unroll
+    for (int jj = 0; jj < kRotJ; ++jj) z[jj] = T(0);
+    for (int q = i; q <= p; ++q) {
+      const T av = a[(q - i) * cw];
+      const int kq = q * q + q + m;
+#pragma unroll
+      for (int jj = 0; jj < kRotJ; ++jj) z[jj] = fma(av, yv[jj * ld + kq], z[jj]);
     }
-    __syncthreads();
-    // 3. rotate the local back: acc += G_n x_n
-    if (active) {
-      const T *g = bwd + size_t(t) * rb + bo + r;
-      T v = T(0);
-      for (int c = 0; c < w; ++c) v = fma(g[c * w], x[n * n + c], v);
-      acc += v;
+    T *xo = X + size_t(jg) * kRotJ * ld + k;
+#pragma unroll
+    for (int jj = 0; jj < kRotJ; ++jj) xo[jj * ld] = z[jj];
+  }
+  __syncthreads();
+  // 3. rotate back: row_j = G X_j, into the pairs' scratch rows
+  for (int it = tid; it < items; it += kRotThreads) {
+    const int jg = it / nc, k = it - jg * nc;
+    int n = 0;
+    while ((n + 1) * (n + 1) <= k) ++n;
+    const int r = k - n * n, w = 2 * n + 1, bo = n * (2 * n - 1) * (2 * n + 1) / 3;
+    const T *x = X + size_t(jg) * kRotJ * ld + n * n;
+    T y[kRotJ];
+#pragma unroll
+    for (int jj = 0; jj < kRotJ; ++jj) y[jj] = T(0);
+    for (int c = 0; c < w; ++c) {
+      const T g = tG[bo + r + c * w];
+#pragma unroll
+      for (int jj = 0; jj < kRotJ; ++jj) y[jj] = fma(g, x[jj * ld + c], y[jj]);
+    }
+#pragma unroll
+    for (int jj = 0; jj < kRotJ; ++jj) {
+      const uint32_t j = uint32_t(jg * kRotJ + jj);
+      if (j < np) S[size_t(q0 + j) * nc + k] = y[jj];
     }
   }
-  if (active) L[size_t(box) * nc + k] += double(acc);
 }
 
 template <typename T>
-void m2l_rotation(hipStream_t s, int p, uint32_t rows, const uint32_t *row_box, const uint32_t *row_off,
-                  const uint32_t *ent_src, const uint16_t *ent_offset, const uint16_t *dist_of, const T *fwd,
-                  const T *bwd, const T *coax, int rb, int cb, const double *M, double *L) {
-  if (rows == 0) return;
-  hipLaunchKernelGGL((k_m2l_rotation<T>), dim3(xcd_grid(rows)), dim3(256), 0, s, p, rows, row_box, row_off, ent_src,
-                     ent_offset, dist_of, fwd, bwd, coax, rb, cb, M, L);
+int rotation_pairs_per_block(int p, int rb, int cb) {
+  int pb = 32;
+  while (pb > kRotJ && rot_lds<T>(p, rb, cb, pb) > 160 * 1024) pb /= 2;
+  return pb;
 }
 
-template void m2l_rotation<double>(hipStream_t, int, uint32_t, const uint32_t *, const uint32_t *, const uint32_t *,
-                                   const uint16_t *, const uint16_t *, const double *, const double *,
-                                   const double *, int, int, const double *, double *);
-template void m2l_rotation<float>(hipStream_t, int, uint32_t, const uint32_t *, const uint32_t *, const uint32_t *,
-                                  const uint16_t *, const uint16_t *, const float *, const float *, const float *,
-                                  int, int, const double *, double *);
+template <typename T>
+hipError_t rotation_allow_lds(int p, int rb, int cb) {
+  const int pb = rotation_pairs_per_block<T>(p, rb, cb);
+  const size_t bytes = rot_lds<T>(p, rb, cb, pb);
+  if (bytes > 160 * 1024) return hipErrorInvalidValue;
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_rot_pairs<T>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+}
+
+template <typename T>
+void m2l_rotation(hipStream_t s, int p, uint32_t nblocks, const uint32_t *blk_first, const uint16_t *blk_offset,
+                  const uint32_t *pair_src, const uint16_t *dist_of, const T *fwd, const T *bwd, const T *coax, int rb,
+                  int cb, const double *M, T *S) {
+  if (nblocks == 0) return;
+  const int pb = rotation_pairs_per_block<T>(p, rb, cb);
+  hipLaunchKernelGGL((k_m2l_rot_pairs<T>), dim3(xcd_grid(nblocks)), dim3(kRotThreads), rot_lds<T>(p, rb, cb, pb), s,
+                     p, pb, nblocks, blk_first, blk_offset, pair_src, dist_of, fwd, bwd, coax, rb, cb, M, S);
+}
+
+template int rotation_pairs_per_block<double>(int, int, int);
+template int rotation_pairs_per_block<float>(int, int, int);
+template hipError_t rotation_allow_lds<double>(int, int, int);
+template hipError_t rotation_allow_lds<float>(int, int, int);
+template void m2l_rotation<double>(hipStream_t, int, uint32_t, const uint32_t *, const uint16_t *, const uint32_t *,
+                                   const uint16_t *, const double *, const double *, const double *, int, int,
+                                   const double *, double *);
+template void m2l_rotation<float>(hipStream_t, int, uint32_t, const uint32_t *, const uint16_t *, const uint32_t *,
+                                  const uint16_t *, const float *, const float *, const float *, int, int,
+                                  const double *, float *);
 
 }  // namespace dev
 }  // namespace fmmh
