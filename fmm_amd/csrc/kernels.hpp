@@ -66,7 +66,9 @@ hipError_t rotation_allow_lds(int p, int rb, int cb);
 template <typename T>
 void m2l_rotation(hipStream_t s, int p, uint32_t nblocks, const uint32_t *blk_first, const uint16_t *blk_offset,
                   const uint32_t *pair_src, const uint16_t *dist_of, const T *fwd, const T *bwd, const T *coax, int rb,
-                  int cb, const double *M, T *S);
+                  int cb, const T *M, T *S);
+// b[i] = f32(a[i]), i < n
+void to_f32(hipStream_t s, size_t n, const double *a, float *b);
 
 // Compensated f64 direct sum (unscaled): chunks of sources per block, partials in
 // part[nchunks][nt][8], then added per target in chunk order.

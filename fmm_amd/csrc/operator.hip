@@ -577,10 +577,12 @@ void DeviceOperator::build_rotation(const Plan &plan) {
   rot_blk_offset_.upload(blk_offset);
   rot_pair_src_.upload(pair_src);
   rot_pair_of_.upload(pair_of);
-  if (mixed)
+  if (mixed) {
     rot_slots32_.alloc(size_t(npairs) * nc_);
-  else
+    mult32_.alloc(nbox_total_ * nc_);
+  } else {
     rot_slots_.alloc(size_t(npairs) * nc_);
+  }
 }
 
 size_t DeviceOperator::device_bytes() const {
@@ -613,11 +615,16 @@ void DeviceOperator::launch_m2l(int level_first, int level_last) {
   if (opt_.m2l == kM2lRotation) {
     const uint32_t b0 = rot_blk_level_[level_first], b1 = rot_blk_level_[level_last + 1];
     const uint32_t r0 = rot_level_[level_first], r1 = rot_level_[level_last + 1];
+    if (opt_.precision == kMixed) {
+      // the multipoles of these levels in f32, read by every pair that uses them
+      const size_t x0 = box_base_[level_first], x1 = box_base_[level_last + 1];
+      dev::to_f32(stream_, (x1 - x0) * nc_, mult_.ptr + x0 * nc_, mult32_.ptr + x0 * nc_);
+    }
     timed(kKindM2lContract, [&] {
       if (opt_.precision == kMixed)
         dev::m2l_rotation<float>(stream_, opt_.p, b1 - b0, rot_blk_first_.ptr + b0, rot_blk_offset_.ptr + b0,
                                  rot_pair_src_.ptr, rot_dist_.ptr, rot_fwd32_.ptr, rot_bwd32_.ptr, rot_coax32_.ptr,
-                                 rot_rb_, rot_cb_, mult_.ptr, rot_slots32_.ptr);
+                                 rot_rb_, rot_cb_, mult32_.ptr, rot_slots32_.ptr);
       else
         dev::m2l_rotation<double>(stream_, opt_.p, b1 - b0, rot_blk_first_.ptr + b0, rot_blk_offset_.ptr + b0,
                                   rot_pair_src_.ptr, rot_dist_.ptr, rot_fwd_.ptr, rot_bwd_.ptr, rot_coax_.ptr, rot_rb_,

@@ -197,7 +197,7 @@ class DeviceOperator {
   DevBuf<uint16_t> rot_blk_offset_;
   std::vector<uint32_t> rot_blk_level_;
   DevBuf<double> rot_slots_;
-  DevBuf<float> rot_slots32_;
+  DevBuf<float> rot_slots32_, mult32_;
   void build_rotation(const Plan &plan);
   void build_plane_wave(const Plan &plan);
   // plane-wave M2L: quadrature tables, the waves of the boxes of levels >= 2

@@ -14,7 +14,7 @@
 // pairs; the pairs' coefficients of one degree are read by all of that degree's threads
 // at once).  Each pair's result goes to its own scratch row; a second pass adds a
 // target's rows into its f64 local in the order of its V list.  T = double (plain path)
-// or float (mixed: multipoles rounded to f32, f32 steps and rows).  Tables are
+// or float (mixed: multipoles rounded to f32 once per evaluation, f32 steps and rows).  Tables are
 // column-major per block (entry (row r, column c) of a degree-n block at r + c (2n+1)).
 #include <hip/hip_runtime.h>
 
@@ -44,7 +44,7 @@ __global__ __launch_bounds__(kRotThreads) void k_m2l_rot_pairs(int p, int pb, ui
                                                                const uint16_t *__restrict__ dist_of,
                                                                const T *__restrict__ fwd, const T *__restrict__ bwd,
                                                                const T *__restrict__ coax, int rb, int cb,
-                                                               const double *__restrict__ M, T *__restrict__ S) {
+                                                               const T *__restrict__ M, T *__restrict__ S) {
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t blk = xcd_index(nblocks);
   if (blk >= nblocks) return;  // uniform over the workgroup
@@ -58,14 +58,22 @@ __global__ __launch_bounds__(kRotThreads) void k_m2l_rot_pairs(int p, int pb, ui
   const uint32_t t = blk_offset[blk];
   const uint32_t q0 = blk_first[blk], np = blk_first[blk + 1] - q0;  // np <= pb
   const uint32_t dist = dist_of[t];
+  // the block's source boxes first, so the multipole loads below carry no dependent
+  // index load and can all be in flight together
+  __shared__ uint32_t src[64];
+  if (tid < pb) src[tid] = uint32_t(tid) < np ? pair_src[q0 + tid] : 0u;
+#pragma unroll 4
   for (int i = tid; i < rb; i += kRotThreads) {
     tF[i] = fwd[size_t(t) * rb + i];
     tG[i] = bwd[size_t(t) * rb + i];
   }
+#pragma unroll 4
   for (int i = tid; i < cb; i += kRotThreads) tA[i] = coax[size_t(dist) * cb + i];
+  __syncthreads();
+#pragma unroll 4
   for (int i = tid; i < pb * nc; i += kRotThreads) {
     const int j = i / nc, k = i - j * nc;
-    X[j * ld + k] = uint32_t(j) < np ? T(M[size_t(pair_src[q0 + j]) * nc + k]) : T(0);
+    X[j * ld + k] = uint32_t(j) < np ? M[size_t(src[j]) * nc + k] : T(0);
   }
   __syncthreads();
   const int items = nc * (pb / kRotJ);
@@ -136,10 +144,13 @@ __global__ __launch_bounds__(kRotThreads) void k_m2l_rot_pairs(int p, int pb, ui
   }
 }
 
+// 32 pairs per block, or the most that fit (the dynamic part; the 256 bytes of source
+// indices come on top)
+constexpr size_t kRotLdsMax = 160 * 1024 - 1024;
 template <typename T>
 int rotation_pairs_per_block(int p, int rb, int cb) {
   int pb = 32;
-  while (pb > kRotJ && rot_lds<T>(p, rb, cb, pb) > 160 * 1024) pb /= 2;
+  while (pb > kRotJ && rot_lds<T>(p, rb, cb, pb) > kRotLdsMax) pb /= 2;
   return pb;
 }
 
@@ -147,15 +158,26 @@ template <typename T>
 hipError_t rotation_allow_lds(int p, int rb, int cb) {
   const int pb = rotation_pairs_per_block<T>(p, rb, cb);
   const size_t bytes = rot_lds<T>(p, rb, cb, pb);
-  if (bytes > 160 * 1024) return hipErrorInvalidValue;
+  if (bytes > kRotLdsMax) return hipErrorInvalidValue;
   return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_rot_pairs<T>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+}
+
+__global__ void k_to_f32(size_t n, const double *__restrict__ a, float *__restrict__ b) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    b[i] = float(a[i]);
+}
+
+void to_f32(hipStream_t s, size_t n, const double *a, float *b) {
+  if (n == 0) return;
+  const size_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(k_to_f32, dim3(blocks < 65536 ? blocks : 65536), dim3(256), 0, s, n, a, b);
 }
 
 template <typename T>
 void m2l_rotation(hipStream_t s, int p, uint32_t nblocks, const uint32_t *blk_first, const uint16_t *blk_offset,
                   const uint32_t *pair_src, const uint16_t *dist_of, const T *fwd, const T *bwd, const T *coax, int rb,
-                  int cb, const double *M, T *S) {
+                  int cb, const T *M, T *S) {
   if (nblocks == 0) return;
   const int pb = rotation_pairs_per_block<T>(p, rb, cb);
   hipLaunchKernelGGL((k_m2l_rot_pairs<T>), dim3(xcd_grid(nblocks)), dim3(kRotThreads), rot_lds<T>(p, rb, cb, pb), s,
@@ -171,7 +193,7 @@ template void m2l_rotation<double>(hipStream_t, int, uint32_t, const uint32_t *,
                                    const double *, double *);
 template void m2l_rotation<float>(hipStream_t, int, uint32_t, const uint32_t *, const uint16_t *, const uint32_t *,
                                   const uint16_t *, const float *, const float *, const float *, int, int,
-                                  const double *, float *);
+                                  const float *, float *);
 
 }  // namespace dev
 }  // namespace fmmh
