@@ -297,7 +297,8 @@ def main():
 
 def batched(args):
     """256 independent clouds (sizes log-uniform in [2k, 256k], one Gaussian blob each,
-    charges N(0,1)), p = 6, f32 P2P + bf16x3 M2L, <= 64 points per leaf: one forest
+    charges N(0,1)), p = 6, f32 P2P + bf16x3 M2L, <= 64 points per leaf, near rule at
+    (p+1)^2 / 2: one forest
     operator, reset + evaluate replayed as one hipGraph launch; then the adjoint of
     L = sum(phi^2)/N (one more evaluation and the combine kernel)."""
     import torch
@@ -311,7 +312,9 @@ def batched(args):
     n = int(sum(len(c) for c in clouds))
     log(f"batched: {len(clouds)} clouds, {n} points, drawn in {time.time() - t0:.1f}s")
     t0 = time.time()
-    fmm = fmm_amd.FmmBuilder(6).precision("mixed").max_points_per_leaf(64).build_batched(clouds)
+    # the near rule on, as the reference's FmmBuilder has it by default (fmm-exec/src/fmm.rs:1029),
+    # at this path's crossovers (p+1)^2 / 2
+    fmm = fmm_amd.FmmBuilder(6).precision("mixed").max_points_per_leaf(64).near_rule("device").build_batched(clouds)
     log(f"forest plan + operator in {time.time() - t0:.1f}s: {fmm.operator.stats()}")
     op = fmm.operator
     q_dev = torch.from_numpy(np.concatenate(charges)).to(dev)
@@ -366,7 +369,7 @@ def batched(args):
         "ms_per_step": med * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 P2P (double-float separations) + bf16x3 M2L + f64 other", "data": "synthetic",
         "config": {"workload": "batched-gaussian-clouds", "clouds": len(clouds), "n": n, "p": 6, "leaf_points": 64,
-                   "sizes": [int(len(c)) for c in clouds][:8] + ["..."]},
+                   "near_rule": [24.5, 24.5], "sizes": [int(len(c)) for c in clouds][:8] + ["..."]},
         "errors": {"phi": (num / den) ** 0.5, "grad": (numg / deng) ** 0.5, "sampled_per_cloud": 32},
         "adjoint_ms": float(np.median(vt)) * 1e3,
     }
