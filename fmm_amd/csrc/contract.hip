@@ -329,11 +329,14 @@ __device__ __forceinline__ bf8 as_bf8(u4 v) {
 // them) for the 32 columns of pair w / 2, so each A fragment read from LDS feeds two
 // N-tiles (half the LDS traffic of one column tile per wave).
 //
-// Pipeline: the table slices of k-steps go global -> LDS by DMA (global_load_lds) into a
-// ring of NST = 3 buffers, issued two k-steps ahead; the source expansions (B operand)
-// go to registers one k-step ahead.  One barrier per k-step: after it every wave's DMA
-// for this step has landed and every wave is done with the buffer the next DMA refills.
-template <int MT>
+// Pipeline: the table slices go global -> LDS by DMA (global_load_lds) into a ring of
+// NST = 3 buffers, issued two stages ahead; the source expansions (B operand) go to
+// registers one stage ahead.  A stage is KG consecutive k-steps of one group (KG = 2 for
+// the short tables of p <= 7, whose two k-steps would otherwise each pay a full
+// wait-and-barrier round for 24 MFMAs per wave; else 1).  One barrier per stage: after
+// it every wave's DMA for this stage has landed and every wave is done with the buffer
+// the next DMA refills.
+template <int MT, int KG>
 __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, uint32_t ntasks, uint32_t xchunk,
                                                        const uint32_t *__restrict__ task_group_off,
                                                        const uint16_t *__restrict__ group_table,
@@ -342,15 +345,17 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, uint3
                                                        const u4 *__restrict__ X16, double *__restrict__ Y,
                                                        uint32_t first_slot, float *__restrict__ S) {
   constexpr int CHUNK = MT * 3 * 64;    // u4 per k-step: [plane][mt][lane]
-  constexpr int SEGS = CHUNK / 64;      // 1 KB segments, one DMA instruction each
-  constexpr int SEGW = (SEGS + 7) / 8;  // DMA instructions per wave per k-step
+  constexpr int STAGE = KG * CHUNK;     // u4 per stage (KG consecutive k-steps)
+  constexpr int SEGS = STAGE / 64;      // 1 KB segments, one DMA instruction each
+  constexpr int SEGW = (SEGS + 7) / 8;  // DMA instructions per wave per stage
   constexpr int NST = 3;
   constexpr int MH = (MT + 1) / 2;
-  __shared__ u4 As[NST][CHUNK];
+  __shared__ u4 As[NST][STAGE];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int half = wave & 1, pair = wave >> 1, g = lane >> 4;
   const int mt0 = half * MH;
   const int kq = ksteps * 4;  // u4 per plane of a split expansion
+  const int spg = ksteps / KG;  // stages per group
   const uint32_t task = xcd_index(ntasks, xchunk);
   if (task >= ntasks) return;  // uniform over the workgroup
   int col[2];
@@ -361,7 +366,7 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, uint3
     out[nt] = col_out[size_t(task) * COLS + col[nt]];
   }
   const uint32_t g0 = task_group_off[task], g1 = task_group_off[task + 1];
-  const int nsteps = int(g1 - g0) * ksteps;
+  const int nsteps = int(g1 - g0) * spg;
   if (nsteps == 0) return;
   f4 acc[MH][2];
   double sum[MH][2][4];
@@ -373,9 +378,9 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, uint3
 #pragma unroll
       for (int r = 0; r < 4; ++r) sum[i][nt][r] = 0.0;
     }
-  u4 pb[2][3], b[2][3];
+  u4 pb[KG][2][3], b[KG][2][3];
   const u4 zero4 = {0u, 0u, 0u, 0u};
-  // table slice of k-step `step` into ring buffer `buf`: wave w copies the segments
+  // table slice of stage `step` into ring buffer `buf`: wave w copies the segments
   // w, w + 8, ...; a wave past the last segment copies the last one again (same bytes,
   // same place), so every wave issues SEGW DMA instructions
   // the table index of a group, through a 32-bit load of the pair holding it (uniform:
@@ -385,8 +390,8 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, uint3
     return (gi & 1) ? (w >> 16) : (w & 0xFFFFu);
   };
   auto dma = [&](int step, int buf) {
-    const uint32_t gi = g0 + uint32_t(step / ksteps);
-    const int ks = step % ksteps;
+    const uint32_t gi = g0 + uint32_t(step / spg);
+    const int ks = (step % spg) * KG;
     const u4 *gsrc = tsw + (size_t(table_of(gi)) * ksteps + ks) * CHUNK;
 #pragma unroll
     for (int i = 0; i < SEGW; ++i) {
@@ -396,30 +401,32 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, uint3
   };
   // Every lane issues the same loads whether its column has a source or not (the value
   // is zeroed after the load), so the vector-memory counter advances the same way on
-  // every path and the waits can count on it.  The source indices of a step are loaded
-  // a step before its B loads, which are loaded a step before use.
+  // every path and the waits can count on it.  The source indices of a stage are loaded
+  // a stage before its B loads, which are loaded a stage before use.
   uint32_t src_n[2];
   auto fetch_src = [&](int step) {
-    const uint32_t gi = g0 + uint32_t(step / ksteps);
+    const uint32_t gi = g0 + uint32_t(step / spg);
 #pragma unroll
     for (int nt = 0; nt < 2; ++nt) src_n[nt] = group_src[size_t(gi) * COLS + col[nt]];
   };
   auto fetch_b = [&](int step) {
-    const int ks = step % ksteps;
+    const int ks = (step % spg) * KG;
 #pragma unroll
     for (int nt = 0; nt < 2; ++nt) {
       const uint32_t src = src_n[nt];
       const size_t s = src != 0xFFFFFFFFu ? src : 0;
 #pragma unroll
-      for (int pl = 0; pl < 3; ++pl) {
-        const u4 v = X16[(s * 3 + pl) * kq + ks * 4 + g];
-        pb[nt][pl] = src != 0xFFFFFFFFu ? v : zero4;
-      }
+      for (int kg = 0; kg < KG; ++kg)
+#pragma unroll
+        for (int pl = 0; pl < 3; ++pl) {
+          const u4 v = X16[(s * 3 + pl) * kq + (ks + kg) * 4 + g];
+          pb[kg][nt][pl] = src != 0xFFFFFFFFu ? v : zero4;
+        }
     }
   };
-  // Issue order in step s: B loads of s + 1, source indices of s + 2, DMA of s + 2
-  // (steps past the end repeat the last step, into a ring slot nobody reads again).  At
-  // the top of step s + 1 only the DMA of s + 2 is newer than the B loads of s + 1 and
+  // Issue order in stage s: B loads of s + 1, source indices of s + 2, DMA of s + 2
+  // (stages past the end repeat the last one, into a ring slot nobody reads again).  At
+  // the top of stage s + 1 only the DMA of s + 2 is newer than the B loads of s + 1 and
   // the DMA of s + 1, so a wait for all but SEGW operations covers both.
   const int last = nsteps - 1;
   fetch_src(0);
@@ -429,13 +436,15 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, uint3
   dma(min(1, last), 1);
   for (int step = 0; step < nsteps; ++step) {
     const int buf = step % NST;
-    wait_vm<SEGW>();               // step s and its B loads landed (only the DMA of s + 1 may not)
+    wait_vm<SEGW>();               // stage s and its B loads landed (only the DMA of s + 1 may not)
     __builtin_amdgcn_s_barrier();  // every wave's part; ring slot s - 1 free
-    __builtin_amdgcn_sched_barrier(0);  // nothing that reads this step's data moves above
+    __builtin_amdgcn_sched_barrier(0);  // nothing that reads this stage's data moves above
 #pragma unroll
-    for (int nt = 0; nt < 2; ++nt)
+    for (int kg = 0; kg < KG; ++kg)
 #pragma unroll
-      for (int pl = 0; pl < 3; ++pl) b[nt][pl] = pb[nt][pl];
+      for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int pl = 0; pl < 3; ++pl) b[kg][nt][pl] = pb[kg][nt][pl];
     fetch_b(min(step + 1, last));
     fetch_src(min(step + 2, last));
     // keep the issue order: the DMA after the loads (a wait for the B loads must not
@@ -443,44 +452,47 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, uint3
     __builtin_amdgcn_sched_barrier(0);
     asm volatile("" ::: "memory");
     dma(min(step + 2, last), (step + 2) % NST);
-    bf8 bh[2], bm[2], bl[2];
 #pragma unroll
-    for (int nt = 0; nt < 2; ++nt) {
-      bh[nt] = as_bf8(b[nt][0]);
-      bm[nt] = as_bf8(b[nt][1]);
-      bl[nt] = as_bf8(b[nt][2]);
-    }
-    // the A fragments of M-tile i + 1 are read from LDS while M-tile i is multiplied
-    u4 af[2][3];
-    auto read_a = [&](int i, u4 *f) {
-      const int mt = min(mt0 + i, MT - 1);
+    for (int kg = 0; kg < KG; ++kg) {
+      bf8 bh[2], bm[2], bl[2];
 #pragma unroll
-      for (int pl = 0; pl < 3; ++pl) f[pl] = As[buf][(pl * MT + mt) * 64 + lane];
-    };
-    read_a(0, af[0]);
+      for (int nt = 0; nt < 2; ++nt) {
+        bh[nt] = as_bf8(b[kg][nt][0]);
+        bm[nt] = as_bf8(b[kg][nt][1]);
+        bl[nt] = as_bf8(b[kg][nt][2]);
+      }
+      // the A fragments of M-tile i + 1 are read from LDS while M-tile i is multiplied
+      u4 af[2][3];
+      auto read_a = [&](int i, u4 *f) {
+        const int mt = min(mt0 + i, MT - 1);
 #pragma unroll
-    for (int i = 0; i < MH; ++i) {
-      const int mt = mt0 + i;
-      if (i + 1 < MH) read_a(i + 1, af[(i + 1) & 1]);
-      if (mt < MT) {
-        const bf8 ah = as_bf8(af[i & 1][0]);
-        const bf8 am = as_bf8(af[i & 1][1]);
-        const bf8 al = as_bf8(af[i & 1][2]);
+        for (int pl = 0; pl < 3; ++pl) f[pl] = As[buf][kg * CHUNK + (pl * MT + mt) * 64 + lane];
+      };
+      read_a(0, af[0]);
 #pragma unroll
-        for (int nt = 0; nt < 2; ++nt) {
-          // smallest terms first
-          f4 a4 = acc[i][nt];
-          a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[nt], a4, 0, 0, 0);
-          a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm[nt], a4, 0, 0, 0);
-          a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[nt], a4, 0, 0, 0);
-          a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh[nt], a4, 0, 0, 0);
-          a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm[nt], a4, 0, 0, 0);
-          a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[nt], a4, 0, 0, 0);
-          acc[i][nt] = a4;
+      for (int i = 0; i < MH; ++i) {
+        const int mt = mt0 + i;
+        if (i + 1 < MH) read_a(i + 1, af[(i + 1) & 1]);
+        if (mt < MT) {
+          const bf8 ah = as_bf8(af[i & 1][0]);
+          const bf8 am = as_bf8(af[i & 1][1]);
+          const bf8 al = as_bf8(af[i & 1][2]);
+#pragma unroll
+          for (int nt = 0; nt < 2; ++nt) {
+            // smallest terms first
+            f4 a4 = acc[i][nt];
+            a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[nt], a4, 0, 0, 0);
+            a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm[nt], a4, 0, 0, 0);
+            a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[nt], a4, 0, 0, 0);
+            a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh[nt], a4, 0, 0, 0);
+            a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm[nt], a4, 0, 0, 0);
+            a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[nt], a4, 0, 0, 0);
+            acc[i][nt] = a4;
+          }
         }
       }
     }
-    if ((step + 1) % ksteps == 0) {
+    if ((step + 1) % spg == 0) {
       // end of a group (one offset): its f32 sum goes into the f64 accumulators
 #pragma unroll
       for (int i = 0; i < MH; ++i)
@@ -638,18 +650,29 @@ void contract_bf16(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_g
   const int nc = (p + 1) * (p + 1), ks = k_steps16(p);
   const u4 *t4 = reinterpret_cast<const u4 *>(tsw16);
   const u4 *x4 = reinterpret_cast<const u4 *>(X16);
+  // two k-steps per stage for the short tables (at most 4 M-tiles and 2 k-steps)
+  const bool pair_steps = m_tiles(p) <= 4 && ks == 2;
 #define FMMH_CONTRACT(MT)                                                                                        \
   case MT:                                                                                                       \
-    hipLaunchKernelGGL(k_contract_bf16<MT>, dim3(xcd_grid(ntasks, xchunk)), dim3(512), 0, s, nc, ks, ntasks,     \
+    hipLaunchKernelGGL((k_contract_bf16<MT, 1>), dim3(xcd_grid(ntasks, xchunk)), dim3(512), 0, s, nc, ks, ntasks, \
                        xchunk,                                                                                   \
                        task_group_off, group_table,                                                              \
                        group_src, col_out, t4, x4, Y, first_slot, S);                                            \
     break;
+#define FMMH_CONTRACT2(MT)                                                                                       \
+  case MT:                                                                                                       \
+    if (pair_steps)                                                                                              \
+      hipLaunchKernelGGL((k_contract_bf16<MT, 2>), dim3(xcd_grid(ntasks, xchunk)), dim3(512), 0, s, nc, ks,      \
+                         ntasks, xchunk, task_group_off, group_table, group_src, col_out, t4, x4, Y, first_slot, S); \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_contract_bf16<MT, 1>), dim3(xcd_grid(ntasks, xchunk)), dim3(512), 0, s, nc, ks,      \
+                         ntasks, xchunk, task_group_off, group_table, group_src, col_out, t4, x4, Y, first_slot, S); \
+    break;
   switch (m_tiles(p)) {
-    FMMH_CONTRACT(1)
-    FMMH_CONTRACT(2)
-    FMMH_CONTRACT(3)
-    FMMH_CONTRACT(4)
+    FMMH_CONTRACT2(1)
+    FMMH_CONTRACT2(2)
+    FMMH_CONTRACT2(3)
+    FMMH_CONTRACT2(4)
     FMMH_CONTRACT(5)
     FMMH_CONTRACT(6)
     FMMH_CONTRACT(7)
@@ -666,6 +689,7 @@ void contract_bf16(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_g
       break;
   }
 #undef FMMH_CONTRACT
+#undef FMMH_CONTRACT2
 }
 
 void split_bf16(hipStream_t s, int p, size_t nboxes, const double *X, uint16_t *X16) {
