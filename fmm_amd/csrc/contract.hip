@@ -222,7 +222,7 @@ __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, uint3
   }
   wait_vm<0>();  // no DMA may land in LDS after the workgroup ends
   if (out != 0xFFFFFFFFu) {
-    // a box row adds; a scratch slot (pair mode: one group, one writer) is stored
+    // a box row adds; a scratch slot (one writer) is stored
     const bool slot = out >= first_slot;
     double *y = slot ? S + size_t(out - first_slot) * nc : Y + size_t(out) * nc;
     // accumulator element r of lane (g, j): row 4 r + g of the M-tile (f64 16x16x4)

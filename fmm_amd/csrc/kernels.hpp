@@ -109,8 +109,8 @@ void evaluate(hipStream_t s, int p, int irregular, int gradients, uint32_t rows,
 // Grouped dense contraction on MFMA (contract.hip): for every task (COLS output columns)
 // and every group of the task, Y[out[c]] += T_group X[src[g, c]]; f64, or bf16x3 split
 // with f32 accumulation per group and f64 across groups.  An output id >= first_slot is
-// a scratch slot (pair mode, written by exactly one group): its row out - first_slot of S
-// is stored, not added to (f32 rows for bf16x3: one group's f32 sum).
+// a scratch slot (written by exactly one task): its row out - first_slot of S is stored,
+// not added to (f32 rows for bf16x3: the task's f64 sum rounded to f32).
 // width: columns per task, COLS (wide) or 16 (narrow: the waves split the rows)
 // xchunk: tasks dealt to the XCDs in runs of xchunk consecutive tasks (0: block b runs
 // task b).

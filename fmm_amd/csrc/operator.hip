@@ -117,7 +117,7 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
     src_rec_.upload(rec);
   }
 
-  // expansions and outputs (the pair-mode scratch rows are sized in build_schedules)
+  // expansions and outputs (the clustered-mode scratch rows are sized in build_schedules)
   mult_.alloc(nbox_total_ * nc_);
   phi_hat_.alloc(nt_);
   grad_hat_.alloc(opt_.gradients ? 3 * nt_ : 0);
@@ -353,8 +353,8 @@ void DeviceOperator::build_schedules(const Plan &plan) {
   }
   // M2L of every level in one schedule.  Dense levels: tiles of COLS targets of one
   // octant, whose groups are the offsets their parity allows.  Sparse levels (most of
-  // a tile's column slots without a source): pair mode, tasks of COLS pairs of one
-  // offset writing scratch rows, then an ordered reduction per target.
+  // a tile's column slots without a source) and small ones: clustered mode (below),
+  // tasks writing scratch rows, then an ordered reduction per target.
   ScheduleBuilder all;
   std::vector<uint32_t> red_id, red_cell, red_off{0}, red_ent;
   m2l_level_task_.assign(nlevels_ + 1, 0);
@@ -598,7 +598,7 @@ void DeviceOperator::launch_contract(const Schedule &s, uint32_t task_first, uin
 }
 
 // M2L of the levels level_first .. level_last: the contraction tasks, then the ordered
-// reduction of the pair-mode levels.
+// reduction of the clustered-mode levels.
 void DeviceOperator::launch_m2l(int level_first, int level_last) {
   if (opt_.m2l == kM2lPlaneWave) {
     const uint32_t r0 = rot_level_[level_first], r1 = rot_level_[level_last + 1];

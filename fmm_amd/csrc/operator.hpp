@@ -205,7 +205,7 @@ class DeviceOperator {
   DevBuf<double> pw_lambda_, pw_hw_, pw_hc_, pw_hs_, pw_fwd_, pw_bwd_, pw_shift_, pw_waves_;
   DevBuf<int> pw_first_, pw_nalpha_, pw_hk_, pw_dir_;
   uint32_t pw_box0_ = 0, pw_nbox_ = 0;
-  size_t nslots_ = 0;                  // pair-mode scratch rows (slot ids follow the box ids)
+  size_t nslots_ = 0;                  // clustered-mode scratch rows (slot ids follow the box ids)
   DevBuf<double> slots_;               // plain f64: scratch rows
   DevBuf<float> slots32_;              // mixed: scratch rows (one group's f32 sum each)
   DevBuf<float> src_rec_;              // mixed: P2P source records, 8 floats per source
@@ -228,7 +228,7 @@ class DeviceOperator {
   std::vector<Schedule> m2m_, l2l_;                          // per level
   Schedule m2l_;                                             // every level, by level
   std::vector<uint32_t> m2l_level_task_;                     // task range per level
-  Rows m2l_red_;                                             // pair-mode reductions
+  Rows m2l_red_;                                             // clustered-mode reductions
   std::vector<uint32_t> m2l_red_level_;
 };
 
