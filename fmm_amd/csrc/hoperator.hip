@@ -251,8 +251,11 @@ HelmholtzOperator::HelmholtzOperator(const Plan &plan, const HelmholtzOptions &o
     t_m2m_.upload(m2m_sw);
     t_l2l_.upload(l2l_sw);
     for (int l = 0; l + 1 < nlevels_; ++l) {
-      const int width = 2 * plan.nboxes(l) < 64u * dev::COLS ? 16 : dev::COLS;
-      // M2M into the parents of level l
+      // M2M into the parents of level l (boxes with children); narrow tasks (waves split
+      // the rows) while wide ones would leave the GPU idle
+      uint32_t nparents = 0;
+      for (uint32_t b = 0; b < plan.nboxes(l); ++b) nparents += plan.first_child[l][b] != kNone;
+      const int width = 2 * nparents < 64u * dev::COLS ? 16 : dev::COLS;
       ScheduleBuilder sb(width);
       std::vector<uint32_t> cols;
       std::vector<std::vector<std::pair<uint16_t, uint32_t>>> lists;
