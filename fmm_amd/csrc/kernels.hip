@@ -220,13 +220,13 @@ __global__ void k_vjp_combine(size_t n, const double *q, const double *g, const 
 // tile sums are added in f64.  Coincident points give d = 0 exactly in both forms.
 
 // Lane mapping (one wave per target leaf, targets processed in chunks of 128): a chunk of
-// n <= 64 targets uses np = the next power of two >= n lanes per source phase and
-// 64 / np phases, phase f taking the sources j = f, f + 64/np, ... of every tile; a chunk
-// of 65..128 targets puts two targets in each lane.  A tile is padded with zero charges
-// far away to a multiple of the phase count, so every lane runs the same number of
-// steps; the phases' partial sums are added by a fixed butterfly of lane shuffles, so
-// results are the same on every run.  The next tile's coordinates and charges are
-// loaded into registers while the current tile is computed.
+// n <= 64 targets uses np = n lanes per source phase and floor(64 / n) phases, phase f
+// taking the sources j = f, f + phases, ... of every tile (tiles of phases x floor(64 /
+// phases) sources); a chunk of 65..128 targets puts two targets in each lane.  A tile is
+// padded with zero charges far away to a multiple of the phase count, so every lane runs
+// the same number of steps; the phases' partial sums are added in phase order through
+// LDS, so results are the same on every run.  The next tile's coordinates and charges
+// are loaded into registers while the current tile is computed.
 //
 // Only a target's own leaf can hold a source at r^2 == 0 (equal coordinates give equal
 // leaves): that tile selects 0 for r^2 == 0; other tiles clamp r^2 below at the smallest
@@ -401,20 +401,23 @@ __global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, uint32_t count,
         p2p_uniform<GRAD, false>(lf, tb, n, e0, e1, near_ent, leaf_cell, src_start, rec, tx, ty, tz, phi, grad);
       continue;
     }
+    // np = n lanes per phase and floor(64 / n) phases (a chunk of 17..21 targets runs 3
+    // phases, where the next power of two would allow 2); lanes past np x phases idle
     int np = 64, phases = 1;
     if (n <= 64) {
-      np = 1;
-      while (np < int(n)) np <<= 1;
+      np = int(n);
       phases = 64 / np;
     }
-    const int slot = lane & (np - 1), phase = lane / np;
+    const int tsz = phases * (64 / phases);  // sources per tile: a multiple of the phases
+    const bool lane_ok = lane < np * phases;
+    const int slot = lane % np, phase = lane_ok ? lane / np : 0;
     const bool two = n > 64;
     T ux[2], uy[2], uz[2], lx[2], ly[2], lz[2], ap[2], ax[2], ay[2], az[2];
     bool ok[2];
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
       const uint32_t i = tb + slot + a * 64;
-      ok[a] = (a == 0) ? (uint32_t(slot) < n) : (two && uint32_t(slot + 64) < n);
+      ok[a] = lane_ok && ((a == 0) ? (uint32_t(slot) < n) : (two && uint32_t(slot + 64) < n));
       const double x = ok[a] ? tx[i] : 0.0, y = ok[a] ? ty[i] : 0.0, z = ok[a] ? tz[i] : 0.0;
       ux[a] = T(x);
       uy[a] = T(y);
@@ -528,8 +531,8 @@ __global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, uint32_t count,
       if (has_own) {
         // the own leaf: y = u_s; coincident points add nothing
         const uint32_t s0 = src_start[lf], s1 = src_start[lf + 1];
-        for (uint32_t sb = s0; sb < s1; sb += TILE) {
-          const int cn = int((s1 - sb) < uint32_t(TILE) ? (s1 - sb) : uint32_t(TILE));
+        for (uint32_t sb = s0; sb < s1; sb += uint32_t(tsz)) {
+          const int cn = int((s1 - sb) < uint32_t(tsz) ? (s1 - sb) : uint32_t(tsz));
           const int steps = (cn + phases - 1) / phases;
           const uint32_t s = sb + uint32_t(lane);
           const bool valid = s < s1;
@@ -562,8 +565,8 @@ __global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, uint32_t count,
         }
       };
       if (total > 0) prefetch(0);
-      for (uint32_t base = 0; base < total; base += TILE) {
-        const int cn = int((total - base) < uint32_t(TILE) ? (total - base) : uint32_t(TILE));
+      for (uint32_t base = 0; base < total; base += uint32_t(tsz)) {
+        const int cn = int((total - base) < uint32_t(tsz) ? (total - base) : uint32_t(tsz));
         const int steps = (cn + phases - 1) / phases;
         double yx = 0.0, yy = 0.0, yz = 0.0;
         if (pvalid) {
@@ -573,7 +576,7 @@ __global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, uint32_t count,
         }
         stage(cn, steps, pvalid, yx, yy, yz, pq);
         // start the next tile's loads before computing this one
-        if (base + TILE < total) prefetch(base + TILE);
+        if (base + uint32_t(tsz) < total) prefetch(base + uint32_t(tsz));
         int k2 = 0;
         for (; k2 + 2 <= steps; k2 += 2) {
           step(phase + k2 * phases, false);
@@ -592,14 +595,26 @@ __global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, uint32_t count,
         cz[a] = double(az[a]);
       }
     }
-    // add the phases' partial sums (lanes slot + np f share a target)
-    for (int off = np; off < 64; off <<= 1) {
-      cp[0] += __shfl_xor(cp[0], off);
-      if (GRAD) {
-        cx[0] += __shfl_xor(cx[0], off);
-        cy[0] += __shfl_xor(cy[0], off);
-        cz[0] += __shfl_xor(cz[0], off);
-      }
+    // add the phases' partial sums (lanes slot + np f share a target) in phase order,
+    // through the frame table's LDS (free once the row's tiles are done)
+    if (phases > 1) {
+      __syncthreads();
+      fr[lane][0] = cp[0];
+      fr[lane][1] = cx[0];
+      fr[lane][2] = cy[0];
+      fr[lane][3] = cz[0];
+      __syncthreads();
+      if (phase == 0)
+        for (int f = 1; f < phases; ++f) {
+          const int o = f * np + slot;
+          cp[0] += fr[o][0];
+          if (GRAD) {
+            cx[0] += fr[o][1];
+            cy[0] += fr[o][2];
+            cz[0] += fr[o][3];
+          }
+        }
+      __syncthreads();  // the next chunk writes the frame table again
     }
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
