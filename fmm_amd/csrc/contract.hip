@@ -582,6 +582,10 @@ void contract_f64(hipStream_t s, int p, int width, uint32_t ntasks, const uint32
                          ntasks, xchunk,                                                                         \
                          task_group_off,                                                                         \
                          group_table, group_src, col_out, tsw, X, Y, first_slot, S);                             \
+    else if (width == 64)                                                                                        \
+      hipLaunchKernelGGL((k_contract_f64<MT, 2>), dim3(xcd_grid(ntasks, xchunk)), dim3(512), 0, s, nc, nch,      \
+                         ntasks, xchunk, task_group_off, group_table, group_src, col_out, tsw, X, Y, first_slot, \
+                         S);                                                                                     \
     else                                                                                                         \
       hipLaunchKernelGGL((k_contract_f64<MT, 1>), dim3(xcd_grid(ntasks, xchunk)), dim3(512), 0, s, nc, nch,      \
                          ntasks, xchunk,                                                                         \

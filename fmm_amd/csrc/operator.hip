@@ -329,8 +329,11 @@ void DeviceOperator::build_schedules(const Plan &plan) {
       sb.finish(m2m_[l]);
     }
     if (l >= 1) {
-      // L2L into the boxes of level l, grouped by octant so a task shares one table
-      const int width = plan.nboxes(l) < 64u * dev::COLS ? 16 : dev::COLS;
+      // L2L into the boxes of level l, grouped by octant so a task shares one table;
+      // 16 columns per task (waves split the rows) on small levels, 64 while 128-column
+      // tasks would fill fewer than four rounds of the GPU, else 128
+      const uint32_t nb = plan.nboxes(l);
+      const int width = nb < 64u * dev::COLS ? 16 : (nb < 4u * 512u * dev::COLS ? 64 : dev::COLS);
       ScheduleBuilder sb(width);
       const int C = width;
       for (int o = 0; o < 8; ++o) {
