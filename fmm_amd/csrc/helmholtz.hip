@@ -548,17 +548,134 @@ struct PairMath<float> {
   }
 };
 
+// Leaves of more than 32 targets in the f32 path: each lane holds one target (TWO: two,
+// the second 64 further on) and all lanes take the same source at each step, read by
+// uniform loads from its 32-byte record (u_hi x, y, z, u_lo x | u_lo y, z, q re, q im: the
+// source's own leaf frame, f32); no LDS staging, no barriers.  Per near entry the targets
+// move into the source leaf's frame, x' = c'' + rho u_t (rho = r_t / r_s), and the pair
+// term is formed in the source leaf's units with that leaf's k' = k r_s: in target units
+// e^{i k_t r_t} / r_t = rho e^{i k' r'} / r' and its gradient is rho^2 (i k' r' - 1)
+// e^{i k' r'} d' / r'^3, so an entry's f32 sums (at most 128 sources) go into the f64
+// sums scaled by rho and rho^2 (powers of two).
+template <bool GRAD, bool DAMP, bool TWO>
+__device__ __forceinline__ void h_p2p_uniform(uint32_t lf, uint32_t tb, uint32_t n, uint32_t e0, uint32_t e1,
+                                              const uint32_t *__restrict__ near_ent,
+                                              const uint32_t *__restrict__ leaf_cell,
+                                              const uint32_t *__restrict__ src_start,
+                                              const float4 *__restrict__ rec, const double *__restrict__ hlev,
+                                              const double *__restrict__ tx, const double *__restrict__ ty,
+                                              const double *__restrict__ tz, double *__restrict__ phi_hat,
+                                              double *__restrict__ grad_hat) {
+  constexpr int A = TWO ? 2 : 1;
+  const int lane = threadIdx.x;
+  double u[A][3];
+  bool ok[A];
+#pragma unroll
+  for (int a = 0; a < A; ++a) {
+    const uint32_t i = tb + uint32_t(lane + 64 * a);
+    ok[a] = uint32_t(lane + 64 * a) < n;
+    u[a][0] = ok[a] ? tx[i] : 0.0;
+    u[a][1] = ok[a] ? ty[i] : 0.0;
+    u[a][2] = ok[a] ? tz[i] : 0.0;
+  }
+  double acc[A][8];
+#pragma unroll
+  for (int a = 0; a < A; ++a)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[a][q] = 0.0;
+  const uint32_t *ct = leaf_cell + 4 * size_t(lf);
+  for (uint32_t e = e0; e < e1; ++e) {
+    const uint32_t sl = near_ent[e];
+    const uint32_t s0 = src_start[sl], s1 = src_start[sl + 1];
+    if (s0 == s1) continue;
+    const uint32_t *cs = leaf_cell + 4 * size_t(sl);
+    double c[3], rho;
+    cell_frame(ct, cs, c, rho);  // the target leaf in the source leaf's frame
+    const int ls = int(cs[0]);
+    const float kre = float(hlev[4 * ls]), kim = float(hlev[4 * ls + 1]);
+    float xh[A][3], xl[A][3];
+#pragma unroll
+    for (int a = 0; a < A; ++a)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double x = fma(rho, u[a][k], c[k]);
+        xh[a][k] = float(x);
+        xl[a][k] = float(x - double(xh[a][k]));
+      }
+    for (uint32_t jb = s0; jb < s1; jb += 128) {
+      const uint32_t je = (s1 - jb) < 128u ? s1 : jb + 128;
+      float loc[A][8];
+#pragma unroll
+      for (int a = 0; a < A; ++a)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) loc[a][q] = 0.0f;
+#pragma unroll 2
+      for (uint32_t j = jb; j < je; ++j) {
+        const float4 h = rec[2 * size_t(j)], l = rec[2 * size_t(j) + 1];
+#pragma unroll
+        for (int a = 0; a < A; ++a) {
+          const float dx = (xh[a][0] - h.x) + (xl[a][0] - h.w);
+          const float dy = (xh[a][1] - h.y) + (xl[a][1] - l.x);
+          const float dz = (xh[a][2] - h.z) + (xl[a][2] - l.y);
+          const float r2 = dx * dx + dy * dy + dz * dz;
+          float rr, ir;
+          PairMath<float>::rinv(r2, rr, ir);
+          float sn, cs2;
+          PairMath<float>::sincos(kre * rr, &sn, &cs2);
+          const float dmp = DAMP ? expf(-kim * rr) : 1.0f;
+          const float er = dmp * cs2 * ir, ei = dmp * sn * ir;
+          const float q0 = l.z, q1 = l.w;
+          loc[a][0] += q0 * er - q1 * ei;
+          loc[a][1] += q0 * ei + q1 * er;
+          if (GRAD) {
+            const float ar = -kim * rr - 1.0f, ai = kre * rr, ir2 = ir * ir;
+            const float gr = (ar * er - ai * ei) * ir2, gi = (ar * ei + ai * er) * ir2;
+            const float qgr = q0 * gr - q1 * gi, qgi = q0 * gi + q1 * gr;
+            loc[a][2] += qgr * dx;
+            loc[a][3] += qgi * dx;
+            loc[a][4] += qgr * dy;
+            loc[a][5] += qgi * dy;
+            loc[a][6] += qgr * dz;
+            loc[a][7] += qgi * dz;
+          }
+        }
+      }
+      const double s2 = rho * rho;
+#pragma unroll
+      for (int a = 0; a < A; ++a) {
+        acc[a][0] += double(loc[a][0]) * rho;
+        acc[a][1] += double(loc[a][1]) * rho;
+        if (GRAD)
+#pragma unroll
+          for (int q = 2; q < 8; ++q) acc[a][q] += double(loc[a][q]) * s2;
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < A; ++a) {
+    if (!ok[a]) continue;
+    const size_t t = tb + uint32_t(lane + 64 * a);
+    phi_hat[2 * t] += acc[a][0];
+    phi_hat[2 * t + 1] += acc[a][1];
+    if (GRAD)
+#pragma unroll
+      for (int q = 0; q < 6; ++q) grad_hat[6 * t + q] += acc[a][2 + q];
+  }
+}
+
 template <typename T, bool GRAD, bool DAMP>
 __global__ __launch_bounds__(64) void k_h_p2p(uint32_t leaf_first, uint32_t count,
                                               const uint32_t *__restrict__ near_off,
                                               const uint32_t *__restrict__ near_ent,
                                               const uint32_t *__restrict__ leaf_cell,
                                               const uint32_t *__restrict__ src_start,
-                                              const uint32_t *__restrict__ tgt_start, const double *sx,
-                                              const double *sy, const double *sz, const double *qr,
-                                              const double *qi, const double *tx, const double *ty,
-                                              const double *tz, const double *hlev, double *phi_hat,
-                                              double *grad_hat) {
+                                              const uint32_t *__restrict__ tgt_start, const double *__restrict__ sx,
+                                              const double *__restrict__ sy, const double *__restrict__ sz,
+                                              const double *__restrict__ qr, const double *__restrict__ qi,
+                                              const float4 *__restrict__ rec, const double *__restrict__ tx,
+                                              const double *__restrict__ ty, const double *__restrict__ tz,
+                                              const double *__restrict__ hlev, double *__restrict__ phi_hat,
+                                              double *__restrict__ grad_hat) {
   constexpr bool F32 = sizeof(T) == 4;
   __shared__ T ys[64][8];  // y hi (3), y lo (3, f32 only), q re, q im
   __shared__ double red[64][8];
@@ -572,6 +689,19 @@ __global__ __launch_bounds__(64) void k_h_p2p(uint32_t leaf_first, uint32_t coun
   const int lev = int(ct[0]);
   const T kre = T(hlev[4 * lev]), kim = T(hlev[4 * lev + 1]);
   const uint32_t ntl = t1 - t0;
+  if (F32 && ntl > 32) {
+    const uint32_t e0 = near_off[lf], e1 = near_off[lf + 1];
+    for (uint32_t tb = t0; tb < t1; tb += 128) {
+      const uint32_t n = (t1 - tb) < 128u ? (t1 - tb) : 128u;
+      if (n > 64)
+        h_p2p_uniform<GRAD, DAMP, true>(lf, tb, n, e0, e1, near_ent, leaf_cell, src_start, rec, hlev, tx, ty, tz,
+                                        phi_hat, grad_hat);
+      else
+        h_p2p_uniform<GRAD, DAMP, false>(lf, tb, n, e0, e1, near_ent, leaf_cell, src_start, rec, hlev, tx, ty, tz,
+                                         phi_hat, grad_hat);
+    }
+    return;
+  }
   const int phases = ntl >= 64 ? 1 : (ntl > 32 ? 1 : int(64 / ntl));
   const int span = ntl >= 64 ? 64 : int(ntl);  // targets per chunk
   for (uint32_t tb = t0; tb < t1; tb += uint32_t(span)) {
@@ -671,11 +801,15 @@ __global__ void k_h_negate(int nc, size_t b0, size_t b1, double *coeffs) {
   }
 }
 
-__global__ void k_h_gather(const double *q, const uint32_t *order, size_t n, double *qr, double *qi) {
+__global__ void k_h_gather(const double *q, const uint32_t *order, size_t n, double *qr, double *qi, float *rec) {
   for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
     const size_t c = order[i];
     qr[i] = q[2 * c];
     qi[i] = q[2 * c + 1];
+    if (rec) {
+      rec[8 * i + 6] = float(q[2 * c]);
+      rec[8 * i + 7] = float(q[2 * c + 1]);
+    }
   }
 }
 
@@ -825,13 +959,14 @@ template <typename T>
 void h_p2p(hipStream_t s, uint32_t leaf_first, uint32_t count, int gradients, int damped, const uint32_t *near_off,
            const uint32_t *near_ent, const uint32_t *leaf_cell, const uint32_t *src_start, const uint32_t *tgt_start,
            const double *sx, const double *sy, const double *sz, const double *qr, const double *qi,
-           const double *tx, const double *ty, const double *tz, const double *hlev, double *phi_hat,
-           double *grad_hat) {
+           const float *rec, const double *tx, const double *ty, const double *tz, const double *hlev,
+           double *phi_hat, double *grad_hat) {
   if (count == 0) return;
+  const float4 *r4 = reinterpret_cast<const float4 *>(rec);
 #define FMMH_HP2P(G, D)                                                                                         \
   hipLaunchKernelGGL((k_h_p2p<T, G, D>), dim3(xcd_grid(count)), dim3(64), 0, s, leaf_first, count, near_off,   \
-                     near_ent, leaf_cell, src_start, tgt_start, sx, sy, sz, qr, qi, tx, ty, tz, hlev, phi_hat,  \
-                     grad_hat)
+                     near_ent, leaf_cell, src_start, tgt_start, sx, sy, sz, qr, qi, r4, tx, ty, tz, hlev,       \
+                     phi_hat, grad_hat)
   if (gradients) {
     if (damped)
       FMMH_HP2P(true, true);
@@ -848,12 +983,12 @@ void h_p2p(hipStream_t s, uint32_t leaf_first, uint32_t count, int gradients, in
 
 template void h_p2p<double>(hipStream_t, uint32_t, uint32_t, int, int, const uint32_t *, const uint32_t *,
                             const uint32_t *, const uint32_t *, const uint32_t *, const double *, const double *,
-                            const double *, const double *, const double *, const double *, const double *,
-                            const double *, const double *, double *, double *);
+                            const double *, const double *, const double *, const float *, const double *,
+                            const double *, const double *, const double *, double *, double *);
 template void h_p2p<float>(hipStream_t, uint32_t, uint32_t, int, int, const uint32_t *, const uint32_t *,
                            const uint32_t *, const uint32_t *, const uint32_t *, const double *, const double *,
-                           const double *, const double *, const double *, const double *, const double *,
-                           const double *, const double *, double *, double *);
+                           const double *, const double *, const double *, const float *, const double *,
+                           const double *, const double *, const double *, double *, double *);
 
 void h_negate(hipStream_t s, int p, size_t b0, size_t b1, double *coeffs) {
   if (b1 <= b0) return;
@@ -863,10 +998,30 @@ void h_negate(hipStream_t s, int p, size_t b0, size_t b1, double *coeffs) {
   hipLaunchKernelGGL(k_h_negate, dim3(blocks), dim3(256), 0, s, nc, b0, b1, coeffs);
 }
 
-void h_gather(hipStream_t s, const double *q, const uint32_t *order, size_t n, double *qr, double *qi) {
+void h_gather(hipStream_t s, const double *q, const uint32_t *order, size_t n, double *qr, double *qi, float *rec) {
   if (n == 0) return;
   const unsigned blocks = unsigned(n / 256 + 1 < 4096 ? n / 256 + 1 : 4096);
-  hipLaunchKernelGGL(k_h_gather, dim3(blocks), dim3(256), 0, s, q, order, n, qr, qi);
+  hipLaunchKernelGGL(k_h_gather, dim3(blocks), dim3(256), 0, s, q, order, n, qr, qi, rec);
+}
+
+void h_source_records(const double *sx, const double *sy, const double *sz, size_t n, float *host_rec) {
+  for (size_t j = 0; j < n; ++j) {
+    const double u[3] = {sx[j], sy[j], sz[j]};
+    float hi[3], lo[3];
+    for (int k = 0; k < 3; ++k) {
+      hi[k] = float(u[k]);
+      lo[k] = float(u[k] - double(hi[k]));
+    }
+    float *r = host_rec + 8 * j;
+    r[0] = hi[0];
+    r[1] = hi[1];
+    r[2] = hi[2];
+    r[3] = lo[0];
+    r[4] = lo[1];
+    r[5] = lo[2];
+    r[6] = 0.0f;
+    r[7] = 0.0f;
+  }
 }
 
 void h_output(hipStream_t s, uint32_t nleaves, const uint32_t *tgt_start, const double *leaf_r,

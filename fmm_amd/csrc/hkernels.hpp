@@ -29,17 +29,22 @@ void h_evaluate(hipStream_t s, int p, int singular, int gradients, uint32_t rows
                 double *grad_hat);
 // P2P over the leaves leaf_first .. leaf_first + count: e^{i kh r} / r in the target leaf's
 // units, computed in T (double, or float with double-float separations and per-leaf f32
-// sums added in f64).
+// sums added in f64; T = float also reads the source records of h_source_records for the
+// leaves of more than 32 targets, rec may be null for T = double).
 template <typename T>
 void h_p2p(hipStream_t s, uint32_t leaf_first, uint32_t count, int gradients, int damped, const uint32_t *near_off,
            const uint32_t *near_ent, const uint32_t *leaf_cell, const uint32_t *src_start, const uint32_t *tgt_start,
            const double *sx, const double *sy, const double *sz, const double *qr, const double *qi,
-           const double *tx, const double *ty, const double *tz, const double *hlev, double *phi_hat,
-           double *grad_hat);
+           const float *rec, const double *tx, const double *ty, const double *tz, const double *hlev,
+           double *phi_hat, double *grad_hat);
+// f32 P2P source records, 8 floats per source in leaf order: (u_hi x, y, z, u_lo x, y, z,
+// q re, q im) of the own-leaf coordinates u = hi + lo; the charges are written by h_gather
+void h_source_records(const double *sx, const double *sy, const double *sz, size_t n, float *host_rec);
 // planes 3b + 2 = -(planes 3b + 1) for boxes b0 .. b1
 void h_negate(hipStream_t s, int p, size_t b0, size_t b1, double *coeffs);
 // caller-order interleaved complex charges to leaf-order planes
-void h_gather(hipStream_t s, const double *q, const uint32_t *order, size_t n, double *qr, double *qi);
+void h_gather(hipStream_t s, const double *q, const uint32_t *order, size_t n, double *qr, double *qi,
+              float *rec = nullptr);
 // phi = phi_hat / (4 pi r_t), grad = grad_hat / (4 pi r_t^2), in caller order
 void h_output(hipStream_t s, uint32_t nleaves, const uint32_t *tgt_start, const double *leaf_r,
               const uint32_t *order, const double *phi_hat, const double *grad_hat, double *phi, double *grad,

@@ -147,6 +147,11 @@ HelmholtzOperator::HelmholtzOperator(const Plan &plan, const HelmholtzOptions &o
     sx_.upload(a);
     sy_.upload(b);
     sz_.upload(c);
+    if (opt_.precision == kMixed && ns_ > 0) {
+      std::vector<float> rec(8 * size_t(ns_));
+      dev::h_source_records(a.data(), b.data(), c.data(), size_t(ns_), rec.data());
+      src_rec_.upload(rec);
+    }
     a.resize(nt_);
     b.resize(nt_);
     c.resize(nt_);
@@ -536,7 +541,7 @@ void HelmholtzOperator::load_charges(const double *charges, bool on_device) {
               "charges");
     src = q_in_.ptr;
   }
-  dev::h_gather(stream_, src, src_order_.ptr, ns_, qr_.ptr, qi_.ptr);
+  dev::h_gather(stream_, src, src_order_.ptr, ns_, qr_.ptr, qi_.ptr, src_rec_.count ? src_rec_.ptr : nullptr);
 }
 
 void HelmholtzOperator::reset() {
@@ -550,11 +555,11 @@ void HelmholtzOperator::launch_p2p() {
   const int damped = opt_.kim != 0.0;
   if (opt_.precision == kMixed)
     dev::h_p2p<float>(stream_, 0, nleaves_, opt_.gradients, damped, near_off_.ptr, near_ent_.ptr, leaf_cell_.ptr,
-                      src_start_.ptr, tgt_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, qr_.ptr, qi_.ptr, tx_.ptr, ty_.ptr,
-                      tz_.ptr, hlev_.ptr, phi_hat_.ptr, grad_hat_.ptr);
+                      src_start_.ptr, tgt_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, qr_.ptr, qi_.ptr, src_rec_.ptr,
+                      tx_.ptr, ty_.ptr, tz_.ptr, hlev_.ptr, phi_hat_.ptr, grad_hat_.ptr);
   else
     dev::h_p2p<double>(stream_, 0, nleaves_, opt_.gradients, damped, near_off_.ptr, near_ent_.ptr, leaf_cell_.ptr,
-                       src_start_.ptr, tgt_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, qr_.ptr, qi_.ptr, tx_.ptr,
+                       src_start_.ptr, tgt_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, qr_.ptr, qi_.ptr, nullptr, tx_.ptr,
                        ty_.ptr, tz_.ptr, hlev_.ptr, phi_hat_.ptr, grad_hat_.ptr);
 }
 

@@ -69,6 +69,7 @@ class HelmholtzOperator {
   DevBuf<uint32_t> leaf_cell_, box_cell_, src_start_, tgt_start_, src_order_, tgt_order_;
   DevBuf<uint32_t> near_off_, near_ent_;
   DevBuf<double> sx_, sy_, sz_, tx_, ty_, tz_, qr_, qi_, q_in_;
+  DevBuf<float> src_rec_;  // mixed: P2P source records (dev::h_source_records), charges by h_gather
   DevBuf<double> leaf_r_, hlev_, norm_;
   DevBuf<double> mult_, loc_, phi_hat_, grad_hat_, phi_, grad_;
   DevBuf<double> t_m2m_, t_l2l_, t_m2l_;
