@@ -634,10 +634,12 @@ __global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, uint32_t count,
 // ---- expansions from points: P2M (regular, unit frame) and P2L (irregular) ----------
 // One wave per output box; the row's source leaves in order, their points in tiles of
 // 32.  Per tile, lane j < 32 forms the diagonal X_m^m (m = 0..p) of point j into LDS; then
-// lane (order m, phase f) walks column m (degrees m..p) of the points j = f, f + F, ... of
-// the tile (F = 64 / (p+1) phases), adding q conj X_n^m to its column in registers.  The
-// phases' columns meet in LDS (over the diagonals' space) and are added in phase order.
-// About 10 KB of LDS per wave, so 16 waves share a CU.
+// lane (column pair g, phase f) walks columns g and p - g (degrees g..p, then p-g..p: p + 2
+// entries, one column of p/2 + 1 entries for g = p/2) of the points j = f, f + F, ... of
+// the tile (F = 64 / ceil((p+1)/2) phases), adding q conj X_n^m to its entries in
+// registers, so every lane takes the same number of steps per point.  The phases' columns
+// meet in LDS (over the diagonals' space, as many phases at a time as fit) and are added
+// in phase order.  About 10 KB of LDS per wave, so 16 waves share a CU.
 
 template <bool IRREG>
 __global__ __launch_bounds__(64) void k_expand(int p, uint32_t rows, const uint32_t *row_box, const uint32_t *row_cell,
@@ -648,27 +650,32 @@ __global__ __launch_bounds__(64) void k_expand(int p, uint32_t rows, const uint3
   constexpr int TP = 32;                       // points per tile
   constexpr int LD = TP + 1;                   // row stride of a diagonal plane (no bank conflicts)
   constexpr int DW = (kMaxDegree + 1) * LD;    // doubles per diagonal plane
+  constexpr int NE = kMaxDegree + 2;           // register entries per lane (two columns)
   __shared__ double pt[3][TP];                 // z, q, r2 (regular) or 1/r2 (irregular)
   __shared__ double diag[2 * DW];              // Re, Im of X_m^m at [m][j]; then the phase sums
   __shared__ double inv_tab[kMaxDegree + 1][kMaxDegree + 1];  // 1 / ((n + m)(n - m)) at [n - m][m]
-  static_assert(2 * DW >= 1024, "phase sums need 64 (p + 1) <= 1024 doubles");
   double *dre = diag, *dim = diag + DW, *red = diag;
   const int lane = threadIdx.x;
   const int P1 = p + 1, nc = P1 * P1;
-  const int phases = 64 / P1;
-  const int m = lane % P1, ph = lane / P1;
+  const int groups = (P1 + 1) / 2;
+  const int phases = 64 / groups;
+  const int g = lane % groups, ph = lane / groups;
   const bool active = ph < phases;
+  // this lane's columns: m1 = g (entries 0 .. len1 - 1), then m2 = p - g (entries
+  // len1 .. tot - 1) unless it is the same column
+  const int m1 = g, m2 = p - g;
+  const int len1 = P1 - m1, tot = m2 != m1 ? p + 2 : len1;
   const uint32_t row = xcd_index(rows);
   if (row >= rows) return;
   const uint32_t box = row_box[row];
   if (!IRREG)
     for (int t = lane; t < (kMaxDegree + 1) * (kMaxDegree + 1); t += 64) {
       const int kk = t / (kMaxDegree + 1), mm = t % (kMaxDegree + 1);
-      inv_tab[kk][mm] = kk >= 2 ? 1.0 / double((2 * mm + kk) * kk) : 0.0;
+      inv_tab[kk][mm] = kk >= 1 ? 1.0 / double((2 * mm + kk) * kk) : 0.0;
     }
-  double are[kMaxDegree + 1], aim[kMaxDegree + 1];
+  double are[NE], aim[NE];
 #pragma unroll
-  for (int k = 0; k <= kMaxDegree; ++k) are[k] = aim[k] = 0.0;
+  for (int k = 0; k < NE; ++k) are[k] = aim[k] = 0.0;
   for (uint32_t e = row_off[row]; e < row_off[row + 1]; ++e) {
     const uint32_t sl = row_ent[e];
     const uint32_t s0 = src_start[sl], s1 = src_start[sl + 1];
@@ -713,66 +720,77 @@ __global__ __launch_bounds__(64) void k_expand(int p, uint32_t rows, const uint3
       if (active)
         for (int j = ph; j < cnt; j += phases) {
           const double z = pt[0][j], q = pt[1][j], w = pt[2][j];
-          double pr = 0.0, pi = 0.0, cr = dre[m * LD + j], ci = dim[m * LD + j];
+          const double d2r = dre[m2 * LD + j], d2i = dim[m2 * LD + j];
+          double pr = 0.0, pi = 0.0, cr = dre[m1 * LD + j], ci = dim[m1 * LD + j];
+          int mm = m1, k = 0;  // current column and degree above its diagonal
 #pragma unroll
-          for (int k = 0; k <= kMaxDegree; ++k) {
-            const int n = m + k;
-            if (n > p) break;
-            if (k > 0) {
+          for (int kk = 0; kk < NE; ++kk) {
+            if (kk >= tot) continue;
+            if (kk > 0) {
+              // X_n^m from X_{n-1}^m and X_{n-2}^m (n = mm + k + 1); at the column change
+              // the second column's diagonal instead
+              ++k;
+              const int n = mm + k;
+              const double a = (2.0 * n - 1.0) * z;
               double nr, ni;
               if (IRREG) {
-                if (k == 1) {
-                  const double f = (2.0 * m + 1.0) * z * w;
-                  nr = f * cr;
-                  ni = f * ci;
-                } else {
-                  const double a = (2.0 * n - 1.0) * z, b = double((n - 1) * (n - 1) - m * m);
-                  nr = (a * cr - b * pr) * w;
-                  ni = (a * ci - b * pi) * w;
-                }
+                const double b = double((n - 1) * (n - 1) - mm * mm);
+                nr = (a * cr - b * pr) * w;
+                ni = (a * ci - b * pi) * w;
               } else {
-                if (k == 1) {
-                  nr = z * cr;
-                  ni = z * ci;
-                } else {
-                  const double a = (2.0 * n - 1.0) * z, inv = inv_tab[k][m];
-                  nr = (a * cr - w * pr) * inv;
-                  ni = (a * ci - w * pi) * inv;
-                }
+                const double inv = inv_tab[k][mm];
+                nr = (a * cr - w * pr) * inv;
+                ni = (a * ci - w * pi) * inv;
               }
-              pr = cr;
-              pi = ci;
-              cr = nr;
-              ci = ni;
+              const bool restart = kk == len1;
+              pr = restart ? 0.0 : cr;
+              pi = restart ? 0.0 : ci;
+              cr = restart ? d2r : nr;
+              ci = restart ? d2i : ni;
+              if (restart) {
+                mm = m2;
+                k = 0;
+              }
             }
-            are[k] += q * cr;
-            aim[k] -= q * ci;
+            are[kk] += q * cr;
+            aim[kk] -= q * ci;
           }
         }
     }
   }
-  __syncthreads();
-  if (active)
+  // phase sums, as many phases at a time as the diagonals' space holds, in phase order
+  const int chunk = (2 * DW) / nc;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};  // slots lane, lane + 64, ... (nc <= 256)
+  for (int f0 = 0; f0 < phases; f0 += chunk) {
+    __syncthreads();
+    if (active && ph >= f0 && ph < f0 + chunk)
 #pragma unroll
-    for (int k = 0; k <= kMaxDegree; ++k) {
-      const int n = m + k;
-      if (n > p) break;
-      red[ph * nc + idx(n, m)] = are[k];
-      if (m > 0) red[ph * nc + idx(n, -m)] = aim[k];
+      for (int kk = 0; kk < NE; ++kk) {
+        const int mm = kk < len1 ? m1 : m2, n = kk < len1 ? m1 + kk : m2 + (kk - len1);
+        if (kk < tot) red[(ph - f0) * nc + idx(n, mm)] = are[kk];
+        if (kk < tot && mm > 0) red[(ph - f0) * nc + idx(n, -mm)] = aim[kk];
+      }
+    __syncthreads();
+    const int f1 = f0 + chunk < phases ? f0 + chunk : phases;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int slot = lane + 64 * u;
+      if (slot < nc)
+        for (int f = f0; f < f1; ++f) acc[u] += red[(f - f0) * nc + slot];
     }
-  __syncthreads();
-  for (int slot = lane; slot < nc; slot += 64) {
-    double s = 0.0;
-    for (int f = 0; f < phases; ++f) s += red[f * nc + slot];
-    coeffs[size_t(box) * nc + slot] += s;
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int slot = lane + 64 * u;
+    if (slot < nc) coeffs[size_t(box) * nc + slot] += acc[u];
   }
 }
 
 // ---- evaluation at targets: L2P (regular) and M2P (irregular) ------------------------
-// One workgroup per target leaf; every entry's coefficients go to LDS with the derived
+// One wave per target leaf; every entry's coefficients go to LDS with the derived
 // coefficients of the gradient (the ladders of CONVENTIONS §3.11 "L2P and M2P" moved
-// onto the coefficients), then each thread walks the recursion of its target column by
-// column and contracts on the fly.  Regular: grad_k = sum_{n<p} G_k R_n, with
+// onto the coefficients), then the lanes of a target (one, or up to 8 for leaves of at
+// most 32 targets) walk the recursion of its columns and contract on the fly.  Regular: grad_k = sum_{n<p} G_k R_n, with
 // Gz_n^m = L_{n+1}^m, Gx_n^m = (L_{n+1}^{m+1} - L_{n+1}^{m-1})/2,
 // Gy_n^m = i (L_{n+1}^{m+1} + L_{n+1}^{m-1})/2.  Irregular: grad_k = sum_{1<=n<=p+1}
 // G_k I_n, with Gz_n^m = -M_{n-1}^m, Gx_n^m = (M_{n-1}^{m+1} - M_{n-1}^{m-1})/2,
@@ -794,14 +812,15 @@ __device__ __forceinline__ void read_order(const double *c, int n, int q, int pm
 }
 
 template <bool IRREG, bool GRAD>
-__global__ __launch_bounds__(EVAL_THREADS) void k_evaluate(int p, uint32_t rows, const uint32_t *row_leaf,
+__global__ __launch_bounds__(64) void k_evaluate(int p, uint32_t rows, const uint32_t *row_leaf,
                                                            const uint32_t *row_off,
                                                            const uint32_t *row_box, const uint32_t *box_cell,
                                                            const uint32_t *leaf_cell, const uint32_t *tgt_start,
                                                            const double *tx, const double *ty, const double *tz,
                                                            const double *coeffs, double *phi, double *grad) {
-  __shared__ double C[kMaxCoef];
-  __shared__ double G[3][kMaxCoefGrad];
+  // LDS sized to the degree (eval_lds_bytes): the coefficients, then the three gradient
+  // coefficient sets, so that more waves share a CU at low p
+  extern __shared__ __align__(16) double eval_smem[];
   const int nc = (p + 1) * (p + 1);
   const uint32_t row = xcd_index(rows);
   if (row >= rows) return;
@@ -810,15 +829,23 @@ __global__ __launch_bounds__(EVAL_THREADS) void k_evaluate(int p, uint32_t rows,
   if (t0 == t1) return;
   const int pg = IRREG ? p + 1 : p - 1;  // degree of the gradient coefficients
   const int ng = pg >= 0 ? (pg + 1) * (pg + 1) : 0;
+  double *C = eval_smem, *G0 = C + nc, *G1 = G0 + ng, *G2 = G1 + ng;
+  // one wave per leaf: leaves of more than 32 targets one lane per target, in passes of
+  // 64; smaller ones L = min(8, 64 / n) sub-lanes per target, which share its columns
+  const int lane = threadIdx.x;
+  const uint32_t nt = t1 - t0;
+  const int L = nt > 32 ? 1 : (64 / int(nt) < 8 ? 64 / int(nt) : 8);
+  const int per = 64 / L;  // targets per pass
+  const int u = lane % L, tl = lane / L;
   for (uint32_t e = row_off[row]; e < row_off[row + 1]; ++e) {
     const uint32_t box = row_box[e];
     double c[3] = {0.0, 0.0, 0.0}, r = 1.0;
     if (IRREG) relative_frame<double>(box_cell + 4 * size_t(box), leaf_cell + 4 * size_t(lf), c, r);
     __syncthreads();
-    for (int k = threadIdx.x; k < nc; k += EVAL_THREADS) C[k] = coeffs[size_t(box) * nc + k];
+    for (int k = lane; k < nc; k += 64) C[k] = coeffs[size_t(box) * nc + k];
     __syncthreads();
     if (GRAD)
-      for (int k = threadIdx.x; k < ng; k += EVAL_THREADS) {
+      for (int k = lane; k < ng; k += 64) {
         int n = 0;
         while ((n + 1) * (n + 1) <= k) ++n;
         const int m = k - n * n - n;
@@ -834,86 +861,117 @@ __global__ __launch_bounds__(EVAL_THREADS) void k_evaluate(int p, uint32_t rows,
           zr = -zr;
           zi = -zi;
         }
-        G[0][idx(n, m)] = xr;
-        G[1][idx(n, m)] = yr;
-        G[2][idx(n, m)] = zr;
+        G0[idx(n, m)] = xr;
+        G1[idx(n, m)] = yr;
+        G2[idx(n, m)] = zr;
         if (m > 0) {
-          G[0][idx(n, -m)] = xi;
-          G[1][idx(n, -m)] = yi;
-          G[2][idx(n, -m)] = zi;
+          G0[idx(n, -m)] = xi;
+          G1[idx(n, -m)] = yi;
+          G2[idx(n, -m)] = zi;
         }
       }
     __syncthreads();
     const double inv_r = 1.0 / r;
-    for (uint32_t i = t0 + threadIdx.x; i < t1; i += EVAL_THREADS) {
-      double x = tx[i], y = ty[i], z = tz[i];
-      if (IRREG) {
-        x = (x - c[0]) * inv_r;
-        y = (y - c[1]) * inv_r;
-        z = (z - c[2]) * inv_r;
-      }
-      const double r2 = x * x + y * y + z * z;
-      const double inv_r2 = IRREG ? 1.0 / r2 : 0.0;
+    for (uint32_t b = 0; b < nt; b += uint32_t(per)) {
+      const uint32_t ti = b + uint32_t(tl);
+      const bool valid = tl < per && ti < nt;
+      const uint32_t i = t0 + (valid ? ti : 0u);
       double sp = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-      double dr = IRREG ? rsqrt_t<double>(r2) : 1.0, di = 0.0;
-      const int ptop = IRREG ? (GRAD ? p + 1 : p) : p;
-      for (int m = 0; m <= ptop; ++m) {
-        if (m > 0) {
-          const double f = IRREG ? (2.0 * m - 1.0) * inv_r2 : c_inv_2m[m];
-          const double nr = f * (x * dr - y * di), ni = f * (x * di + y * dr);
-          dr = nr;
-          di = ni;
+      if (valid) {
+        double x = tx[i], y = ty[i], z = tz[i];
+        if (IRREG) {
+          x = (x - c[0]) * inv_r;
+          y = (y - c[1]) * inv_r;
+          z = (z - c[2]) * inv_r;
         }
-        double pr = 0.0, pi = 0.0, cr = dr, ci = di;
-        const double w = m == 0 ? 1.0 : 2.0;
-        for (int n = m; n <= ptop; ++n) {
-          if (n > m) {
-            double nr, ni;
-            if (IRREG) {
-              if (n == m + 1) {
-                const double f = (2.0 * m + 1.0) * z * inv_r2;
-                nr = f * cr;
-                ni = f * ci;
+        const double r2 = x * x + y * y + z * z;
+        const double inv_r2 = IRREG ? 1.0 / r2 : 0.0;
+        double dr = IRREG ? rsqrt_t<double>(r2) : 1.0, di = 0.0;
+        int md = 0;  // order of the diagonal entry (dr, di)
+        const int ptop = IRREG ? (GRAD ? p + 1 : p) : p;
+        // columns m of this sub-lane: block jb holds m = jb L .. jb L + L - 1, dealt to
+        // the sub-lanes forwards in even blocks and backwards in odd ones
+        for (int jb = 0; jb * L <= ptop; ++jb) {
+          const int m = jb * L + ((jb & 1) ? L - 1 - u : u);
+          if (m > ptop) continue;
+          while (md < m) {
+            ++md;
+            const double f = IRREG ? (2.0 * md - 1.0) * inv_r2 : c_inv_2m[md];
+            const double nr = f * (x * dr - y * di), ni = f * (x * di + y * dr);
+            dr = nr;
+            di = ni;
+          }
+          double pr = 0.0, pi = 0.0, cr = dr, ci = di;
+          const double w = m == 0 ? 1.0 : 2.0;
+          for (int n = m; n <= ptop; ++n) {
+            if (n > m) {
+              double nr, ni;
+              if (IRREG) {
+                if (n == m + 1) {
+                  const double f = (2.0 * m + 1.0) * z * inv_r2;
+                  nr = f * cr;
+                  ni = f * ci;
+                } else {
+                  const double a = (2.0 * n - 1.0) * z, bb = double((n - 1) * (n - 1) - m * m);
+                  nr = (a * cr - bb * pr) * inv_r2;
+                  ni = (a * ci - bb * pi) * inv_r2;
+                }
               } else {
-                const double a = (2.0 * n - 1.0) * z, b = double((n - 1) * (n - 1) - m * m);
-                nr = (a * cr - b * pr) * inv_r2;
-                ni = (a * ci - b * pi) * inv_r2;
+                if (n == m + 1) {
+                  nr = z * cr;
+                  ni = z * ci;
+                } else {
+                  const double a = (2.0 * n - 1.0) * z, inv = c_inv_reg[idx(n, m)];
+                  nr = (a * cr - r2 * pr) * inv;
+                  ni = (a * ci - r2 * pi) * inv;
+                }
               }
-            } else {
-              if (n == m + 1) {
-                nr = z * cr;
-                ni = z * ci;
-              } else {
-                const double a = (2.0 * n - 1.0) * z, inv = c_inv_reg[idx(n, m)];
-                nr = (a * cr - r2 * pr) * inv;
-                ni = (a * ci - r2 * pi) * inv;
-              }
+              pr = cr;
+              pi = ci;
+              cr = nr;
+              ci = ni;
             }
-            pr = cr;
-            pi = ci;
-            cr = nr;
-            ci = ni;
-          }
-          if (n <= p) {
-            const double lr = C[idx(n, m)], li = m > 0 ? C[idx(n, -m)] : 0.0;
-            sp += w * (lr * cr - li * ci);
-          }
-          if (GRAD && n <= pg && (!IRREG || n >= 1)) {
-            const int a = idx(n, m), b = idx(n, -m);
-            const double xr = G[0][a], yr = G[1][a], zr = G[2][a];
-            const double xi = m > 0 ? G[0][b] : 0.0, yi = m > 0 ? G[1][b] : 0.0, zi = m > 0 ? G[2][b] : 0.0;
-            sx += w * (xr * cr - xi * ci);
-            sy += w * (yr * cr - yi * ci);
-            sz += w * (zr * cr - zi * ci);
+            if (n <= p) {
+              const double lr = C[idx(n, m)], li = m > 0 ? C[idx(n, -m)] : 0.0;
+              sp += w * (lr * cr - li * ci);
+            }
+            if (GRAD && n <= pg && (!IRREG || n >= 1)) {
+              const int ia = idx(n, m), ib = idx(n, -m);
+              const double xr = G0[ia], yr = G1[ia], zr = G2[ia];
+              const double xi = m > 0 ? G0[ib] : 0.0, yi = m > 0 ? G1[ib] : 0.0, zi = m > 0 ? G2[ib] : 0.0;
+              sx += w * (xr * cr - xi * ci);
+              sy += w * (yr * cr - yi * ci);
+              sz += w * (zr * cr - zi * ci);
+            }
           }
         }
       }
-      phi[i] += sp * inv_r;
-      if (GRAD) {
-        const double s2 = inv_r * inv_r;
-        grad[3 * size_t(i) + 0] += sx * s2;
-        grad[3 * size_t(i) + 1] += sy * s2;
-        grad[3 * size_t(i) + 2] += sz * s2;
+      if (L > 1) {
+        // the target's sub-lanes add their sums in sub-lane order (every lane takes part)
+        const int base = lane - u;
+        double tp = 0.0, ux = 0.0, uy = 0.0, uz = 0.0;
+        for (int k = 0; k < L; ++k) {
+          const int src = base + k < 64 ? base + k : 63;
+          tp += __shfl(sp, src);
+          if (GRAD) {
+            ux += __shfl(sx, src);
+            uy += __shfl(sy, src);
+            uz += __shfl(sz, src);
+          }
+        }
+        sp = tp;
+        sx = ux;
+        sy = uy;
+        sz = uz;
+      }
+      if (valid && u == 0) {
+        phi[i] += sp * inv_r;
+        if (GRAD) {
+          const double s2 = inv_r * inv_r;
+          grad[3 * size_t(i) + 0] += sx * s2;
+          grad[3 * size_t(i) + 1] += sy * s2;
+          grad[3 * size_t(i) + 2] += sz * s2;
+        }
       }
     }
   }
@@ -1027,8 +1085,12 @@ void evaluate(hipStream_t s, int p, int irregular, int gradients, uint32_t rows,
               const uint32_t *leaf_cell, const uint32_t *tgt_start, const double *tx, const double *ty,
               const double *tz, const double *coeffs, double *phi, double *grad) {
   if (rows == 0) return;
+  // LDS: (p+1)^2 coefficients and, with gradients, three sets of (pg+1)^2 (pg = p + 1 for
+  // M2P, p - 1 for L2P); at most 1,123 doubles at p = 15
+  const int pg = irregular ? p + 1 : p - 1;
+  const size_t lds = sizeof(double) * (size_t((p + 1) * (p + 1)) + (gradients && pg >= 0 ? 3 * size_t((pg + 1) * (pg + 1)) : 0));
 #define FMMH_EVAL(I, G)                                                                                          \
-  hipLaunchKernelGGL((k_evaluate<I, G>), dim3(xcd_grid(rows)), dim3(EVAL_THREADS), 0, s, p, rows, row_leaf,     \
+  hipLaunchKernelGGL((k_evaluate<I, G>), dim3(xcd_grid(rows)), dim3(64), lds, s, p, rows, row_leaf,     \
                      row_off, row_box, box_cell, leaf_cell, tgt_start, tx, ty, tz, coeffs, phi, grad)
   if (irregular) {
     if (gradients)

@@ -11,7 +11,6 @@ namespace dev {
 constexpr int kMaxDegree = 15;  // device path: p <= 15, (p+1)^2 <= 256 coefficients
 constexpr int kMaxCoef = (kMaxDegree + 1) * (kMaxDegree + 1);
 constexpr int kMaxCoefGrad = (kMaxDegree + 2) * (kMaxDegree + 2);
-constexpr int EVAL_THREADS = 128;
 constexpr int COLS = 128;  // output columns (boxes) per contraction task: 8 waves x 16
 constexpr int KC = 4;      // MFMA k-steps per LDS chunk of a table
 
