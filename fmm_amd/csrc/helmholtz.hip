@@ -302,9 +302,11 @@ __global__ __launch_bounds__(64) void k_h_evaluate(int p, uint32_t rows, const u
                                                    const double *ty, const double *tz, const double *hlev,
                                                    const double *norm, const double *coeffs, double *phi_hat,
                                                    double *grad_hat) {
-  __shared__ double Cr[kMaxCoef], Ci[kMaxCoef], Nm[kMaxCoef];
-  __shared__ double Dr[3][kMaxCoef], Di[3][kMaxCoef];
-  __shared__ double CD[(kMaxDegree + 1) * 4][64];
+  // coefficients scaled by their normalisation, (re, im) planes interleaved: C[t] =
+  // N_t (c_re[t], c_im[t]); derivative coefficients likewise, D[q][t] = N_t (D_re, D_im)
+  __shared__ double2 C[kMaxCoef];
+  __shared__ double2 D[3][kMaxCoef];
+  __shared__ double Nm[kMaxCoef];
   const uint32_t row = xcd_index(rows);
   if (row >= rows) return;
   const int P1 = p + 1, nc = P1 * P1;
@@ -326,10 +328,9 @@ __global__ __launch_bounds__(64) void k_h_evaluate(int p, uint32_t rows, const u
     for (uint32_t e = row_off[row]; e < row_off[row + 1]; ++e) {
       const uint32_t box = row_box[e];
       __syncthreads();
-      for (int i = lane; i < nc; i += 64) {
-        Cr[i] = coeffs[(3 * size_t(box)) * nc + i];
-        Ci[i] = coeffs[(3 * size_t(box) + 1) * nc + i];
-      }
+      for (int i = lane; i < nc; i += 64)
+        C[i] = make_double2(Nm[i] * coeffs[(3 * size_t(box)) * nc + i],
+                            Nm[i] * coeffs[(3 * size_t(box) + 1) * nc + i]);
       __syncthreads();
       if (GRAD)
         for (int i = lane; i < p * p; i += 64) {
@@ -338,30 +339,29 @@ __global__ __launch_bounds__(64) void k_h_evaluate(int p, uint32_t rows, const u
           while ((np + 1) * (np + 1) <= i) ++np;
           const int mp = i - np * np - np, q = mp < 0 ? -mp : mp, n = np + 1;
           const int base = n * n + n;
-          // alpha_m of degree n for both planes: (re plane) ar + i ai, (im plane) br + i bi
-          auto alpha = [&](const double *a, int m, double &xr, double &xi) {
+          // alpha_m of degree n for both planes (the coefficients carry N already)
+          auto alpha = [&](int pl, int m, double &xr, double &xi) {
             if (m > n) {
               xr = xi = 0.0;
             } else if (m == 0) {
-              xr = Nm[base] * a[base];
+              xr = pl == 0 ? C[base].x : C[base].y;
               xi = 0.0;
             } else {
-              xr = Nm[base + m] * a[base + m];
-              xi = -Nm[base - m] * a[base - m];
+              xr = pl == 0 ? C[base + m].x : C[base + m].y;
+              xi = -(pl == 0 ? C[base - m].x : C[base - m].y);
             }
           };
-          const double inv = 1.0 / Nm[np * np + np + q];
+          const double inv = Nm[i] / Nm[np * np + np + q];
+          double out[2][3];
 #pragma unroll
           for (int pl = 0; pl < 2; ++pl) {
-            const double *a = pl == 0 ? Cr : Ci;
-            double (*D)[kMaxCoef] = pl == 0 ? Dr : Di;
             double ur, ui, vr, vi, wr, wi;  // alpha_{q+1}, alpha_{q-1}, alpha_q
-            alpha(a, q + 1, ur, ui);
+            alpha(pl, q + 1, ur, ui);
             if (q >= 1)
-              alpha(a, q - 1, vr, vi);
+              alpha(pl, q - 1, vr, vi);
             else
               vr = vi = 0.0;
-            alpha(a, q, wr, wi);
+            alpha(pl, q, wr, wi);
             double bx_r, bx_i, by_r, by_i;
             if (q == 0) {
               bx_r = 0.5 * ur;
@@ -379,12 +379,14 @@ __global__ __launch_bounds__(64) void k_h_evaluate(int p, uint32_t rows, const u
               by_r = -0.5 * (ui + vi);
               by_i = 0.5 * (ur + vr);
             }
-            // slot +q takes Re(beta) / N, slot -q takes -Im(beta) / N
+            // slot +q takes Re(beta) / N, slot -q takes -Im(beta) / N (times N_i)
             const bool neg = mp < 0;
-            D[0][i] = (neg ? -bx_i : bx_r) * inv;
-            D[1][i] = (neg ? -by_i : by_r) * inv;
-            D[2][i] = (neg ? -wi : wr) * inv;
+            out[pl][0] = (neg ? -bx_i : bx_r) * inv;
+            out[pl][1] = (neg ? -by_i : by_r) * inv;
+            out[pl][2] = (neg ? -wi : wr) * inv;
           }
+#pragma unroll
+          for (int d = 0; d < 3; ++d) D[d][i] = make_double2(out[0][d], out[1][d]);
         }
       double c[3] = {0.0, 0.0, 0.0}, r = 1.0;
       if (SING) cell_frame(box_cell + 4 * size_t(box), lcell, c, r);  // c^(s|t), r^(s|t)
@@ -392,7 +394,8 @@ __global__ __launch_bounds__(64) void k_h_evaluate(int p, uint32_t rows, const u
       const double inv_r = 1.0 / r;
       const double v0 = (u[0] - c[0]) * inv_r, v1 = (u[1] - c[1]) * inv_r, v2 = (u[2] - c[2]) * inv_r;
       const double r2 = v0 * v0 + v1 * v1 + v2 * v2, rho = sqrt(r2);
-      // radial factors into CD (this lane's column)
+      // radial factors c_n, d_n of this lane's target, in registers (degree loops unrolled)
+      cx cn[kMaxDegree + 2], dn[kMaxDegree + 1];
       {
         const cx zz = cscale(L.k, rho);
         cx v[kRad];
@@ -401,15 +404,11 @@ __global__ __launch_bounds__(64) void k_h_evaluate(int p, uint32_t rows, const u
           cx f = {exp(-L.beta), 0.0};
           const cx step = cscale(L.k, 1.0 / L.s), mk2 = cscale(cmul(L.k, L.k), -1.0);
 #pragma unroll
-          for (int d = 0; d <= kMaxDegree; ++d)
-            if (d <= p) {
-              const cx cn = cmul(f, v[d]), dn = cmul(mk2, cmul(f, v[d + 1]));
-              CD[4 * d][lane] = cn.re;
-              CD[4 * d + 1][lane] = cn.im;
-              CD[4 * d + 2][lane] = dn.re;
-              CD[4 * d + 3][lane] = dn.im;
-              f = cmul(f, step);
-            }
+          for (int d = 0; d <= kMaxDegree; ++d) {
+            cn[d] = cmul(f, v[d]);
+            dn[d] = cmul(mk2, cmul(f, v[d + 1]));
+            f = cmul(f, step);
+          }
         } else {
           cx w0;
           radial_h(p, zz, v, w0);
@@ -417,19 +416,15 @@ __global__ __launch_bounds__(64) void k_h_evaluate(int p, uint32_t rows, const u
           const cx step = cdiv({L.s / r2, 0.0}, L.k);
           const double inv_rho = 1.0 / rho;
 #pragma unroll
-          for (int d = 0; d <= kMaxDegree; ++d)
-            if (d <= p) {
-              const cx ed = v[d], dd = d == 0 ? w0 : cmul(zz, v[d > 0 ? d - 1 : 0]);
-              const cx cn = cmul(w, ed);
-              const cx hr = cscale(csub(cmul(L.k, dd), cscale(ed, (2.0 * d + 1.0) * inv_rho)), inv_rho);
-              const cx dn = cmul(w, hr);
-              CD[4 * d][lane] = cn.re;
-              CD[4 * d + 1][lane] = cn.im;
-              CD[4 * d + 2][lane] = dn.re;
-              CD[4 * d + 3][lane] = dn.im;
-              w = cmul(w, step);
-            }
+          for (int d = 0; d <= kMaxDegree; ++d) {
+            const cx ed = v[d], dd = d == 0 ? w0 : cmul(zz, v[d > 0 ? d - 1 : 0]);
+            cn[d] = cmul(w, ed);
+            const cx hr = cscale(csub(cmul(L.k, dd), cscale(ed, (2.0 * d + 1.0) * inv_rho)), inv_rho);
+            dn[d] = cmul(w, hr);
+            w = cmul(w, step);
+          }
         }
+        cn[kMaxDegree + 1] = {0.0, 0.0};
       }
       __syncthreads();
       cx f = {0.0, 0.0}, rad = {0.0, 0.0}, gr[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
@@ -442,7 +437,9 @@ __global__ __launch_bounds__(64) void k_h_evaluate(int p, uint32_t rows, const u
           di = ni;
         }
         double rr = dr, ri = di, pr = 0.0, pi = 0.0;
-        for (int n = m; n <= p; ++n) {
+#pragma unroll
+        for (int n = 0; n <= kMaxDegree; ++n) {
+          if (n < m || n > p) continue;
           if (n == m + 1) {
             pr = rr;
             pi = ri;
@@ -458,20 +455,18 @@ __global__ __launch_bounds__(64) void k_h_evaluate(int p, uint32_t rows, const u
             ri = ni;
           }
           const int tp = n * n + n + m, tm = n * n + n - m;
-          const double sp = Nm[tp] * rr, sm = m > 0 ? Nm[tm] * ri : 0.0;
-          const cx tsum = {Cr[tp] * sp + (m > 0 ? Cr[tm] * sm : 0.0), Ci[tp] * sp + (m > 0 ? Ci[tm] * sm : 0.0)};
-          const cx cn = {CD[4 * n][lane], CD[4 * n + 1][lane]};
-          f = cadd(f, cmul(cn, tsum));
+          const double sm = m > 0 ? ri : 0.0;
+          const double2 a = C[tp], b = C[tm];
+          const cx tsum = {a.x * rr + b.x * sm, a.y * rr + b.y * sm};
+          f = cadd(f, cmul(cn[n], tsum));
           if (GRAD) {
-            const cx dn = {CD[4 * n + 2][lane], CD[4 * n + 3][lane]};
-            rad = cadd(rad, cmul(dn, tsum));
+            rad = cadd(rad, cmul(dn[n], tsum));
             if (n < p) {
-              const cx cn1 = {CD[4 * (n + 1)][lane], CD[4 * (n + 1) + 1][lane]};
 #pragma unroll
               for (int q = 0; q < 3; ++q) {
-                const cx ds = {Dr[q][tp] * sp + (m > 0 ? Dr[q][tm] * sm : 0.0),
-                               Di[q][tp] * sp + (m > 0 ? Di[q][tm] * sm : 0.0)};
-                gr[q] = cadd(gr[q], cmul(cn1, ds));
+                const double2 da = D[q][tp], db = D[q][tm];
+                const cx ds = {da.x * rr + db.x * sm, da.y * rr + db.y * sm};
+                gr[q] = cadd(gr[q], cmul(cn[n + 1], ds));
               }
             }
           }
