@@ -160,10 +160,17 @@ __global__ __launch_bounds__(256) void k_h_expand(int p, uint32_t rows, const ui
   __shared__ double H[HT][kMaxCoef];
   __shared__ double Wr[HT][kMaxDegree + 1], Wi[HT][kMaxDegree + 1];
   __shared__ double Pt[HT][4];
+  __shared__ double rinv[kMaxDegree + 1][kMaxDegree + 1];  // [d][m]: 1/((d+m)(d-m)), d > m
+  __shared__ double dinv[kMaxDegree + 1];                   // [q]: 1/(2q), q > 0
   const uint32_t row = xcd_index(rows);
   if (row >= rows) return;
   const int P1 = p + 1, nc = P1 * P1;
   const int k = threadIdx.x;
+  {
+    const int d = k / (kMaxDegree + 1), m = k % (kMaxDegree + 1);
+    rinv[d][m] = d > m ? 1.0 / double((d + m) * (d - m)) : 0.0;
+    if (k <= kMaxDegree) dinv[k] = k > 0 ? 1.0 / (2.0 * k) : 0.0;
+  }
   int n = 0;
   while ((n + 1) * (n + 1) <= k) ++n;
   const uint32_t *cell = row_cell + 4 * size_t(row);
@@ -232,7 +239,7 @@ __global__ __launch_bounds__(256) void k_h_expand(int p, uint32_t rows, const ui
           const double x = Pt[j][0], y = Pt[j][1], z = Pt[j][2], r2 = Pt[j][3];
           double rr = 1.0, ri = 0.0;
           for (int q = 1; q <= m; ++q) {
-            const double inv = 1.0 / (2.0 * q);
+            const double inv = dinv[q];
             const double nr = (x * rr - y * ri) * inv, ni = (x * ri + y * rr) * inv;
             rr = nr;
             ri = ni;
@@ -245,7 +252,7 @@ __global__ __launch_bounds__(256) void k_h_expand(int p, uint32_t rows, const ui
               rr *= z;
               ri *= z;
             } else if (d >= m + 2) {
-              const double inv = 1.0 / double((d + m) * (d - m));
+              const double inv = rinv[d][m];
               const double nr = ((2.0 * d - 1.0) * z * rr - r2 * pr) * inv;
               const double ni = ((2.0 * d - 1.0) * z * ri - r2 * pi) * inv;
               pr = rr;
@@ -253,9 +260,10 @@ __global__ __launch_bounds__(256) void k_h_expand(int p, uint32_t rows, const ui
               rr = nr;
               ri = ni;
             }
+            // unnormalised: slot k's normalisation multiplies its sum at the end
             const int t = d * d + d;
-            H[j][t + m] = norm[t + m] * rr;
-            if (m > 0) H[j][t - m] = norm[t - m] * ri;
+            H[j][t + m] = rr;
+            if (m > 0) H[j][t - m] = ri;
           }
         }
       }
@@ -270,8 +278,9 @@ __global__ __launch_bounds__(256) void k_h_expand(int p, uint32_t rows, const ui
   }
   if (k < nc) {
     const size_t box = row_box[row];
-    coeffs[(3 * box) * nc + k] += ar;
-    coeffs[(3 * box + 1) * nc + k] += ai;
+    const double nk = norm[k];
+    coeffs[(3 * box) * nc + k] += nk * ar;
+    coeffs[(3 * box + 1) * nc + k] += nk * ai;
   }
 }
 
@@ -307,11 +316,16 @@ __global__ __launch_bounds__(64) void k_h_evaluate(int p, uint32_t rows, const u
   __shared__ double2 C[kMaxCoef];
   __shared__ double2 D[3][kMaxCoef];
   __shared__ double Nm[kMaxCoef];
+  __shared__ double rinv[kMaxDegree + 1][kMaxDegree + 1];  // [n][m]: 1/((n+m)(n-m)), n >= m + 2
   const uint32_t row = xcd_index(rows);
   if (row >= rows) return;
   const int P1 = p + 1, nc = P1 * P1;
   const int lane = threadIdx.x;
   for (int t = lane; t < nc; t += 64) Nm[t] = norm[t];
+  for (int t = lane; t < (kMaxDegree + 1) * (kMaxDegree + 1); t += 64) {
+    const int n = t / (kMaxDegree + 1), m = t % (kMaxDegree + 1);
+    rinv[n][m] = n > m ? 1.0 / double((n + m) * (n - m)) : 0.0;
+  }
   const uint32_t lf = row_leaf[row];
   const uint32_t t0 = tgt_start[lf], t1 = tgt_start[lf + 1];
   const uint32_t *lcell = leaf_cell + 4 * size_t(lf);
@@ -446,7 +460,7 @@ __global__ __launch_bounds__(64) void k_h_evaluate(int p, uint32_t rows, const u
             rr *= v2;
             ri *= v2;
           } else if (n >= m + 2) {
-            const double inv = 1.0 / double((n + m) * (n - m));
+            const double inv = rinv[n][m];
             const double nr = ((2.0 * n - 1.0) * v2 * rr - r2 * pr) * inv;
             const double ni = ((2.0 * n - 1.0) * v2 * ri - r2 * pi) * inv;
             pr = rr;
