@@ -317,6 +317,80 @@ __global__ __launch_bounds__(512) void k_m2m_split(int nc, int nchunks, uint32_t
   }
 }
 
+// The same M2M with one workgroup per (task, M-tile): wave w takes group g0 + w (and
+// + 8, ... if a task has more), issues every table and source load of that group and
+// M-tile at once (nchunks <= MT chunks of KC k-steps, in registers) and then runs the
+// MFMA chain, so a task pays the memory latency once per group instead of once per
+// chunk.  The per-tile MFMA order and the wave tree are those of k_m2m_split, so the
+// sums are the same bits.  For MT <= 11 (p <= 12: 88 doubles of operands per lane) and
+// the levels near the root, where a few tasks would leave most CUs idle.
+template <int MT>
+__global__ __launch_bounds__(512) void k_m2m_rows(int nc, int nchunks, uint32_t ntasks,
+                                                  const uint32_t *__restrict__ task_group_off,
+                                                  const uint16_t *__restrict__ group_table,
+                                                  const uint32_t *__restrict__ group_src,
+                                                  const uint32_t *__restrict__ col_out,
+                                                  const double *__restrict__ tsw, const double *__restrict__ X,
+                                                  double *__restrict__ Y) {
+  __shared__ d4 red[4][64];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int col = lane & 15, g = lane >> 4;
+  const uint32_t task = blockIdx.x / MT;
+  const int mt = int(blockIdx.x % MT);
+  if (task >= ntasks) return;  // uniform over the workgroup
+  const uint32_t g0 = task_group_off[task], g1 = task_group_off[task + 1];
+  const int KS = nchunks * KC;
+  const size_t tsize = size_t(nchunks) * MT * KC * 64;  // one swizzled table
+  d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+  for (uint32_t gi = g0 + uint32_t(wave); gi < g1; gi += 8) {
+    const uint32_t w2 = reinterpret_cast<const uint32_t *>(group_table)[gi >> 1];
+    const uint32_t table = (gi & 1) ? (w2 >> 16) : (w2 & 0xFFFFu);
+    const double *a = tsw + size_t(table) * tsize;
+    const uint32_t src = group_src[size_t(gi) * 16 + col];
+    const bool has = src != 0xFFFFFFFFu;
+    // the run of k = g KS .. g KS + KS - 1 (the caller's X holds 15 readable doubles past
+    // its last expansion; values past nc are masked)
+    const double *xb = X + size_t(has ? src : 0) * nc + g * KS;
+    double av[MT * KC], bv[MT * KC];
+#pragma unroll
+    for (int c = 0; c < MT; ++c)
+      if (c < nchunks)
+#pragma unroll
+        for (int kk = 0; kk < KC; ++kk) {
+          const int k = c * KC + kk + g * KS;
+          const double v = xb[c * KC + kk];
+          bv[c * KC + kk] = (has && k < nc) ? v : 0.0;
+          av[c * KC + kk] = a[((size_t(c) * MT + mt) * KC + kk) * 64 + lane];
+        }
+#pragma unroll
+    for (int c = 0; c < MT; ++c)
+      if (c < nchunks)
+#pragma unroll
+        for (int kk = 0; kk < KC; ++kk)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[c * KC + kk], bv[c * KC + kk], acc, 0, 0, 0);
+  }
+  // fixed-order tree over the waves' partial sums
+#pragma unroll
+  for (int half = 4; half >= 1; half >>= 1) {
+    __syncthreads();
+    if (wave >= half && wave < 2 * half) red[wave - half][lane] = acc;
+    __syncthreads();
+    if (wave < half) acc += red[wave][lane];
+  }
+  if (wave == 0) {
+    const uint32_t out = col_out[size_t(task) * 16 + col];
+    if (out != 0xFFFFFFFFu) {
+      double *y = Y + size_t(out) * nc;
+      // accumulator element r of lane (g, j): row 4 r + g of the M-tile (f64 16x16x4)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = mt * 16 + 4 * r + g;
+        if (row < nc) y[row] += acc[r];
+      }
+    }
+  }
+}
+
 // ---- bf16x3 contraction ---------------------------------------------------------------------
 
 __device__ __forceinline__ bf8 as_bf8(u4 v) {
@@ -619,6 +693,30 @@ void m2m_split(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group
                const uint32_t *group_src, const uint32_t *col_out, const double *tsw, const double *X, double *Y) {
   if (ntasks == 0) return;
   const int nc = (p + 1) * (p + 1), nch = k_chunks(p);
+#define FMMH_M2M_ROWS(MT)                                                                                        \
+  case MT:                                                                                                       \
+    hipLaunchKernelGGL((k_m2m_rows<MT>), dim3(ntasks * MT), dim3(512), 0, s, nc, nch, ntasks, task_group_off,   \
+                       group_table, group_src, col_out, tsw, X, Y);                                              \
+    return;
+  // p <= 12 and fewer (task, M-tile) pairs than CUs: one workgroup per pair (each
+  // re-reads the source expansions, so a large level keeps the split form: 4M-point
+  // tree, level 0: 57 -> 14 us, level 5: 226 us split, 787 us per pair)
+  if (ntasks * uint32_t(m_tiles(p)) < 256) switch (m_tiles(p)) {
+    FMMH_M2M_ROWS(1)
+    FMMH_M2M_ROWS(2)
+    FMMH_M2M_ROWS(3)
+    FMMH_M2M_ROWS(4)
+    FMMH_M2M_ROWS(5)
+    FMMH_M2M_ROWS(6)
+    FMMH_M2M_ROWS(7)
+    FMMH_M2M_ROWS(8)
+    FMMH_M2M_ROWS(9)
+    FMMH_M2M_ROWS(10)
+    FMMH_M2M_ROWS(11)
+    default:
+      break;
+  }
+#undef FMMH_M2M_ROWS
 #define FMMH_M2M(MT)                                                                                             \
   case MT:                                                                                                       \
     hipLaunchKernelGGL((k_m2m_split<MT>), dim3(ntasks), dim3(512), 0, s, nc, nch, ntasks, task_group_off,       \

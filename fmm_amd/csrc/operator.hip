@@ -118,7 +118,9 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
   }
 
   // expansions and outputs (the clustered-mode scratch rows are sized in build_schedules)
-  mult_.alloc(nbox_total_ * nc_);
+  // 64 doubles past the last box: the per-M-tile M2M reads a source expansion as four
+  // runs of 4 ceil(nc / 16) values (at most 15 past its end) and masks the tail
+  mult_.alloc(nbox_total_ * nc_ + 64);
   phi_hat_.alloc(nt_);
   grad_hat_.alloc(opt_.gradients ? 3 * nt_ : 0);
   phi_.alloc(nt_);
