@@ -812,7 +812,7 @@ __device__ __forceinline__ void read_order(const double *c, int n, int q, int pm
 }
 
 template <bool IRREG, bool GRAD>
-__global__ __launch_bounds__(64) void k_evaluate(int p, uint32_t rows, const uint32_t *row_leaf,
+__global__ __launch_bounds__(IRREG ? 128 : 64) void k_evaluate(int p, uint32_t rows, const uint32_t *row_leaf,
                                                            const uint32_t *row_off,
                                                            const uint32_t *row_box, const uint32_t *box_cell,
                                                            const uint32_t *leaf_cell, const uint32_t *tgt_start,
@@ -830,22 +830,25 @@ __global__ __launch_bounds__(64) void k_evaluate(int p, uint32_t rows, const uin
   const int pg = IRREG ? p + 1 : p - 1;  // degree of the gradient coefficients
   const int ng = pg >= 0 ? (pg + 1) * (pg + 1) : 0;
   double *C = eval_smem, *G0 = C + nc, *G1 = G0 + ng, *G2 = G1 + ng;
-  // one wave per leaf: leaves of more than 32 targets one lane per target, in passes of
-  // 64; smaller ones L = min(8, 64 / n) sub-lanes per target, which share its columns
+  // L2P: one wave per leaf; leaves of more than 32 targets one lane per target, in
+  // passes of 64, smaller ones L = min(8, 64 / n) sub-lanes per target, which share its
+  // columns.  M2P: two waves, one thread per target (a row's W-list boxes are taken in
+  // turn, so the per-box set-up and passes stay short on long rows).
+  constexpr int NT = IRREG ? 128 : 64;
   const int lane = threadIdx.x;
   const uint32_t nt = t1 - t0;
-  const int L = nt > 32 ? 1 : (64 / int(nt) < 8 ? 64 / int(nt) : 8);
-  const int per = 64 / L;  // targets per pass
+  const int L = (IRREG || nt > 32) ? 1 : (64 / int(nt) < 8 ? 64 / int(nt) : 8);
+  const int per = NT / L;  // targets per pass
   const int u = lane % L, tl = lane / L;
   for (uint32_t e = row_off[row]; e < row_off[row + 1]; ++e) {
     const uint32_t box = row_box[e];
     double c[3] = {0.0, 0.0, 0.0}, r = 1.0;
     if (IRREG) relative_frame<double>(box_cell + 4 * size_t(box), leaf_cell + 4 * size_t(lf), c, r);
     __syncthreads();
-    for (int k = lane; k < nc; k += 64) C[k] = coeffs[size_t(box) * nc + k];
+    for (int k = lane; k < nc; k += NT) C[k] = coeffs[size_t(box) * nc + k];
     __syncthreads();
     if (GRAD)
-      for (int k = lane; k < ng; k += 64) {
+      for (int k = lane; k < ng; k += NT) {
         int n = 0;
         while ((n + 1) * (n + 1) <= k) ++n;
         const int m = k - n * n - n;
@@ -1090,7 +1093,7 @@ void evaluate(hipStream_t s, int p, int irregular, int gradients, uint32_t rows,
   const int pg = irregular ? p + 1 : p - 1;
   const size_t lds = sizeof(double) * (size_t((p + 1) * (p + 1)) + (gradients && pg >= 0 ? 3 * size_t((pg + 1) * (pg + 1)) : 0));
 #define FMMH_EVAL(I, G)                                                                                          \
-  hipLaunchKernelGGL((k_evaluate<I, G>), dim3(xcd_grid(rows)), dim3(64), lds, s, p, rows, row_leaf,     \
+  hipLaunchKernelGGL((k_evaluate<I, G>), dim3(xcd_grid(rows)), dim3(I ? 128 : 64), lds, s, p, rows, row_leaf, \
                      row_off, row_box, box_cell, leaf_cell, tgt_start, tx, ty, tz, coeffs, phi, grad)
   if (irregular) {
     if (gradients)
