@@ -514,9 +514,8 @@ __global__ __launch_bounds__(64) void k_h_evaluate(int p, uint32_t rows, const u
 // from double-float coordinates (hi + lo), the pair arithmetic in f32, each source leaf's
 // f32 sums added to f64 accumulators.
 // Pair kernel pieces per precision.  f64: sqrt, division and sincos of the math library.
-// f32: 1/r from v_rsq_f32 (1 ulp) and r = r^2 / r; sine and cosine by quadrant reduction,
-// x = q pi/2 + y with pi/2 split in two f32 parts (q exact while |x| < 2^17), and the Taylor
-// polynomials of sin y to y^9 and cos y to y^8 on |y| <= pi/4 (truncation below 3e-9).
+// f32: 1/r from v_rsq_f32 (1 ulp) and r = r^2 / r; sine and cosine from v_sin_f32 and
+// v_cos_f32 of the fraction of x / (2 pi).
 template <typename T>
 struct PairMath;
 
@@ -537,23 +536,11 @@ struct PairMath<float> {
     r = r2 * ir;
   }
   __device__ static __forceinline__ void sincos(float x, float *s, float *c) {
-    const float q = rintf(x * 0.636619772367581343f);  // 2/pi
-    // pi/2 = 1.57079637050628662109375 - 4.37113900018624283e-8 (two f32 parts)
-    float y = fmaf(-q, 1.57079637050628662109375f, x);
-    y = fmaf(q, 4.37113900018624283e-8f, y);
-    const float y2 = y * y;
-    float ps = fmaf(y2, 2.75573192e-6f, -1.98412698e-4f);  // 1/9!, -1/7!
-    ps = fmaf(y2, ps, 8.33333333e-3f);                     // 1/5!
-    ps = fmaf(y2, ps, -1.66666667e-1f);                    // -1/3!
-    const float sy = fmaf(y * y2, ps, y);
-    float pc = fmaf(y2, 2.48015873e-5f, -1.38888889e-3f);  // 1/8!, -1/6!
-    pc = fmaf(y2, pc, 4.16666667e-2f);                     // 1/4!
-    pc = fmaf(y2, pc, -0.5f);
-    const float cy = fmaf(y2, pc, 1.0f);
-    const int qi = int(q) & 3;
-    const float ss = (qi & 1) ? cy : sy, cc = (qi & 1) ? sy : cy;
-    *s = (qi & 2) ? -ss : ss;
-    *c = ((qi + 1) & 2) ? -cc : cc;
+    // in revolutions: t = x / (2 pi), reduced to [0, 1) by v_fract_f32; v_sin_f32 and
+    // v_cos_f32 take revolutions
+    const float t = __builtin_amdgcn_fractf(x * 0.159154943091895336f);
+    *s = __builtin_amdgcn_sinf(t);
+    *c = __builtin_amdgcn_cosf(t);
   }
 };
 
