@@ -624,7 +624,7 @@ __device__ __forceinline__ void h_p2p_uniform(uint32_t lf, uint32_t tb, uint32_t
           loc[a][0] += q0 * er - q1 * ei;
           loc[a][1] += q0 * ei + q1 * er;
           if (GRAD) {
-            const float ar = -kim * rr - 1.0f, ai = kre * rr, ir2 = ir * ir;
+            const float ar = DAMP ? -kim * rr - 1.0f : -1.0f, ai = kre * rr, ir2 = ir * ir;
             const float gr = (ar * er - ai * ei) * ir2, gi = (ar * ei + ai * er) * ir2;
             const float qgr = q0 * gr - q1 * gi, qgi = q0 * gi + q1 * gr;
             loc[a][2] += qgr * dx;
@@ -756,7 +756,7 @@ __global__ __launch_bounds__(64) void k_h_p2p(uint32_t leaf_first, uint32_t coun
             loc[0] += q0 * er - q1 * ei;
             loc[1] += q0 * ei + q1 * er;
             if (GRAD) {
-              const T ar = -kim * rr - T(1), ai = kre * rr, ir2 = ir * ir;
+              const T ar = DAMP ? -kim * rr - T(1) : T(-1), ai = kre * rr, ir2 = ir * ir;
               const T gr = (ar * er - ai * ei) * ir2, gi = (ar * ei + ai * er) * ir2;
               const T qgr = q0 * gr - q1 * gi, qgi = q0 * gi + q1 * gr;
               loc[2] += qgr * dx;
