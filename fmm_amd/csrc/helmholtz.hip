@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "hkernels.hpp"
 #include "kernels.hpp"
@@ -605,37 +606,51 @@ __device__ __forceinline__ void h_p2p_uniform(uint32_t lf, uint32_t tb, uint32_t
       for (int a = 0; a < A; ++a)
 #pragma unroll
         for (int q = 0; q < 8; ++q) loc[a][q] = 0.0f;
+      // off the own leaf the smallest normal number enters the first product: it keeps a
+      // separation rounded to zero finite (no r^2 == 0 test) and changes no r^2 above
+      // 2^-102; the own leaf keeps the test (a pair with r^2 == 0 adds nothing)
+      auto run = [&](auto check) {
 #pragma unroll 2
-      for (uint32_t j = jb; j < je; ++j) {
-        const float4 h = rec[2 * size_t(j)], l = rec[2 * size_t(j) + 1];
+        for (uint32_t j = jb; j < je; ++j) {
+          const float4 h = rec[2 * size_t(j)], l = rec[2 * size_t(j) + 1];
 #pragma unroll
-        for (int a = 0; a < A; ++a) {
-          const float dx = (xh[a][0] - h.x) + (xl[a][0] - h.w);
-          const float dy = (xh[a][1] - h.y) + (xl[a][1] - l.x);
-          const float dz = (xh[a][2] - h.z) + (xl[a][2] - l.y);
-          const float r2 = dx * dx + dy * dy + dz * dz;
-          float rr, ir;
-          PairMath<float>::rinv(r2, rr, ir);
-          float sn, cs2;
-          PairMath<float>::sincos(kre * rr, &sn, &cs2);
-          const float dmp = DAMP ? expf(-kim * rr) : 1.0f;
-          const float er = dmp * cs2 * ir, ei = dmp * sn * ir;
-          const float q0 = l.z, q1 = l.w;
-          loc[a][0] += q0 * er - q1 * ei;
-          loc[a][1] += q0 * ei + q1 * er;
-          if (GRAD) {
-            const float ar = DAMP ? -kim * rr - 1.0f : -1.0f, ai = kre * rr, ir2 = ir * ir;
-            const float gr = (ar * er - ai * ei) * ir2, gi = (ar * ei + ai * er) * ir2;
-            const float qgr = q0 * gr - q1 * gi, qgi = q0 * gi + q1 * gr;
-            loc[a][2] += qgr * dx;
-            loc[a][3] += qgi * dx;
-            loc[a][4] += qgr * dy;
-            loc[a][5] += qgi * dy;
-            loc[a][6] += qgr * dz;
-            loc[a][7] += qgi * dz;
+          for (int a = 0; a < A; ++a) {
+            const float dx = (xh[a][0] - h.x) + (xl[a][0] - h.w);
+            const float dy = (xh[a][1] - h.y) + (xl[a][1] - l.x);
+            const float dz = (xh[a][2] - h.z) + (xl[a][2] - l.y);
+            float rr, ir;
+            if (decltype(check)::value) {
+              PairMath<float>::rinv(dx * dx + dy * dy + dz * dz, rr, ir);
+            } else {
+              const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, 1.17549435e-38f)));
+              ir = __builtin_amdgcn_rsqf(r2);
+              rr = r2 * ir;
+            }
+            float sn, cs2;
+            PairMath<float>::sincos(kre * rr, &sn, &cs2);
+            const float dmp = DAMP ? expf(-kim * rr) : 1.0f;
+            const float er = dmp * cs2 * ir, ei = dmp * sn * ir;
+            const float q0 = l.z, q1 = l.w;
+            loc[a][0] += q0 * er - q1 * ei;
+            loc[a][1] += q0 * ei + q1 * er;
+            if (GRAD) {
+              const float ar = DAMP ? -kim * rr - 1.0f : -1.0f, ai = kre * rr, ir2 = ir * ir;
+              const float gr = (ar * er - ai * ei) * ir2, gi = (ar * ei + ai * er) * ir2;
+              const float qgr = q0 * gr - q1 * gi, qgi = q0 * gi + q1 * gr;
+              loc[a][2] += qgr * dx;
+              loc[a][3] += qgi * dx;
+              loc[a][4] += qgr * dy;
+              loc[a][5] += qgi * dy;
+              loc[a][6] += qgr * dz;
+              loc[a][7] += qgi * dz;
+            }
           }
         }
-      }
+      };
+      if (sl == lf)
+        run(std::true_type{});
+      else
+        run(std::false_type{});
       const double s2 = rho * rho;
 #pragma unroll
       for (int a = 0; a < A; ++a) {
