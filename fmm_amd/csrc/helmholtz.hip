@@ -409,19 +409,20 @@ __global__ __launch_bounds__(64) void k_h_evaluate(int p, uint32_t rows, const u
       const double inv_r = 1.0 / r;
       const double v0 = (u[0] - c[0]) * inv_r, v1 = (u[1] - c[1]) * inv_r, v2 = (u[2] - c[2]) * inv_r;
       const double r2 = v0 * v0 + v1 * v1 + v2 * v2, rho = sqrt(r2);
-      // radial factors c_n, d_n of this lane's target, in registers (degree loops unrolled)
-      cx cn[kMaxDegree + 2], dn[kMaxDegree + 1];
+      // radial factors c_n (and for M2P d_n) of this lane's target, in registers (degree
+      // loops unrolled).  L2P: d_n = -k^2 e^-beta (k/s)^n g_{n+1} = -k s c_{n+1}, so the
+      // radial sum runs over c_{n+1} and takes the factor -k s once at the end.
+      cx cn[kMaxDegree + 2], dn[SING ? kMaxDegree + 1 : 1];
       {
         const cx zz = cscale(L.k, rho);
         cx v[kRad];
         if (!SING) {
           radial_j(p + 1, zz, v);
           cx f = {exp(-L.beta), 0.0};
-          const cx step = cscale(L.k, 1.0 / L.s), mk2 = cscale(cmul(L.k, L.k), -1.0);
+          const cx step = cscale(L.k, 1.0 / L.s);
 #pragma unroll
           for (int d = 0; d <= kMaxDegree; ++d) {
             cn[d] = cmul(f, v[d]);
-            dn[d] = cmul(mk2, cmul(f, v[d + 1]));
             f = cmul(f, step);
           }
         } else {
@@ -435,7 +436,7 @@ __global__ __launch_bounds__(64) void k_h_evaluate(int p, uint32_t rows, const u
             const cx ed = v[d], dd = d == 0 ? w0 : cmul(zz, v[d > 0 ? d - 1 : 0]);
             cn[d] = cmul(w, ed);
             const cx hr = cscale(csub(cmul(L.k, dd), cscale(ed, (2.0 * d + 1.0) * inv_rho)), inv_rho);
-            dn[d] = cmul(w, hr);
+            if (SING) dn[SING ? d : 0] = cmul(w, hr);
             w = cmul(w, step);
           }
         }
@@ -475,7 +476,7 @@ __global__ __launch_bounds__(64) void k_h_evaluate(int p, uint32_t rows, const u
           const cx tsum = {a.x * rr + b.x * sm, a.y * rr + b.y * sm};
           f = cadd(f, cmul(cn[n], tsum));
           if (GRAD) {
-            rad = cadd(rad, cmul(dn[n], tsum));
+            rad = cadd(rad, cmul(SING ? dn[SING ? n : 0] : cn[n + 1], tsum));
             if (n < p) {
 #pragma unroll
               for (int q = 0; q < 3; ++q) {
@@ -489,6 +490,7 @@ __global__ __launch_bounds__(64) void k_h_evaluate(int p, uint32_t rows, const u
       }
       phi = cadd(phi, cscale(f, inv_r));
       if (GRAD) {
+        if (!SING) rad = cmul(cscale(L.k, -L.s), rad);
         const double s2 = inv_r * inv_r, vv[3] = {v0, v1, v2};
 #pragma unroll
         for (int q = 0; q < 3; ++q) g[q] = cadd(g[q], cscale(cadd(gr[q], cscale(rad, vv[q])), s2));

@@ -40,6 +40,7 @@ CASES = [
     ("cube", 8000, 6, 48),
     ("plummer", 8000, 8, 32),
     ("mixed", 10000, 12, 64),
+    ("cube", 4000, 15, 40),
 ]
 
 
