@@ -273,20 +273,46 @@ __global__ __launch_bounds__(512) void k_m2m_split(int nc, int nchunks, uint32_t
     const uint32_t src = group_src[size_t(gi) * 16 + col];
     const bool has = src != 0xFFFFFFFFu;
     const double *x = X + size_t(has ? src : 0) * nc;
-    for (int c = 0; c < nchunks; ++c) {
-      double b[KC];
+    // half chunks (KH k-steps) in two register sets: the loads of half h + 1 are issued
+    // before the MFMAs of half h, so their memory latency runs under those MFMAs (the same
+    // MFMA order per M-tile as one loop over the chunks)
+    constexpr int KH = KC / 2;
+    double a0[KH][MT], b0[KH], a1[KH][MT], b1[KH];
+    auto load = [&](int h, double (&av)[KH][MT], double (&bv)[KH]) {
+      const int c = h / 2, k0 = (h % 2) * KH;
 #pragma unroll
-      for (int kk = 0; kk < KC; ++kk) {
-        const int k = c * KC + kk + g * KS;
+      for (int kk = 0; kk < KH; ++kk) {
+        const int k = c * KC + k0 + kk + g * KS;
         const double v = x[min(k, nc - 1)];
-        b[kk] = (has && k < nc) ? v : 0.0;
-      }
+        bv[kk] = (has && k < nc) ? v : 0.0;
 #pragma unroll
-      for (int kk = 0; kk < KC; ++kk)
+        for (int mt = 0; mt < MT; ++mt) av[kk][mt] = a[((size_t(c) * MT + mt) * KC + k0 + kk) * 64 + lane];
+      }
+    };
+    auto run = [&](const double (&av)[KH][MT], const double (&bv)[KH]) {
+#pragma unroll
+      for (int kk = 0; kk < KH; ++kk)
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
-          acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[((size_t(c) * MT + mt) * KC + kk) * 64 + lane], b[kk],
-                                                         acc[mt], 0, 0, 0);
+          acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk][mt], bv[kk], acc[mt], 0, 0, 0);
+    };
+    const int nh = 2 * nchunks;
+    if constexpr (MT <= 14) {
+      load(0, a0, b0);
+      for (int h = 0; h < nh; h += 2) {
+        if (h + 1 < nh) load(h + 1, a1, b1);
+        run(a0, b0);
+        if (h + 1 < nh) {
+          if (h + 2 < nh) load(h + 2, a0, b0);
+          run(a1, b1);
+        }
+      }
+    } else {
+      // p >= 14: the two register sets would spill; one half at a time
+      for (int h = 0; h < nh; ++h) {
+        load(h, a0, b0);
+        run(a0, b0);
+      }
     }
   }
   // fixed-order tree over the waves' partial sums
