@@ -661,9 +661,19 @@ __global__ void k_reduce_slots(int nc, uint32_t rows, const uint32_t *row_box, c
   const uint32_t r = xcd_index(rows);
   if (r >= rows) return;
   const uint32_t box = row_box[r];
+  const uint32_t e0 = row_off[r], e1 = row_off[r + 1];
   for (int i = threadIdx.x; i < nc; i += blockDim.x) {
     double acc = Y[size_t(box) * nc + i];
-    for (uint32_t e = row_off[r]; e < row_off[r + 1]; ++e) acc += double(S[size_t(row_ent[e] - first_slot) * nc + i]);
+    // eight rows' loads in flight at a time, added in row order
+    uint32_t e = e0;
+    for (; e + 8 <= e1; e += 8) {
+      T v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] = S[size_t(row_ent[e + k] - first_slot) * nc + i];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc += double(v[k]);
+    }
+    for (; e < e1; ++e) acc += double(S[size_t(row_ent[e] - first_slot) * nc + i]);
     Y[size_t(box) * nc + i] = acc;
   }
 }
