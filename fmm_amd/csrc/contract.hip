@@ -348,8 +348,9 @@ __global__ __launch_bounds__(512) void k_m2m_split(int nc, int nchunks, uint32_t
 // M-tile at once (nchunks <= MT chunks of KC k-steps, in registers) and then runs the
 // MFMA chain, so a task pays the memory latency once per group instead of once per
 // chunk.  The per-tile MFMA order and the wave tree are those of k_m2m_split, so the
-// sums are the same bits.  For MT <= 11 (p <= 12: 88 doubles of operands per lane) and
-// the levels near the root, where a few tasks would leave most CUs idle.
+// sums are the same bits.  Up to MT = 11 (p <= 12: 88 doubles of operands per lane) all
+// chunks at once, above in two batches; for the levels near the root, where a few tasks
+// would leave most CUs idle.
 template <int MT>
 __global__ __launch_bounds__(512) void k_m2m_rows(int nc, int nchunks, uint32_t ntasks,
                                                   const uint32_t *__restrict__ task_group_off,
@@ -377,23 +378,31 @@ __global__ __launch_bounds__(512) void k_m2m_rows(int nc, int nchunks, uint32_t 
     // the run of k = g KS .. g KS + KS - 1 (the caller's X holds 15 readable doubles past
     // its last expansion; values past nc are masked)
     const double *xb = X + size_t(has ? src : 0) * nc + g * KS;
-    double av[MT * KC], bv[MT * KC];
+    // the chunks in NB batches of CH (one batch up to MT = 11; two above, whose operands
+    // would not fit the registers at once)
+    constexpr int NB = MT <= 11 ? 1 : 2, CH = (MT + NB - 1) / NB;
 #pragma unroll
-    for (int c = 0; c < MT; ++c)
-      if (c < nchunks)
+    for (int bt = 0; bt < NB; ++bt) {
+      double av[CH * KC], bv[CH * KC];
 #pragma unroll
-        for (int kk = 0; kk < KC; ++kk) {
-          const int k = c * KC + kk + g * KS;
-          const double v = xb[c * KC + kk];
-          bv[c * KC + kk] = (has && k < nc) ? v : 0.0;
-          av[c * KC + kk] = a[((size_t(c) * MT + mt) * KC + kk) * 64 + lane];
-        }
+      for (int cc = 0; cc < CH; ++cc) {
+        const int c = bt * CH + cc;
+        if (c < nchunks)
 #pragma unroll
-    for (int c = 0; c < MT; ++c)
-      if (c < nchunks)
+          for (int kk = 0; kk < KC; ++kk) {
+            const int k = c * KC + kk + g * KS;
+            const double v = xb[c * KC + kk];
+            bv[cc * KC + kk] = (has && k < nc) ? v : 0.0;
+            av[cc * KC + kk] = a[((size_t(c) * MT + mt) * KC + kk) * 64 + lane];
+          }
+      }
 #pragma unroll
-        for (int kk = 0; kk < KC; ++kk)
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[c * KC + kk], bv[c * KC + kk], acc, 0, 0, 0);
+      for (int cc = 0; cc < CH; ++cc)
+        if (bt * CH + cc < nchunks)
+#pragma unroll
+          for (int kk = 0; kk < KC; ++kk)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[cc * KC + kk], bv[cc * KC + kk], acc, 0, 0, 0);
+    }
   }
   // fixed-order tree over the waves' partial sums
 #pragma unroll
@@ -734,7 +743,7 @@ void m2m_split(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group
     hipLaunchKernelGGL((k_m2m_rows<MT>), dim3(ntasks * MT), dim3(512), 0, s, nc, nch, ntasks, task_group_off,   \
                        group_table, group_src, col_out, tsw, X, Y);                                              \
     return;
-  // p <= 12 and fewer (task, M-tile) pairs than CUs: one workgroup per pair (each
+  // fewer (task, M-tile) pairs than CUs: one workgroup per pair (each
   // re-reads the source expansions, so a large level keeps the split form: 4M-point
   // tree, level 0: 57 -> 14 us, level 5: 226 us split, 787 us per pair)
   if (ntasks * uint32_t(m_tiles(p)) < 256) switch (m_tiles(p)) {
@@ -749,6 +758,11 @@ void m2m_split(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group
     FMMH_M2M_ROWS(9)
     FMMH_M2M_ROWS(10)
     FMMH_M2M_ROWS(11)
+    FMMH_M2M_ROWS(12)
+    FMMH_M2M_ROWS(13)
+    FMMH_M2M_ROWS(14)
+    FMMH_M2M_ROWS(15)
+    FMMH_M2M_ROWS(16)
     default:
       break;
   }

@@ -167,7 +167,9 @@ HelmholtzOperator::HelmholtzOperator(const Plan &plan, const HelmholtzOptions &o
   qr_.alloc(ns_);
   qi_.alloc(ns_);
   q_in_.alloc(2 * ns_);
-  mult_.alloc(3 * nbox_total_ * nc_);
+  // 64 doubles past the last row: the M2M's per-tile form reads a source row as four runs
+  // of 4 ceil(nc / 16) values (at most 15 past its end) and masks the tail
+  mult_.alloc(3 * nbox_total_ * nc_ + 64);
   loc_.alloc(3 * nbox_total_ * nc_);
   phi_hat_.alloc(2 * nt_);
   grad_hat_.alloc(opt_.gradients ? 6 * nt_ : 0);
@@ -578,7 +580,14 @@ void HelmholtzOperator::evaluate() {
     timed(1, [&] {
       dev::h_negate(stream_, p, box_base_[l + 1], box_base_[l + 2], mult_.ptr);
       const Schedule &s = m2m_[l];
-      if (s.ntasks)
+      // levels near the root (fewer (task, M-tile) pairs than CUs): one workgroup per
+      // task and M-tile (dev::m2m_split's per-tile form); else the narrow or wide
+      // contraction with its LDS ring
+      const uint32_t mtiles = uint32_t((nc_ + 15) / 16);
+      if (s.ntasks && s.width == 16 && s.ntasks * mtiles < 256)
+        dev::m2m_split(stream_, p, s.ntasks, s.task_group_off.ptr, s.group_table.ptr, s.group_src.ptr,
+                       s.col_out.ptr, t_m2m_.ptr, mult_.ptr, mult_.ptr);
+      else if (s.ntasks)
         dev::contract_f64(stream_, p, s.width, s.ntasks, s.task_group_off.ptr, s.group_table.ptr, s.group_src.ptr,
                           s.col_out.ptr, t_m2m_.ptr, mult_.ptr, mult_.ptr, none, nullptr, 0);
     });

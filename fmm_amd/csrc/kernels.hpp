@@ -119,7 +119,7 @@ void contract_f64(hipStream_t s, int p, int width, uint32_t ntasks, const uint32
                   uint32_t xchunk);
 // M2M tasks of 16 parent columns whose groups are the child octants' tables: wave w
 // takes groups w, w + 8, ..., the eight waves' sums added in a fixed tree
-// (contract.hip k_m2m_split; near the root, p <= 12, one workgroup per task and M-tile,
+// (contract.hip k_m2m_split; near the root one workgroup per task and M-tile,
 // k_m2m_rows, which reads up to 15 doubles past a source's row: X must stay readable
 // 15 doubles past its last expansion).  Y may be X.
 void m2m_split(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off, const uint16_t *group_table,
