@@ -60,6 +60,60 @@ def hbm_traffic(kernel):
     return fk * 1024 * scale.get("fetch_scale", 1.0) + wk * 1024 * scale.get("write_scale", 1.0)
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(args, pts, q, sample, near, exact_phi, exact_grad):
+    """The reference's CPU path restated (oracle/cpu_path.c: OpenMP threads over targets,
+    AVX-512 targets-in-lanes P2P with vrsqrt14 + two Newton steps, the batched M2L column
+    kernel) on this host's cores, in this run: configuration 1 (100,000 uniform points,
+    p = 6, <= 64 per leaf, charges uniform in [-1, 1)) and one evaluation of the
+    benchmark workload itself (same points, charges, degree, leaf size and near rule).
+    Tree and table builds are excluded from the times, as for the GPU."""
+    from oracle import oracle as O  # the CPU baseline only
+
+    threads = O.cpu_threads()
+    # configuration 1
+    p1, q1, s1 = O.bench_problem(100_000, 1000)
+    f1 = O.Fmm(p1, p=6, leaf_points=64)
+    t1 = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        phi1, grad1 = f1.evaluate_cpu_path(q1)
+        t1.append(time.perf_counter() - t0)
+    e1p, e1g = O.direct_sum(p1, q1, p1[s1])
+    c1 = {"n": 100_000, "p": 6, "leaf_points": 64, "ms": min(t1) * 1e3,
+          "mpts_per_s": 100_000 / min(t1) / 1e6,
+          "err_phi": O.error_norms(phi1[s1], e1p / (4 * np.pi))[0],
+          "err_grad": O.error_norms(grad1[s1], e1g / (4 * np.pi), True)[0]}
+    del f1
+    # the benchmark workload
+    t0 = time.perf_counter()
+    f = O.Fmm(pts, p=args.p, leaf_points=args.leaf, near_rule=near)
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    phi, grad = f.evaluate_cpu_path(q)
+    t_eval = time.perf_counter() - t0
+    err_phi = float(np.linalg.norm(phi[sample] - exact_phi) / np.linalg.norm(exact_phi))
+    err_grad = float(np.linalg.norm(grad[sample] - exact_grad) / np.linalg.norm(exact_grad))
+    del f
+    n = len(pts)
+    return {"value": n / t_eval / 1e6, "unit": "Mpts/s", "cores": threads, "kind": "port",
+            "sample": f"restatement of the reference CPU path (oracle/cpu_path.c, {O.cpu_isa()}, {threads} OpenMP "
+                      f"threads) on the whole benchmark workload: N={n}, p={args.p}, <= {args.leaf} per leaf, one "
+                      f"evaluation {t_eval:.2f} s (tree and tables {t_build:.1f} s excluded)",
+            "ms_per_eval": t_eval * 1e3, "err_phi": err_phi, "err_grad": err_grad,
+            "cpu_model": cpu_model(), "isa": O.cpu_isa(), "nproc": os.cpu_count(),
+            "config1": c1}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,7 +123,6 @@ def main():
     ap.add_argument("--p", type=int, default=12)
     ap.add_argument("--leaf", type=int, default=128)
     ap.add_argument("--sampled", type=int, default=4096)
-    ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--near", default="device", help="near rule: off, device, or N_W,N_X")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain f64 leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -163,6 +216,11 @@ def main():
             step()
             torch.cuda.synchronize()
             times.append(time.perf_counter() - s)
+        # the errors of the timed (eager) steps' own output
+        phi = phi_dev.cpu().numpy()
+        grad = grad_dev.cpu().numpy().reshape(-1, 3)
+        e_phi = error_norm(phi[sample], exact_phi)
+        e_grad = error_norm(grad[sample], exact_grad)
         # per-kind device times in a pass of their own (HIP events on the operator's own
         # stream around each launch; kept out of the timed steps above)
         op.kind_timing(True)
@@ -186,19 +244,18 @@ def main():
             step_graph()
             torch.cuda.synchronize()
             gtimes.append(time.perf_counter() - s)
-        phi = phi_dev.cpu().numpy()
-        grad = grad_dev.cpu().numpy().reshape(-1, 3)
-        e_phi = error_norm(phi[sample], exact_phi)
-        e_grad = error_norm(grad[sample], exact_grad)
+        # the graph replay's output against the eager output (same launches, same bits)
+        graph_diff = float(np.abs(phi_dev.cpu().numpy() - phi).max())
         med = float(np.median(times))
         log(f"[{precision}] median {med * 1e3:.2f} ms ({args.n / med / 1e6:.1f} Mpts/s), "
             f"err phi {e_phi:.3e} grad {e_grad:.3e}")
         log(f"[{precision}] kinds (ms, launches): " +
             ", ".join(f"{k} {v[0] / max(v[1], 1):.3f}x{v[1]}" for k, v in kinds.items() if v[1]))
         gmed = float(np.median(gtimes))
-        log(f"[{precision}] hipGraph replay: median {gmed * 1e3:.2f} ms ({args.n / gmed / 1e6:.1f} Mpts/s)")
+        log(f"[{precision}] hipGraph replay: median {gmed * 1e3:.2f} ms ({args.n / gmed / 1e6:.1f} Mpts/s), "
+            f"max |graph - eager| phi {graph_diff:.1e}")
         res = dict(ms=med * 1e3, times=times, mpts=args.n / med / 1e6, err_phi=e_phi, err_grad=e_grad,
-                   kinds=kinds, stats=stats, graph_ms=gmed * 1e3)
+                   kinds=kinds, stats=stats, graph_ms=gmed * 1e3, graph_diff=graph_diff)
         del op
         return res
 
@@ -231,7 +288,7 @@ def main():
         pairs = stats["p2p_pairs"]
         avg_ms = kinds["p2p"][0] / kinds["p2p"][1]
         achieved = 30.0 * pairs / (avg_ms * 1e-3) / 1e12
-        roof = {"kernel": "k_p2p<float> (P2P)", "bound": "mfma", "achieved": achieved, "peak": PEAK_F32,
+        roof = {"kernel": "k_p2p<float> (P2P)", "bound": "valu", "achieved": achieved, "peak": PEAK_F32,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_F32,
                 "traffic": hbm_traffic("k_p2p") if default_workload else None,
                 "algorithmic": "15 lane-ops (30 flops) per source-target pair x p2p_pairs per launch, FP32 peak",
@@ -249,19 +306,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle as O  # the CPU baseline only
-
-        m = min(args.cpu_sample, args.n)
-        t0 = time.time()
-        f = O.Fmm(pts[:m], p=args.p, leaf_points=args.leaf, near_rule=near)
-        t_build = time.time() - t0
-        t0 = time.time()
-        f.evaluate(q[:m])
-        t_eval = time.time() - t0
-        cpu = {"value": m / t_eval / 1e6, "unit": "Mpts/s", "cores": 1, "kind": "port",
-               "sample": f"oracle serial f64 FMM evaluation (dense M2L tables) of the first {m} points of the "
-                         f"same workload, p={args.p}, <= {args.leaf} per leaf; {t_eval:.1f} s "
-                         f"(build {t_build:.1f} s excluded)"}
+        cpu = cpu_baseline(args, pts, q, sample, near, exact_phi, exact_grad)
         log(f"cpu baseline: {cpu}")
 
     line = {
@@ -295,6 +340,10 @@ def main():
         line["errors"]["plain_f64"] = {"phi": plain["err_phi"], "grad": plain["err_grad"]}
     line["p2p_pairs"] = stats["p2p_pairs"]
     line["m2l_pairs"] = stats["m2l_pairs"]
+    # interactions per second of the timed evaluation (BASELINE's definitions)
+    line["interactions"] = {"p2p_pairs_per_s": stats["p2p_pairs"] / (mixed["ms"] * 1e-3),
+                            "m2l_v_pairs_per_s": stats["m2l_pairs"] / (mixed["ms"] * 1e-3)}
+    line["graph_max_abs_diff_phi"] = mixed["graph_diff"]
     print(json.dumps(line), flush=True)
 
 
@@ -433,6 +482,11 @@ def nonuniform(args):
             step()
             torch.cuda.synchronize()
             times.append(time.perf_counter() - s)
+        # the errors of the timed (eager) steps' own output
+        phi = phi_dev.cpu().numpy()
+        grad = grad_dev.cpu().numpy().reshape(-1, 3)
+        e_phi = error_norm(phi[sample], exact_phi)
+        e_grad = error_norm(grad[sample], exact_grad)
         # per-kind device times in a pass of their own, outside the timed steps
         op.kind_timing(True)
         for _ in range(args.steps):
@@ -497,7 +551,7 @@ def helmholtz(args):
     ep, eg = fmm_amd.helmholtz_direct_sum(args.k, pts, q, pts[sample])
     ep, eg = ep / (4 * np.pi), eg / (4 * np.pi)
     log(f"direct sum at {len(sample)} targets in {time.time() - t0:.1f}s")
-    q_dev = torch.from_numpy(np.ascontiguousarray(q.view(np.float64))).to(dev)
+    q_dev = torch.from_numpy(np.ascontiguousarray(q, dtype=np.complex128)).to(dev)
 
     def run(precision):
         t0 = time.time()

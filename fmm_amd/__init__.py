@@ -29,6 +29,33 @@ KINDS = ("p2m", "m2m", "l2l", "m2l", "p2l", "l2p", "m2p", "p2p", "other", "m2l_c
 LISTS = {"near": 0, "v": 1, "w": 2, "x": 3}
 
 
+def _device_buffer(t, count: int, name: str, device: int, complex_=False):
+    """The data pointer of a caller's device tensor after checking it against what the
+    operator reads or writes: float64 (or complex128 for complex values), contiguous,
+    `count` elements, on the operator's GPU.  Raises FmmError(InvalidArgument) else."""
+    import torch
+
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise FmmError(9, f"{name}: expected a CUDA tensor")
+    want = torch.complex128 if complex_ else torch.float64
+    if t.dtype != want:
+        raise FmmError(9, f"{name}: dtype {t.dtype}, expected {want}")
+    if not t.is_contiguous():
+        raise FmmError(9, f"{name}: not contiguous")
+    if t.numel() != count:
+        raise FmmError(9, f"{name}: {t.numel()} elements, expected {count}")
+    if t.device.index != device:
+        raise FmmError(6, f"{name}: on cuda:{t.device.index}, the operator runs on cuda:{device}")
+    return C.c_void_p(t.data_ptr())
+
+
+def _torch_stream(device: int):
+    """The caller's current torch stream on `device` (a hipStream_t), for fmmh_stream_wait."""
+    import torch
+
+    return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
 def device_count() -> int:
     n = C.c_int32(0)
     _lib.check(_lib.lib().fmmh_device_count(C.byref(n)))
@@ -160,6 +187,7 @@ class Operator:
         self.gradients = gradients
         self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         self.m2l = M2L_METHODS[m2l] if isinstance(m2l, str) else int(m2l)
+        self.device = int(device)
         o = _lib.OperatorOptions(int(p), self.precision, int(bool(gradients)), int(device), self.m2l)
         h = C.c_void_p()
         _lib.check(_lib.lib().fmmh_operator_create(plan.handle, C.byref(o), C.byref(h)))
@@ -179,13 +207,17 @@ class Operator:
         return {name: getattr(s, name) for name, _ in _lib.Stats._fields_}
 
     def load_charges(self, charges):
+        """Charges in caller order: a numpy array, or a float64 CUDA tensor on the
+        operator's GPU (the operator's stream is ordered after torch's current stream)."""
         if hasattr(charges, "data_ptr") and getattr(charges, "is_cuda", False):
-            _lib.check(_lib.lib().fmmh_load_charges(self._h, C.c_void_p(charges.data_ptr()), 1))
+            ptr = _device_buffer(charges, self.nsources, "charges", self.device)
+            _lib.check(_lib.lib().fmmh_stream_wait(self._h, _torch_stream(self.device)))
+            _lib.check(_lib.lib().fmmh_load_charges(self._h, ptr, self.nsources, 1))
             return
         q = np.ascontiguousarray(charges, dtype=np.float64).reshape(-1)
         if len(q) != self.nsources:
             raise FmmError(9, f"expected {self.nsources} charges, got {len(q)}")
-        _lib.check(_lib.lib().fmmh_load_charges(self._h, _lib.ptr(q), 0))
+        _lib.check(_lib.lib().fmmh_load_charges(self._h, _lib.ptr(q), len(q), 0))
 
     def reset(self):
         _lib.check(_lib.lib().fmmh_reset(self._h))
@@ -203,13 +235,20 @@ class Operator:
     def read_output(self) -> Output:
         phi = np.zeros(self.ntargets)
         grad = np.zeros((self.ntargets, 3)) if self.gradients else None
-        _lib.check(_lib.lib().fmmh_read_output(self._h, _lib.ptr(phi), _lib.ptr(grad), 0))
+        _lib.check(_lib.lib().fmmh_read_output(self._h, _lib.ptr(phi), _lib.ptr(grad), self.ntargets, 0))
         return Output(phi, grad)
 
     def read_output_device(self, phi, grad=None):
-        """Output pass into device buffers (e.g. torch tensors on cuda:0)."""
-        g = C.c_void_p(grad.data_ptr()) if grad is not None else None
-        _lib.check(_lib.lib().fmmh_read_output(self._h, C.c_void_p(phi.data_ptr()), g, 1))
+        """Output pass into device buffers (float64 CUDA tensors on the operator's GPU:
+        phi of ntargets, grad of 3 ntargets); complete on return."""
+        pp = _device_buffer(phi, self.ntargets, "phi", self.device)
+        g = None
+        if grad is not None:
+            if not self.gradients:
+                raise FmmError(7, "read_output_device: the operator computes no gradients")
+            g = _device_buffer(grad, 3 * self.ntargets, "grad", self.device)
+        _lib.check(_lib.lib().fmmh_stream_wait(self._h, _torch_stream(self.device)))
+        _lib.check(_lib.lib().fmmh_read_output(self._h, pp, g, self.ntargets, 1))
 
     def sync(self):
         _lib.check(_lib.lib().fmmh_sync(self._h))
@@ -224,15 +263,25 @@ class Operator:
         if field is not None:
             f = np.ascontiguousarray(field, dtype=np.float64).reshape(-1, 3)
             gp = np.zeros((len(q), 3))
-        _lib.check(_lib.lib().fmmh_vjp(self._h, _lib.ptr(q), _lib.ptr(g), _lib.ptr(f), 0, _lib.ptr(gq),
+        if len(q) != self.nsources or len(g) != self.nsources:
+            raise FmmError(9, f"vjp: expected {self.nsources} charges and cotangents")
+        if f is not None and len(f) != self.nsources:
+            raise FmmError(9, f"vjp: expected a field of {self.nsources} x 3")
+        _lib.check(_lib.lib().fmmh_vjp(self._h, _lib.ptr(q), _lib.ptr(g), _lib.ptr(f), len(q), 0, _lib.ptr(gq),
                                        _lib.ptr(gp)))
         return gq, gp
 
     def vjp_device(self, charges, cotangent, field, grad_charges, grad_points=None):
-        """The same on device buffers (e.g. torch tensors on cuda:0, float64, contiguous)."""
-        vp = lambda t: None if t is None else C.c_void_p(t.data_ptr())
-        _lib.check(_lib.lib().fmmh_vjp(self._h, vp(charges), vp(cotangent), vp(field), 1, vp(grad_charges),
-                                       vp(grad_points)))
+        """The same on device buffers (float64 CUDA tensors on the operator's GPU, n and
+        3 n elements); the operator's stream waits for torch's current stream first, and
+        the outputs are complete on return."""
+        n, d = self.nsources, self.device
+        args = [_device_buffer(charges, n, "charges", d), _device_buffer(cotangent, n, "cotangent", d),
+                None if field is None else _device_buffer(field, 3 * n, "field", d)]
+        outs = [_device_buffer(grad_charges, n, "grad_charges", d),
+                None if grad_points is None else _device_buffer(grad_points, 3 * n, "grad_points", d)]
+        _lib.check(_lib.lib().fmmh_stream_wait(self._h, _torch_stream(d)))
+        _lib.check(_lib.lib().fmmh_vjp(self._h, *args, n, 1, *outs))
 
     def expansions(self, locals_: bool, level: int) -> np.ndarray:
         n = (self.p + 1) ** 2
@@ -441,6 +490,7 @@ class HelmholtzOperator:
         self.k = complex(k)
         self.gradients = gradients
         self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        self.device = int(device)
         o = _lib.HelmholtzOptions(int(p), self.k.real, self.k.imag, self.precision, int(bool(gradients)),
                                   int(device))
         h = C.c_void_p()
@@ -462,12 +512,14 @@ class HelmholtzOperator:
 
     def load_charges(self, charges):
         if hasattr(charges, "data_ptr") and getattr(charges, "is_cuda", False):
-            _lib.check(_lib.lib().fmmh_helmholtz_load_charges(self._h, C.c_void_p(charges.data_ptr()), 1))
+            ptr = _device_buffer(charges, self.nsources, "charges", self.device, complex_=True)
+            _lib.check(_lib.lib().fmmh_helmholtz_stream_wait(self._h, _torch_stream(self.device)))
+            _lib.check(_lib.lib().fmmh_helmholtz_load_charges(self._h, ptr, self.nsources, 1))
             return
         q = np.ascontiguousarray(np.asarray(charges, dtype=np.complex128).reshape(-1))
         if len(q) != self.nsources:
             raise FmmError(9, f"expected {self.nsources} charges, got {len(q)}")
-        _lib.check(_lib.lib().fmmh_helmholtz_load_charges(self._h, _lib.ptr(q.view(np.float64)), 0))
+        _lib.check(_lib.lib().fmmh_helmholtz_load_charges(self._h, _lib.ptr(q.view(np.float64)), len(q), 0))
 
     def reset(self):
         _lib.check(_lib.lib().fmmh_helmholtz_reset(self._h))
@@ -478,7 +530,7 @@ class HelmholtzOperator:
     def read_output(self) -> Output:
         phi = np.zeros(2 * self.ntargets)
         grad = np.zeros(6 * self.ntargets) if self.gradients else None
-        _lib.check(_lib.lib().fmmh_helmholtz_read_output(self._h, _lib.ptr(phi), _lib.ptr(grad), 0))
+        _lib.check(_lib.lib().fmmh_helmholtz_read_output(self._h, _lib.ptr(phi), _lib.ptr(grad), self.ntargets, 0))
         return Output(phi.view(np.complex128),
                       grad.view(np.complex128).reshape(-1, 3) if self.gradients else None)
 

@@ -41,7 +41,7 @@ EXPORTS = [
     "fmmh_helmholtz_load_charges", "fmmh_helmholtz_reset", "fmmh_helmholtz_evaluate",
     "fmmh_helmholtz_read_output", "fmmh_helmholtz_read_expansions", "fmmh_helmholtz_kind_timing",
     "fmmh_helmholtz_kind_times", "fmmh_helmholtz_table", "fmmh_helmholtz_direct_sum",
-    "fmmh_plane_wave_quadrature", "fmmh_plane_wave_m2l",
+    "fmmh_plane_wave_quadrature", "fmmh_plane_wave_m2l", "fmmh_stream_wait", "fmmh_helmholtz_stream_wait",
 ]
 
 
@@ -126,19 +126,20 @@ def lib():
     L.fmmh_operator_free.argtypes = [vp]
     L.fmmh_operator_free.restype = None
     L.fmmh_operator_stats.argtypes = [vp, C.POINTER(Stats)]
-    L.fmmh_load_charges.argtypes = [vp, vp, i32]
+    L.fmmh_load_charges.argtypes = [vp, vp, i64, i32]
     L.fmmh_reset.argtypes = [vp]
     for name in ("p2m", "m2m", "m2l", "p2l", "l2l", "l2p", "m2p", "p2p"):
         getattr(L, "fmmh_" + name).argtypes = [vp, i32]
     L.fmmh_evaluate.argtypes = [vp]
     L.fmmh_evaluate_graph.argtypes = [vp]
     L.fmmh_direct_sum.argtypes = [i32, vp, vp, i64, vp, i64, vp, vp, i32]
-    L.fmmh_read_output.argtypes = [vp, vp, vp, i32]
+    L.fmmh_read_output.argtypes = [vp, vp, vp, i64, i32]
+    L.fmmh_stream_wait.argtypes = [vp, vp]
     L.fmmh_sync.argtypes = [vp]
     L.fmmh_kind_timing.argtypes = [vp, i32]
     L.fmmh_read_expansions.argtypes = [vp, i32, i32, vp]
     L.fmmh_kind_times.argtypes = [vp, vp, vp]
-    L.fmmh_vjp.argtypes = [vp, vp, vp, vp, i32, vp, vp]
+    L.fmmh_vjp.argtypes = [vp, vp, vp, vp, i64, i32, vp, vp]
     L.fmmh_forest_build.argtypes = [vp, vp, i32, C.POINTER(PlanOptions), C.POINTER(vp)]
     L.fmmh_plane_wave_quadrature.argtypes = [i32, dbl, vp, vp, vp, vp, vp]
     L.fmmh_plane_wave_m2l.argtypes = [i32, dbl, i32, vp, vp]
@@ -147,10 +148,11 @@ def lib():
     L.fmmh_helmholtz_free.argtypes = [vp]
     L.fmmh_helmholtz_free.restype = None
     L.fmmh_helmholtz_stats.argtypes = [vp, C.POINTER(Stats)]
-    L.fmmh_helmholtz_load_charges.argtypes = [vp, vp, i32]
+    L.fmmh_helmholtz_load_charges.argtypes = [vp, vp, i64, i32]
     L.fmmh_helmholtz_reset.argtypes = [vp]
     L.fmmh_helmholtz_evaluate.argtypes = [vp]
-    L.fmmh_helmholtz_read_output.argtypes = [vp, vp, vp, i32]
+    L.fmmh_helmholtz_read_output.argtypes = [vp, vp, vp, i64, i32]
+    L.fmmh_helmholtz_stream_wait.argtypes = [vp, vp]
     L.fmmh_helmholtz_read_expansions.argtypes = [vp, i32, i32, vp]
     L.fmmh_helmholtz_kind_timing.argtypes = [vp, i32]
     L.fmmh_helmholtz_kind_times.argtypes = [vp, vp, vp]

@@ -63,7 +63,7 @@ extern "C" {
 
 const char *fmmh_last_error(void) { return g_last_error.c_str(); }
 
-int32_t fmmh_abi_version(void) { return 4; }
+int32_t fmmh_abi_version(void) { return 5; }
 
 int32_t fmmh_plan_build(const double *sources, int64_t nsources, const double *targets, int64_t ntargets,
                         const fmmh_plan_options *options, fmmh_plan **out) {
@@ -275,9 +275,10 @@ int32_t fmmh_operator_stats(const fmmh_operator *op, fmmh_stats *out) {
   });
 }
 
-int32_t fmmh_load_charges(fmmh_operator *op, const double *charges, int32_t on_device) {
+int32_t fmmh_load_charges(fmmh_operator *op, const double *charges, int64_t count, int32_t on_device) {
   return guard([&] {
     need(op && charges, "fmmh_load_charges: NULL argument");
+    need(count == op->op->nsources(), "fmmh_load_charges: count differs from the plan's source count");
     op->op->load_charges(charges, on_device != 0);
   });
 }
@@ -321,17 +322,20 @@ int32_t fmmh_evaluate_graph(fmmh_operator *op) {
   });
 }
 
-int32_t fmmh_read_output(fmmh_operator *op, double *phi, double *grad, int32_t on_device) {
+int32_t fmmh_read_output(fmmh_operator *op, double *phi, double *grad, int64_t count, int32_t on_device) {
   return guard([&] {
     need(op && phi, "fmmh_read_output: NULL argument");
+    need(count == op->op->ntargets(), "fmmh_read_output: count differs from the plan's target count");
     op->op->read_output(phi, grad, on_device != 0);
   });
 }
 
 int32_t fmmh_vjp(fmmh_operator *op, const double *charges, const double *cotangent, const double *field,
-                 int32_t on_device, double *grad_charges, double *grad_points) {
+                 int64_t count, int32_t on_device, double *grad_charges, double *grad_points) {
   return guard([&] {
     need(op && charges && cotangent && grad_charges, "fmmh_vjp: NULL argument");
+    need(count == op->op->nsources() && count == op->op->ntargets(),
+         "fmmh_vjp: count differs from the plan's point count");
     need(grad_points == nullptr || field != nullptr, "fmmh_vjp: grad_points needs the forward field");
     op->op->vjp(charges, cotangent, field, on_device != 0, grad_charges, grad_points);
   });
@@ -343,6 +347,13 @@ int32_t fmmh_direct_sum(int32_t device, const double *sources, const double *cha
     need(nsources == 0 || (sources && charges), "fmmh_direct_sum: NULL sources");
     need(ntargets == 0 || (targets && phi), "fmmh_direct_sum: NULL targets or output");
     fmmh::direct_sum(device, sources, charges, nsources, targets, ntargets, phi, grad, on_device != 0);
+  });
+}
+
+int32_t fmmh_stream_wait(fmmh_operator *op, void *stream) {
+  return guard([&] {
+    need(op != nullptr, "fmmh_stream_wait: NULL operator");
+    op->op->wait_for(static_cast<hipStream_t>(stream));
   });
 }
 
@@ -417,10 +428,18 @@ int32_t fmmh_helmholtz_stats(const fmmh_helmholtz *op, fmmh_stats *out) {
   });
 }
 
-int32_t fmmh_helmholtz_load_charges(fmmh_helmholtz *op, const double *charges, int32_t on_device) {
+int32_t fmmh_helmholtz_load_charges(fmmh_helmholtz *op, const double *charges, int64_t count, int32_t on_device) {
   return guard([&] {
     need(op && charges, "fmmh_helmholtz_load_charges: NULL argument");
+    need(count == op->op->nsources(), "fmmh_helmholtz_load_charges: count differs from the plan's source count");
     op->op->load_charges(charges, on_device != 0);
+  });
+}
+
+int32_t fmmh_helmholtz_stream_wait(fmmh_helmholtz *op, void *stream) {
+  return guard([&] {
+    need(op != nullptr, "fmmh_helmholtz_stream_wait: NULL operator");
+    op->op->wait_for(static_cast<hipStream_t>(stream));
   });
 }
 
@@ -438,9 +457,11 @@ int32_t fmmh_helmholtz_evaluate(fmmh_helmholtz *op) {
   });
 }
 
-int32_t fmmh_helmholtz_read_output(fmmh_helmholtz *op, double *phi, double *grad, int32_t on_device) {
+int32_t fmmh_helmholtz_read_output(fmmh_helmholtz *op, double *phi, double *grad, int64_t count,
+                                   int32_t on_device) {
   return guard([&] {
     need(op && phi, "fmmh_helmholtz_read_output: NULL argument");
+    need(count == op->op->ntargets(), "fmmh_helmholtz_read_output: count differs from the plan's target count");
     op->op->read_output(phi, grad, on_device != 0);
   });
 }

@@ -129,15 +129,16 @@ __device__ __forceinline__ void wait_vm() {
 //
 // Pipeline as in the bf16x3 kernel below: table chunks global -> LDS by DMA into a ring
 // of 3 buffers two chunks ahead, source coefficients to registers one chunk ahead, one
-// barrier per chunk.
+// barrier per chunk.  X and Y may be one array (L2L reads the parents' locals and adds
+// into the children's), so neither is declared restrict.
 template <int MT, int WM>
 __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, uint32_t ntasks, uint32_t xchunk,
                                                       const uint32_t *__restrict__ task_group_off,
                                                       const uint16_t *__restrict__ group_table,
                                                       const uint32_t *__restrict__ group_src,
                                                       const uint32_t *__restrict__ col_out,
-                                                      const double *__restrict__ tsw, const double *__restrict__ X,
-                                                      double *__restrict__ Y, uint32_t first_slot,
+                                                      const double *__restrict__ tsw, const double *X,
+                                                      double *Y, uint32_t first_slot,
                                                       double *__restrict__ S) {
   constexpr int CHUNK2 = MT * KC * 32;  // 16-byte units per chunk
   constexpr int SEGS = CHUNK2 / 64;     // 1 KB segments (MT KC / 2 of them)
@@ -246,15 +247,16 @@ __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, uint3
 // expansions as B) over all M-tiles, with no LDS staging and no barrier per chunk, so a
 // task is 11 chunks long (p = 12) instead of 8 x 11.  The eight waves' sums then meet in
 // LDS in a fixed tree, ((w0 + w4) + (w2 + w6)) + ((w1 + w5) + (w3 + w7)), and wave 0
-// adds the result to the parents' rows.
+// adds the result to the parents' rows.  X and Y are one array (children read, parents
+// written), so neither is declared restrict.
 template <int MT>
 __global__ __launch_bounds__(512) void k_m2m_split(int nc, int nchunks, uint32_t ntasks,
                                                    const uint32_t *__restrict__ task_group_off,
                                                    const uint16_t *__restrict__ group_table,
                                                    const uint32_t *__restrict__ group_src,
                                                    const uint32_t *__restrict__ col_out,
-                                                   const double *__restrict__ tsw, const double *__restrict__ X,
-                                                   double *__restrict__ Y) {
+                                                   const double *__restrict__ tsw, const double *X,
+                                                   double *Y) {
   __shared__ d4 red[4][MT][64];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int col = lane & 15, g = lane >> 4;
@@ -357,8 +359,8 @@ __global__ __launch_bounds__(512) void k_m2m_rows(int nc, int nchunks, uint32_t 
                                                   const uint16_t *__restrict__ group_table,
                                                   const uint32_t *__restrict__ group_src,
                                                   const uint32_t *__restrict__ col_out,
-                                                  const double *__restrict__ tsw, const double *__restrict__ X,
-                                                  double *__restrict__ Y) {
+                                                  const double *__restrict__ tsw, const double *X,
+                                                  double *Y) {
   __shared__ d4 red[4][64];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int col = lane & 15, g = lane >> 4;
@@ -375,9 +377,9 @@ __global__ __launch_bounds__(512) void k_m2m_rows(int nc, int nchunks, uint32_t 
     const double *a = tsw + size_t(table) * tsize;
     const uint32_t src = group_src[size_t(gi) * 16 + col];
     const bool has = src != 0xFFFFFFFFu;
-    // the run of k = g KS .. g KS + KS - 1 (the caller's X holds 15 readable doubles past
-    // its last expansion; values past nc are masked)
-    const double *xb = X + size_t(has ? src : 0) * nc + g * KS;
+    // the run of k = g KS .. g KS + KS - 1; k past nc is clamped to the row's last
+    // element (in bounds) and masked
+    const double *xr = X + size_t(has ? src : 0) * nc;
     // the chunks in NB batches of CH (one batch up to MT = 11; two above, whose operands
     // would not fit the registers at once)
     constexpr int NB = MT <= 11 ? 1 : 2, CH = (MT + NB - 1) / NB;
@@ -391,7 +393,7 @@ __global__ __launch_bounds__(512) void k_m2m_rows(int nc, int nchunks, uint32_t 
 #pragma unroll
           for (int kk = 0; kk < KC; ++kk) {
             const int k = c * KC + kk + g * KS;
-            const double v = xb[c * KC + kk];
+            const double v = xr[k < nc ? k : nc - 1];
             bv[cc * KC + kk] = (has && k < nc) ? v : 0.0;
             av[cc * KC + kk] = a[((size_t(c) * MT + mt) * KC + kk) * 64 + lane];
           }

@@ -61,6 +61,7 @@ HelmholtzOperator::HelmholtzOperator(const Plan &plan, const HelmholtzOptions &o
   require(opt_.device >= 0 && opt_.device < ndev, "device ordinal out of range", kNoDevice);
   hip_check(hipSetDevice(opt_.device), "hipSetDevice");
   hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipEventCreateWithFlags(&order_event_, hipEventDisableTiming), "hipEventCreate");
 
   const int p = opt_.p;
   nlevels_ = plan.nlevels;
@@ -479,7 +480,13 @@ HelmholtzOperator::HelmholtzOperator(const Plan &plan, const HelmholtzOptions &o
 
 HelmholtzOperator::~HelmholtzOperator() {
   for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+  if (order_event_) (void)hipEventDestroy(order_event_);
   if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void HelmholtzOperator::wait_for(hipStream_t producer) {
+  hip_check(hipEventRecord(order_event_, producer), "hipEventRecord");
+  hip_check(hipStreamWaitEvent(stream_, order_event_, 0), "hipStreamWaitEvent");
 }
 
 size_t HelmholtzOperator::device_bytes() const {

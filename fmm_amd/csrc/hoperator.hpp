@@ -40,12 +40,16 @@ class HelmholtzOperator {
   // phi[2 nt], grad[6 nt] (x re, x im, y re, ...) in caller order; the one sync
   void read_output(double *phi, double *grad, bool on_device);
   void sync();
+  // order this operator's stream after the work queued on `producer` so far
+  void wait_for(hipStream_t producer);
   // the stored coefficients of one level, box-major, real plane then imaginary plane
   void read_expansions(bool locals, int level, double *out);
   void set_timing(bool enable);
   void kind_times(double *ms, int64_t *launches);
 
   int nlevels() const { return nlevels_; }
+  int64_t nsources() const { return ns_; }
+  int64_t ntargets() const { return nt_; }
   uint64_t p2p_pairs() const { return p2p_pairs_; }
   uint64_t m2l_pairs() const { return m2l_pairs_; }
   size_t device_bytes() const;
@@ -60,6 +64,7 @@ class HelmholtzOperator {
   int nlevels_ = 0, nc_ = 0;
   bool m2m_complex_ = true;  // M2M / L2L tables have an imaginary part (not for real k)
   hipStream_t stream_ = nullptr;
+  hipEvent_t order_event_ = nullptr;  // wait_for: a caller's stream before this one
   int64_t ns_ = 0, nt_ = 0;
   uint32_t nleaves_ = 0;
   size_t nbox_total_ = 0;

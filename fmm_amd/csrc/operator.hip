@@ -32,6 +32,7 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
   require(opt_.device >= 0 && opt_.device < ndev, "device ordinal out of range", kNoDevice);
   hip_check(hipSetDevice(opt_.device), "hipSetDevice");
   hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipEventCreateWithFlags(&order_event_, hipEventDisableTiming), "hipEventCreate");
   hip_check(dev::upload_constants(), "upload constants");
 
   nlevels_ = plan.nlevels;
@@ -229,7 +230,13 @@ DeviceOperator::~DeviceOperator() {
   if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
   if (graph_) (void)hipGraphDestroy(graph_);
   for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+  if (order_event_) (void)hipEventDestroy(order_event_);
   if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void DeviceOperator::wait_for(hipStream_t producer) {
+  hip_check(hipEventRecord(order_event_, producer), "hipEventRecord");
+  hip_check(hipStreamWaitEvent(stream_, order_event_, 0), "hipStreamWaitEvent");
 }
 
 void DeviceOperator::set_timing(bool enable) {

@@ -134,6 +134,9 @@ class DeviceOperator {
   void vjp(const double *q, const double *g, const double *field, bool on_device, double *grad_q,
            double *grad_pos);
   void sync();
+  // order this operator's stream after the work queued on `producer` so far (a caller's
+  // stream that writes the charges or cotangents this operator reads from device memory)
+  void wait_for(hipStream_t producer);
   // copy the multipoles (locals = false) or locals of one level to host, box-major
   void read_expansions(bool locals, int level, double *out);
   // per-kind device times (HIP events on this operator's stream)
@@ -141,6 +144,8 @@ class DeviceOperator {
   void kind_times(double *ms, int64_t *launches);
 
   int nlevels() const { return nlevels_; }
+  int64_t nsources() const { return ns_; }
+  int64_t ntargets() const { return nt_; }
   hipStream_t stream() const { return stream_; }
   const OperatorOptions &options() const { return opt_; }
   // statistics for reports and the roofline
@@ -166,6 +171,7 @@ class DeviceOperator {
   OperatorOptions opt_;
   int nlevels_ = 0, nc_ = 0;
   hipStream_t stream_ = nullptr;
+  hipEvent_t order_event_ = nullptr;  // wait_for: a caller's stream before this one
   int64_t ns_ = 0, nt_ = 0;
   uint32_t nleaves_ = 0;
   size_t nbox_total_ = 0;

@@ -145,8 +145,10 @@ int32_t fmmh_device_count(int32_t *count);
 int32_t fmmh_operator_create(const fmmh_plan *plan, const fmmh_operator_options *options, fmmh_operator **out);
 void fmmh_operator_free(fmmh_operator *op);
 int32_t fmmh_operator_stats(const fmmh_operator *op, fmmh_stats *out);
-/* Charges of the sources in caller order, on the host (on_device = 0) or on the GPU. */
-int32_t fmmh_load_charges(fmmh_operator *op, const double *charges, int32_t on_device);
+/* Charges of the sources in caller order, on the host (on_device = 0) or on the GPU;
+ * count must equal the plan's source count (begin_evaluation asserts charges.len(),
+ * fmm-exec/src/device.rs:4386-4390), else FMMH_INVALID_ARGUMENT and nothing is read. */
+int32_t fmmh_load_charges(fmmh_operator *op, const double *charges, int64_t count, int32_t on_device);
 /* HostData::Reset: zero multipoles, locals and target output. */
 int32_t fmmh_reset(fmmh_operator *op);
 /* FmmOperator level calls (fmm-plan/src/operator.rs:434-485). */
@@ -166,8 +168,9 @@ int32_t fmmh_evaluate(fmmh_operator *op);
  * fmm-exec/src/device.rs:5212-5446) by one graph launch. */
 int32_t fmmh_evaluate_graph(fmmh_operator *op);
 /* Output pass and download: phi[nt], grad[3 nt] (may be NULL) in caller order, to the
- * host (on_device = 0) or to device buffers.  Synchronises. */
-int32_t fmmh_read_output(fmmh_operator *op, double *phi, double *grad, int32_t on_device);
+ * host (on_device = 0) or to device buffers; count must equal nt (read_output,
+ * fmm-exec/src/device.rs:4687).  Synchronises. */
+int32_t fmmh_read_output(fmmh_operator *op, double *phi, double *grad, int64_t count, int32_t on_device);
 /* Adjoint (vector-Jacobian product) of the potentials, for a plan built with one array
  * for sources and targets.  Inputs in caller order: charges q[n], the cotangent
  * g[n] = dL/dphi (e.g. 2 phi / n for L = sum phi^2 / n) and the forward gradient
@@ -178,7 +181,14 @@ int32_t fmmh_read_output(fmmh_operator *op, double *phi, double *grad, int32_t o
  * the reference's Rust API (its north-star addition: the adjoint of Fmm::evaluate,
  * fmm-exec/src/fmm.rs:3691).  Synchronises. */
 int32_t fmmh_vjp(fmmh_operator *op, const double *charges, const double *cotangent, const double *field,
-                 int32_t on_device, double *grad_charges, double *grad_points);
+                 int64_t count, int32_t on_device, double *grad_charges, double *grad_points);
+/* Order the operator's stream after the work queued on `stream` (a hipStream_t; NULL is
+ * the null stream) so far: call it before passing device pointers that another stream
+ * writes (charges, cotangents).  Outputs are complete when fmmh_read_output or fmmh_vjp
+ * returns.  No counterpart in the reference (its Device owns every buffer it reads,
+ * fmm-kernels/src/error.rs:56-57); it is the ordering half of accepting a caller's
+ * device buffers. */
+int32_t fmmh_stream_wait(fmmh_operator *op, void *stream);
 int32_t fmmh_sync(fmmh_operator *op);
 /* The compensated f64 direct sum of nd_fmm_ref::p2p::direct_sum (fmm-ref/src/p2p.rs:147-205):
  * per target phi = sum q / r and grad = -sum q d / r^3 (d = target - source, no 1/(4 pi)),
@@ -232,12 +242,17 @@ int32_t fmmh_helmholtz_degree(double k_re, double radius, double eps);
 int32_t fmmh_helmholtz_create(const fmmh_plan *plan, const fmmh_helmholtz_options *options, fmmh_helmholtz **out);
 void fmmh_helmholtz_free(fmmh_helmholtz *op);
 int32_t fmmh_helmholtz_stats(const fmmh_helmholtz *op, fmmh_stats *out);
-int32_t fmmh_helmholtz_load_charges(fmmh_helmholtz *op, const double *charges, int32_t on_device);
+/* count: the number of complex charges (the plan's source count). */
+int32_t fmmh_helmholtz_load_charges(fmmh_helmholtz *op, const double *charges, int64_t count, int32_t on_device);
 int32_t fmmh_helmholtz_reset(fmmh_helmholtz *op);
 /* P2M, M2M, M2L, P2L, L2L, L2P, M2P, P2P over every level (the order of
  * fmm-exec/src/helmholtz/fmm.rs's evaluation). */
 int32_t fmmh_helmholtz_evaluate(fmmh_helmholtz *op);
-int32_t fmmh_helmholtz_read_output(fmmh_helmholtz *op, double *phi, double *grad, int32_t on_device);
+/* count: the number of targets. */
+int32_t fmmh_helmholtz_read_output(fmmh_helmholtz *op, double *phi, double *grad, int64_t count,
+                                   int32_t on_device);
+/* As fmmh_stream_wait, for a Helmholtz operator. */
+int32_t fmmh_helmholtz_stream_wait(fmmh_helmholtz *op, void *stream);
 /* Stored multipoles or locals of one level: per box the real plane then the imaginary
  * plane of (p+1)^2 values. */
 int32_t fmmh_helmholtz_read_expansions(fmmh_helmholtz *op, int32_t locals, int32_t level, double *out);

@@ -96,6 +96,12 @@ orc_fmm *orc_fmm_new(const double *src, int64_t ns, const double *tgt, int64_t n
 void orc_fmm_free(orc_fmm *f);
 /* Evaluate with one charge vector; output in caller order: phi[nt], grad[3 nt]. */
 void orc_fmm_evaluate(orc_fmm *f, const double *q, double *phi, double *grad);
+/* The CPU baseline (cpu_path.c): the reference's multi-threaded host evaluation restated
+ * (threads over targets, SIMD P2P, batched M2L column kernel) on the same tree; threads
+ * <= 0 keeps OpenMP's default.  orc_cpu_isa: 2 = AVX-512F kernels, 0 = scalar loops. */
+void orc_fmm_evaluate_cpu(orc_fmm *f, const double *q, double *phi, double *grad, int threads);
+int orc_cpu_isa(void);
+int orc_cpu_threads(void);
 
 /* Inspection for the parity tests. */
 /* Multipoles (locals = 0) or locals of `level` after the last evaluation, box-major. */

@@ -84,6 +84,9 @@ def lib():
         L.orc_fmm_new.restype = C.c_void_p
         L.orc_fmm_free.argtypes = [C.c_void_p]
         L.orc_fmm_evaluate.argtypes = [C.c_void_p, _f64p, _f64p, pd]
+        L.orc_fmm_evaluate_cpu.argtypes = [C.c_void_p, _f64p, _f64p, pd, C.c_int]
+        L.orc_cpu_isa.restype = C.c_int
+        L.orc_cpu_threads.restype = C.c_int
         L.orc_fmm_nlevels.argtypes = [C.c_void_p]
         L.orc_fmm_nlevels.restype = C.c_int
         L.orc_fmm_domain.argtypes = [C.c_void_p, _f64p]
@@ -185,6 +188,15 @@ def bench_problem(n: int, sampled: int, charges: str = "uniform"):
     q = rng.charges(n) if charges == "uniform" else rng.normals(n)
     sample = rng.sample(n, sampled)
     return pts, q, sample
+
+
+def cpu_isa() -> str:
+    """The kernels the CPU baseline runs here: "avx512f" or "scalar"."""
+    return "avx512f" if lib().orc_cpu_isa() == 2 else "scalar"
+
+
+def cpu_threads() -> int:
+    return int(lib().orc_cpu_threads())
 
 
 def direct_sum(src, q, tgt, gradients=True):
@@ -471,6 +483,16 @@ class Fmm:
         phi = np.zeros(len(self.targets))
         grad = np.zeros((len(self.targets), 3)) if self.gradients else None
         lib().orc_fmm_evaluate(self._h, q, phi, _ptr(grad))
+        return phi, grad
+
+    def evaluate_cpu_path(self, charges, threads=0):
+        """The same FMM by the CPU baseline (cpu_path.c): the reference's multi-threaded
+        host evaluation restated (threads over targets, AVX-512 P2P, batched M2L column
+        kernel with fused steps); threads <= 0 uses OpenMP's default."""
+        q = np.ascontiguousarray(charges, np.float64)
+        phi = np.zeros(len(self.targets))
+        grad = np.zeros((len(self.targets), 3)) if self.gradients else None
+        lib().orc_fmm_evaluate_cpu(self._h, q, phi, _ptr(grad), int(threads))
         return phi, grad
 
     def evaluate_helmholtz(self, k, charges):

@@ -16,6 +16,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include "oracle.h"
+#include "tree_internal.h"
 
 #define DEEPEST 16
 #define LEVEL_BITS 15
@@ -157,7 +158,7 @@ static void linearize(kvec *a) {
   a->n = out;
 }
 
-static int64_t find(const uint64_t *v, int64_t n, uint64_t key) {
+int64_t orc_find_key(const uint64_t *v, int64_t n, uint64_t key) {
   int64_t lo = 0, hi = n;
   while (lo < hi) {
     int64_t mid = (lo + hi) / 2;
@@ -234,39 +235,10 @@ static void balance(kvec *leaves) {
 
 /* ---- the FMM ------------------------------------------------------------------------ */
 
-typedef struct {
-  int64_t *off, *ent, *aux; /* CSR rows; aux = offset index for V */
-} csr;
-
-struct orc_fmm {
-  int p, gradients, nlevels;
-  double box[6], lower[3], side;
-  int64_t ns, nt;
-  /* box index per level */
-  int64_t *nbox;
-  uint64_t **keys;
-  int64_t **box_leaf;  /* leaf index or -1 */
-  int64_t **first_child; /* box index of child 0 on level+1 or -1 */
-  /* leaves by (level, key) */
-  int64_t nleaves;
-  uint64_t *leaf_key;
-  int64_t *leaf_box, *leaf_off; /* leaf_off[nlevels+1] */
-  /* lists */
-  csr *v, *x;  /* per level, rows = boxes of the level */
-  csr *u, *w;  /* per level, rows = leaves of the level (near = U + self) */
-  /* points */
-  int64_t *src_start, *tgt_start; /* per leaf, CSR */
-  int64_t *src_order, *tgt_order; /* leaf order -> caller index */
-  int64_t *src_leaf, *tgt_leaf;   /* caller index -> leaf */
-  double *us, *ut;                /* leaf-scaled coordinates, leaf order */
-  double *m2m, *l2l, *m2l;        /* tables */
-  double **mult, **loc;           /* expansions of the last evaluation, per level */
-};
-
 static int64_t box_index(const orc_fmm *f, uint64_t key) {
   int l = key_level(key);
   if (l >= f->nlevels) return -1;
-  return find(f->keys[l], f->nbox[l], key);
+  return orc_find_key(f->keys[l], f->nbox[l], key);
 }
 
 static int is_leaf_key(const orc_fmm *f, uint64_t key) {
@@ -566,7 +538,7 @@ orc_fmm *orc_fmm_new(const double *src, int64_t ns, const double *tgt, int64_t n
   }
   for (int64_t i = 0; i < f->nleaves; ++i) {
     int l = key_level(f->leaf_key[i]);
-    int64_t b = find(f->keys[l], f->nbox[l], f->leaf_key[i]);
+    int64_t b = orc_find_key(f->keys[l], f->nbox[l], f->leaf_key[i]);
     f->leaf_box[i] = b;
     f->box_leaf[l][b] = i;
     f->leaf_off[l + 1]++;
@@ -576,7 +548,7 @@ orc_fmm *orc_fmm_new(const double *src, int64_t ns, const double *tgt, int64_t n
     for (int64_t b = 0; b < f->nbox[l]; ++b)
       f->first_child[l][b] =
           (f->box_leaf[l][b] < 0 && l + 1 < f->nlevels)
-              ? find(f->keys[l + 1], f->nbox[l + 1], child_key(f->keys[l][b], 0))
+              ? orc_find_key(f->keys[l + 1], f->nbox[l + 1], child_key(f->keys[l][b], 0))
               : -1;
   /* lists */
   f->v = calloc(f->nlevels, sizeof(csr));
@@ -598,7 +570,7 @@ orc_fmm *orc_fmm_new(const double *src, int64_t ns, const double *tgt, int64_t n
       for (int64_t e = 0; e < cntv; ++e) {
         orc_decode(vr[b].v[e], ds);
         tmp[2 * e] = orc_offset_index((int)(dt[1] - ds[1]), (int)(dt[2] - ds[2]), (int)(dt[3] - ds[3]));
-        tmp[2 * e + 1] = find(f->keys[l], nb, vr[b].v[e]);
+        tmp[2 * e + 1] = orc_find_key(f->keys[l], nb, vr[b].v[e]);
       }
       /* insertion sort by offset index */
       for (int64_t i = 1; i < cntv; ++i)
@@ -720,7 +692,7 @@ void orc_fmm_free(orc_fmm *f) {
 }
 
 /* ĉ(s|t), r̂(s|t): CONVENTIONS §3.13 "Relative frames", exact from integer centres. */
-static void relative_frame(uint64_t s, uint64_t t, double c[3], double *r) {
+void orc_relative_frame(uint64_t s, uint64_t t, double c[3], double *r) {
   int64_t ds[4], dt[4];
   orc_decode(s, ds);
   orc_decode(t, dt);
@@ -781,7 +753,7 @@ void orc_fmm_evaluate(orc_fmm *f, const double *q_caller, double *phi, double *g
   for (int l = 1; l <= L; ++l) {
     for (int64_t b = 0; b < f->nbox[l]; ++b) {
       uint64_t k = f->keys[l][b];
-      int64_t pb = find(f->keys[l - 1], f->nbox[l - 1], orc_parent(k));
+      int64_t pb = orc_find_key(f->keys[l - 1], f->nbox[l - 1], orc_parent(k));
       int64_t d[4];
       orc_decode(k, d);
       int o = (int)(((d[1] & 1) << 2) | ((d[2] & 1) << 1) | (d[3] & 1));
@@ -795,7 +767,7 @@ void orc_fmm_evaluate(orc_fmm *f, const double *q_caller, double *phi, double *g
       for (int64_t e = f->x[l].off[b]; e < f->x[l].off[b + 1]; ++e) {
         int64_t sl = f->x[l].ent[e], s0 = f->src_start[sl], s1 = f->src_start[sl + 1];
         double c[3], r;
-        relative_frame(f->keys[l][b], f->leaf_key[sl], c, &r); /* ĉ(t|s), r̂(t|s) */
+        orc_relative_frame(f->keys[l][b], f->leaf_key[sl], c, &r); /* ĉ(t|s), r̂(t|s) */
         orc_p2l(p, c, r, f->us + 3 * s0, q + s0, s1 - s0, loc[l] + (size_t)b * n);
       }
   }
@@ -812,7 +784,7 @@ void orc_fmm_evaluate(orc_fmm *f, const double *q_caller, double *phi, double *g
         int64_t sb = f->w[l].ent[e];
         int sl = (int)f->w[l].aux[e];
         double c[3], rr;
-        relative_frame(f->keys[sl][sb], f->leaf_key[lf], c, &rr); /* ĉ(s|t), r̂(s|t) */
+        orc_relative_frame(f->keys[sl][sb], f->leaf_key[lf], c, &rr); /* ĉ(s|t), r̂(s|t) */
         orc_m2p(p, c, rr, mult[sl] + (size_t)sb * n, f->ut + 3 * t0, t1 - t0, ph + t0,
                 gp ? gp + 3 * t0 : NULL);
       }
@@ -826,7 +798,7 @@ void orc_fmm_evaluate(orc_fmm *f, const double *q_caller, double *phi, double *g
           memcpy(y, f->us + 3 * s0, sizeof(double) * 3 * (s1 - s0));
         } else {
           double c[3], rr;
-          relative_frame(f->leaf_key[sl], f->leaf_key[lf], c, &rr); /* ĉ(s|t), r̂(s|t) */
+          orc_relative_frame(f->leaf_key[sl], f->leaf_key[lf], c, &rr); /* ĉ(s|t), r̂(s|t) */
           for (int64_t j = 0; j < s1 - s0; ++j)
             for (int k = 0; k < 3; ++k) y[3 * j + k] = c[k] + rr * f->us[3 * (s0 + j) + k];
         }
@@ -903,7 +875,7 @@ void orc_hfmm_evaluate(orc_fmm *f, double kre, double kim, const double *q_calle
         int64_t d[4];
         orc_decode(f->keys[l][b], d);
         if ((int)(((d[1] & 1) << 2) | ((d[2] & 1) << 1) | (d[3] & 1)) != o) continue;
-        int64_t pb = find(f->keys[l - 1], f->nbox[l - 1], orc_parent(f->keys[l][b]));
+        int64_t pb = orc_find_key(f->keys[l - 1], f->nbox[l - 1], orc_parent(f->keys[l][b]));
         orc_h_apply(p, T, loc[l - 1] + pb * w, loc[l] + b * w);
       }
     }
@@ -923,7 +895,7 @@ void orc_hfmm_evaluate(orc_fmm *f, double kre, double kim, const double *q_calle
       for (int64_t e = f->x[l].off[b]; e < f->x[l].off[b + 1]; ++e) {
         int64_t sl = f->x[l].ent[e], s0 = f->src_start[sl], s1 = f->src_start[sl + 1];
         double c[3], r;
-        relative_frame(f->keys[l][b], f->leaf_key[sl], c, &r); /* ĉ(t|s), r̂(t|s) */
+        orc_relative_frame(f->keys[l][b], f->leaf_key[sl], c, &r); /* ĉ(t|s), r̂(t|s) */
         orc_h_p2l(p, kh[2 * l], kh[2 * l + 1], c, r, f->us + 3 * s0, q + 2 * s0, s1 - s0, loc[l] + b * w);
       }
   }
@@ -942,7 +914,7 @@ void orc_hfmm_evaluate(orc_fmm *f, double kre, double kim, const double *q_calle
         int64_t sb = f->w[l].ent[e];
         int sl = (int)f->w[l].aux[e];
         double c[3], rr;
-        relative_frame(f->keys[sl][sb], f->leaf_key[lf], c, &rr); /* ĉ(s|t), r̂(s|t) */
+        orc_relative_frame(f->keys[sl][sb], f->leaf_key[lf], c, &rr); /* ĉ(s|t), r̂(s|t) */
         orc_h_m2p(p, kh[2 * sl], kh[2 * sl + 1], c, rr, mult[sl] + sb * w, f->ut + 3 * t0, t1 - t0, ph + 2 * t0,
                   gp ? gp + 6 * t0 : NULL);
       }
@@ -956,7 +928,7 @@ void orc_hfmm_evaluate(orc_fmm *f, double kre, double kim, const double *q_calle
           memcpy(y, f->us + 3 * s0, sizeof(double) * 3 * (s1 - s0));
         } else {
           double c[3], rr;
-          relative_frame(f->leaf_key[sl], f->leaf_key[lf], c, &rr); /* ĉ(s|t), r̂(s|t) */
+          orc_relative_frame(f->leaf_key[sl], f->leaf_key[lf], c, &rr); /* ĉ(s|t), r̂(s|t) */
           for (int64_t j = 0; j < s1 - s0; ++j)
             for (int k = 0; k < 3; ++k) y[3 * j + k] = c[k] + rr * f->us[3 * (s0 + j) + k];
         }

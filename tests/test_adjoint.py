@@ -130,3 +130,87 @@ def test_torch_autograd(_gpu):
     assert _rel(qt.grad.numpy(), gq) < 1e-12
     assert _rel(zt.grad.numpy(), gz) < 1e-12
     assert abs(loss.item() - float(np.mean(out.potential ** 2))) <= 1e-12 * loss.item()
+
+
+@pytest.mark.gpu
+def test_torch_autograd_device_inputs(_gpu):
+    # charges and the cotangent are produced by queued torch kernels on the GPU (a long
+    # matrix product first, so they are still in flight when the library is called): the
+    # operator's stream must wait for torch's stream before it reads them
+    torch = pytest.importorskip("torch")
+    from fmm_amd.autograd import FmmPotential, fmm_for
+    rng = O.Rng(35)
+    z = rng.cube(3000)
+    q = rng.normals(3000)
+    f = fmm_for(z, p=8, leaf_points=32)
+    dev = torch.device("cuda", 0)
+    zt = torch.tensor(z, dtype=torch.float64, device=dev, requires_grad=True)
+    base = torch.tensor(q, dtype=torch.float64, device=dev)
+    big = torch.randn(2048, 2048, dtype=torch.float64, device=dev)
+    for _ in range(4):
+        big = big @ big / 2048.0  # keeps torch's stream busy
+    qt = (base * 2.0 + 0.0 * big[0, 0]).requires_grad_(True)
+    phi = FmmPotential.apply(zt, qt, f)
+    w = (big.abs().sum() * 0.0 + 3.0)  # the loss weight also comes out of queued work
+    loss = w * (phi * phi).mean()
+    loss.backward()
+    out = f.evaluate(2.0 * q)
+    g = 3.0 * 2.0 * out.potential / len(q)
+    gq, gz = f.vjp(2.0 * q, g, out.gradient)
+    assert _rel(phi.detach().cpu().numpy(), out.potential) < 1e-12
+    assert _rel(qt.grad.cpu().numpy(), gq) < 1e-12
+    assert _rel(zt.grad.cpu().numpy(), gz) < 1e-12
+
+
+@pytest.mark.gpu
+def test_autograd_moved_points(_gpu):
+    # FmmPotential refuses points other than the FMM's; Potential rebuilds and so
+    # evaluates at the moved points
+    torch = pytest.importorskip("torch")
+    import fmm_amd
+    from fmm_amd.autograd import FmmPotential, Potential, fmm_for
+    rng = O.Rng(36)
+    z = rng.cube(1500)
+    q = rng.normals(1500)
+    f = fmm_for(z, p=8, leaf_points=32)
+    moved = z.copy()
+    moved[7] += (0.01, -0.02, 0.005)
+    zt = torch.tensor(moved, dtype=torch.float64, requires_grad=True)
+    qt = torch.tensor(q, dtype=torch.float64, requires_grad=True)
+    with pytest.raises(fmm_amd.FmmError):
+        FmmPotential.apply(zt, qt, f)
+    pot = Potential(p=8, leaf_points=32)
+    phi0 = pot(torch.tensor(z, dtype=torch.float64), qt).detach().cpu().numpy()
+    phi1 = pot(zt, qt)
+    assert pot.builds == 2
+    ref, _ = _direct(moved, q)
+    assert _rel(phi1.detach().cpu().numpy(), ref) < 1e-5
+    assert np.abs(phi1.detach().cpu().numpy() - phi0).max() > 1e-4
+    (phi1 * phi1).mean().backward()
+    assert zt.grad is not None and np.all(np.isfinite(zt.grad.numpy()))
+
+
+@pytest.mark.gpu
+def test_device_buffers_checked(_gpu):
+    # device pointers of the wrong length, dtype or layout are refused before any launch
+    torch = pytest.importorskip("torch")
+    import fmm_amd
+    rng = O.Rng(37)
+    z = rng.cube(500)
+    f = fmm_amd.FmmBuilder(4).max_points_per_leaf(32).build(z)
+    op = f.operator
+    dev = torch.device("cuda", 0)
+    for bad in (torch.zeros(499, dtype=torch.float64, device=dev), torch.zeros(500, dtype=torch.float32, device=dev),
+                torch.zeros(1000, dtype=torch.float64, device=dev)[::2]):
+        with pytest.raises(fmm_amd.FmmError):
+            op.load_charges(bad)
+    op.load_charges(torch.ones(500, dtype=torch.float64, device=dev))
+    op.reset()
+    op.evaluate()
+    with pytest.raises(fmm_amd.FmmError):
+        op.read_output_device(torch.zeros(500, dtype=torch.float64, device=dev),
+                              torch.zeros(499, 3, dtype=torch.float64, device=dev))
+    phi = torch.zeros(500, dtype=torch.float64, device=dev)
+    op.read_output_device(phi)
+    ref, _ = _direct(z, np.ones(500))
+    assert _rel(phi.cpu().numpy(), ref) < 1e-4
