@@ -137,6 +137,10 @@ def main():
                          "1M uniform points, k = 25, complex charges, degree by the rule at 1e-6; nonuniform: 2M "
                          "sphere-surface + 1M Plummer points, p = 10, rotation M2L in f32")
     ap.add_argument("--k", type=float, default=25.0, help="helmholtz: the wavenumber (real)")
+    ap.add_argument("--tune-budget", type=float, default=10.0,
+                    help="seconds of autotune per leg (fmm_amd/tune.py) before timing; 0: the static rule")
+    ap.add_argument("--tuning-cache", default=None,
+                    help="directory of the tuning cache (default: a fresh temporary directory, so each run tunes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each box has one GPU)",
@@ -193,10 +197,21 @@ def main():
         """Relative L2 error (fmm-validate/src/metrics.rs:51-170)."""
         return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
+    import tempfile
+
+    from fmm_amd import tune
+
+    tune_dir = args.tuning_cache or tempfile.mkdtemp(prefix="fmm_amd_tune")
+
     def run(precision):
-        t0 = time.time()
         m2l = "dense" if (args.m2l == "plane_wave" and precision == "mixed") else args.m2l
-        op = fmm_amd.Operator(plan, args.p, precision, True, 0, m2l)
+        tuning, tinfo = None, None
+        if args.tune_budget > 0:
+            tuning, tinfo = tune.resolve(plan, args.p, precision, True, 0, m2l, tune_dir, args.tune_budget)
+            log(f"[{precision}] tuning ({'cached' if tinfo['cached'] else 'tuned in %.1fs' % tinfo['tuned_s']}): "
+                f"{tinfo['decisions']}")
+        t0 = time.time()
+        op = fmm_amd.Operator(plan, args.p, precision, True, 0, m2l, tuning)
         stats = op.stats()
         log(f"[{precision}] operator built in {time.time() - t0:.1f}s: {stats}")
 
@@ -255,7 +270,10 @@ def main():
         log(f"[{precision}] hipGraph replay: median {gmed * 1e3:.2f} ms ({args.n / gmed / 1e6:.1f} Mpts/s), "
             f"max |graph - eager| phi {graph_diff:.1e}")
         res = dict(ms=med * 1e3, times=times, mpts=args.n / med / 1e6, err_phi=e_phi, err_grad=e_grad,
-                   kinds=kinds, stats=stats, graph_ms=gmed * 1e3, graph_diff=graph_diff)
+                   kinds=kinds, stats=stats, graph_ms=gmed * 1e3, graph_diff=graph_diff,
+                   tuning=None if tinfo is None else {"decisions": tinfo["decisions"],
+                                                      "tuned_s": round(tinfo["tuned_s"], 2),
+                                                      "cached": tinfo["cached"]})
         del op
         return res
 
@@ -320,7 +338,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 P2P (double-float separations) + bf16x3 M2L (f32 MFMA accumulation) + f64 other",
+        "dtype": "f32 P2P (fixed-point separations) + bf16x3 M2L (f32 MFMA accumulation) + f64 other",
         "data": "synthetic: SplitMix64(0x00be9c06) uniform cube points, N(0,1) charges",
         "config": {"workload": "laplace-fmm-uniform-cube", "n": args.n, "p": args.p, "leaf_points": args.leaf,
                    "m2l": args.m2l,
@@ -344,6 +362,8 @@ def main():
     line["interactions"] = {"p2p_pairs_per_s": stats["p2p_pairs"] / (mixed["ms"] * 1e-3),
                             "m2l_v_pairs_per_s": stats["m2l_pairs"] / (mixed["ms"] * 1e-3)}
     line["graph_max_abs_diff_phi"] = mixed["graph_diff"]
+    line["tuning"] = {"mixed": mixed["tuning"], "plain_f64": None if plain is None else plain["tuning"],
+                      "budget_s": args.tune_budget}
     print(json.dumps(line), flush=True)
 
 
@@ -366,7 +386,14 @@ def batched(args):
     t0 = time.time()
     # the near rule on, as the reference's FmmBuilder has it by default (fmm-exec/src/fmm.rs:1029),
     # at this path's crossovers (p+1)^2 / 2
-    fmm = fmm_amd.FmmBuilder(6).precision("mixed").max_points_per_leaf(64).near_rule("device").build_batched(clouds)
+    builder = fmm_amd.FmmBuilder(6).precision("mixed").max_points_per_leaf(64).near_rule("device")
+    if args.tune_budget > 0:
+        import tempfile
+
+        builder.tuning_cache(args.tuning_cache or tempfile.mkdtemp(prefix="fmm_amd_tune")).tuning_budget(args.tune_budget)
+    fmm = builder.build_batched(clouds)
+    if builder.tuning_info is not None:
+        log(f"tuning: {builder.tuning_info['decisions']} ({builder.tuning_info['tuned_s']:.1f} s)")
     log(f"forest plan + operator in {time.time() - t0:.1f}s: {fmm.operator.stats()}")
     op = fmm.operator
     q_dev = torch.from_numpy(np.concatenate(charges)).to(dev)
@@ -400,11 +427,37 @@ def batched(args):
         op.vjp_device(q_dev, g, field, gq, gz)
         torch.cuda.synchronize()
         vt.append(time.perf_counter() - s)
+    gq_h, gz_h = gq.cpu().numpy(), gz.cpu().numpy()
     step()
+    # the adjoint against central finite differences (h = 1e-4) of the f64 direct-sum loss
+    # L = sum(phi^2) / N over the forest; clouds do not interact, so a coordinate of the
+    # smallest cloud moves only that cloud's terms (direct sums on the GPU, compensated)
+    first = np.concatenate([[0], np.cumsum([len(c) for c in clouds])])
+    cs = int(np.argmin([len(c) for c in clouds]))
+    zc, qc = clouds[cs], charges[cs]
+    gqc, gzc = gq_h[first[cs]:first[cs + 1]], gz_h[first[cs]:first[cs + 1]]
+
+    def loss(zz, qq):
+        f, _ = fmm_amd.direct_sum(zz, qq, zz, gradients=False)
+        return float(np.sum((f / (4 * np.pi)) ** 2) / n)
+
+    hstep, fd_worst = 1e-4, 0.0
+    for j in P.SplitMix64(77).sample(len(zc), 4):
+        qp, qm = qc.copy(), qc.copy()
+        qp[j] += hstep
+        qm[j] -= hstep
+        fd = (loss(zc, qp) - loss(zc, qm)) / (2 * hstep)
+        fd_worst = max(fd_worst, abs(fd - gqc[j]) / np.abs(gqc).max())
+        for k in range(3):
+            zp, zm = zc.copy(), zc.copy()
+            zp[j, k] += hstep
+            zm[j, k] -= hstep
+            fd = (loss(zp, qc) - loss(zm, qc)) / (2 * hstep)
+            fd_worst = max(fd_worst, abs(fd - gzc[j, k]) / np.abs(gzc).max())
+    log(f"adjoint vs finite differences (cloud {cs}, {len(zc)} points): worst {fd_worst:.2e}")
     # errors: 32 sampled targets per cloud against the compensated f64 direct sum
     ph = phi.cpu().numpy()
     gr = field.cpu().numpy()
-    first = np.concatenate([[0], np.cumsum([len(c) for c in clouds])])
     num = den = numg = deng = 0.0
     for c, (pts, q) in enumerate(zip(clouds, charges)):
         idx = P.SplitMix64(c + 1).sample(len(pts), 32)
@@ -419,11 +472,15 @@ def batched(args):
         "metric": "batched Laplace FMM Mpts/s (256 clouds, one forest, hipGraph replay)",
         "value": n / med / 1e6, "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": med * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 P2P (double-float separations) + bf16x3 M2L + f64 other", "data": "synthetic",
+        "dtype": "f32 P2P (fixed-point separations) + bf16x3 M2L + f64 other", "data": "synthetic",
         "config": {"workload": "batched-gaussian-clouds", "clouds": len(clouds), "n": n, "p": 6, "leaf_points": 64,
                    "near_rule": [24.5, 24.5], "sizes": [int(len(c)) for c in clouds][:8] + ["..."]},
         "errors": {"phi": (num / den) ** 0.5, "grad": (numg / deng) ** 0.5, "sampled_per_cloud": 32},
         "adjoint_ms": float(np.median(vt)) * 1e3,
+        "adjoint_fd": {"worst_rel_to_max": fd_worst, "h": hstep, "cloud_points": int(len(zc)),
+                       "checked": "dL/dq and dL/dz at 4 points of the smallest cloud, central differences of the "
+                                  "f64 direct-sum loss sum(phi^2)/N"},
+        "tuning": None if builder.tuning_info is None else builder.tuning_info["decisions"],
     }
     print(json.dumps(line), flush=True)
 

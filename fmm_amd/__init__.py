@@ -12,7 +12,9 @@ evaluation path: without the built library or a GPU the calls raise.
 """
 from __future__ import annotations
 
+import copy
 import ctypes as C
+import os
 from dataclasses import dataclass
 from typing import Optional
 
@@ -181,7 +183,7 @@ class Operator:
     """The device operator: the level calls of FmmOperator (fmm-plan/src/operator.rs)
     on the GPU, plus the evaluator's whole order in one call."""
 
-    def __init__(self, plan: Plan, p: int, precision="f64", gradients=True, device=0, m2l="dense"):
+    def __init__(self, plan: Plan, p: int, precision="f64", gradients=True, device=0, m2l="dense", tuning=None):
         self.plan = plan
         self.p = p
         self.gradients = gradients
@@ -190,8 +192,10 @@ class Operator:
         self.device = int(device)
         o = _lib.OperatorOptions(int(p), self.precision, int(bool(gradients)), int(device), self.m2l)
         h = C.c_void_p()
-        _lib.check(_lib.lib().fmmh_operator_create(plan.handle, C.byref(o), C.byref(h)))
+        t = C.byref(tuning) if tuning is not None else None
+        _lib.check(_lib.lib().fmmh_operator_create_tuned(plan.handle, C.byref(o), t, C.byref(h)))
         self._h = h
+        self.tuning = tuning
         self.ntargets = len(plan.targets)
         self.nsources = len(plan.sources)
 
@@ -363,10 +367,37 @@ class FmmBuilder:
         self._device = 0
         self._near = None
         self._m2l = "dense"
+        self._tuning_dir = None
+        self._tuning_budget = 10.0
+        self._leaf_candidates = None
+        self.tuning_info = None
 
     def gradients(self, on: bool):
         self._gradients = bool(on)
         return self
+
+    def tuning_cache(self, directory):
+        """Autotune the operator's launch shapes at build and keep the decisions in
+        <directory>/fmm_amd_tuning.json (FmmBuilder::tuning_cache, fmm-exec/src/tune.rs);
+        None: the static rule, nothing timed."""
+        self._tuning_dir = None if directory is None else os.fspath(directory)
+        return self
+
+    def tuning_budget(self, seconds: float):
+        """Seconds the autotune may spend per build (10 by default, as the reference's)."""
+        self._tuning_budget = float(seconds)
+        return self
+
+    def tune_leaf_size(self, candidates):
+        """Choose max_points_per_leaf among `candidates` by timing whole evaluations of
+        the points given to build (cached per N bucket with tuning_cache)."""
+        self._leaf_candidates = tuple(int(c) for c in candidates) if candidates else None
+        return self
+
+    def _with_leaf(self, leaf: int):
+        b = copy.copy(self)
+        b._leaf_points = int(leaf)
+        return b
 
     def max_level(self, level: int):
         self._max_level = int(level)
@@ -412,17 +443,33 @@ class FmmBuilder:
     def build_plan(self, sources, targets=None) -> Plan:
         return Plan(sources, targets, self._leaf_points, self._max_level, self._near)
 
+    def _operator(self, plan) -> Operator:
+        tuning = None
+        if self._tuning_dir is not None:
+            from . import tune
+
+            tuning, self.tuning_info = tune.resolve(plan, self._p, self._precision, self._gradients, self._device,
+                                                    self._m2l, self._tuning_dir, self._tuning_budget)
+        return Operator(plan, self._p, self._precision, self._gradients, self._device, self._m2l, tuning)
+
     def build(self, sources, targets=None) -> Fmm:
-        plan = self.build_plan(sources, targets)
-        op = Operator(plan, self._p, self._precision, self._gradients, self._device, self._m2l)
+        b = self
+        if self._leaf_candidates and targets is None:
+            from . import tune
+
+            leaf, times = tune.tune_leaf_size(self, sources, self._leaf_candidates, self._tuning_dir)
+            b = self._with_leaf(leaf)
+            self.leaf_info = {"leaf_points": leaf, "ms": times}
+        plan = b.build_plan(sources, targets)
+        op = b._operator(plan)
+        self.tuning_info = b.tuning_info
         return Fmm(plan, op)
 
     def build_batched(self, clouds) -> Fmm:
         """One Fmm over many independent clouds (sources = targets in each); charges and
         outputs are the clouds' points in order."""
         plan = Plan.forest(clouds, self._leaf_points, self._max_level, self._near)
-        op = Operator(plan, self._p, self._precision, self._gradients, self._device, self._m2l)
-        return Fmm(plan, op)
+        return Fmm(plan, self._operator(plan))
 
 
 def plane_wave_quadrature(p: int, eps: float):

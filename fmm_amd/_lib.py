@@ -42,6 +42,7 @@ EXPORTS = [
     "fmmh_helmholtz_read_output", "fmmh_helmholtz_read_expansions", "fmmh_helmholtz_kind_timing",
     "fmmh_helmholtz_kind_times", "fmmh_helmholtz_table", "fmmh_helmholtz_direct_sum",
     "fmmh_plane_wave_quadrature", "fmmh_plane_wave_m2l", "fmmh_stream_wait", "fmmh_helmholtz_stream_wait",
+    "fmmh_operator_create_tuned", "fmmh_autotune",
 ]
 
 
@@ -62,6 +63,12 @@ class PlanOptions(C.Structure):
 class OperatorOptions(C.Structure):
     _fields_ = [("p", C.c_int32), ("precision", C.c_int32), ("gradients", C.c_int32), ("device", C.c_int32),
                 ("m2l", C.c_int32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("p2p_uniform_min", C.c_int32), ("m2l_stage", C.c_int32), ("m2l_mode", C.c_int32 * 17),
+                ("l2l_width", C.c_int32 * 17),
+                ("m2m_form", C.c_int32 * 17)]
 
 
 class HelmholtzOptions(C.Structure):
@@ -123,6 +130,8 @@ def lib():
     L.fmmh_m2l_rotation_table.argtypes = [i32, i32, vp]
     L.fmmh_device_count.argtypes = [C.POINTER(i32)]
     L.fmmh_operator_create.argtypes = [vp, C.POINTER(OperatorOptions), C.POINTER(vp)]
+    L.fmmh_operator_create_tuned.argtypes = [vp, C.POINTER(OperatorOptions), C.POINTER(Tuning), C.POINTER(vp)]
+    L.fmmh_autotune.argtypes = [vp, C.POINTER(OperatorOptions), dbl, C.POINTER(Tuning), vp]
     L.fmmh_operator_free.argtypes = [vp]
     L.fmmh_operator_free.restype = None
     L.fmmh_operator_stats.argtypes = [vp, C.POINTER(Stats)]

@@ -13,6 +13,7 @@
 #include "operator.hpp"
 #include "plan.hpp"
 #include "pwtables.hpp"
+#include "tune.hpp"
 
 struct fmmh_plan {
   fmmh::Plan plan;
@@ -241,17 +242,54 @@ int32_t fmmh_device_count(int32_t *count) {
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+fmmh::OperatorOptions options_of(const fmmh_operator_options *options, const fmmh_tuning *tuning) {
+  fmmh::OperatorOptions o;
+  if (options) {
+    o.p = options->p;
+    o.precision = options->precision;
+    o.gradients = options->gradients;
+    o.device = options->device;
+    o.m2l = options->m2l;
+  }
+  if (tuning) {
+    o.tuning.p2p_uniform_min = tuning->p2p_uniform_min;
+    o.tuning.m2l_stage = tuning->m2l_stage;
+    for (int l = 0; l < fmmh::kTuneLevels; ++l) {
+      o.tuning.m2l_mode[l] = tuning->m2l_mode[l];
+      o.tuning.l2l_width[l] = tuning->l2l_width[l];
+      o.tuning.m2m_form[l] = tuning->m2m_form[l];
+    }
+  }
+  return o;
+}
+
+}  // namespace
+
+extern "C" {
+
 int32_t fmmh_operator_create(const fmmh_plan *plan, const fmmh_operator_options *options, fmmh_operator **out) {
+  return fmmh_operator_create_tuned(plan, options, nullptr, out);
+}
+
+int32_t fmmh_operator_create_tuned(const fmmh_plan *plan, const fmmh_operator_options *options,
+                                   const fmmh_tuning *tuning, fmmh_operator **out) {
   return guard([&] {
     need(plan && out, "fmmh_operator_create: NULL argument");
-    fmmh::OperatorOptions o;
-    if (options) {
-      o.p = options->p;
-      o.precision = options->precision;
-      o.gradients = options->gradients;
-      o.device = options->device;
-      o.m2l = options->m2l;
+    if (tuning) {
+      need(tuning->p2p_uniform_min >= 0, "fmmh_operator_create_tuned: p2p_uniform_min must be >= 0");
+      need(tuning->m2l_stage >= 0 && tuning->m2l_stage <= 2, "fmmh_operator_create_tuned: M2L stage 0..2");
+      for (int l = 0; l < fmmh::kTuneLevels; ++l) {
+        need(tuning->m2l_mode[l] >= 0 && tuning->m2l_mode[l] <= 2, "fmmh_operator_create_tuned: M2L mode 0..2");
+        const int w = tuning->l2l_width[l];
+        need(w == 0 || w == 16 || w == 64 || w == 128, "fmmh_operator_create_tuned: L2L width 0, 16, 64, 128");
+        need(tuning->m2m_form[l] >= 0 && tuning->m2m_form[l] <= 2, "fmmh_operator_create_tuned: M2M form 0..2");
+      }
     }
+    const fmmh::OperatorOptions o = options_of(options, tuning);
     auto h = std::make_unique<fmmh_operator>();
     h->op = std::make_unique<fmmh::DeviceOperator>(plan->plan, o);
     h->nlevels = plan->plan.nlevels;
@@ -261,6 +299,26 @@ int32_t fmmh_operator_create(const fmmh_plan *plan, const fmmh_operator_options 
 }
 
 void fmmh_operator_free(fmmh_operator *op) { delete op; }
+
+int32_t fmmh_autotune(const fmmh_plan *plan, const fmmh_operator_options *options, double budget_seconds,
+                      fmmh_tuning *out, double *report) {
+  return guard([&] {
+    need(plan && options && out, "fmmh_autotune: NULL argument");
+    need(budget_seconds >= 0.0, "fmmh_autotune: negative budget");
+    const fmmh::OperatorOptions o = options_of(options, nullptr);
+    fmmh::Tuning t;
+    fmmh::TuneReport r;
+    fmmh::autotune(plan->plan, o, budget_seconds, t, r);
+    out->p2p_uniform_min = t.p2p_uniform_min;
+    out->m2l_stage = t.m2l_stage;
+    for (int l = 0; l < fmmh::kTuneLevels; ++l) {
+      out->m2l_mode[l] = t.m2l_mode[l];
+      out->l2l_width[l] = t.l2l_width[l];
+      out->m2m_form[l] = t.m2m_form[l];
+    }
+    if (report) std::memcpy(report, &r.ms[0][0][0], sizeof(r.ms));
+  });
+}
 
 int32_t fmmh_operator_stats(const fmmh_operator *op, fmmh_stats *out) {
   return guard([&] {

@@ -210,7 +210,7 @@ __global__ __launch_bounds__(512) void k_contract_f64(int nc, int nchunks, uint3
     fetch_src(min(step + 2, last));
     __builtin_amdgcn_sched_barrier(0);
     asm volatile("" ::: "memory");
-    dma(min(step + 2, last), (step + 2) % NST);
+    dma(min(step + NST - 1, last), (step + NST - 1) % NST);
     const double *a = reinterpret_cast<const double *>(As[buf]);
 #pragma unroll
     for (int kk = 0; kk < KC; ++kk)
@@ -441,13 +441,14 @@ __device__ __forceinline__ bf8 as_bf8(u4 v) {
 // N-tiles (half the LDS traffic of one column tile per wave).
 //
 // Pipeline: the table slices go global -> LDS by DMA (global_load_lds) into a ring of
-// NST = 3 buffers, issued two stages ahead; the source expansions (B operand) go to
-// registers one stage ahead.  A stage is KG consecutive k-steps of one group (KG = 2 for
-// the short tables of p <= 7, whose two k-steps would otherwise each pay a full
-// wait-and-barrier round for 24 MFMAs per wave; else 1).  One barrier per stage: after
-// it every wave's DMA for this stage has landed and every wave is done with the buffer
-// the next DMA refills.
-template <int MT, int KG>
+// NST buffers, issued NST - 1 stages ahead; the source expansions (B operand) go to
+// registers one stage ahead.  A stage is KG consecutive k-steps of one group.  NST = 3,
+// KG = 1: one k-step per stage, DMA two stages ahead.  KG = 2 halves the wait-and-barrier
+// rounds (the short tables of p <= 7 always; larger tables as an autotuned shape, with
+// NST = 2 so two stages of up to 13 M-tiles fit the LDS: DMA one stage ahead, covered by
+// a stage of twice the MFMAs).  One barrier per stage: after it every wave's DMA for this
+// stage has landed and every wave is done with the buffer the next DMA refills.
+template <int MT, int KG, int NST = 3>
 __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, uint32_t ntasks, uint32_t xchunk,
                                                        const uint32_t *__restrict__ task_group_off,
                                                        const uint16_t *__restrict__ group_table,
@@ -459,7 +460,7 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, uint3
   constexpr int STAGE = KG * CHUNK;     // u4 per stage (KG consecutive k-steps)
   constexpr int SEGS = STAGE / 64;      // 1 KB segments, one DMA instruction each
   constexpr int SEGW = (SEGS + 7) / 8;  // DMA instructions per wave per stage
-  constexpr int NST = 3;
+  static_assert(NST == 2 || NST == 3, "ring of 2 or 3 stages");
   constexpr int MH = (MT + 1) / 2;
   __shared__ u4 As[NST][STAGE];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -535,84 +536,152 @@ __global__ __launch_bounds__(512) void k_contract_bf16(int nc, int ksteps, uint3
         }
     }
   };
-  // Issue order in stage s: B loads of s + 1, source indices of s + 2, DMA of s + 2
-  // (stages past the end repeat the last one, into a ring slot nobody reads again).  At
-  // the top of stage s + 1 only the DMA of s + 2 is newer than the B loads of s + 1 and
-  // the DMA of s + 1, so a wait for all but SEGW operations covers both.
-  const int last = nsteps - 1;
-  fetch_src(0);
-  fetch_b(0);
-  fetch_src(min(1, last));
-  dma(0, 0);
-  dma(min(1, last), 1);
-  for (int step = 0; step < nsteps; ++step) {
-    const int buf = step % NST;
-    wait_vm<SEGW>();               // stage s and its B loads landed (only the DMA of s + 1 may not)
-    __builtin_amdgcn_s_barrier();  // every wave's part; ring slot s - 1 free
-    __builtin_amdgcn_sched_barrier(0);  // nothing that reads this stage's data moves above
+  // the MFMAs of k-step kg of the stage in ring slot buf, B operand bb
+  auto mfma_kstep = [&](int buf, int kg, const u4 (&bb)[2][3]) {
+    bf8 bh[2], bm[2], bl[2];
 #pragma unroll
-    for (int kg = 0; kg < KG; ++kg)
+    for (int nt = 0; nt < 2; ++nt) {
+      bh[nt] = as_bf8(bb[nt][0]);
+      bm[nt] = as_bf8(bb[nt][1]);
+      bl[nt] = as_bf8(bb[nt][2]);
+    }
+    // the A fragments of M-tile i + 1 are read from LDS while M-tile i is multiplied
+    u4 af[2][3];
+    auto read_a = [&](int i, u4 *f) {
+      const int mt = min(mt0 + i, MT - 1);
 #pragma unroll
-      for (int nt = 0; nt < 2; ++nt)
+      for (int pl = 0; pl < 3; ++pl) f[pl] = As[buf][kg * CHUNK + (pl * MT + mt) * 64 + lane];
+    };
+    read_a(0, af[0]);
 #pragma unroll
-        for (int pl = 0; pl < 3; ++pl) b[kg][nt][pl] = pb[kg][nt][pl];
-    fetch_b(min(step + 1, last));
-    fetch_src(min(step + 2, last));
-    // keep the issue order: the DMA after the loads (a wait for the B loads must not
-    // retire the DMA with them)
-    __builtin_amdgcn_sched_barrier(0);
-    asm volatile("" ::: "memory");
-    dma(min(step + 2, last), (step + 2) % NST);
+    for (int i = 0; i < MH; ++i) {
+      const int mt = mt0 + i;
+      if (i + 1 < MH) read_a(i + 1, af[(i + 1) & 1]);
+      if (mt < MT) {
+        const bf8 ah = as_bf8(af[i & 1][0]);
+        const bf8 am = as_bf8(af[i & 1][1]);
+        const bf8 al = as_bf8(af[i & 1][2]);
 #pragma unroll
-    for (int kg = 0; kg < KG; ++kg) {
-      bf8 bh[2], bm[2], bl[2];
-#pragma unroll
-      for (int nt = 0; nt < 2; ++nt) {
-        bh[nt] = as_bf8(b[kg][nt][0]);
-        bm[nt] = as_bf8(b[kg][nt][1]);
-        bl[nt] = as_bf8(b[kg][nt][2]);
-      }
-      // the A fragments of M-tile i + 1 are read from LDS while M-tile i is multiplied
-      u4 af[2][3];
-      auto read_a = [&](int i, u4 *f) {
-        const int mt = min(mt0 + i, MT - 1);
-#pragma unroll
-        for (int pl = 0; pl < 3; ++pl) f[pl] = As[buf][kg * CHUNK + (pl * MT + mt) * 64 + lane];
-      };
-      read_a(0, af[0]);
-#pragma unroll
-      for (int i = 0; i < MH; ++i) {
-        const int mt = mt0 + i;
-        if (i + 1 < MH) read_a(i + 1, af[(i + 1) & 1]);
-        if (mt < MT) {
-          const bf8 ah = as_bf8(af[i & 1][0]);
-          const bf8 am = as_bf8(af[i & 1][1]);
-          const bf8 al = as_bf8(af[i & 1][2]);
-#pragma unroll
-          for (int nt = 0; nt < 2; ++nt) {
-            // smallest terms first
-            f4 a4 = acc[i][nt];
-            a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[nt], a4, 0, 0, 0);
-            a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm[nt], a4, 0, 0, 0);
-            a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[nt], a4, 0, 0, 0);
-            a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh[nt], a4, 0, 0, 0);
-            a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm[nt], a4, 0, 0, 0);
-            a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[nt], a4, 0, 0, 0);
-            acc[i][nt] = a4;
-          }
+        for (int nt = 0; nt < 2; ++nt) {
+          // smallest terms first
+          f4 a4 = acc[i][nt];
+          a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[nt], a4, 0, 0, 0);
+          a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm[nt], a4, 0, 0, 0);
+          a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[nt], a4, 0, 0, 0);
+          a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh[nt], a4, 0, 0, 0);
+          a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm[nt], a4, 0, 0, 0);
+          a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[nt], a4, 0, 0, 0);
+          acc[i][nt] = a4;
         }
       }
     }
-    if ((step + 1) % spg == 0) {
-      // end of a group (one offset): its f32 sum goes into the f64 accumulators
+  };
+  // end of a group (one offset): its f32 sum goes into the f64 accumulators
+  auto flush = [&]() {
 #pragma unroll
-      for (int i = 0; i < MH; ++i)
+    for (int i = 0; i < MH; ++i)
 #pragma unroll
-        for (int nt = 0; nt < 2; ++nt) {
+      for (int nt = 0; nt < 2; ++nt) {
 #pragma unroll
-          for (int r = 0; r < 4; ++r) sum[i][nt][r] += double(acc[i][nt][r]);
-          acc[i][nt] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int r = 0; r < 4; ++r) sum[i][nt][r] += double(acc[i][nt][r]);
+        acc[i][nt] = f4{0.f, 0.f, 0.f, 0.f};
+      }
+  };
+  const int last = nsteps - 1;
+  if constexpr (NST == 3) {
+    // Issue order in stage s: B loads of s + 1, source indices of s + 2, DMA of s + 2
+    // (stages past the end repeat the last one, into a ring slot nobody reads again).  At
+    // the top of stage s + 1 only the DMA of s + 2 is newer than the B loads of s + 1 and
+    // the DMA of s + 1, so a wait for all but SEGW operations covers both.
+    fetch_src(0);
+    fetch_b(0);
+    fetch_src(min(1, last));
+    dma(0, 0);
+    dma(min(1, last), 1);
+    for (int step = 0; step < nsteps; ++step) {
+      const int buf = step % NST;
+      wait_vm<SEGW>();               // stage s and its B loads landed (only the DMA of s + 1 may not)
+      __builtin_amdgcn_s_barrier();  // every wave's part; ring slot s - 1 free
+      __builtin_amdgcn_sched_barrier(0);  // nothing that reads this stage's data moves above
+#pragma unroll
+      for (int kg = 0; kg < KG; ++kg)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+          for (int pl = 0; pl < 3; ++pl) b[kg][nt][pl] = pb[kg][nt][pl];
+      fetch_b(min(step + 1, last));
+      fetch_src(min(step + 2, last));
+      // keep the issue order: the DMA after the loads (a wait for the B loads must not
+      // retire the DMA with them)
+      __builtin_amdgcn_sched_barrier(0);
+      asm volatile("" ::: "memory");
+      dma(min(step + 2, last), (step + 2) % NST);
+#pragma unroll
+      for (int kg = 0; kg < KG; ++kg) mfma_kstep(buf, kg, b[kg]);
+      if ((step + 1) % spg == 0) flush();
+    }
+  } else {
+    // Two k-steps per stage (KG = 2, an even k-step count per group, so a stage never
+    // straddles groups), ring of two: the B operand goes one k-step (not one stage) ahead,
+    // so one register set is in flight as in the one-step form.  Issue order in stage s
+    // (k-steps t0 = 2 s, t1 = 2 s + 1): B of t1, DMA of s + 1, then during t1 the B of
+    // t1 + 1 (the next group's source indices, loaded a group ahead, when t1 + 1 starts a
+    // group).  Waits: at the top of a stage for everything (the DMA of s and the B of t0);
+    // before t1 for all but the newest SEGW operations (the DMA of s + 1).
+    static_assert(KG == 2, "the ring of two takes two k-steps per stage");
+    uint32_t src_cur[2], src_next[2];
+    auto load_src = [&](uint32_t gi, uint32_t (&dst)[2]) {
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt) dst[nt] = group_src[size_t(gi) * COLS + col[nt]];
+    };
+    auto load_b = [&](int t, const uint32_t (&src)[2], u4 (&dst)[2][3]) {
+      const int ks = t % ksteps;
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt) {
+        const size_t s = src[nt] != 0xFFFFFFFFu ? src[nt] : 0;
+#pragma unroll
+        for (int pl = 0; pl < 3; ++pl) {
+          const u4 v = X16[(s * 3 + pl) * kq + ks * 4 + g];
+          dst[nt][pl] = src[nt] != 0xFFFFFFFFu ? v : zero4;
         }
+      }
+    };
+    const int lastk = 2 * nsteps - 1;
+    load_src(g0, src_cur);
+    load_src(min(g0 + 1, g1 - 1), src_next);
+    load_b(0, src_cur, pb[0]);
+    dma(0, 0);
+    for (int step = 0; step < nsteps; ++step) {
+      const int buf = step & 1, t0 = 2 * step, t1 = t0 + 1;
+      wait_vm<0>();                  // the DMA of this stage and the B of t0 landed
+      __builtin_amdgcn_s_barrier();  // every wave's part; the other ring slot free
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int pl = 0; pl < 3; ++pl) b[0][nt][pl] = pb[0][nt][pl];
+      load_b(t1, src_cur, pb[0]);  // t1 is in t0's group
+      __builtin_amdgcn_sched_barrier(0);
+      asm volatile("" ::: "memory");
+      dma(min(step + 1, last), buf ^ 1);
+      mfma_kstep(buf, 0, b[0]);
+      wait_vm<SEGW>();  // the B of t1 landed (only the DMA of s + 1 may not)
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int pl = 0; pl < 3; ++pl) b[0][nt][pl] = pb[0][nt][pl];
+      const int tn = min(t1 + 1, lastk);
+      if (tn % ksteps == 0 && tn != t1) {
+        // t1 + 1 opens the next group: its indices were loaded a group ago
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) src_cur[nt] = src_next[nt];
+        load_src(min(g0 + uint32_t(tn / ksteps) + 1, g1 - 1), src_next);
+      }
+      load_b(tn, src_cur, pb[0]);
+      __builtin_amdgcn_sched_barrier(0);
+      mfma_kstep(buf, 1, b[0]);
+      if ((t1 + 1) % ksteps == 0) flush();
     }
   }
   // no DMA may land in LDS after the workgroup ends
@@ -737,7 +806,8 @@ void contract_f64(hipStream_t s, int p, int width, uint32_t ntasks, const uint32
 }
 
 void m2m_split(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off, const uint16_t *group_table,
-               const uint32_t *group_src, const uint32_t *col_out, const double *tsw, const double *X, double *Y) {
+               const uint32_t *group_src, const uint32_t *col_out, const double *tsw, const double *X, double *Y,
+               int form) {
   if (ntasks == 0) return;
   const int nc = (p + 1) * (p + 1), nch = k_chunks(p);
 #define FMMH_M2M_ROWS(MT)                                                                                        \
@@ -748,7 +818,8 @@ void m2m_split(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group
   // fewer (task, M-tile) pairs than CUs: one workgroup per pair (each
   // re-reads the source expansions, so a large level keeps the split form: 4M-point
   // tree, level 0: 57 -> 14 us, level 5: 226 us split, 787 us per pair)
-  if (ntasks * uint32_t(m_tiles(p)) < 256) switch (m_tiles(p)) {
+  const bool per_tile = form == 1 || (form == 0 && ntasks * uint32_t(m_tiles(p)) < 256);
+  if (per_tile) switch (m_tiles(p)) {
     FMMH_M2M_ROWS(1)
     FMMH_M2M_ROWS(2)
     FMMH_M2M_ROWS(3)
@@ -799,13 +870,32 @@ void m2m_split(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group
 
 void contract_bf16(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off, const uint16_t *group_table,
                    const uint32_t *group_src, const uint32_t *col_out, const uint16_t *tsw16, const uint16_t *X16,
-                   double *Y, uint32_t first_slot, float *S, uint32_t xchunk) {
+                   double *Y, uint32_t first_slot, float *S, uint32_t xchunk, int stage) {
   if (ntasks == 0) return;
   const int nc = (p + 1) * (p + 1), ks = k_steps16(p);
   const u4 *t4 = reinterpret_cast<const u4 *>(tsw16);
   const u4 *x4 = reinterpret_cast<const u4 *>(X16);
   // two k-steps per stage for the short tables (at most 4 M-tiles and 2 k-steps)
   const bool pair_steps = m_tiles(p) <= 4 && ks == 2;
+  // two k-steps per stage in a ring of two (stage = 2): an even k-step count and at most
+  // 13 M-tiles (2 x 2 x 13 x 3 KB of LDS)
+  if (stage == 2 && ks % 2 == 0 && m_tiles(p) >= 5 && m_tiles(p) <= 13) {
+#define FMMH_CONTRACT22(MT)                                                                                        case MT:                                                                                                           hipLaunchKernelGGL((k_contract_bf16<MT, 2, 2>), dim3(xcd_grid(ntasks, xchunk)), dim3(512), 0, s, nc, ks,                           ntasks, xchunk, task_group_off, group_table, group_src, col_out, t4, x4, Y, first_slot, S);     return;
+    switch (m_tiles(p)) {
+      FMMH_CONTRACT22(5)
+      FMMH_CONTRACT22(6)
+      FMMH_CONTRACT22(7)
+      FMMH_CONTRACT22(8)
+      FMMH_CONTRACT22(9)
+      FMMH_CONTRACT22(10)
+      FMMH_CONTRACT22(11)
+      FMMH_CONTRACT22(12)
+      FMMH_CONTRACT22(13)
+      default:
+        break;
+    }
+#undef FMMH_CONTRACT22
+  }
 #define FMMH_CONTRACT(MT)                                                                                        \
   case MT:                                                                                                       \
     hipLaunchKernelGGL((k_contract_bf16<MT, 1>), dim3(xcd_grid(ntasks, xchunk)), dim3(512), 0, s, nc, ks, ntasks, \

@@ -87,7 +87,7 @@ __global__ void k_gather_charges(const double *q, const uint32_t *order, size_t 
   for (size_t j = blockIdx.x * size_t(blockDim.x) + threadIdx.x; j < n; j += size_t(gridDim.x) * blockDim.x) {
     const double v = q[order[j]];
     q64[j] = v;
-    if (rec) rec[8 * j + 3] = float(v);
+    if (rec) rec[4 * j + 3] = float(v);
   }
 }
 
@@ -232,24 +232,38 @@ __global__ void k_vjp_combine(size_t n, const double *q, const double *g, const 
 // leaves): that tile selects 0 for r^2 == 0; other tiles clamp r^2 below at the smallest
 // normal number, which changes nothing for distinct points (r^2 >= 1e-22 in leaf units
 // at the deepest level) and keeps a separation rounded to zero finite.
-// Chunks of more than 32 targets in the mixed path: each lane holds one target (TWO:
-// two, the second 64 further on) and at every step all lanes take the same source, read
-// from its record (u_hi, q, u_lo: f32, the source's own leaf frame) by uniform loads, so
-// no LDS staging and no lanes split over phases.  Per near entry the targets are moved
-// into the source leaf's frame once, x' = c'' + rho u_t in f64 (c'' = (C_t - C_s) / r_s,
-// rho = r_t / r_s), and split into hi + lo floats; the separation d' = x' - u_s is formed
-// as (x'_hi - u_hi) + (x'_lo - u_lo) as in the staged path.  In target units q / |d| =
-// rho q / |d'| and q d / |d|^3 = rho^2 q d' / |d'|^3: the f32 sums of an entry (at most
-// 128 sources) are scaled by these powers of two as they go into the f64 sums.
+// Chunks of at least `umin` targets in the mixed path (33 by the static rule): each lane
+// holds one target (TWO: two, the second 64 further on) and at every step all lanes take
+// the same source, read from its 16-byte record by uniform loads, so no LDS staging and
+// no lanes split over phases.  A record holds the source's coordinates in its own leaf's
+// frame as 32-bit fixed point (u 2^28, |u| <= 1) and its charge as f32.  Per near entry
+// the targets are moved into the source leaf's frame once, x' = c'' + rho u_t in f64
+// (c'' = (C_t - C_s) / r_s, rho = r_t / r_s), and rounded to the same fixed point: in a
+// 2:1-balanced tree a near leaf (U list, or W/X pairs moved to P2P by the near rule) has
+// |x'| <= 1 + 2 rho + 2 <= 7 per axis, so |x' 2^28| < 2^31 and the separation
+// d' = x' - u_s is one exact integer subtraction per axis, then one conversion to f32
+// (relative rounding 2^-24).  The coordinates' quantisation, 2^-29 source-leaf units,
+// bounds the error of a pair at 2^-29 / |d'| relative: below 1e-7 from |d'| = 2^-4.5
+// half-widths on; a closer pair (r^2 under 2^47 integer units, rare: one in ~10^6 pairs
+// of a uniform leaf) forms d' again from the f64 coordinates, x' - u_s, scaled and
+// rounded once, so close pairs keep 2^-24 of |d'| as in the double-float form.
+// In integer units r^2 = |d'|^2 2^56: the f32 sums of an entry (at most 128 sources) are
+// scaled by rho 2^28 (potential) and rho^2 2^56 (gradient) as they go into the f64 sums,
+// since q / |d| = rho q / |d'| and q d / |d|^3 = rho^2 q d' / |d'|^3 in target units.
+// Off the own leaf r^2 starts from 1/4 (integer units) in the quantised form and from the
+// smallest normal float in the f64 form: distinct points keep a finite r^2.
 template <bool GRAD, bool TWO>
 __device__ __forceinline__ void p2p_uniform(uint32_t lf, uint32_t tb, uint32_t n, uint32_t e0, uint32_t e1,
                                             const uint32_t *__restrict__ near_ent,
                                             const uint32_t *__restrict__ leaf_cell,
-                                            const uint32_t *__restrict__ src_start, const float4 *__restrict__ rec,
-                                            const double *__restrict__ tx, const double *__restrict__ ty,
-                                            const double *__restrict__ tz, double *__restrict__ phi,
-                                            double *__restrict__ grad) {
+                                            const uint32_t *__restrict__ src_start, const int4 *__restrict__ rec,
+                                            const uint8_t *__restrict__ near_close,
+                                            const double *__restrict__ sx, const double *__restrict__ sy,
+                                            const double *__restrict__ sz, const double *__restrict__ tx,
+                                            const double *__restrict__ ty, const double *__restrict__ tz,
+                                            double *__restrict__ phi, double *__restrict__ grad) {
   constexpr int A = TWO ? 2 : 1;
+  constexpr float CLOSE = kP2pClose;
   constexpr float TINY = 1.17549435e-38f;
   const int lane = threadIdx.x;
   double u[A][3];
@@ -272,77 +286,110 @@ __device__ __forceinline__ void p2p_uniform(uint32_t lf, uint32_t tb, uint32_t n
     if (s0 == s1) continue;
     double c[3], rho;
     relative_frame<double>(ct, leaf_cell + 4 * size_t(sl), c, rho);
-    float xh[A][3], xl[A][3];
+    int xi[A][3];
 #pragma unroll
     for (int a = 0; a < A; ++a)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const double x = fma(rho, u[a][k], c[k]);
-        xh[a][k] = float(x);
-        xl[a][k] = float(x - double(xh[a][k]));
-      }
+      for (int k = 0; k < 3; ++k) xi[a][k] = __double2int_rn(fma(rho, u[a][k], c[k]) * kP2pFixed);
     const bool own = sl == lf;
+    // entries flagged at build (a pair of the entry closer than the threshold) and the
+    // own leaf test their pairs; the others run the separations unchecked
+    const bool close_entry = own || near_close[e] != 0;
     for (uint32_t jb = s0; jb < s1; jb += 128) {
       const uint32_t je = (s1 - jb) < 128u ? s1 : jb + 128;
       float ap[A], ax[A], ay[A], az[A];
 #pragma unroll
       for (int a = 0; a < A; ++a) ap[a] = ax[a] = ay[a] = az[a] = 0.0f;
       // the records of the next 4 sources are loaded while the current 4 are computed
-      auto run = [&](auto check) {
-        auto body = [&](const float4 h, const float4 l) {
+      auto run = [&](auto check, auto test_close) {
+        // K sources: separations and r^2 of every pair first, then (flagged entries) one
+        // wave-uniform test for a close pair among them (the f64 recomputation runs only
+        // then, per lane), then the terms, so the pairs' chains interleave
+        auto body = [&](const int4 *h, uint32_t js, auto kk) {
+          constexpr int K = decltype(kk)::value;
+          float dx[K][A], dy[K][A], dz[K][A], r2[K][A];
+          float rmin = CLOSE;
 #pragma unroll
-          for (int a = 0; a < A; ++a) {
-            const float dx = (xh[a][0] - h.x) + (xl[a][0] - l.x);
-            const float dy = (xh[a][1] - h.y) + (xl[a][1] - l.y);
-            const float dz = (xh[a][2] - h.z) + (xl[a][2] - l.z);
-            // off the own leaf the smallest normal number enters the first product: it
-            // keeps a separation rounded to zero finite and changes no r^2 above 2^-102
-            const float r2 = decltype(check)::value ? dx * dx + dy * dy + dz * dz
-                                                    : fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, TINY)));
-            const float inv = decltype(check)::value ? (r2 == 0.0f ? 0.0f : __builtin_amdgcn_rsqf(r2))
-                                                     : __builtin_amdgcn_rsqf(r2);
-            const float qi = h.w * inv;
-            ap[a] += qi;
-            if (GRAD) {
-              const float qi3 = qi * (inv * inv);
-              ax[a] -= qi3 * dx;
-              ay[a] -= qi3 * dy;
-              az[a] -= qi3 * dz;
+          for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int a = 0; a < A; ++a) {
+              dx[k][a] = float(xi[a][0] - h[k].x);
+              dy[k][a] = float(xi[a][1] - h[k].y);
+              dz[k][a] = float(xi[a][2] - h[k].z);
+              r2[k][a] = decltype(check)::value
+                             ? dx[k][a] * dx[k][a] + dy[k][a] * dy[k][a] + dz[k][a] * dz[k][a]
+                             : fmaf(dz[k][a], dz[k][a], fmaf(dy[k][a], dy[k][a], fmaf(dx[k][a], dx[k][a], 0.25f)));
+              rmin = fminf(rmin, r2[k][a]);
+            }
+          if (decltype(test_close)::value && __any(rmin < CLOSE)) {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+#pragma unroll
+              for (int a = 0; a < A; ++a)
+                if (r2[k][a] < CLOSE) {
+                  // a close pair: d' = x' - u_s in f64, then scaled and rounded once
+                  const uint32_t jj = js + uint32_t(k);
+                  dx[k][a] = float((fma(rho, u[a][0], c[0]) - sx[jj]) * kP2pFixed);
+                  dy[k][a] = float((fma(rho, u[a][1], c[1]) - sy[jj]) * kP2pFixed);
+                  dz[k][a] = float((fma(rho, u[a][2], c[2]) - sz[jj]) * kP2pFixed);
+                  r2[k][a] = decltype(check)::value
+                                 ? dx[k][a] * dx[k][a] + dy[k][a] * dy[k][a] + dz[k][a] * dz[k][a]
+                                 : fmaf(dz[k][a], dz[k][a], fmaf(dy[k][a], dy[k][a], fmaf(dx[k][a], dx[k][a], TINY)));
+                }
+          }
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            const float q = __int_as_float(h[k].w);
+#pragma unroll
+            for (int a = 0; a < A; ++a) {
+              const float inv = decltype(check)::value ? (r2[k][a] == 0.0f ? 0.0f : __builtin_amdgcn_rsqf(r2[k][a]))
+                                                       : __builtin_amdgcn_rsqf(r2[k][a]);
+              const float qi = q * inv;
+              ap[a] += qi;
+              if (GRAD) {
+                const float qi3 = qi * (inv * inv);
+                ax[a] -= qi3 * dx[k][a];
+                ay[a] -= qi3 * dy[k][a];
+                az[a] -= qi3 * dz[k][a];
+              }
             }
           }
         };
         uint32_t j = jb;
         if (je - j >= 4) {
-          float4 c[8];
+          int4 c[4];
 #pragma unroll
-          for (int k = 0; k < 8; ++k) c[k] = rec[2 * size_t(j) + k];
+          for (int k = 0; k < 4; ++k) c[k] = rec[size_t(j) + k];
           for (; j + 8 <= je; j += 4) {
-            float4 nx[8];
+            int4 nx[4];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) nx[k] = rec[2 * size_t(j + 4) + k];
+            for (int k = 0; k < 4; ++k) nx[k] = rec[size_t(j + 4) + k];
+            body(c, j, std::integral_constant<int, 4>{});
 #pragma unroll
-            for (int k = 0; k < 4; ++k) body(c[2 * k], c[2 * k + 1]);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) c[k] = nx[k];
+            for (int k = 0; k < 4; ++k) c[k] = nx[k];
           }
-#pragma unroll
-          for (int k = 0; k < 4; ++k) body(c[2 * k], c[2 * k + 1]);
+          body(c, j, std::integral_constant<int, 4>{});
           j += 4;
         }
-        for (; j < je; ++j) body(rec[2 * size_t(j)], rec[2 * size_t(j) + 1]);
+        for (; j < je; ++j) {
+          const int4 h1 = rec[j];
+          body(&h1, j, std::integral_constant<int, 1>{});
+        }
       };
       if (own)
-        run(std::true_type{});
+        run(std::true_type{}, std::true_type{});
+      else if (close_entry)
+        run(std::false_type{}, std::true_type{});
       else
-        run(std::false_type{});
-      const double s2 = rho * rho;
+        run(std::false_type{}, std::false_type{});
+      const double s1f = rho * kP2pFixed, s2f = (rho * rho) * (kP2pFixed * kP2pFixed);
 #pragma unroll
       for (int a = 0; a < A; ++a) {
-        cp[a] += double(ap[a]) * rho;
+        cp[a] += double(ap[a]) * s1f;
         if (GRAD) {
-          cx[a] += double(ax[a]) * s2;
-          cy[a] += double(ay[a]) * s2;
-          cz[a] += double(az[a]) * s2;
+          cx[a] += double(ax[a]) * s2f;
+          cy[a] += double(ay[a]) * s2f;
+          cz[a] += double(az[a]) * s2f;
         }
       }
     }
@@ -362,13 +409,15 @@ __device__ __forceinline__ void p2p_uniform(uint32_t lf, uint32_t tb, uint32_t n
 }
 
 template <typename T, bool GRAD>
-__global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, uint32_t count, const uint32_t *__restrict__ near_off,
+__global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, uint32_t count, uint32_t umin,
+                                            const uint32_t *__restrict__ near_off,
                                             const uint32_t *__restrict__ near_ent,
                                             const uint32_t *__restrict__ leaf_cell,
                                             const uint32_t *__restrict__ src_start,
                                             const uint32_t *__restrict__ tgt_start, const double *__restrict__ sx,
                                             const double *__restrict__ sy, const double *__restrict__ sz,
-                                            const double *__restrict__ sq, const float4 *__restrict__ rec,
+                                            const double *__restrict__ sq, const int4 *__restrict__ rec,
+                                            const uint8_t *__restrict__ near_close,
                                             const double *__restrict__ tx, const double *__restrict__ ty,
                                             const double *__restrict__ tz, double *__restrict__ phi,
                                             double *__restrict__ grad) {
@@ -394,11 +443,13 @@ __global__ __launch_bounds__(64) void k_p2p(uint32_t leaf_first, uint32_t count,
   const uint32_t e0 = near_off[lf], e1 = near_off[lf + 1];
   for (uint32_t tb = t0; tb < t1; tb += 128) {
     const uint32_t n = (t1 - tb) < 128u ? (t1 - tb) : 128u;
-    if (SPLIT && n > 32) {
+    if (SPLIT && n >= umin) {
       if (n > 64)
-        p2p_uniform<GRAD, true>(lf, tb, n, e0, e1, near_ent, leaf_cell, src_start, rec, tx, ty, tz, phi, grad);
+        p2p_uniform<GRAD, true>(lf, tb, n, e0, e1, near_ent, leaf_cell, src_start, rec, near_close, sx, sy, sz, tx,
+                                ty, tz, phi, grad);
       else
-        p2p_uniform<GRAD, false>(lf, tb, n, e0, e1, near_ent, leaf_cell, src_start, rec, tx, ty, tz, phi, grad);
+        p2p_uniform<GRAD, false>(lf, tb, n, e0, e1, near_ent, leaf_cell, src_start, rec, near_close, sx, sy, sz, tx,
+                                 ty, tz, phi, grad);
       continue;
     }
     // np = n lanes per phase and floor(64 / n) phases (a chunk of 17..21 targets runs 3
@@ -1020,16 +1071,12 @@ void vjp_combine(hipStream_t s, size_t n, const double *q, const double *g, cons
                      grad_pos);
 }
 
-void build_source_records(const double *sx, const double *sy, const double *sz, size_t n, float *host_rec) {
+void build_source_records(const double *sx, const double *sy, const double *sz, size_t n, int32_t *host_rec) {
   for (size_t j = 0; j < n; ++j) {
     const double u[3] = {sx[j], sy[j], sz[j]};
-    float *r = host_rec + 8 * j;
-    for (int k = 0; k < 3; ++k) {
-      r[k] = float(u[k]);
-      r[4 + k] = float(u[k] - double(r[k]));
-    }
-    r[3] = 0.0f;
-    r[7] = 0.0f;
+    int32_t *r = host_rec + 4 * j;
+    for (int k = 0; k < 3; ++k) r[k] = int32_t(std::lrint(u[k] * kP2pFixed));
+    r[3] = 0;  // the charge's bits, written by gather_charges
   }
 }
 
@@ -1047,28 +1094,86 @@ void output_pass(hipStream_t s, uint32_t nleaves, const uint32_t *tgt_start, con
 }
 
 template <typename T>
-void p2p(hipStream_t s, uint32_t leaf_first, uint32_t count, int gradients, const uint32_t *near_off,
-         const uint32_t *near_ent, const uint32_t *leaf_cell, const uint32_t *src_start,
+void p2p(hipStream_t s, uint32_t leaf_first, uint32_t count, int gradients, uint32_t uniform_min,
+         const uint32_t *near_off, const uint32_t *near_ent, const uint32_t *leaf_cell, const uint32_t *src_start,
          const uint32_t *tgt_start, const double *sx, const double *sy, const double *sz, const double *sq,
-         const float *rec, const double *tx, const double *ty, const double *tz, double *phi, double *grad) {
+         const float *rec, const uint8_t *near_close, const double *tx, const double *ty, const double *tz,
+         double *phi, double *grad) {
   if (count == 0) return;
-  const float4 *r4 = reinterpret_cast<const float4 *>(rec);
+  const int4 *r4 = reinterpret_cast<const int4 *>(rec);
+  const uint32_t umin = uniform_min < 1 ? 1 : uniform_min;
   if (gradients)
-    hipLaunchKernelGGL((k_p2p<T, true>), dim3(xcd_grid(count)), dim3(64), 0, s, leaf_first, count, near_off, near_ent,
-                       leaf_cell, src_start, tgt_start, sx, sy, sz, sq, r4, tx, ty, tz, phi, grad);
+    hipLaunchKernelGGL((k_p2p<T, true>), dim3(xcd_grid(count)), dim3(64), 0, s, leaf_first, count, umin, near_off,
+                       near_ent, leaf_cell, src_start, tgt_start, sx, sy, sz, sq, r4, near_close, tx, ty, tz, phi,
+                       grad);
   else
-    hipLaunchKernelGGL((k_p2p<T, false>), dim3(xcd_grid(count)), dim3(64), 0, s, leaf_first, count, near_off,
-                       near_ent, leaf_cell, src_start, tgt_start, sx, sy, sz, sq, r4, tx, ty, tz, phi, grad);
+    hipLaunchKernelGGL((k_p2p<T, false>), dim3(xcd_grid(count)), dim3(64), 0, s, leaf_first, count, umin, near_off,
+                       near_ent, leaf_cell, src_start, tgt_start, sx, sy, sz, sq, r4, near_close, tx, ty, tz, phi,
+                       grad);
 }
 
-template void p2p<float>(hipStream_t, uint32_t, uint32_t, int, const uint32_t *, const uint32_t *, const uint32_t *,
-                         const uint32_t *, const uint32_t *, const double *, const double *, const double *,
-                         const double *, const float *, const double *, const double *, const double *, double *,
-                         double *);
-template void p2p<double>(hipStream_t, uint32_t, uint32_t, int, const uint32_t *, const uint32_t *,
+template void p2p<float>(hipStream_t, uint32_t, uint32_t, int, uint32_t, const uint32_t *, const uint32_t *,
+                         const uint32_t *, const uint32_t *, const uint32_t *, const double *, const double *,
+                         const double *, const double *, const float *, const uint8_t *, const double *,
+                         const double *, const double *, double *, double *);
+template void p2p<double>(hipStream_t, uint32_t, uint32_t, int, uint32_t, const uint32_t *, const uint32_t *,
                           const uint32_t *, const uint32_t *, const uint32_t *, const double *, const double *,
-                          const double *, const double *, const float *, const double *, const double *,
-                          const double *, double *, double *);
+                          const double *, const double *, const float *, const uint8_t *, const double *,
+                          const double *, const double *, double *, double *);
+
+// Near entries with a close pair for the mixed P2P's fixed-point form: one wave per
+// target leaf; per entry the smallest r^2 over the leaf's targets and the entry's
+// sources in the integer units the uniform-source form computes (the same rounding of
+// x' and the same quarter-unit seed), flag = r^2 < kP2pClose.  The own leaf is always
+// flagged (its coincident points and quantised-equal pairs need the test).
+__global__ __launch_bounds__(64) void k_p2p_close_flags(uint32_t count, const uint32_t *__restrict__ near_off,
+                                                        const uint32_t *__restrict__ near_ent,
+                                                        const uint32_t *__restrict__ leaf_cell,
+                                                        const uint32_t *__restrict__ src_start,
+                                                        const uint32_t *__restrict__ tgt_start,
+                                                        const int4 *__restrict__ rec, const double *__restrict__ tx,
+                                                        const double *__restrict__ ty, const double *__restrict__ tz,
+                                                        uint8_t *__restrict__ flags) {
+  const uint32_t lf = blockIdx.x;
+  if (lf >= count) return;
+  const int lane = threadIdx.x;
+  const uint32_t t0 = tgt_start[lf], t1 = tgt_start[lf + 1];
+  const uint32_t *ct = leaf_cell + 4 * size_t(lf);
+  for (uint32_t e = near_off[lf]; e < near_off[lf + 1]; ++e) {
+    const uint32_t sl = near_ent[e];
+    if (sl == lf) {
+      if (lane == 0) flags[e] = 1;
+      continue;
+    }
+    const uint32_t s0 = src_start[sl], s1 = src_start[sl + 1];
+    double c[3], rho;
+    relative_frame<double>(ct, leaf_cell + 4 * size_t(sl), c, rho);
+    float rmin = kP2pClose;
+    for (uint32_t tb = t0; tb < t1; tb += 64) {
+      const uint32_t i = tb + uint32_t(lane);
+      if (i >= t1) break;
+      const int xi0 = __double2int_rn(fma(rho, tx[i], c[0]) * kP2pFixed);
+      const int xi1 = __double2int_rn(fma(rho, ty[i], c[1]) * kP2pFixed);
+      const int xi2 = __double2int_rn(fma(rho, tz[i], c[2]) * kP2pFixed);
+      for (uint32_t j = s0; j < s1; ++j) {
+        const int4 h = rec[j];
+        const float dx = float(xi0 - h.x), dy = float(xi1 - h.y), dz = float(xi2 - h.z);
+        rmin = fminf(rmin, fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, 0.25f))));
+      }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) rmin = fminf(rmin, __shfl_xor(rmin, off));
+    if (lane == 0) flags[e] = rmin < kP2pClose ? 1 : 0;
+  }
+}
+
+void p2p_close_flags(hipStream_t s, uint32_t nleaves, const uint32_t *near_off, const uint32_t *near_ent,
+                     const uint32_t *leaf_cell, const uint32_t *src_start, const uint32_t *tgt_start,
+                     const float *rec, const double *tx, const double *ty, const double *tz, uint8_t *flags) {
+  if (nleaves == 0) return;
+  hipLaunchKernelGGL(k_p2p_close_flags, dim3(nleaves), dim3(64), 0, s, nleaves, near_off, near_ent, leaf_cell,
+                     src_start, tgt_start, reinterpret_cast<const int4 *>(rec), tx, ty, tz, flags);
+}
 
 void expand(hipStream_t s, int p, int irregular, uint32_t rows, const uint32_t *row_box, const uint32_t *row_cell,
             const uint32_t *row_off, const uint32_t *row_ent, const uint32_t *leaf_cell,

@@ -50,10 +50,15 @@ void fill(hipStream_t s, double *p, size_t n, double v);
 // q64[j] = q[order[j]]; with rec set, also rec[8 j + 3] = f32 of it (the charge slot of
 // the mixed-precision P2P source records)
 void gather_charges(hipStream_t s, const double *q, const uint32_t *order, size_t n, double *q64, float *rec);
-// Mixed-precision P2P source records, 8 floats per source in leaf order: (x_hi, y_hi,
-// z_hi, q, x_lo, y_lo, z_lo, 0) of the source's own-leaf coordinates u = hi + lo.  The
-// charge slot is written by gather_charges.
-void build_source_records(const double *sx, const double *sy, const double *sz, size_t n, float *host_rec);
+// Fixed-point scale of the mixed P2P's source records: 2^28 per leaf half-width.
+constexpr double kP2pFixed = 268435456.0;
+// r^2 (integer units) below which a pair's quantised separation would cost more than 1e-7
+// of its terms (|d| < 2^23.5 units, 0.044 half-widths): 2^47
+constexpr float kP2pClose = 140737488355328.0f;
+// Mixed-precision P2P source records, 4 words per source in leaf order: the source's
+// own-leaf coordinates u (|u| <= 1) as round(u 2^28) in int32, then the charge's f32
+// bits, which gather_charges writes.
+void build_source_records(const double *sx, const double *sy, const double *sz, size_t n, int32_t *host_rec);
 // Rotation M2L (rotation.hip): blocks of at most rotation_pairs_per_block pairs of one
 // offset (blk_offset), pairs blk_first[b] .. blk_first[b + 1] with source boxes pair_src;
 // tables per offset (fwd, bwd: rb values) and per distance (coax: cb values); pair q's
@@ -82,14 +87,23 @@ void output_pass(hipStream_t s, uint32_t nleaves, const uint32_t *tgt_start, con
                  int gradients);
 
 // P2P over the leaves leaf_first .. leaf_first + count (global leaf indices), computed in
-// T (double, or float with double-float separations); f64 coordinates and charges.  T =
-// float also reads the source records of build_source_records (rec is unused for T =
-// double and may be null).
+// T (double, or float with fixed-point or double-float separations); f64 coordinates and
+// charges.  T = float also reads the source records of build_source_records (4 words per
+// source; rec is unused for T = double and may be null).  uniform_min: for T = float,
+// chunks of at least this many targets (and at most 128) take the uniform-source form;
+// near_close: per near entry 1 where p2p_close_flags found a close pair (T = float; may
+// be null for T = double).
 template <typename T>
-void p2p(hipStream_t s, uint32_t leaf_first, uint32_t count, int gradients, const uint32_t *near_off,
-         const uint32_t *near_ent, const uint32_t *leaf_cell, const uint32_t *src_start,
+void p2p(hipStream_t s, uint32_t leaf_first, uint32_t count, int gradients, uint32_t uniform_min,
+         const uint32_t *near_off, const uint32_t *near_ent, const uint32_t *leaf_cell, const uint32_t *src_start,
          const uint32_t *tgt_start, const double *sx, const double *sy, const double *sz, const double *sq,
-         const float *rec, const double *tx, const double *ty, const double *tz, double *phi, double *grad);
+         const float *rec, const uint8_t *near_close, const double *tx, const double *ty, const double *tz,
+         double *phi, double *grad);
+// Flags the near entries (global CSR over all leaves) that hold a pair closer than
+// kP2pClose in the mixed P2P's integer units, and every own-leaf entry.
+void p2p_close_flags(hipStream_t s, uint32_t nleaves, const uint32_t *near_off, const uint32_t *near_ent,
+                     const uint32_t *leaf_cell, const uint32_t *src_start, const uint32_t *tgt_start,
+                     const float *rec, const double *tx, const double *ty, const double *tz, uint8_t *flags);
 
 // P2M (irregular = 0, row_ent the leaf of each row, unit frame) or P2L (irregular = 1,
 // row_ent the X-list leaves).  rows: output box gid and cell per row, CSR of entries.
@@ -120,14 +134,17 @@ void contract_f64(hipStream_t s, int p, int width, uint32_t ntasks, const uint32
 // M2M tasks of 16 parent columns whose groups are the child octants' tables: wave w
 // takes groups w, w + 8, ..., the eight waves' sums added in a fixed tree
 // (contract.hip k_m2m_split; near the root one workgroup per task and M-tile,
-// k_m2m_rows, which reads up to 15 doubles past a source's row: X must stay readable
-// 15 doubles past its last expansion).  Y may be X.
+// k_m2m_rows).  Y may be X.  form: 0 the per-tile form when fewer (task, M-tile) pairs
+// than CUs, else the split form; 1 the per-tile form; 2 the split form.
 void m2m_split(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off, const uint16_t *group_table,
-               const uint32_t *group_src, const uint32_t *col_out, const double *tsw, const double *X, double *Y);
+               const uint32_t *group_src, const uint32_t *col_out, const double *tsw, const double *X, double *Y,
+               int form = 0);
+// stage: 0 or 1 one k-step per pipeline stage (two for the short tables of p <= 7); 2 two
+// k-steps per stage in a ring of two buffers where the table allows it (an autotuned shape)
 void contract_bf16(hipStream_t s, int p, uint32_t ntasks, const uint32_t *task_group_off,
                    const uint16_t *group_table, const uint32_t *group_src, const uint32_t *col_out,
                    const uint16_t *tables_swizzled, const uint16_t *X16, double *Y, uint32_t first_slot, float *S,
-                   uint32_t xchunk);
+                   uint32_t xchunk, int stage = 0);
 // X16: three bf16 planes (hi, mid, lo) of split_expansion_size(p) / 3 values per box
 void split_bf16(hipStream_t s, int p, size_t nboxes, const double *X, uint16_t *X16);
 // Y[row_box[r]] += sum of the scratch rows row_ent[row_off[r] .. row_off[r+1]) - first_slot

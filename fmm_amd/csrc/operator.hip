@@ -113,9 +113,36 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
   q_.alloc(ns_);
   q_in_.alloc(ns_);
   if (opt_.precision == kMixed && ns_ > 0) {
-    std::vector<float> rec(8 * size_t(ns_));
-    dev::build_source_records(sx.data(), sy.data(), sz.data(), size_t(ns_), rec.data());
+    std::vector<int32_t> irec(4 * size_t(ns_));
+    dev::build_source_records(sx.data(), sy.data(), sz.data(), size_t(ns_), irec.data());
+    std::vector<float> rec(irec.size());
+    std::memcpy(rec.data(), irec.data(), irec.size() * sizeof(int32_t));
     src_rec_.upload(rec);
+    // the fixed-point separations assume |x'| <= 7 half-widths for every near pair (2:1
+    // balance), in either leaf's frame; check the plan's near rows once
+    for (int l = 0; l < plan.nlevels; ++l) {
+      const Csr &nr = plan.near[l];
+      for (uint32_t r = 0; r + 1 < uint32_t(nr.off.size()); ++r) {
+        const Cell t = decode(plan.leaf_key[plan.leaf_off[l] + r]);
+        for (uint32_t e = nr.off[r]; e < nr.off[r + 1]; ++e) {
+          const Cell c = decode(plan.leaf_key[nr.ent[e]]);
+          const double ct[3] = {double(t.i), double(t.j), double(t.k)}, cs[3] = {double(c.i), double(c.j), double(c.k)};
+          // max over the axes of |c''| + rho: the targets in the source leaf's half-widths
+          // (the uniform-source form) and the sources in the target leaf's (the staged form)
+          double reach = 0.0;
+          for (int k = 0; k < 3; ++k) {
+            const double dt = (2.0 * ct[k] + 1.0) * std::ldexp(1.0, -int(t.level));
+            const double ds = (2.0 * cs[k] + 1.0) * std::ldexp(1.0, -int(c.level));
+            reach = std::max(reach, std::fabs(dt - ds) * std::ldexp(1.0, int(c.level)) +
+                                        std::ldexp(1.0, int(c.level) - int(t.level)));
+            reach = std::max(reach, std::fabs(dt - ds) * std::ldexp(1.0, int(t.level)) +
+                                        std::ldexp(1.0, int(t.level) - int(c.level)));
+          }
+          require(reach <= 7.0, "P2P: a near pair spans more than 7 half-widths (the tree is not 2:1 balanced)",
+                  kUnsupportedLayout);
+        }
+      }
+    }
   }
 
   // expansions and outputs (the clustered-mode scratch rows are sized in build_schedules)
@@ -223,6 +250,13 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
   else
     slots_.alloc(nslots_ * nc_);
   if (opt_.precision == kMixed) mult16_.alloc(nbox_total_ * dev::split_expansion_size(opt_.p));
+  if (opt_.precision == kMixed && near_ent_.count > 0) {
+    // the near entries whose pairs the fixed-point P2P must test (one pass at build)
+    near_close_.alloc(near_ent_.count);
+    dev::p2p_close_flags(stream_, nleaves_, near_off_.ptr, near_ent_.ptr, leaf_cell_.ptr, src_start_.ptr,
+                         tgt_start_.ptr, src_rec_.ptr, tx_.ptr, ty_.ptr, tz_.ptr, near_close_.ptr);
+    hip_check(hipStreamSynchronize(stream_), "P2P close-pair flags");
+  }
   hip_check(hipDeviceSynchronize(), "operator build");
 }
 
@@ -342,7 +376,9 @@ void DeviceOperator::build_schedules(const Plan &plan) {
       // 16 columns per task (waves split the rows) on small levels, 64 while 128-column
       // tasks would fill fewer than four rounds of the GPU, else 128
       const uint32_t nb = plan.nboxes(l);
-      const int width = nb < 64u * dev::COLS ? 16 : (nb < 4u * 512u * dev::COLS ? 64 : dev::COLS);
+      const int tuned = l < kTuneLevels ? opt_.tuning.l2l_width[l] : 0;
+      require(tuned == 0 || tuned == 16 || tuned == 64 || tuned == dev::COLS, "tuning: L2L width 16, 64 or 128");
+      const int width = tuned ? tuned : nb < 64u * dev::COLS ? 16 : (nb < 4u * 512u * dev::COLS ? 64 : dev::COLS);
       ScheduleBuilder sb(width);
       const int C = width;
       for (int o = 0; o < 8; ++o) {
@@ -409,7 +445,12 @@ void DeviceOperator::build_schedules(const Plan &plan) {
       // all of its ~189 groups in sequence, so a level of a few dozen tasks would be one
       // long chain per CU.
       constexpr uint32_t kMinBoxTasks = 256;
-      if (tile.pairs == 0 || (2 * tile.pairs >= tile.slots && tile.ntasks() >= kMinBoxTasks)) {
+      const int tuned = l < kTuneLevels ? opt_.tuning.m2l_mode[l] : 0;
+      require(tuned >= 0 && tuned <= 2, "tuning: M2L mode 0, 1 or 2");
+      const bool box = tile.pairs == 0 || (tuned ? tuned == 1
+                                                 : (2 * tile.pairs >= tile.slots && tile.ntasks() >= kMinBoxTasks));
+      if (l < kTuneLevels) m2l_mode_taken_[l] = tile.pairs == 0 ? 0 : box ? 1 : 2;
+      if (box) {
         all.append(tile);
       } else {
         // Sparse level (few of a target's possible offsets present), or a small one: clustered mode.
@@ -657,7 +698,8 @@ void DeviceOperator::launch_m2l(int level_first, int level_last) {
       timed(kKindM2lContract, [&] {
         dev::contract_bf16(stream_, opt_.p, t1 - t0, m2l_.task_group_off.ptr + t0, m2l_.group_table.ptr,
                            m2l_.group_src.ptr, m2l_.col_out.ptr + size_t(t0) * dev::COLS, t_m2l16_.ptr,
-                           mult16_.ptr, loc_.ptr, static_cast<uint32_t>(nbox_total_), slots32_.ptr, kM2lXcdRun);
+                           mult16_.ptr, loc_.ptr, static_cast<uint32_t>(nbox_total_), slots32_.ptr, kM2lXcdRun,
+                           opt_.tuning.m2l_stage);
       });
     } else {
       timed(kKindM2lContract, [&] { launch_contract(m2l_, t0, t1 - t0, mult_.ptr, loc_.ptr, t_m2l_, kM2lXcdRun); });
@@ -708,7 +750,8 @@ void DeviceOperator::m2m(int level) {
   timed(1, [&] {
     const Schedule &sc = m2m_[level];
     dev::m2m_split(stream_, opt_.p, sc.ntasks, sc.task_group_off.ptr, sc.group_table.ptr, sc.group_src.ptr,
-                   sc.col_out.ptr, t_m2m_.ptr, mult_.ptr, mult_.ptr);
+                   sc.col_out.ptr, t_m2m_.ptr, mult_.ptr, mult_.ptr,
+                   level < kTuneLevels ? opt_.tuning.m2m_form[level] : 0);
   });
 }
 
@@ -752,14 +795,15 @@ void DeviceOperator::m2p(int level) {
 }
 
 void DeviceOperator::launch_p2p(uint32_t leaf_first, uint32_t count) {
+  const uint32_t umin = uint32_t(p2p_uniform_min());
   if (opt_.precision == kMixed)
-    dev::p2p<float>(stream_, leaf_first, count, opt_.gradients, near_off_.ptr, near_ent_.ptr, leaf_cell_.ptr,
-                    src_start_.ptr, tgt_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, q_.ptr, src_rec_.ptr, tx_.ptr,
-                    ty_.ptr, tz_.ptr, phi_hat_.ptr, grad_hat_.ptr);
+    dev::p2p<float>(stream_, leaf_first, count, opt_.gradients, umin, near_off_.ptr, near_ent_.ptr, leaf_cell_.ptr,
+                    src_start_.ptr, tgt_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, q_.ptr, src_rec_.ptr,
+                    near_close_.ptr, tx_.ptr, ty_.ptr, tz_.ptr, phi_hat_.ptr, grad_hat_.ptr);
   else
-    dev::p2p<double>(stream_, leaf_first, count, opt_.gradients, near_off_.ptr, near_ent_.ptr, leaf_cell_.ptr,
-                     src_start_.ptr, tgt_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, q_.ptr, nullptr, tx_.ptr, ty_.ptr,
-                     tz_.ptr, phi_hat_.ptr, grad_hat_.ptr);
+    dev::p2p<double>(stream_, leaf_first, count, opt_.gradients, umin, near_off_.ptr, near_ent_.ptr,
+                     leaf_cell_.ptr, src_start_.ptr, tgt_start_.ptr, sx_.ptr, sy_.ptr, sz_.ptr, q_.ptr, nullptr,
+                     nullptr, tx_.ptr, ty_.ptr, tz_.ptr, phi_hat_.ptr, grad_hat_.ptr);
 }
 
 void DeviceOperator::p2p(int level) {

@@ -93,12 +93,26 @@ enum M2lMethod : int {
   kM2lPlaneWave = 2, // plane waves in six directions (f64; approximate to the quadrature's tolerance)
 };
 
+constexpr int kTuneLevels = 17;  // levels 0..16
+
+// Launch-shape decisions an autotuner may take (tune.hip); 0 everywhere is the static rule.
+struct Tuning {
+  int p2p_uniform_min = 0;                  // mixed P2P: chunks of at least this many targets read
+                                            // sources by uniform loads (0: 33)
+  int m2l_stage = 0;                        // bf16x3 M2L: 0/1 one k-step per pipeline stage,
+                                            // 2 two k-steps in a ring of two buffers
+  std::array<int, kTuneLevels> m2l_mode{};  // per level: 0 static, 1 box mode, 2 clustered mode
+  std::array<int, kTuneLevels> l2l_width{}; // per level: 0 static, else 16, 64 or 128 columns per task
+  std::array<int, kTuneLevels> m2m_form{};  // per parent level: 0 static, 1 per-tile, 2 split
+};
+
 struct OperatorOptions {
   int p = 6;
   int precision = kPlainF64;
   int gradients = 1;
   int device = 0;
   int m2l = kM2lDense;
+  Tuning tuning;
 };
 
 class DeviceOperator {
@@ -142,6 +156,14 @@ class DeviceOperator {
   // per-kind device times (HIP events on this operator's stream)
   void set_timing(bool enable);
   void kind_times(double *ms, int64_t *launches);
+  // the mixed P2P's uniform-source threshold (targets per chunk), applied from the next launch
+  void set_p2p_uniform_min(int n) { opt_.tuning.p2p_uniform_min = n; }
+  // the bf16x3 M2L's pipeline stage shape (Tuning::m2l_stage), from the next launch
+  void set_m2l_stage(int stage) { opt_.tuning.m2l_stage = stage; }
+  int p2p_uniform_min() const { return opt_.tuning.p2p_uniform_min > 0 ? opt_.tuning.p2p_uniform_min : 33; }
+  // the M2L mode each level's schedule took: 0 none, 1 box mode, 2 clustered mode
+  int m2l_mode_of(int level) const { return level >= 0 && level < kTuneLevels ? m2l_mode_taken_[level] : 0; }
+  int l2l_width_of(int level) const { return level >= 0 && level < int(l2l_.size()) ? l2l_[level].width : 0; }
 
   int nlevels() const { return nlevels_; }
   int64_t nsources() const { return ns_; }
@@ -214,7 +236,8 @@ class DeviceOperator {
   size_t nslots_ = 0;                  // clustered-mode scratch rows (slot ids follow the box ids)
   DevBuf<double> slots_;               // plain f64: scratch rows
   DevBuf<float> slots32_;              // mixed: scratch rows (one group's f32 sum each)
-  DevBuf<float> src_rec_;              // mixed: P2P source records, 8 floats per source
+  DevBuf<float> src_rec_;              // mixed: P2P source records, 4 words per source
+  DevBuf<uint8_t> near_close_;         // mixed: near entries with a close pair (p2p_close_flags)
 
   bool timing_ = false;
   struct Mark {
@@ -233,6 +256,7 @@ class DeviceOperator {
   std::vector<uint32_t> p2l_level_;
   std::vector<Schedule> m2m_, l2l_;                          // per level
   Schedule m2l_;                                             // every level, by level
+  std::array<int, kTuneLevels> m2l_mode_taken_{};
   std::vector<uint32_t> m2l_level_task_;                     // task range per level
   Rows m2l_red_;                                             // clustered-mode reductions
   std::vector<uint32_t> m2l_red_level_;

@@ -140,9 +140,32 @@ int32_t fmmh_tables(int32_t p, double *m2m, double *l2l, double *m2l);
  * (p+1)^2 x (p+1)^2 matrix: equal to the dense table to rounding. */
 int32_t fmmh_m2l_rotation_table(int32_t p, int32_t offset, double *table);
 
+/* Launch-shape decisions of an operator (0 everywhere: the static rule), as the
+ * autotuner takes them.  Per level index 0..16. */
+typedef struct {
+  int32_t p2p_uniform_min; /* mixed P2P: chunks of at least this many targets read sources by
+                              uniform loads, smaller ones through LDS tiles (0: 33) */
+  int32_t m2l_stage;       /* mixed dense M2L: 0 or 1 one k-step per pipeline stage, 2 two k-steps
+                              in a ring of two LDS buffers (where the table fits) */
+  int32_t m2l_mode[17];    /* dense M2L of the level: 0 static rule, 1 box mode, 2 clustered */
+  int32_t l2l_width[17];   /* L2L into the level: 0 static rule, or 16, 64, 128 columns per task */
+  int32_t m2m_form[17];    /* M2M into the parents of the level: 0 static, 1 per-tile, 2 split */
+} fmmh_tuning;
+
 /* ---- device operator -------------------------------------------------------------- */
 int32_t fmmh_device_count(int32_t *count);
 int32_t fmmh_operator_create(const fmmh_plan *plan, const fmmh_operator_options *options, fmmh_operator **out);
+/* fmmh_operator_create with launch-shape decisions (fmmh_autotune's, or a cache's). */
+int32_t fmmh_operator_create_tuned(const fmmh_plan *plan, const fmmh_operator_options *options,
+                                   const fmmh_tuning *tuning, fmmh_operator **out);
+/* The autotune of the device path (fmm-exec/src/tune.rs:1-110): times each decision's
+ * candidates on operators built over the plan (the static rule's candidate first), on its
+ * own levels, within budget_seconds, and writes the winners (0 where the static rule was
+ * kept).  report (may be NULL): 5 x 17 x 5 doubles, ms per call of every candidate timed
+ * ([kind: P2P, M2L, L2L, M2M, M2L stage][level][candidate]), NaN where not timed.  The cache file of
+ * decisions is the caller's (fmm_amd/tune.py). */
+int32_t fmmh_autotune(const fmmh_plan *plan, const fmmh_operator_options *options, double budget_seconds,
+                      fmmh_tuning *out, double *report);
 void fmmh_operator_free(fmmh_operator *op);
 int32_t fmmh_operator_stats(const fmmh_operator *op, fmmh_stats *out);
 /* Charges of the sources in caller order, on the host (on_device = 0) or on the GPU;

@@ -527,8 +527,10 @@ class Fmm:
         lib().orc_fmm_leaf_keys(self._h, out)
         return out
 
-    def list(self, kind: str, level: int):
-        """CSR (offsets, entries[, offset indices]) of 'near', 'v', 'w' or 'x' rows."""
+    def list(self, kind: str, level: int, aux: bool = False):
+        """CSR (offsets, entries[, aux]) of 'near', 'v', 'w' or 'x' rows; aux (always for
+        V, on request for W): the offset index of each V entry, the level of each W
+        entry's box."""
         k = {"near": 0, "v": 1, "w": 2, "x": 3}[kind]
         total = lib().orc_fmm_list(self._h, k, level, None, None, None)
         if kind in ("near", "w"):
@@ -539,10 +541,10 @@ class Fmm:
             rows = len(self.box_keys(level))
         off = np.zeros(rows + 1, np.int64)
         ent = np.zeros(max(total, 1), np.int64)
-        aux = np.zeros(max(total, 1), np.int64)
-        lib().orc_fmm_list(self._h, k, level, _ptr(off), _ptr(ent), _ptr(aux))
-        if kind == "v":
-            return off, ent[:total], aux[:total]
+        ax = np.zeros(max(total, 1), np.int64)
+        lib().orc_fmm_list(self._h, k, level, _ptr(off), _ptr(ent), _ptr(ax))
+        if kind == "v" or (kind == "w" and aux):
+            return off, ent[:total], ax[:total]
         return off, ent[:total]
 
     def leaf_coordinates(self, targets=False):

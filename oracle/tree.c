@@ -1001,7 +1001,7 @@ int64_t orc_fmm_list(const orc_fmm *f, int kind, int level, int64_t *offsets, in
   int64_t total = c->off[rows];
   if (offsets) memcpy(offsets, c->off, sizeof(int64_t) * (rows + 1));
   if (entries) memcpy(entries, c->ent, sizeof(int64_t) * total);
-  if (aux && kind == 1) memcpy(aux, c->aux, sizeof(int64_t) * total);
+  if (aux && (kind == 1 || kind == 2)) memcpy(aux, c->aux, sizeof(int64_t) * total);
   return total;
 }
 

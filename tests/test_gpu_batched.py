@@ -111,12 +111,17 @@ def test_forest_adjoint(_gpu, small_forest, precision):
 
 
 @pytest.mark.gpu
-def test_forest_adjoint_finite_differences(_gpu, small_forest):
+@pytest.mark.parametrize("degree,bar", [(6, 5e-5), (10, 1e-5)])
+def test_forest_adjoint_finite_differences(_gpu, small_forest, degree, bar):
     # central differences (h = 1e-4) of the f64 direct-sum loss L = sum(phi^2) / N over the
     # whole forest; the clouds do not interact, so a coordinate of cloud c moves only
-    # cloud c's terms, and the FD of L is that of cloud c's part (N the forest's count)
+    # cloud c's terms, and the FD of L is that of cloud c's part (N the forest's count).
+    # The adjoint is one more FMM evaluation, so it carries the FMM's truncation: at the
+    # batched workload's p = 6 (relative L2 error ~1e-4) the worst component differs from
+    # the FD by a few 1e-5 of the largest; at p = 10 by less than 1e-5
     clouds, charges, first, oracles, ref = small_forest
-    forest = _forest(clouds, "mixed")
+    b = fmm_amd.FmmBuilder(degree).precision("mixed").max_points_per_leaf(LEAF).near_rule("device")
+    forest = b.build_batched(clouds)
     q = np.concatenate(charges)
     ntot = len(q)
     out = forest.evaluate(q)
@@ -145,8 +150,9 @@ def test_forest_adjoint_finite_differences(_gpu, small_forest):
             zm[j, k] -= h
             fd = (loss(zp, qc) - loss(zm, qc)) / (2 * h)
             worst = max(worst, abs(fd - gzc[j, k]) / scale_z)
-    print(f"forest adjoint vs finite differences (cloud {c}, n={len(z)}, p = {p}): worst {worst:.3e}", flush=True)
-    assert worst <= 1e-5
+    print(f"forest adjoint vs finite differences (cloud {c}, n={len(z)}, p = {degree}): worst {worst:.3e}",
+          flush=True)
+    assert worst <= bar
 
 
 @pytest.mark.gpu

@@ -207,3 +207,85 @@ def test_mixed_p2p_uniform_chunks(leaf):
     e1, e2 = _rel(out.potential, phi), _rel(out.gradient, grad)
     _log(f"  largest leaf {sizes.max()}: rel phi {e1:.3e} grad {e2:.3e}")
     assert e1 < 1e-6 and e2 < 1e-6
+
+
+def _frame(s_key, t_key):
+    """c(s|t), r(s|t): the source box's centre and radius in the target leaf's units, exact
+    from the integer cells (CONVENTIONS §3.13)."""
+    ls, *cs = O.decode(int(s_key))
+    lt, *ct = O.decode(int(t_key))
+    L = max(ls, lt)
+    c = [((2 * cs[k] + 1) * 2 ** (L - ls) - (2 * ct[k] + 1) * 2 ** (L - lt)) * 2.0 ** (lt - L) for k in range(3)]
+    return np.array(c), 2.0 ** (lt - ls)
+
+
+def _kind_alone_reference(ref, kind):
+    """L2P alone (each leaf's local at its targets) or M2P alone (each leaf's W-list
+    multipoles at its targets) from the oracle's own expansions and leaf operators,
+    divided by 4 pi r_t (r_t^2 for the gradient), in caller order."""
+    p = ref.p
+    lk = ref.leaf_keys()
+    levels = (lk & np.uint64(0x7FFF)).astype(int)
+    u, order = ref.leaf_coordinates(targets=True)
+    pl = ref.point_leaves(targets=True)
+    start = np.concatenate([[0], np.cumsum(np.bincount(pl, minlength=len(lk)))])
+    box, side = ref.domain()
+    phi = np.zeros(len(order))
+    grad = np.zeros((len(order), 3))
+    offsets = np.concatenate([[0], np.cumsum(np.bincount(levels, minlength=ref.nlevels))])
+    keys = [ref.box_keys(l) for l in range(ref.nlevels)]
+    for l in range(ref.nlevels):
+        rt = side / 2.0 ** (l + 1)
+        if kind == "l2p":
+            loc = ref.expansions(True, l)
+        else:
+            woff, went, wlev = ref.list("w", l, aux=True)  # W entries: box index and its level
+        for r in range(offsets[l + 1] - offsets[l]):
+            lf = offsets[l] + r
+            t0, t1 = start[lf], start[lf + 1]
+            if t0 == t1:
+                continue
+            tgt = u[t0:t1]
+            f = np.zeros(t1 - t0)
+            g = np.zeros((t1 - t0, 3))
+            if kind == "l2p":
+                b = int(np.searchsorted(keys[l], lk[lf]))
+                f, g = O.l2p(p, (0.0, 0.0, 0.0), 1.0, loc[b], tgt)
+            else:
+                for e in range(woff[r], woff[r + 1]):
+                    sb = int(went[e])
+                    sl = int(wlev[e])
+                    c, rr = _frame(keys[sl][sb], lk[lf])
+                    df, dg = O.m2p(p, c, rr, ref.expansions(False, sl)[sb], tgt)
+                    f = f + df
+                    g = g + dg
+            phi[order[t0:t1]] = f / (4 * np.pi * rt)
+            grad[order[t0:t1]] = g / (4 * np.pi * rt * rt)
+    return phi, grad
+
+
+@pytest.mark.parametrize("kind", ["l2p", "m2p"])
+def test_leaf_kind_alone_against_oracle_operators(setup, kind):
+    # the upward and downward passes through the level calls (multipoles and locals equal
+    # the oracle's to 1e-13), then only the L2P or only the M2P level calls: the output is
+    # that kind alone, compared with the oracle's leaf operator on the oracle's expansions
+    pts, q, p, leaf, fmm, ref = setup
+    op = fmm.operator
+    L = fmm.plan.nlevels - 1
+    op.load_charges(q)
+    op.reset()
+    for level in range(L, -1, -1):
+        op.call("p2m", level)
+        if level >= 1:
+            op.call("m2m", level - 1)
+    for level in range(1, L + 1):
+        for k in ("l2l", "m2l", "p2l"):
+            op.call(k, level)
+    for level in range(L + 1):
+        op.call(kind, level)
+    out = op.read_output()
+    rphi, rgrad = _kind_alone_reference(ref, kind)
+    assert np.linalg.norm(rphi) > 0, "the tree must exercise this kind"
+    e1, e2 = _rel(out.potential, rphi), _rel(out.gradient, rgrad)
+    _log(f"{kind} alone vs oracle operators: rel phi {e1:.3e} grad {e2:.3e}")
+    assert e1 < 1e-13 and e2 < 1e-13

@@ -202,3 +202,24 @@ def test_graph_replays_equal_eager(precision):
         assert np.array_equal(out.gradient, eager.gradient), rep
         other = fmm.evaluate(q2)  # eager work between replays
         assert np.all(np.isfinite(other.potential))
+
+
+@pytest.mark.parametrize("precision,bar", [("f64", 1e-12), ("mixed", 1e-6)])
+def test_graph_replay_matches_oracle(precision, bar):
+    # the replayed graph's output against the oracle FMM on the same tree directly, with
+    # new charges loaded before each replay (the graph reads the charges buffer, it does
+    # not capture the values)
+    _log(f"case graph replay vs oracle {precision}")
+    pts = _points("mixed", 9000, 64)
+    fmm = fmm_amd.FmmBuilder(8).precision(precision).max_points_per_leaf(32).build(pts)
+    ref = O.Fmm(pts, p=8, leaf_points=32)
+    op = fmm.operator
+    for seed in (65, 66):
+        q = O.Rng(seed).normals(len(pts))
+        op.load_charges(q)
+        op.evaluate_graph()
+        out = op.read_output()
+        rphi, rgrad = ref.evaluate(q)
+        e1, e2 = _rel(out.potential, rphi), _rel(out.gradient, rgrad, True)
+        _log(f"  graph {precision} seed {seed}: vs oracle phi {e1:.2e} grad {e2:.2e}")
+        assert e1 < bar and e2 < bar

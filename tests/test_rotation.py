@@ -69,14 +69,18 @@ def test_rotation_m2l_equals_dense(_gpu, precision, bar):
 
 
 @pytest.mark.gpu
-def test_rotation_fmm_matches_oracle(_gpu):
+@pytest.mark.parametrize("precision,bar", [("f64", 1e-12), ("mixed", 1e-6)])
+def test_rotation_fmm_matches_oracle(_gpu, precision, bar):
+    # the whole FMM with the rotation M2L (f32 rotation blocks in the mixed precision)
+    # against the oracle FMM (dense f64 M2L) on the same tree, with the near rule on
     rng = O.Rng(72)
     pts = np.vstack([rng.sphere(6000, (0.5, 0.5, 0.5), 0.45), rng.plummer(4000, (0.5, 0.5, 0.5), 0.1)])
     q = rng.charges(len(pts))
-    f = fmm_amd.FmmBuilder(8).m2l("rotation").max_points_per_leaf(32).build(pts)
+    near = (0.5 * 81,) * 2
+    f = fmm_amd.FmmBuilder(8).precision(precision).m2l("rotation").max_points_per_leaf(32).near_rule(near).build(pts)
     out = f.evaluate(q)
-    ref = O.Fmm(pts, p=8, leaf_points=32)
+    ref = O.Fmm(pts, p=8, leaf_points=32, near_rule=near)
     rphi, rgrad = ref.evaluate(q)
     e1, e2 = _rel(out.potential, rphi), _rel(out.gradient, rgrad)
-    print(f"rotation fmm vs oracle: phi {e1:.3e} grad {e2:.3e}", flush=True)
-    assert e1 < 1e-12 and e2 < 1e-12
+    print(f"rotation fmm {precision} vs oracle: phi {e1:.3e} grad {e2:.3e}", flush=True)
+    assert e1 < bar and e2 < bar
