@@ -42,7 +42,7 @@ EXPORTS = [
     "fmmh_helmholtz_read_output", "fmmh_helmholtz_read_expansions", "fmmh_helmholtz_kind_timing",
     "fmmh_helmholtz_kind_times", "fmmh_helmholtz_table", "fmmh_helmholtz_direct_sum",
     "fmmh_plane_wave_quadrature", "fmmh_plane_wave_m2l", "fmmh_stream_wait", "fmmh_helmholtz_stream_wait",
-    "fmmh_operator_create_tuned", "fmmh_autotune",
+    "fmmh_direct_sum_noalloc", "fmmh_operator_create_tuned", "fmmh_autotune",
 ]
 
 
@@ -142,6 +142,7 @@ def lib():
     L.fmmh_evaluate.argtypes = [vp]
     L.fmmh_evaluate_graph.argtypes = [vp]
     L.fmmh_direct_sum.argtypes = [i32, vp, vp, i64, vp, i64, vp, vp, i32]
+    L.fmmh_direct_sum_noalloc.argtypes = [i32, vp, vp, i64, vp, i64, vp, vp, vp]
     L.fmmh_read_output.argtypes = [vp, vp, vp, i64, i32]
     L.fmmh_stream_wait.argtypes = [vp, vp]
     L.fmmh_sync.argtypes = [vp]

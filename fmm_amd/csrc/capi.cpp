@@ -408,6 +408,16 @@ int32_t fmmh_direct_sum(int32_t device, const double *sources, const double *cha
   });
 }
 
+int32_t fmmh_direct_sum_noalloc(int32_t device, const double *sources, const double *charges, int64_t nsources,
+                                const double *targets, int64_t ntargets, double *phi, double *grad, void *stream) {
+  return guard([&] {
+    need(nsources == 0 || (sources && charges), "fmmh_direct_sum_noalloc: NULL sources");
+    need(ntargets == 0 || (targets && phi), "fmmh_direct_sum_noalloc: NULL targets or output");
+    fmmh::direct_sum_noalloc(device, sources, charges, nsources, targets, ntargets, phi, grad,
+                             static_cast<hipStream_t>(stream));
+  });
+}
+
 int32_t fmmh_stream_wait(fmmh_operator *op, void *stream) {
   return guard([&] {
     need(op != nullptr, "fmmh_stream_wait: NULL operator");

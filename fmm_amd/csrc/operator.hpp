@@ -266,5 +266,10 @@ class DeviceOperator {
 // inputs and outputs on the host (on_device = false) or device pointers; synchronous.
 void direct_sum(int device, const double *sources, const double *charges, int64_t ns, const double *targets,
                 int64_t nt, double *phi, double *grad, bool on_device);
+// The same on device pointers, queued on `stream` without allocating: the partial sums
+// live in a fixed per-GPU array (noalloc.hip), which bounds the targets per call.
+void direct_sum_noalloc(int device, const double *sources, const double *charges, int64_t ns, const double *targets,
+                        int64_t nt, double *phi, double *grad, hipStream_t stream);
+int64_t direct_sum_noalloc_max_targets();
 
 }  // namespace fmmh

@@ -220,6 +220,16 @@ int32_t fmmh_sync(fmmh_operator *op);
  * `device`.  Synchronous; allocates its own scratch. */
 int32_t fmmh_direct_sum(int32_t device, const double *sources, const double *charges, int64_t nsources,
                         const double *targets, int64_t ntargets, double *phi, double *grad, int32_t on_device);
+/* fmmh_direct_sum without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Device pointers only; queued on `stream` (a
+ * hipStream_t, NULL = the null stream) and not synchronised: the outputs are ready when
+ * the caller's stream reaches them.  The partial sums use a fixed 32 MiB array per GPU,
+ * held until the process ends: at most 262,144 targets per call (FMMH_TOO_LARGE above).
+ * Calls that overlap on one GPU share the array and race: order them (one stream, or
+ * events).  The same source chunks and the same bits as fmmh_direct_sum. */
+int32_t fmmh_direct_sum_noalloc(int32_t device, const double *sources, const double *charges, int64_t nsources,
+                                const double *targets, int64_t ntargets, double *phi, double *grad, void *stream);
 /* Multipoles (locals = 0) or locals of one level, box-major (p+1)^2 doubles per box in
  * the box index's order: the values an operator hands to the host store at the "send"
  * events of HostData (fmm-plan/src/operator.rs:344-404).  Synchronises. */
