@@ -64,6 +64,23 @@ def device_count() -> int:
     return int(n.value)
 
 
+def set_table_cache(directory) -> None:
+    """Keep the Laplace tables (M2M, L2L, dense M2L) of every degree built from now on in
+    `directory`, and load them from there when valid (the reference's TableCache file
+    format, fmm-tables/src/cache.rs); None: build every time."""
+    d = None if directory is None else os.fsencode(os.fspath(directory))
+    if d is not None:
+        os.makedirs(d, exist_ok=True)
+    _lib.check(_lib.lib().fmmh_set_table_cache(d))
+
+
+def table_cache_stats():
+    """(hits, misses) of the table cache since it was set."""
+    h, m = C.c_int64(0), C.c_int64(0)
+    _lib.check(_lib.lib().fmmh_table_cache_stats(C.byref(h), C.byref(m)))
+    return h.value, m.value
+
+
 def tables(p: int):
     """(M2M[8], L2L[8], M2L[316]) matrices A, output = A @ input (CONVENTIONS §3.12)."""
     n = (p + 1) ** 2

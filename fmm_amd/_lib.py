@@ -43,6 +43,7 @@ EXPORTS = [
     "fmmh_helmholtz_kind_times", "fmmh_helmholtz_table", "fmmh_helmholtz_direct_sum",
     "fmmh_plane_wave_quadrature", "fmmh_plane_wave_m2l", "fmmh_stream_wait", "fmmh_helmholtz_stream_wait",
     "fmmh_direct_sum_noalloc", "fmmh_operator_create_tuned", "fmmh_autotune",
+    "fmmh_set_table_cache", "fmmh_table_cache_stats",
 ]
 
 
@@ -127,6 +128,8 @@ def lib():
     L.fmmh_plan_point_leaves.argtypes = [vp, i32, vp]
     L.fmmh_plan_leaf_coordinates.argtypes = [vp, i32, vp, vp]
     L.fmmh_tables.argtypes = [i32, vp, vp, vp]
+    L.fmmh_set_table_cache.argtypes = [C.c_char_p]
+    L.fmmh_table_cache_stats.argtypes = [vp, vp]
     L.fmmh_m2l_rotation_table.argtypes = [i32, i32, vp]
     L.fmmh_device_count.argtypes = [C.POINTER(i32)]
     L.fmmh_operator_create.argtypes = [vp, C.POINTER(OperatorOptions), C.POINTER(vp)]

@@ -13,6 +13,7 @@
 #include "operator.hpp"
 #include "plan.hpp"
 #include "pwtables.hpp"
+#include "table_cache.hpp"
 #include "tune.hpp"
 
 struct fmmh_plan {
@@ -223,10 +224,19 @@ int32_t fmmh_m2l_rotation_table(int32_t p, int32_t offset, double *table) {
   });
 }
 
+int32_t fmmh_set_table_cache(const char *directory) {
+  return guard([&] { fmmh::set_table_cache(directory); });
+}
+
+int32_t fmmh_table_cache_stats(int64_t *hits, int64_t *misses) {
+  return guard([&] { fmmh::table_cache_stats(hits, misses); });
+}
+
 int32_t fmmh_tables(int32_t p, double *m2m, double *l2l, double *m2l) {
   return guard([&] {
+    need(p >= 0 && p <= 30, "fmmh_tables: p must lie in 0..30");
     std::vector<double> a, b, c;
-    fmmh::build_tables(p, a, b, c);
+    fmmh::cached_tables(p, a, b, c);
     if (m2m) std::memcpy(m2m, a.data(), a.size() * sizeof(double));
     if (l2l) std::memcpy(l2l, b.data(), b.size() * sizeof(double));
     if (m2l) std::memcpy(m2l, c.data(), c.size() * sizeof(double));

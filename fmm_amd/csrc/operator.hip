@@ -11,6 +11,7 @@
 #include "operator.hpp"
 #include "pwtables.hpp"
 #include "schedule.hpp"
+#include "table_cache.hpp"
 
 namespace fmmh {
 
@@ -157,7 +158,7 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
   // tables, swizzled for the MFMA contraction
   {
     std::vector<double> m2m, l2l, m2l;
-    build_tables(opt_.p, m2m, l2l, m2l);
+    cached_tables(opt_.p, m2m, l2l, m2l);
     const size_t n2 = size_t(nc_) * nc_, sw = dev::swizzled_table_size(opt_.p);
     auto swz = [&](const std::vector<double> &t, int count, DevBuf<double> &dst) {
       std::vector<double> out(sw * count);

@@ -133,8 +133,17 @@ int32_t fmmh_plan_point_leaves(const fmmh_plan *plan, int32_t targets, uint32_t 
 /* Leaf-scaled coordinates (3 per point, leaf order) and the caller index of each. */
 int32_t fmmh_plan_leaf_coordinates(const fmmh_plan *plan, int32_t targets, double *coords, uint32_t *order);
 
-/* Operator tables of degree p (column-major, 8 / 8 / 316 matrices of (p+1)^4 doubles). */
+/* Operator tables of degree p (column-major, 8 / 8 / 316 matrices of (p+1)^4 doubles),
+ * through the table cache when one is set. */
 int32_t fmmh_tables(int32_t p, double *m2m, double *l2l, double *m2l);
+/* The on-disk table cache of this process (fmm-tables/src/cache.rs, its file format:
+ * "<kind>-p<pp>-f64-c4-v1.bin", a 44-byte header with magic NDFMMTAB, versions, key,
+ * payload length and FNV-1a checksum, then the matrices): operators and fmmh_tables load
+ * the M2M, L2L and dense M2L tables of degree p from `directory` when its files are valid,
+ * else build and store them (temporary file, then rename).  NULL disables the cache. */
+int32_t fmmh_set_table_cache(const char *directory);
+/* Loads that hit (every file of a degree valid) or missed since fmmh_set_table_cache. */
+int32_t fmmh_table_cache_stats(int64_t *hits, int64_t *misses);
 /* The M2L of offset index `offset` composed from the rotation-M2L tables, G(Q^T) A(|d|)
  * F(Q) (fmm-tables/src/rotation.rs:307 ShiftTables::apply), as a dense column-major
  * (p+1)^2 x (p+1)^2 matrix: equal to the dense table to rounding. */
