@@ -32,7 +32,7 @@ EXPORTS = [
     "fmmh_last_error", "fmmh_abi_version", "fmmh_plan_build", "fmmh_plan_free", "fmmh_plan_nlevels",
     "fmmh_plan_nboxes", "fmmh_plan_nleaves", "fmmh_plan_domain", "fmmh_plan_box_keys", "fmmh_plan_leaf_keys",
     "fmmh_plan_list", "fmmh_plan_point_leaves", "fmmh_plan_leaf_coordinates", "fmmh_tables",
-    "fmmh_device_count", "fmmh_operator_create", "fmmh_operator_free", "fmmh_operator_stats",
+    "fmmh_device_count", "fmmh_device_describe", "fmmh_operator_create", "fmmh_operator_free", "fmmh_operator_stats",
     "fmmh_load_charges", "fmmh_reset", "fmmh_p2m", "fmmh_m2m", "fmmh_m2l", "fmmh_p2l", "fmmh_l2l",
     "fmmh_l2p", "fmmh_m2p", "fmmh_p2p", "fmmh_evaluate", "fmmh_read_output", "fmmh_sync",
     "fmmh_kind_timing", "fmmh_kind_times", "fmmh_read_expansions", "fmmh_vjp", "fmmh_forest_build",
@@ -132,6 +132,7 @@ def lib():
     L.fmmh_table_cache_stats.argtypes = [vp, vp]
     L.fmmh_m2l_rotation_table.argtypes = [i32, i32, vp]
     L.fmmh_device_count.argtypes = [C.POINTER(i32)]
+    L.fmmh_device_describe.argtypes = [i32, C.c_char_p, i32]
     L.fmmh_operator_create.argtypes = [vp, C.POINTER(OperatorOptions), C.POINTER(vp)]
     L.fmmh_operator_create_tuned.argtypes = [vp, C.POINTER(OperatorOptions), C.POINTER(Tuning), C.POINTER(vp)]
     L.fmmh_autotune.argtypes = [vp, C.POINTER(OperatorOptions), dbl, C.POINTER(Tuning), vp]

@@ -252,6 +252,20 @@ int32_t fmmh_device_count(int32_t *count) {
   });
 }
 
+int32_t fmmh_device_describe(int32_t device, char *name, int32_t capacity) {
+  return guard([&] {
+    need(name != nullptr && capacity > 0, "fmmh_device_describe: NULL or empty name buffer");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    need(device >= 0 && device < n, "fmmh_device_describe: no such device");
+    hipDeviceProp_t prop;
+    fmmh::hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    const std::string s = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount) + "cu";
+    need(int64_t(s.size()) < capacity, "fmmh_device_describe: name buffer too small");
+    std::memcpy(name, s.c_str(), s.size() + 1);
+  });
+}
+
 }  // extern "C"
 
 namespace {

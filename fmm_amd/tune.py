@@ -46,12 +46,11 @@ def bucket(n: float) -> int:
 
 
 def device_name(device: int = 0) -> str:
-    try:
-        import torch
-
-        return torch.cuda.get_device_name(device).replace("|", "/")
-    except Exception:  # noqa: BLE001 - any runtime without torch: one name for every device
-        return "hip"
+    """The HIP runtime's architecture name and CU count of `device` (the cache key's
+    device part; the same under a profiler as without one)."""
+    buf = C.create_string_buffer(256)
+    _lib.check(_lib.lib().fmmh_device_describe(int(device), buf, len(buf)))
+    return buf.value.decode().replace("|", "/")
 
 
 def cache_key(p, precision, gradients, m2l, device=0) -> str:

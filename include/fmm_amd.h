@@ -163,6 +163,11 @@ typedef struct {
 
 /* ---- device operator -------------------------------------------------------------- */
 int32_t fmmh_device_count(int32_t *count);
+/* The device's architecture name and compute-unit count, as "<gcnArchName>/<CUs>cu"
+ * (for example "gfx950:sramecc+:xnack-/256cu"), NUL-terminated in name[capacity]: the key
+ * under which tuned decisions are cached.  FMMH_INVALID_ARGUMENT for a device out of
+ * range or a buffer too small. */
+int32_t fmmh_device_describe(int32_t device, char *name, int32_t capacity);
 int32_t fmmh_operator_create(const fmmh_plan *plan, const fmmh_operator_options *options, fmmh_operator **out);
 /* fmmh_operator_create with launch-shape decisions (fmmh_autotune's, or a cache's). */
 int32_t fmmh_operator_create_tuned(const fmmh_plan *plan, const fmmh_operator_options *options,

@@ -64,6 +64,19 @@ def test_level_buckets_of_a_plan():
     assert set(b["l2l"]) == set(range(1, plan.nlevels))
 
 
+def test_device_key_names_the_architecture():
+    # the cache key's device part comes from the HIP runtime, never from a name that a
+    # profiler or another binding may report differently
+    buf = C.create_string_buffer(8)
+    if fmm_amd.device_count() < 1:
+        assert _lib.lib().fmmh_device_describe(0, buf, len(buf)) == 9
+        return
+    assert _lib.lib().fmmh_device_describe(0, buf, 2) == 9  # too small for any name
+    name = tune.device_name(0)
+    assert name.startswith("gfx") and name.endswith("cu")
+    assert tune.cache_key(8, "mixed", True, "dense").startswith(name + "|p8|")
+
+
 def test_invalid_tuning_refused():
     rng = O.Rng(82)
     plan = fmm_amd.Plan(rng.cube(500), leaf_points=32)
