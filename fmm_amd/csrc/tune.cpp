@@ -19,8 +19,9 @@
 // is timed in batches of launches on the operator's stream, doubling until a batch takes
 // at least 2 ms, and its time is the median of 5 batches; the fastest wins, the first
 // measured on a tie.  Tuning stops at the budget's deadline, checked before every
-// candidate: decisions not reached keep the static rule.  The cache file of decisions
-// per bucket is the host binding's (fmm_amd/tune.py).
+// candidate (and before the three candidate operators of the per-level decisions are
+// built, from the first one's build time): decisions not reached keep the static rule.
+// The cache file of decisions per bucket is the host binding's (fmm_amd/tune.py).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -93,8 +94,10 @@ void autotune(const Plan &plan, const OperatorOptions &options, double budget_s,
   OperatorOptions oa = options;
   oa.tuning = Tuning{};
   oa.tuning.p2p_uniform_min = 0;
+  const auto built0 = Clock::now();
   auto A = std::make_unique<DeviceOperator>(plan, oa);
   prepare(*A, size_t(plan.ns));
+  const auto build_time = Clock::now() - built0;  // one candidate operator's build
   hipStream_t sa = A->stream();
 
   // P2P form (the mixed path only has two forms)
@@ -131,7 +134,9 @@ void autotune(const Plan &plan, const OperatorOptions &options, double budget_s,
     }
     A->set_m2l_stage(0);
   }
-  if (Clock::now() > deadline) return;
+  // the per-level decisions need three more operators: skip them when their builds alone
+  // would overrun the budget (the static rule stays)
+  if (Clock::now() + 3 * build_time > deadline) return;
 
   OperatorOptions ob = oa, oc = oa, od = oa;
   for (int l = 0; l < std::min(L, kTuneLevels); ++l) {

@@ -165,10 +165,11 @@ def test_every_forced_decision_matches_oracle(_gpu, precision):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("p", [10, 12])
-def test_m2l_two_step_stages_match_oracle(_gpu, p):
-    # the bf16x3 contraction with two k-steps per stage in a ring of two (even k-step
-    # counts: p = 10 has 4, p = 12 has 6) against the oracle and the one-step stages
+@pytest.mark.parametrize("p", [6, 10, 12])
+def test_m2l_stages_match_oracle(_gpu, p):
+    # the bf16x3 contraction in both stage shapes (one k-step per stage in a ring of three;
+    # two in a ring of two, even k-step counts: p = 10 has 4, p = 12 has 6; p = 6 always
+    # takes its two k-steps per stage) against the oracle
     rng = O.Rng(84)
     pts = rng.cube(60000)
     q = rng.normals(len(pts))
