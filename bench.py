@@ -131,11 +131,12 @@ def main():
     ap.add_argument("--m2l", default="dense", choices=("dense", "rotation", "plane_wave"),
                     help="M2L formulation of both legs (rotation: f32 in the mixed leg; plane_wave: the plain "
                          "leg only, the mixed leg keeps the dense bf16x3 M2L)")
-    ap.add_argument("--workload", default="cube", choices=("cube", "batched", "helmholtz", "nonuniform"),
+    ap.add_argument("--workload", default="cube", choices=("cube", "batched", "helmholtz", "nonuniform", "grid"),
                     help="cube: 4M uniform points (the headline); batched: 256 independent Gaussian clouds, "
                          "p = 6, one forest evaluation replayed as a hipGraph, plus its adjoint; helmholtz: "
                          "1M uniform points, k = 25, complex charges, degree by the rule at 1e-6; nonuniform: 2M "
-                         "sphere-surface + 1M Plummer points, p = 10, rotation M2L in f32")
+                         "sphere-surface + 1M Plummer points, p = 10, rotation M2L in f32; grid: the reference's published "
+                         "device grid (N = 1e5..1e7, p = 3..12, mixed and f64) with its published times and errors")
     ap.add_argument("--k", type=float, default=25.0, help="helmholtz: the wavenumber (real)")
     ap.add_argument("--tune-budget", type=float, default=10.0,
                     help="seconds of autotune per leg (fmm_amd/tune.py) before timing; 0: the static rule")
@@ -150,6 +151,8 @@ def main():
         return batched(args)
     if args.workload == "helmholtz":
         return helmholtz(args)
+    if args.workload == "grid":
+        return grid(args)
     if args.workload == "nonuniform":
         return nonuniform(args)
 
@@ -540,10 +543,10 @@ def nonuniform(args):
             torch.cuda.synchronize()
             times.append(time.perf_counter() - s)
         # the errors of the timed (eager) steps' own output
-        phi = phi_dev.cpu().numpy()
-        grad = grad_dev.cpu().numpy().reshape(-1, 3)
-        e_phi = error_norm(phi[sample], exact_phi)
-        e_grad = error_norm(grad[sample], exact_grad)
+        ph = phi.cpu().numpy()
+        gr = grad.cpu().numpy().reshape(-1, 3)
+        err = {"phi": float(np.linalg.norm(ph[sample] - ep) / np.linalg.norm(ep)),
+               "grad": float(np.linalg.norm(gr[sample] - eg) / np.linalg.norm(eg))}
         # per-kind device times in a pass of their own, outside the timed steps
         op.kind_timing(True)
         for _ in range(args.steps):
@@ -551,10 +554,6 @@ def nonuniform(args):
         torch.cuda.synchronize()
         kinds = op.kind_times()
         op.kind_timing(False)
-        ph = phi.cpu().numpy()
-        gr = grad.cpu().numpy().reshape(-1, 3)
-        err = {"phi": float(np.linalg.norm(ph[sample] - ep) / np.linalg.norm(ep)),
-               "grad": float(np.linalg.norm(gr[sample] - eg) / np.linalg.norm(eg))}
         med = float(np.median(times))
         per = {k: round(v[0] / args.steps, 3) for k, v in kinds.items() if v[1]}
         log(f"[{precision}, {m2l}] {med * 1e3:.2f} ms ({n / med / 1e6:.1f} Mpts/s), errors {err}, kinds {per}")
@@ -672,6 +671,114 @@ def helmholtz(args):
         "m2l_pairs": stats["m2l_pairs"],
         "cpu_baseline": cpu,
     }
+    print(json.dumps(line), flush=True)
+
+
+# The reference's published device grid on the GH200's H100 (fmm-bench/results/
+# phase4s-gh200.md:89-113), per (precision, N, p): median ms per evaluation, error phi,
+# error grad phi (relative L2 at 1,000 sampled targets against the f64 direct sum).
+PUBLISHED_GRID = {
+    ("f32", 100000, 3): (1.572, 3.39e-3, 1.99e-3), ("f32", 100000, 6): (1.889, 1.42e-4, 1.30e-4),
+    ("f32", 100000, 8): (2.657, 2.12e-5, 2.37e-5), ("f32", 1000000, 3): (12.755, 2.00e-3, 3.80e-3),
+    ("f32", 1000000, 6): (15.917, 8.53e-5, 2.56e-4), ("f32", 1000000, 8): (21.599, 1.36e-5, 4.59e-5),
+    ("f32", 10000000, 3): (220.777, 1.87e-3, 1.87e-3), ("f32", 10000000, 6): (257.726, 7.90e-5, 1.23e-4),
+    ("f32", 10000000, 8): (315.430, 1.21e-5, 2.26e-5),
+    ("f64", 100000, 3): (1.940, 3.39e-3, 1.99e-3), ("f64", 100000, 6): (2.555, 1.42e-4, 1.30e-4),
+    ("f64", 100000, 8): (3.785, 2.12e-5, 2.37e-5), ("f64", 100000, 12): (8.492, 8.07e-7, 1.11e-6),
+    ("f64", 1000000, 3): (20.560, 2.00e-3, 3.80e-3), ("f64", 1000000, 6): (25.475, 8.53e-5, 2.56e-4),
+    ("f64", 1000000, 8): (35.268, 1.36e-5, 4.59e-5), ("f64", 1000000, 12): (76.389, 4.35e-7, 2.17e-6),
+    ("f64", 10000000, 3): (361.077, 1.87e-3, 1.87e-3), ("f64", 10000000, 6): (415.426, 7.90e-5, 1.23e-4),
+    ("f64", 10000000, 8): (517.091, 1.21e-5, 2.26e-5), ("f64", 10000000, 12): (1022.384, 4.40e-7, 1.21e-6),
+}
+
+
+def grid(args):
+    """The reference's published device grid on this GPU: its problems exactly (N = 10^5,
+    10^6, 10^7 points uniform in [0, 1)^3 and charges uniform in [-1, 1) from SplitMix64
+    0x00be9c06, then 1,000 sampled targets, fmm-bench/src/measure.rs:35-67; sources =
+    targets, gradients, max level 16, <= 64 points per leaf, no near rule as in those runs,
+    dense M2L), the mixed path beside the reference's f32 rows (p = 3, 6, 8) and the plain
+    f64 path beside its f64 rows (p = 3, 6, 8, 12; its p = 18 is beyond this path's
+    p <= 15).  One tuning cache for the grid, as the reference shares one per precision
+    over its N.  Times: the median of the timed evaluations with charges and outputs in
+    device memory, and with them in host memory (upload and download included, as the
+    reference's rows).  Prints one JSON line with every row."""
+    import tempfile
+
+    import torch
+
+    import fmm_amd
+    from fmm_amd import points as P, tune
+
+    dev = torch.device("cuda:0")
+    tune_dir = args.tuning_cache or tempfile.mkdtemp(prefix="fmm_amd_tune")
+    rows = []
+    for n in (100_000, 1_000_000, 10_000_000):
+        t0 = time.time()
+        pts, q, sample = P.bench_cube(n, 1000, charges="uniform")
+        plan = fmm_amd.Plan(pts, leaf_points=64)
+        ep, eg = fmm_amd.direct_sum(pts, q, pts[sample])
+        ep, eg = ep / (4 * np.pi), eg / (4 * np.pi)
+        log(f"grid N={n}: {plan.nleaves} leaves on {plan.nlevels} levels, plan and direct sum in "
+            f"{time.time() - t0:.1f}s")
+        q_dev = torch.from_numpy(q).to(dev)
+        phi = torch.zeros(n, dtype=torch.float64, device=dev)
+        grad = torch.zeros(3 * n, dtype=torch.float64, device=dev)
+        for precision, ref_prec, degrees in (("mixed", "f32", (3, 6, 8)), ("f64", "f64", (3, 6, 8, 12))):
+            for p in degrees:
+                tuning = None
+                if args.tune_budget > 0:
+                    tuning, _ = tune.resolve(plan, p, precision, True, 0, "dense", tune_dir, args.tune_budget)
+                op = fmm_amd.Operator(plan, p, precision, True, 0, "dense", tuning)
+                fmm = fmm_amd.Fmm(plan, op)
+
+                def step():
+                    op.load_charges(q_dev)
+                    op.reset()
+                    op.evaluate()
+                    op.read_output_device(phi, grad)
+
+                for _ in range(args.warmup):
+                    step()
+                torch.cuda.synchronize()
+                dt = []
+                for _ in range(args.steps):
+                    torch.cuda.synchronize()
+                    t = time.perf_counter()
+                    step()
+                    torch.cuda.synchronize()
+                    dt.append(time.perf_counter() - t)
+                ph = phi.cpu().numpy()
+                gr = grad.cpu().numpy().reshape(-1, 3)
+                e1 = float(np.linalg.norm(ph[sample] - ep) / np.linalg.norm(ep))
+                e2 = float(np.linalg.norm(gr[sample] - eg) / np.linalg.norm(eg))
+                ht = []
+                for _ in range(max(3, args.steps // 2)):
+                    t = time.perf_counter()
+                    fmm.evaluate(q)
+                    ht.append(time.perf_counter() - t)
+                ms, hms = float(np.median(dt)) * 1e3, float(np.median(ht)) * 1e3
+                pub = PUBLISHED_GRID.get((ref_prec, n, p))
+                row = {"precision": precision, "reference_precision": ref_prec, "n": n, "p": p,
+                       "ms": ms, "host_ms": hms, "mpts_s": n / ms / 1e3, "err_phi": e1, "err_grad": e2,
+                       "published_ms": None if pub is None else pub[0],
+                       "published_err_phi": None if pub is None else pub[1],
+                       "published_err_grad": None if pub is None else pub[2],
+                       "host_speedup_vs_published": None if pub is None else pub[0] / hms}
+                rows.append(row)
+                log(f"  {precision} p={p}: {ms:.3f} ms device-resident, {hms:.3f} ms host round trip "
+                    f"(published {ref_prec} {pub[0] if pub else None} ms), errors {e1:.3g} / {e2:.3g} "
+                    f"(published {pub[1] if pub else None} / {pub[2] if pub else None})")
+                del fmm, op
+        del plan
+    line = {"metric": "the reference's published device grid on one MI355X (ms per evaluation)",
+            "value": None, "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+            "dtype": "mixed: f32 P2P + bf16x3 M2L + f64 other; f64: every operator in f64", "data": "synthetic",
+            "config": {"workload": "reference-grid", "leaf_points": 64, "max_level": 16, "near_rule": None,
+                       "charges": "uniform [-1, 1)", "sampled_targets": 1000, "m2l": "dense",
+                       "reference": "fmm-bench/results/phase4s-gh200.md:89-113 (H100 of a GH200)"},
+            "rows": rows}
     print(json.dumps(line), flush=True)
 
 

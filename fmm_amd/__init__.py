@@ -319,6 +319,12 @@ class Operator:
         _lib.check(_lib.lib().fmmh_kind_times(self._h, _lib.ptr(ms), _lib.ptr(n)))
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KINDS)}
 
+    def inject_error(self):
+        """Test hook (DeviceOperator::inject_error, fmm-exec/src/device.rs:4621): the next
+        call that touches the device fails with an injected device error, which the
+        operator keeps and returns from every such call after it."""
+        _lib.check(_lib.lib().fmmh_inject_error(self._h))
+
 
 def direct_sum(sources, charges, targets, gradients=True, device=0):
     """Compensated f64 direct sum on the GPU (unscaled: sum q / r, -sum q d / r^3), as
@@ -339,6 +345,11 @@ class Fmm:
     def __init__(self, plan: Plan, operator: Operator):
         self.plan = plan
         self.operator = operator
+
+    def inject_device_error(self):
+        """Test hook (Fmm::inject_device_error, fmm-exec/src/fmm.rs:4715): the next
+        evaluation fails with FmmError(FMMH_DEVICE), and so does every later one."""
+        self.operator.inject_error()
 
     def evaluate(self, charges) -> Output:
         """Potential and gradient at the targets (1/(4 pi) included), caller order."""

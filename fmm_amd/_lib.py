@@ -42,7 +42,7 @@ EXPORTS = [
     "fmmh_helmholtz_read_output", "fmmh_helmholtz_read_expansions", "fmmh_helmholtz_kind_timing",
     "fmmh_helmholtz_kind_times", "fmmh_helmholtz_table", "fmmh_helmholtz_direct_sum",
     "fmmh_plane_wave_quadrature", "fmmh_plane_wave_m2l", "fmmh_stream_wait", "fmmh_helmholtz_stream_wait",
-    "fmmh_direct_sum_noalloc", "fmmh_operator_create_tuned", "fmmh_autotune",
+    "fmmh_direct_sum_noalloc", "fmmh_operator_create_tuned", "fmmh_autotune", "fmmh_inject_error",
     "fmmh_set_table_cache", "fmmh_table_cache_stats",
 ]
 
@@ -136,6 +136,7 @@ def lib():
     L.fmmh_operator_create.argtypes = [vp, C.POINTER(OperatorOptions), C.POINTER(vp)]
     L.fmmh_operator_create_tuned.argtypes = [vp, C.POINTER(OperatorOptions), C.POINTER(Tuning), C.POINTER(vp)]
     L.fmmh_autotune.argtypes = [vp, C.POINTER(OperatorOptions), dbl, C.POINTER(Tuning), vp]
+    L.fmmh_inject_error.argtypes = [vp]
     L.fmmh_operator_free.argtypes = [vp]
     L.fmmh_operator_free.restype = None
     L.fmmh_operator_stats.argtypes = [vp, C.POINTER(Stats)]

@@ -24,6 +24,11 @@ struct fmmh_operator {
   std::unique_ptr<fmmh::DeviceOperator> op;
   int64_t nlevels = 0;
   int64_t nleaves = 0;
+  // The first device failure of an evaluation, kept: every later call that touches the
+  // device returns it (fmm-exec/src/device.rs:414-425).  inject: the next such call
+  // fails with a device error of its own (the test hook, device.rs:4621).
+  std::string failed;
+  bool inject = false;
 };
 
 struct fmmh_helmholtz {
@@ -57,6 +62,27 @@ int32_t guard(F &&f) {
 
 void need(bool ok, const char *what) {
   if (!ok) throw fmmh::Error(fmmh::kInvalidArgument, what);
+}
+
+// A call on an operator that touches the device: the kept failure, if any, is returned
+// without device work; a device error raised by the call is kept.  Argument errors are
+// not kept (they are the caller's, found before any device work).
+template <class F>
+int32_t device_call(fmmh_operator *op, const char *name, F &&f) {
+  return guard([&] {
+    need(op != nullptr, (std::string(name) + ": NULL operator").c_str());
+    if (op->inject) {
+      op->inject = false;
+      op->failed = std::string(name) + ": injected device error (test hook)";
+    }
+    if (!op->failed.empty()) throw fmmh::Error(fmmh::kDeviceError, op->failed);
+    try {
+      f();
+    } catch (const fmmh::Error &e) {
+      if (e.status == fmmh::kDeviceError) op->failed = e.what();
+      throw;
+    }
+  });
 }
 
 }  // namespace
@@ -358,27 +384,24 @@ int32_t fmmh_operator_stats(const fmmh_operator *op, fmmh_stats *out) {
 }
 
 int32_t fmmh_load_charges(fmmh_operator *op, const double *charges, int64_t count, int32_t on_device) {
-  return guard([&] {
-    need(op && charges, "fmmh_load_charges: NULL argument");
+  return device_call(op, "fmmh_load_charges", [&] {
+    need(charges != nullptr, "fmmh_load_charges: NULL argument");
     need(count == op->op->nsources(), "fmmh_load_charges: count differs from the plan's source count");
     op->op->load_charges(charges, on_device != 0);
   });
 }
 
 int32_t fmmh_reset(fmmh_operator *op) {
-  return guard([&] {
-    need(op != nullptr, "fmmh_reset: NULL operator");
+  return device_call(op, "fmmh_reset", [&] {
     op->op->reset();
   });
 }
 
 #define FMMH_LEVEL_CALL(NAME)                                                      \
   int32_t fmmh_##NAME(fmmh_operator *op, int32_t level) {                          \
-    return guard([&] {                                                             \
-      need(op != nullptr, "fmmh_" #NAME ": NULL operator");                        \
-      need(level >= 0 && level < op->nlevels, "fmmh_" #NAME ": level out of range"); \
-      op->op->NAME(level);                                                         \
-    });                                                                            \
+    if (op != nullptr && (level < 0 || level >= op->nlevels))                      \
+      return guard([&] { need(false, "fmmh_" #NAME ": level out of range"); });    \
+    return device_call(op, "fmmh_" #NAME, [&] { op->op->NAME(level); });          \
   }
 FMMH_LEVEL_CALL(p2m)
 FMMH_LEVEL_CALL(m2m)
@@ -391,22 +414,20 @@ FMMH_LEVEL_CALL(p2p)
 #undef FMMH_LEVEL_CALL
 
 int32_t fmmh_evaluate(fmmh_operator *op) {
-  return guard([&] {
-    need(op != nullptr, "fmmh_evaluate: NULL operator");
+  return device_call(op, "fmmh_evaluate", [&] {
     op->op->evaluate();
   });
 }
 
 int32_t fmmh_evaluate_graph(fmmh_operator *op) {
-  return guard([&] {
-    need(op != nullptr, "fmmh_evaluate_graph: NULL operator");
+  return device_call(op, "fmmh_evaluate_graph", [&] {
     op->op->evaluate_graph();
   });
 }
 
 int32_t fmmh_read_output(fmmh_operator *op, double *phi, double *grad, int64_t count, int32_t on_device) {
-  return guard([&] {
-    need(op && phi, "fmmh_read_output: NULL argument");
+  return device_call(op, "fmmh_read_output", [&] {
+    need(phi != nullptr, "fmmh_read_output: NULL argument");
     need(count == op->op->ntargets(), "fmmh_read_output: count differs from the plan's target count");
     op->op->read_output(phi, grad, on_device != 0);
   });
@@ -414,8 +435,8 @@ int32_t fmmh_read_output(fmmh_operator *op, double *phi, double *grad, int64_t c
 
 int32_t fmmh_vjp(fmmh_operator *op, const double *charges, const double *cotangent, const double *field,
                  int64_t count, int32_t on_device, double *grad_charges, double *grad_points) {
-  return guard([&] {
-    need(op && charges && cotangent && grad_charges, "fmmh_vjp: NULL argument");
+  return device_call(op, "fmmh_vjp", [&] {
+    need(charges && cotangent && grad_charges, "fmmh_vjp: NULL argument");
     need(count == op->op->nsources() && count == op->op->ntargets(),
          "fmmh_vjp: count differs from the plan's point count");
     need(grad_points == nullptr || field != nullptr, "fmmh_vjp: grad_points needs the forward field");
@@ -443,24 +464,29 @@ int32_t fmmh_direct_sum_noalloc(int32_t device, const double *sources, const dou
 }
 
 int32_t fmmh_stream_wait(fmmh_operator *op, void *stream) {
-  return guard([&] {
-    need(op != nullptr, "fmmh_stream_wait: NULL operator");
+  return device_call(op, "fmmh_stream_wait", [&] {
     op->op->wait_for(static_cast<hipStream_t>(stream));
   });
 }
 
 int32_t fmmh_sync(fmmh_operator *op) {
-  return guard([&] {
-    need(op != nullptr, "fmmh_sync: NULL operator");
+  return device_call(op, "fmmh_sync", [&] {
     op->op->sync();
   });
 }
 
 int32_t fmmh_read_expansions(fmmh_operator *op, int32_t locals, int32_t level, double *out) {
-  return guard([&] {
-    need(op && out, "fmmh_read_expansions: NULL argument");
+  return device_call(op, "fmmh_read_expansions", [&] {
+    need(out != nullptr, "fmmh_read_expansions: NULL argument");
     need(level >= 0 && level < op->nlevels, "fmmh_read_expansions: level out of range");
     op->op->read_expansions(locals != 0, level, out);
+  });
+}
+
+int32_t fmmh_inject_error(fmmh_operator *op) {
+  return guard([&] {
+    need(op != nullptr, "fmmh_inject_error: NULL operator");
+    op->inject = true;
   });
 }
 
@@ -472,8 +498,7 @@ int32_t fmmh_kind_timing(fmmh_operator *op, int32_t enable) {
 }
 
 int32_t fmmh_kind_times(fmmh_operator *op, double *ms, int64_t *launches) {
-  return guard([&] {
-    need(op != nullptr, "fmmh_kind_times: NULL operator");
+  return device_call(op, "fmmh_kind_times", [&] {
     op->op->kind_times(ms, launches);
   });
 }

@@ -252,6 +252,15 @@ int32_t fmmh_read_expansions(fmmh_operator *op, int32_t locals, int32_t level, d
  * call with enable: ms[FMMH_NKINDS]. */
 int32_t fmmh_kind_timing(fmmh_operator *op, int32_t enable);
 int32_t fmmh_kind_times(fmmh_operator *op, double *ms, int64_t *launches);
+/* Errors of an evaluation (fmm-exec/src/device.rs:414-425): the operator keeps the first
+ * device failure (FMMH_DEVICE) of any call that touches the device (charges, reset, the
+ * level calls, evaluate, output, VJP, sync, expansions, kind times, stream wait) and
+ * returns it, with its message, from every such call after it without device work; a new
+ * operator evaluates again.  Argument errors are returned and not kept.
+ * fmmh_inject_error is the test hook (DeviceOperator::inject_error, device.rs:4621, and
+ * Fmm::inject_device_error, fmm-exec/src/fmm.rs:4715): the next such call fails with a
+ * device error whose message says "injected", which the operator then keeps. */
+int32_t fmmh_inject_error(fmmh_operator *op);
 
 /* The plane-wave quadrature built for degree p and tolerance eps (fmm-tables/src/plane_wave/
  * mod.rs:111 uses FMM3D's tables; this build compresses a Gauss-Legendre rule): node count,
