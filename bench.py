@@ -609,10 +609,10 @@ def helmholtz(args):
     log(f"direct sum at {len(sample)} targets in {time.time() - t0:.1f}s")
     q_dev = torch.from_numpy(np.ascontiguousarray(q, dtype=np.complex128)).to(dev)
 
-    def run(precision):
+    def run(precision, m2l="dense"):
         t0 = time.time()
-        op = fmm_amd.HelmholtzOperator(plan, p, args.k, precision, True, 0)
-        log(f"{precision}: operator built in {time.time() - t0:.1f}s: {op.stats()}")
+        op = fmm_amd.HelmholtzOperator(plan, p, args.k, precision, True, 0, m2l)
+        log(f"{precision}, {m2l} M2L: operator built in {time.time() - t0:.1f}s: {op.stats()}")
 
         def step():
             op.load_charges(q_dev)
@@ -637,12 +637,13 @@ def helmholtz(args):
         err = {"phi": float(np.linalg.norm(out.potential[sample] - ep) / np.linalg.norm(ep)),
                "grad": float(np.linalg.norm(out.gradient[sample] - eg) / np.linalg.norm(eg))}
         med = float(np.median(times))
-        log(f"{precision}: {med * 1e3:.2f} ms/step, errors {err}, kinds "
+        log(f"{precision}, {m2l} M2L: {med * 1e3:.2f} ms/step, errors {err}, kinds "
             + ", ".join(f"{k} {v[0]:.2f}" for k, v in kinds.items() if v[1]))
         return med, err, kinds, op.stats()
 
     mixed = run("mixed")
     plain = run("f64")
+    rot = run("f64", "rotation")
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -665,7 +666,11 @@ def helmholtz(args):
         "config": {"workload": "helmholtz-uniform-cube", "n": n, "k": args.k, "p": p, "leaf_points": leaf,
                    "near_wx": near},
         "errors": err,
-        "plain_f64": {"ms_per_step": plain[0] * 1e3, "value": n / plain[0] / 1e6, "errors": plain[1]},
+        "plain_f64": {"ms_per_step": plain[0] * 1e3, "value": n / plain[0] / 1e6, "errors": plain[1],
+                      "device_bytes": plain[3]["device_bytes"]},
+        "plain_f64_rotation_m2l": {"ms_per_step": rot[0] * 1e3, "value": n / rot[0] / 1e6, "errors": rot[1],
+                                   "m2l_ms": rot[2]["m2l"][0], "device_bytes": rot[3]["device_bytes"]},
+        "device_bytes": stats["device_bytes"],
         "kinds_ms": {k: v[0] for k, v in kinds.items() if v[1]},
         "interactions_per_s": stats["p2p_pairs"] / (p2p_ms * 1e-3) if p2p_ms > 0 else None,
         "m2l_pairs": stats["m2l_pairs"],

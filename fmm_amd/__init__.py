@@ -532,11 +532,12 @@ def helmholtz_degree(k, radius: float, tolerance: float) -> int:
 
 def helmholtz_table(p: int, kh, kind: str, index: int) -> np.ndarray:
     """One translation table of a level whose wavenumber in its own units is kh: "m2m" from
-    the child of octant `index`, "l2l" into it, "m2l" of offset index `index`; a complex
-    matrix A with stored output = A @ stored input."""
+    the child of octant `index`, "l2l" into it, "m2l" of offset index `index`, "m2l_rcr" the
+    same M2L composed from the per-pair rotation path's pieces (rotation blocks, scaled
+    coaxial blocks, rotation back); a complex matrix A with stored output = A @ stored input."""
     n = (p + 1) ** 2
     out = np.zeros(2 * n * n)
-    k = {"m2m": 0, "l2l": 1, "m2l": 2}[kind]
+    k = {"m2m": 0, "l2l": 1, "m2l": 2, "m2l_rcr": 3}[kind]
     _lib.check(_lib.lib().fmmh_helmholtz_table(int(p), float(np.real(kh)), float(np.imag(kh)), k, int(index),
                                                _lib.ptr(out)))
     return (out[: n * n] + 1j * out[n * n:]).reshape(n, n).T.copy()
@@ -559,15 +560,25 @@ def helmholtz_direct_sum(k, sources, charges, targets, gradients=True, device=0)
 class HelmholtzOperator:
     """The Helmholtz device operator over a plan (complex charges, kernel e^{ikr}/(4 pi r))."""
 
-    def __init__(self, plan: Plan, p: int, k, precision="f64", gradients=True, device=0):
+    def __init__(self, plan: Plan, p: int, k, precision="f64", gradients=True, device=0, m2l="dense",
+                 dense_budget=0):
+        """m2l (HelmholtzM2l, fmm-exec/src/helmholtz/operator.rs:448-479): "dense", one
+        complex table per offset on the MFMA contraction on the levels whose tables fit
+        dense_budget bytes (316 x 2 (p+1)^4 f64 per level, from the finest level up; 0 the
+        reference's 1 GiB, negative no limit) and per-pair rotation on the others, or
+        "rotation", per pair rotation, coaxial shift, rotation back in f64 (RCR) on every
+        level, with tables O(p^3) per distance."""
+        if m2l not in ("dense", "rotation"):
+            raise FmmError(7, f"Helmholtz M2L: dense or rotation, not {m2l!r}")
         self.plan = plan
         self.p = p
         self.k = complex(k)
+        self.m2l = m2l
         self.gradients = gradients
         self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         self.device = int(device)
         o = _lib.HelmholtzOptions(int(p), self.k.real, self.k.imag, self.precision, int(bool(gradients)),
-                                  int(device))
+                                  int(device), M2L_METHODS[m2l], int(dense_budget))
         h = C.c_void_p()
         _lib.check(_lib.lib().fmmh_helmholtz_create(plan.handle, C.byref(o), C.byref(h)))
         self._h = h
@@ -669,6 +680,8 @@ class HelmholtzFmmBuilder:
         self._precision = "f64"
         self._device = 0
         self._near = None
+        self._m2l = "dense"
+        self._dense_budget = 0
 
     def gradients(self, on: bool):
         self._gradients = bool(on)
@@ -686,6 +699,21 @@ class HelmholtzFmmBuilder:
         if precision not in PRECISIONS:
             raise FmmError(3, f"unknown precision {precision!r}")
         self._precision = precision
+        return self
+
+    def m2l(self, method: str):
+        """The M2L: "dense" (the default) or "rotation" (the reference's RCR per pair,
+        HelmholtzM2l in fmm-exec/src/helmholtz/operator.rs:448-479)."""
+        if method not in ("dense", "rotation"):
+            raise FmmError(7, f"Helmholtz M2L: dense or rotation, not {method!r}")
+        self._m2l = method
+        return self
+
+    def dense_budget(self, nbytes: int):
+        """Bytes of dense M2L tables (316 x 2 (p+1)^4 f64 per level, from the finest level up;
+        0 the reference's DEFAULT_DENSE_BUDGET of 1 GiB, negative no limit); the levels
+        beyond it take the per-pair rotation."""
+        self._dense_budget = int(nbytes)
         return self
 
     def device(self, ordinal: int):
@@ -725,5 +753,6 @@ class HelmholtzFmmBuilder:
         else:
             plan = Plan(sources, targets, self._leaf_points, self._max_level, near)
             p = self.degree_for(plan)
-        op = HelmholtzOperator(plan, p, self._k, self._precision, self._gradients, self._device)
+        op = HelmholtzOperator(plan, p, self._k, self._precision, self._gradients, self._device, self._m2l,
+                               self._dense_budget)
         return HelmholtzFmm(plan, op)

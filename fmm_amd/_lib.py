@@ -74,7 +74,8 @@ class Tuning(C.Structure):
 
 class HelmholtzOptions(C.Structure):
     _fields_ = [("p", C.c_int32), ("k_re", C.c_double), ("k_im", C.c_double), ("precision", C.c_int32),
-                ("gradients", C.c_int32), ("device", C.c_int32)]
+                ("gradients", C.c_int32), ("device", C.c_int32), ("m2l", C.c_int32),
+                ("m2l_dense_budget", C.c_int64)]
 
 
 class Stats(C.Structure):

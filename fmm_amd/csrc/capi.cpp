@@ -521,6 +521,8 @@ int32_t fmmh_helmholtz_create(const fmmh_plan *plan, const fmmh_helmholtz_option
     o.precision = options->precision;
     o.gradients = options->gradients;
     o.device = options->device;
+    o.m2l = options->m2l;
+    o.m2l_dense_budget = options->m2l_dense_budget;
     auto h = std::make_unique<fmmh_helmholtz>();
     h->op = std::make_unique<fmmh::HelmholtzOperator>(plan->plan, o);
     h->nlevels = plan->plan.nlevels;
@@ -620,10 +622,54 @@ int32_t fmmh_helmholtz_table(int32_t p, double kh_re, double kh_im, int32_t kind
       else
         fmmh::h_l2l_tables(p, kh, t);
       std::memcpy(out, t.data() + size_t(index) * sz, sz * sizeof(double));
-    } else {
-      need(kind == 2 && index >= 0 && index < 316, "fmmh_helmholtz_table: kind 0..2, offset index 0..315");
+    } else if (kind == 2) {
+      need(index >= 0 && index < 316, "fmmh_helmholtz_table: offset index 0..315");
       fmmh::h_m2l_tables(p, kh, {uint16_t(index)}, t);
       std::memcpy(out, t.data(), sz * sizeof(double));
+    } else {
+      // the M2L composed from the per-pair pieces the device applies: O^T C O
+      need(kind == 3 && index >= 0 && index < 316, "fmmh_helmholtz_table: kind 0..3, offset index 0..315");
+      need(p <= 15, "fmmh_helmholtz_table: the per-pair pieces support p <= 15");
+      int d[3];
+      fmmh::offset_of_index(index, d);
+      std::vector<double> fwd, coax;
+      fmmh::h_rcr_rotations(p, fwd);
+      fmmh::h_rcr_coaxial(p, kh, {2.0 * std::sqrt(double(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]))}, coax);
+      const int N = fmmh::ncoef(p), rb = fmmh::h_rcr_rotation_size(p), cb = fmmh::h_rcr_coaxial_size(p);
+      const double *F = fwd.data() + size_t(index) * rb;
+      auto bo = [](int n) { return n * (2 * n - 1) * (2 * n + 1) / 3; };
+      auto co = [p](int m) {
+        int c = 0;
+        for (int q = 0; q < m; ++q) c += (p + 1 - q) * (p + 1 - q);
+        return c;
+      };
+      std::vector<double> zr(N), zi(N);
+      for (int k = 0; k < N; ++k) {
+        const int nk = int(std::sqrt(double(k))), ck = k - nk * nk, wk = 2 * nk + 1;
+        std::fill(zr.begin(), zr.end(), 0.0);
+        std::fill(zi.begin(), zi.end(), 0.0);
+        for (int np = 0; np <= p; ++np)
+          for (int m = -std::min(np, nk); m <= std::min(np, nk); ++m) {
+            const int am = m < 0 ? -m : m, w = p + 1 - am;
+            const double x = F[bo(nk) + (m + nk) + ck * wk];
+            const size_t a = size_t(co(am)) + (np - am) + size_t(nk - am) * w;
+            zr[np * np + np + m] = coax[a] * x;
+            zi[np * np + np + m] = coax[cb + a] * x;
+          }
+        for (int np = 0; np <= p; ++np) {
+          const int w = 2 * np + 1;
+          for (int c = 0; c < w; ++c) {
+            double vr = 0.0, vi = 0.0;
+            for (int r = 0; r < w; ++r) {
+              const double o = F[bo(np) + r + c * w];
+              vr += o * zr[np * np + r];
+              vi += o * zi[np * np + r];
+            }
+            out[size_t(k) * N + np * np + c] = vr;
+            out[size_t(N) * N + size_t(k) * N + np * np + c] = vi;
+          }
+        }
+      }
     }
   });
 }

@@ -53,6 +53,7 @@ bool all_zero(const double *v, size_t n) {
 HelmholtzOperator::HelmholtzOperator(const Plan &plan, const HelmholtzOptions &options) : opt_(options) {
   require(opt_.p >= 0 && opt_.p <= dev::kMaxDegree, "the device path supports 0 <= p <= 15", kUnsupportedLayout);
   require(opt_.precision == kPlainF64 || opt_.precision == kMixed, "unknown precision", kUnsupportedPrecision);
+  require(opt_.m2l == kM2lDense || opt_.m2l == kM2lRotation, "Helmholtz: M2L dense or rotation", kUnsupportedLayout);
   require(std::isfinite(opt_.kre) && std::isfinite(opt_.kim) && (opt_.kre != 0.0 || opt_.kim != 0.0),
           "Helmholtz: the wavenumber must be finite and nonzero");
   require(opt_.kre >= 0.0 && opt_.kim >= 0.0, "Helmholtz: Re k >= 0 and Im k >= 0 (outgoing, not growing)");
@@ -325,6 +326,136 @@ HelmholtzOperator::HelmholtzOperator(const Plan &plan, const HelmholtzOptions &o
       sb.finish(l2l_[l]);
     }
   }
+  // M2L levels (HelmholtzM2l, fmm-exec/src/helmholtz/operator.rs:448-479): under the dense
+  // choice one complex table per offset on the levels whose tables fit the byte budget,
+  // chosen from the finest level up as DenseM2l::select does
+  // (fmm-tables/src/helmholtz/dense.rs:74-85: 316 x 2 (p+1)^4 f64 per level, never levels 0
+  // and 1), and per-pair rotation on the others; under rotation, per-pair rotation on
+  // every level.
+  std::vector<char> dense_level(nlevels_, 0);
+  if (opt_.m2l == kM2lDense) {
+    const int64_t level_bytes = int64_t(316) * 2 * nc_ * nc_ * int64_t(sizeof(double));
+    int64_t left = opt_.m2l_dense_budget > 0 ? opt_.m2l_dense_budget : kHelmholtzDenseBudget;
+    for (int l = nlevels_ - 1; l >= 2; --l)
+      if (opt_.m2l_dense_budget < 0 || level_bytes <= left) {
+        left -= level_bytes;
+        dense_level[l] = 1;
+      }
+  }
+  {
+    // Per pair: rotate the source multipole so the offset lies along +z, shift coaxially,
+    // rotate back (the reference's RCR).  Pairs by level, then offset: blocks of at most
+    // pb pairs of one (level, offset), in chunks whose scratch rows (two per pair) stay
+    // within kRotScratchBytes; after each chunk a target adds that chunk's rows in the
+    // order of its V list.
+    std::vector<uint32_t> blk_first, blk_coax, pair_src, red_id, red_off{0}, red_ent;
+    std::vector<uint16_t> blk_offset;
+    std::vector<double> fwd, coax;
+    uint32_t npairs = 0, ncoax = 0;
+    int pb = 0;
+    std::vector<std::vector<uint32_t>> level_pairs(nlevels_);  // pair index of each V entry
+    for (int l = 2; l < nlevels_; ++l) {
+      const Csr &V = plan.v[l];
+      if (V.size() == 0 || dense_level[l]) continue;
+      if (rot_rb_ == 0) {
+        rot_rb_ = h_rcr_rotation_size(p);
+        rot_cb_ = h_rcr_coaxial_size(p);
+        h_rcr_rotations(p, fwd);
+        pb = dev::h_rotation_pairs_per_block(p, rot_rb_, rot_cb_);
+        hip_check(dev::h_rotation_allow_lds(p, rot_rb_, rot_cb_), "Helmholtz rotation M2L LDS");
+      }
+      m2l_pairs_ += V.size();
+      // this level's distances |2 d| (by the exact integer |d|^2) and their coaxial blocks
+      std::map<int, uint32_t> coax_of;
+      for (uint32_t e = 0; e < V.size(); ++e) {
+        int d[3];
+        offset_of_index(V.aux[e], d);
+        coax_of.emplace(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], 0u);
+      }
+      std::vector<double> dist;
+      for (auto &kv : coax_of) {
+        kv.second = ncoax++;
+        dist.push_back(2.0 * std::sqrt(double(kv.first)));
+      }
+      std::vector<double> c;
+      h_rcr_coaxial(p, kh[l], dist, c);
+      coax.insert(coax.end(), c.begin(), c.end());
+      std::vector<std::vector<uint32_t>> by_t(316);
+      for (uint32_t e = 0; e < V.size(); ++e) by_t[V.aux[e]].push_back(e);
+      std::vector<uint32_t> &pair_of = level_pairs[l];
+      pair_of.assign(V.size(), 0);
+      for (int t = 0; t < 316; ++t) {
+        if (by_t[t].empty()) continue;
+        int d[3];
+        offset_of_index(t, d);
+        const uint32_t ci = coax_of.at(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        for (size_t k = 0; k < by_t[t].size(); ++k) {
+          if (k % size_t(pb) == 0) {
+            blk_first.push_back(npairs);
+            blk_offset.push_back(static_cast<uint16_t>(t));
+            blk_coax.push_back(ci);
+          }
+          const uint32_t e = by_t[t][k];
+          pair_src.push_back(static_cast<uint32_t>(gid(l, V.ent[e])));
+          pair_of[e] = npairs++;
+        }
+      }
+    }
+    if (npairs > 0) {
+      const uint32_t nblocks = static_cast<uint32_t>(blk_offset.size());
+      blk_first.push_back(npairs);
+      // chunks of whole blocks, at most chunk_pairs pairs each (at least one block)
+      const size_t chunk_pairs = std::max<size_t>(size_t(pb), kRotScratchBytes / (2 * size_t(nc_) * sizeof(double)));
+      rot_chunk_blk_.assign(1, 0);
+      for (uint32_t bk = 0; bk < nblocks; ++bk)
+        if (blk_first[bk + 1] - blk_first[rot_chunk_blk_.back()] > chunk_pairs) rot_chunk_blk_.push_back(bk);
+      rot_chunk_blk_.push_back(nblocks);
+      const size_t nchunks = rot_chunk_blk_.size() - 1;
+      std::vector<uint32_t> chunk_q(nchunks + 1);
+      for (size_t c = 0; c <= nchunks; ++c) chunk_q[c] = blk_first[rot_chunk_blk_[c]];
+      rot_chunk_q_ = chunk_q;
+      // per chunk, per target (level by level, box order) and plane: its rows of that chunk
+      // in V order
+      std::vector<std::vector<uint32_t>> cid(nchunks), coff(nchunks), cent(nchunks);
+      for (size_t c = 0; c < nchunks; ++c) coff[c].push_back(0);
+      for (int l = 2; l < nlevels_; ++l) {
+        const Csr &V = plan.v[l];
+        if (level_pairs[l].empty()) continue;
+        for (uint32_t b = 0; b < plan.nboxes(l); ++b) {
+          std::map<size_t, std::vector<uint32_t>> by_chunk;
+          for (uint32_t e = V.off[b]; e < V.off[b + 1]; ++e) {
+            const uint32_t q = level_pairs[l][e];
+            const size_t c = size_t(std::upper_bound(chunk_q.begin(), chunk_q.end(), q) - chunk_q.begin()) - 1;
+            by_chunk[c].push_back(q);
+          }
+          for (const auto &kv : by_chunk)
+            for (int plane = 0; plane < 2; ++plane) {
+              cid[kv.first].push_back(static_cast<uint32_t>(3 * gid(l, b) + plane));
+              for (uint32_t q : kv.second) cent[kv.first].push_back(2 * (q - chunk_q[kv.first]) + plane);
+              coff[kv.first].push_back(static_cast<uint32_t>(cent[kv.first].size()));
+            }
+        }
+      }
+      rot_chunk_row_.assign(1, 0);
+      for (size_t c = 0; c < nchunks; ++c) {
+        const uint32_t base = static_cast<uint32_t>(red_ent.size());
+        red_id.insert(red_id.end(), cid[c].begin(), cid[c].end());
+        for (size_t r = 1; r < coff[c].size(); ++r) red_off.push_back(base + coff[c][r]);
+        red_ent.insert(red_ent.end(), cent[c].begin(), cent[c].end());
+        rot_chunk_row_.push_back(static_cast<uint32_t>(red_id.size()));
+      }
+      size_t most = 0;
+      for (size_t c = 0; c < nchunks; ++c) most = std::max<size_t>(most, chunk_q[c + 1] - chunk_q[c]);
+      rot_fwd_.upload(fwd);
+      rot_coax_.upload(coax);
+      rot_blk_first_.upload(blk_first);
+      rot_blk_offset_.upload(blk_offset);
+      rot_blk_coax_.upload(blk_coax);
+      rot_pair_src_.upload(pair_src);
+      rot_slots_.alloc(2 * most * nc_);
+      upload_rows(rot_red_, red_id, {}, red_off, red_ent);
+    }
+  }
   // M2L: per level the offsets present, ids base_l + 2 j + plane; tiles of 64 targets of
   // one octant class, classes interleaved, groups by (parent-level neighbour, octant)
   {
@@ -336,7 +467,7 @@ HelmholtzOperator::HelmholtzOperator(const Plan &plan, const HelmholtzOptions &o
     uint32_t id_base = 0;
     for (int l = 2; l < nlevels_; ++l) {
       const Csr &V = plan.v[l];
-      if (V.size() == 0) continue;
+      if (V.size() == 0 || !dense_level[l]) continue;
       m2l_pairs_ += V.size();
       std::vector<uint16_t> used(V.aux.begin(), V.aux.end());
       std::sort(used.begin(), used.end());
@@ -491,7 +622,8 @@ void HelmholtzOperator::wait_for(hipStream_t producer) {
 
 size_t HelmholtzOperator::device_bytes() const {
   return mult_.bytes() + loc_.bytes() + t_m2m_.bytes() + t_l2l_.bytes() + t_m2l_.bytes() + t_m2l16_.bytes() +
-         mult16_.bytes() + sx_.bytes() * 8;
+         mult16_.bytes() + sx_.bytes() * 8 + slots_.bytes() + slots32_.bytes() + rot_fwd_.bytes() +
+         rot_coax_.bytes() + rot_slots_.bytes();
 }
 
 void HelmholtzOperator::set_timing(bool enable) {
@@ -599,6 +731,19 @@ void HelmholtzOperator::evaluate() {
                           s.col_out.ptr, t_m2m_.ptr, mult_.ptr, mult_.ptr, none, nullptr, 0);
     });
   // M2L of every level in one launch
+  // the per-pair rotation levels, chunk by chunk
+  for (size_t c = 0; c + 1 < rot_chunk_blk_.size(); ++c)
+    timed(3, [&] {
+      const uint32_t b0 = rot_chunk_blk_[c], b1 = rot_chunk_blk_[c + 1];
+      timed(kKindM2lContract, [&] {
+        dev::h_m2l_rotation(stream_, p, b1 - b0, rot_chunk_q_[c], rot_blk_first_.ptr + b0, rot_blk_offset_.ptr + b0,
+                            rot_blk_coax_.ptr + b0, rot_pair_src_.ptr, rot_fwd_.ptr, rot_coax_.ptr, rot_rb_, rot_cb_,
+                            mult_.ptr, rot_slots_.ptr);
+      });
+      const uint32_t r0 = rot_chunk_row_[c], r1 = rot_chunk_row_[c + 1];
+      dev::reduce_slots(stream_, p, r1 - r0, rot_red_.row_id.ptr + r0, rot_red_.row_off.ptr + r0,
+                        rot_red_.row_ent.ptr, 0, rot_slots_.ptr, nullptr, loc_.ptr);
+    });
   if (m2l_.ntasks)
     timed(3, [&] {
       if (opt_.precision == kMixed) {

@@ -26,7 +26,17 @@ struct HelmholtzOptions {
   int precision = kPlainF64;
   int gradients = 1;
   int device = 0;
+  int m2l = 0;  // kM2lDense: dense complex tables per level on MFMA where they fit the
+                // budget, per-pair rotation elsewhere; kM2lRotation: per-pair rotation,
+                // coaxial shift, rotation back (f64) on every level, tables O(p^3) per distance
+  int64_t m2l_dense_budget = 0;  // bytes of dense tables (316 x 2 (p+1)^4 f64 per level): 0 the
+                                 // reference's default (1 GiB), negative no limit
 };
+
+// DEFAULT_DENSE_BUDGET of fmm-exec/src/helmholtz/operator.rs:783, 1 GiB
+constexpr int64_t kHelmholtzDenseBudget = int64_t(1) << 30;
+// scratch rows of the per-pair rotation M2L: at most this many bytes at once
+constexpr size_t kRotScratchBytes = size_t(256) << 20;
 
 class HelmholtzOperator {
  public:
@@ -89,6 +99,14 @@ class HelmholtzOperator {
   DevBuf<double> slots_;
   DevBuf<float> slots32_;
   Rows m2l_red_;
+  // rotation M2L: the offsets' rotation blocks, the (level, distance) coaxial blocks, blocks
+  // of pairs of one (level, offset), two scratch rows per pair and their ordered reduction
+  int rot_rb_ = 0, rot_cb_ = 0;
+  std::vector<uint32_t> rot_chunk_blk_, rot_chunk_q_, rot_chunk_row_;  // per chunk: first block, pair, row
+  DevBuf<double> rot_fwd_, rot_coax_, rot_slots_;
+  DevBuf<uint32_t> rot_blk_first_, rot_blk_coax_, rot_pair_src_;
+  DevBuf<uint16_t> rot_blk_offset_;
+  Rows rot_red_;
 
   bool timing_ = false;
   struct Mark {

@@ -351,6 +351,64 @@ void h_l2l_tables(int p, cplx kh, std::vector<double> &out) {
   }
 }
 
+int h_rcr_rotation_size(int p) { return (p + 1) * (2 * p + 1) * (2 * p + 3) / 3; }
+
+int h_rcr_coaxial_size(int p) {
+  int s = 0;
+  for (int m = 0; m <= p; ++m) s += (p + 1 - m) * (p + 1 - m);
+  return s;
+}
+
+void h_rcr_rotations(int p, std::vector<double> &out) {
+  std::vector<double> norm;
+  h_norm_factors(p, norm);
+  const int rb = h_rcr_rotation_size(p);
+  out.assign(316 * size_t(rb), 0.0);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int i = 0; i < 316; ++i) {
+    int d[3];
+    offset_of_index(i, d);
+    const double t[3] = {2.0 * d[0], 2.0 * d[1], 2.0 * d[2]};
+    double Q[9];
+    rotation_to_z(t, Q);
+    std::vector<double> O;
+    std::vector<int> boff;
+    rotation_blocks(p, Q, norm, O, boff);
+    for (int n = 0; n <= p; ++n) {
+      const int w = 2 * n + 1;
+      for (int r = 0; r < w; ++r)
+        for (int c = 0; c < w; ++c) out[size_t(i) * rb + boff[n] + r + size_t(c) * w] = O[boff[n] + r * w + c];
+    }
+  }
+}
+
+void h_rcr_coaxial(int p, cplx kh, const std::vector<double> &dist, std::vector<double> &out) {
+  const int cb = h_rcr_coaxial_size(p), C = p + 1;
+  out.assign(2 * size_t(cb) * dist.size(), 0.0);
+  const double s = h_scale(kh);
+  std::vector<double> sp(p + 1);
+  for (int n = 0; n <= p; ++n) sp[n] = std::pow(s, n);
+  const cplx common = 4.0 * kPi * cplx(0.0, 1.0) * kh * std::exp(2.0 * h_beta(kh));
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int i = 0; i < int(dist.size()); ++i) {
+    std::vector<cplx> f, E;
+    spherical_h(2 * p, kh * dist[i], f);
+    coaxial(p, f, E);
+    double *re = out.data() + 2 * size_t(cb) * i, *im = re + cb;
+    int co = 0;
+    for (int m = 0; m <= p; ++m) {
+      const int w = p + 1 - m;
+      for (int np = m; np <= p; ++np)
+        for (int n = m; n <= p; ++n) {
+          const cplx v = common * sp[np] * sp[n] * E[(size_t(m) * C + np) * C + n];
+          re[co + (np - m) + (n - m) * w] = v.real();
+          im[co + (np - m) + (n - m) * w] = v.imag();
+        }
+      co += w * w;
+    }
+  }
+}
+
 void h_m2l_tables(int p, cplx kh, const std::vector<uint16_t> &offsets, std::vector<double> &out) {
   std::vector<double> norm;
   h_norm_factors(p, norm);

@@ -73,6 +73,18 @@ void m2l_rotation(hipStream_t s, int p, uint32_t nblocks, const uint32_t *blk_fi
                   int cb, const T *M, T *S);
 // b[i] = f32(a[i]), i < n
 void to_f32(hipStream_t s, size_t n, const double *a, float *b);
+// Helmholtz rotation M2L (rotation.hip): blocks of at most h_rotation_pairs_per_block pairs
+// of one (level, offset); fwd: 316 offsets' rotation blocks (rb each), coax: per (level,
+// distance) entry blk_coax[b] the coaxial blocks, real then imaginary plane (2 cb each);
+// M: three planes per box (re, im, -im); S: two scratch rows per pair (re, im), pair q at
+// rows 2 (q - q_base), 2 (q - q_base) + 1 (blk_first, blk_offset, blk_coax start at the
+// launch's first block; its pairs from q_base on).  h_rotation_allow_lds once per
+// process before the first launch.
+int h_rotation_pairs_per_block(int p, int rb, int cb);
+hipError_t h_rotation_allow_lds(int p, int rb, int cb);
+void h_m2l_rotation(hipStream_t s, int p, uint32_t nblocks, uint32_t q_base, const uint32_t *blk_first,
+                    const uint16_t *blk_offset, const uint32_t *blk_coax, const uint32_t *pair_src, const double *fwd,
+                    const double *coax, int rb, int cb, const double *M, double *S);
 
 // Compensated f64 direct sum (unscaled): chunks of sources per block, partials in
 // part[nchunks][nt][8], then added per target in chunk order.

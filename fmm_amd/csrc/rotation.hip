@@ -184,6 +184,159 @@ void m2l_rotation(hipStream_t s, int p, uint32_t nblocks, const uint32_t *blk_fi
                      p, pb, nblocks, blk_first, blk_offset, pair_src, dist_of, fwd, bwd, coax, rb, cb, M, S);
 }
 
+// ---- Helmholtz: the same three steps on complex expansions ------------------------------------
+//
+// A pair's multipole is two real planes (re, im); the rotation blocks are real, so steps
+// 1 and 3 treat the two planes as two rows, and the back rotation is the transpose of the
+// forward one (orthogonal blocks: only F is staged).  The coaxial blocks are complex (real
+// plane, then imaginary plane) and per (level, distance): step 2 forms
+// Z = (Ar Yr - Ai Yi) + i (Ar Yi + Ai Yr).  Row 2 j of the staging buffers is pair j's
+// real plane, row 2 j + 1 its imaginary plane; the pair's result goes to scratch rows
+// 2 q (real) and 2 q + 1 (imaginary) for the ordered reduction.
+constexpr int kHRotJ = 4;  // complex pairs per thread item of step 2
+
+static size_t h_rot_lds(int p, int rb, int cb, int pb) {
+  const int nc = (p + 1) * (p + 1);
+  return sizeof(double) * (size_t(rb) + 2 * size_t(cb) + size_t(2) * 2 * pb * (nc + 1));
+}
+
+__global__ __launch_bounds__(kRotThreads) void k_h_m2l_rot_pairs(int p, int pb, uint32_t nblocks, uint32_t q_base,
+                                                                 const uint32_t *__restrict__ blk_first,
+                                                                 const uint16_t *__restrict__ blk_offset,
+                                                                 const uint32_t *__restrict__ blk_coax,
+                                                                 const uint32_t *__restrict__ pair_src,
+                                                                 const double *__restrict__ fwd,
+                                                                 const double *__restrict__ coax, int rb, int cb,
+                                                                 const double *__restrict__ M,
+                                                                 double *__restrict__ S) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t blk = xcd_index(nblocks);
+  if (blk >= nblocks) return;  // uniform over the workgroup
+  const int nc = (p + 1) * (p + 1), ld = nc + 1, rows = 2 * pb;
+  double *tF = reinterpret_cast<double *>(smem);
+  double *tAr = tF + rb;
+  double *tAi = tAr + cb;
+  double *X = tAi + cb;
+  double *Y = X + size_t(rows) * ld;
+  const int tid = threadIdx.x;
+  const uint32_t t = blk_offset[blk], a = blk_coax[blk];
+  const uint32_t q0 = blk_first[blk], np = blk_first[blk + 1] - q0;  // np <= pb
+  __shared__ uint32_t src[64];
+  if (tid < pb) src[tid] = uint32_t(tid) < np ? pair_src[q0 + tid] : 0u;
+#pragma unroll 4
+  for (int i = tid; i < rb; i += kRotThreads) tF[i] = fwd[size_t(t) * rb + i];
+#pragma unroll 4
+  for (int i = tid; i < cb; i += kRotThreads) {
+    tAr[i] = coax[2 * size_t(a) * cb + i];
+    tAi[i] = coax[(2 * size_t(a) + 1) * cb + i];
+  }
+  __syncthreads();
+  // the pairs' source planes: box s holds planes 3 s (re) and 3 s + 1 (im)
+#pragma unroll 4
+  for (int i = tid; i < rows * nc; i += kRotThreads) {
+    const int r = i / nc, k = i - r * nc, j = r >> 1;
+    X[r * ld + k] = uint32_t(j) < np ? M[(3 * size_t(src[j]) + (r & 1)) * nc + k] : 0.0;
+  }
+  __syncthreads();
+  const int items = nc * (rows / kRotJ);
+  // 1. rotate every row: Y_r = F X_r, per degree block
+  for (int it = tid; it < items; it += kRotThreads) {
+    const int jg = it / nc, k = it - jg * nc;
+    int n = 0;
+    while ((n + 1) * (n + 1) <= k) ++n;
+    const int r = k - n * n, w = 2 * n + 1, bo = n * (2 * n - 1) * (2 * n + 1) / 3;
+    const double *x = X + size_t(jg) * kRotJ * ld + n * n;
+    double y[kRotJ];
+#pragma unroll
+    for (int jj = 0; jj < kRotJ; ++jj) y[jj] = 0.0;
+    for (int c = 0; c < w; ++c) {
+      const double f = tF[bo + r + c * w];
+#pragma unroll
+      for (int jj = 0; jj < kRotJ; ++jj) y[jj] = fma(f, x[jj * ld + c], y[jj]);
+    }
+    double *yo = Y + size_t(jg) * kRotJ * ld + k;
+#pragma unroll
+    for (int jj = 0; jj < kRotJ; ++jj) yo[jj * ld] = y[jj];
+  }
+  __syncthreads();
+  // 2. complex coaxial M2L, order i = |m|: Z(n, m) = sum_q A_i[n - i][q - i] Y(q, m)
+  const int citems = nc * (pb / kHRotJ);
+  for (int it = tid; it < citems; it += kRotThreads) {
+    const int jg = it / nc, k = it - jg * nc;
+    int n = 0;
+    while ((n + 1) * (n + 1) <= k) ++n;
+    const int m = k - n * n - n, i = m < 0 ? -m : m, cw = p + 1 - i;
+    int co = 0;  // sum of (p+1-i')^2 over i' < i
+    for (int q = 0; q < i; ++q) co += (p + 1 - q) * (p + 1 - q);
+    const double *ar = tAr + co + (n - i), *ai = tAi + co + (n - i);
+    const double *yv = Y + size_t(jg) * 2 * kHRotJ * ld;
+    double zr[kHRotJ], zi[kHRotJ];
+#pragma unroll
+    for (int jj = 0; jj < kHRotJ; ++jj) zr[jj] = zi[jj] = 0.0;
+    for (int q = i; q <= p; ++q) {
+      const double vr = ar[(q - i) * cw], vi = ai[(q - i) * cw];
+      const int kq = q * q + q + m;
+#pragma unroll
+      for (int jj = 0; jj < kHRotJ; ++jj) {
+        const double yr = yv[(2 * jj) * ld + kq], yi = yv[(2 * jj + 1) * ld + kq];
+        zr[jj] = fma(vr, yr, fma(-vi, yi, zr[jj]));
+        zi[jj] = fma(vr, yi, fma(vi, yr, zi[jj]));
+      }
+    }
+    double *xo = X + size_t(jg) * 2 * kHRotJ * ld + k;
+#pragma unroll
+    for (int jj = 0; jj < kHRotJ; ++jj) {
+      xo[(2 * jj) * ld] = zr[jj];
+      xo[(2 * jj + 1) * ld] = zi[jj];
+    }
+  }
+  __syncthreads();
+  // 3. rotate back with the transpose: row_r = F^T X_r, into the pairs' scratch rows
+  for (int it = tid; it < items; it += kRotThreads) {
+    const int jg = it / nc, k = it - jg * nc;
+    int n = 0;
+    while ((n + 1) * (n + 1) <= k) ++n;
+    const int r = k - n * n, w = 2 * n + 1, bo = n * (2 * n - 1) * (2 * n + 1) / 3;
+    const double *x = X + size_t(jg) * kRotJ * ld + n * n;
+    double y[kRotJ];
+#pragma unroll
+    for (int jj = 0; jj < kRotJ; ++jj) y[jj] = 0.0;
+    for (int c = 0; c < w; ++c) {
+      const double f = tF[bo + c + r * w];
+#pragma unroll
+      for (int jj = 0; jj < kRotJ; ++jj) y[jj] = fma(f, x[jj * ld + c], y[jj]);
+    }
+#pragma unroll
+    for (int jj = 0; jj < kRotJ; ++jj) {
+      const uint32_t row = uint32_t(jg * kRotJ + jj);
+      if ((row >> 1) < np) S[(2 * size_t(q0 - q_base) + row) * nc + k] = y[jj];
+    }
+  }
+}
+
+// pairs per block: 32, or the most (a multiple of kHRotJ) whose tables and staging fit
+int h_rotation_pairs_per_block(int p, int rb, int cb) {
+  int pb = 32;
+  while (pb > kHRotJ && h_rot_lds(p, rb, cb, pb) > kRotLdsMax) pb /= 2;
+  return pb;
+}
+
+hipError_t h_rotation_allow_lds(int p, int rb, int cb) {
+  const size_t bytes = h_rot_lds(p, rb, cb, h_rotation_pairs_per_block(p, rb, cb));
+  if (bytes > kRotLdsMax) return hipErrorInvalidValue;
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_h_m2l_rot_pairs),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+}
+
+void h_m2l_rotation(hipStream_t s, int p, uint32_t nblocks, uint32_t q_base, const uint32_t *blk_first,
+                    const uint16_t *blk_offset, const uint32_t *blk_coax, const uint32_t *pair_src, const double *fwd,
+                    const double *coax, int rb, int cb, const double *M, double *S) {
+  if (nblocks == 0) return;
+  const int pb = h_rotation_pairs_per_block(p, rb, cb);
+  hipLaunchKernelGGL(k_h_m2l_rot_pairs, dim3(xcd_grid(nblocks)), dim3(kRotThreads), h_rot_lds(p, rb, cb, pb), s, p,
+                     pb, nblocks, q_base, blk_first, blk_offset, blk_coax, pair_src, fwd, coax, rb, cb, M, S);
+}
+
 template int rotation_pairs_per_block<double>(int, int, int);
 template int rotation_pairs_per_block<float>(int, int, int);
 template hipError_t rotation_allow_lds<double>(int, int, int);

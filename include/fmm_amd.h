@@ -286,6 +286,14 @@ typedef struct {
   int32_t precision; /* fmmh_precision: plain f64, or f32 P2P and bf16x3 M2L */
   int32_t gradients;
   int32_t device;
+  int32_t m2l;       /* fmmh_m2l_method (HelmholtzM2l, fmm-exec/src/helmholtz/operator.rs:448-479):
+                        0 dense: one complex table per offset (MFMA contraction) on the levels
+                        whose tables fit m2l_dense_budget, from the finest level up
+                        (DenseM2l::select), per-pair rotation on the others; 1 rotation:
+                        per pair rotation, coaxial shift, rotation back in f64 (RCR) on every
+                        level, tables O(p^3) per distance instead of O(p^4) per offset */
+  int64_t m2l_dense_budget; /* bytes of dense tables, counted as 316 x 2 (p+1)^4 f64 per level: 0
+                               the reference's default (1 GiB), negative no limit */
 } fmmh_helmholtz_options;
 
 /* The degree rule at tolerance eps for boxes of half-width radius (fmm-exec/src/helmholtz/
@@ -316,8 +324,11 @@ int32_t fmmh_helmholtz_kind_timing(fmmh_helmholtz *op, int32_t enable);
 int32_t fmmh_helmholtz_kind_times(fmmh_helmholtz *op, double *ms, int64_t *launches);
 /* One translation table of a level whose wavenumber in its own units is kh (host):
  * kind 0 M2M from the child of octant `index` (kh of the parent level), 1 L2L into that
- * child, 2 M2L of offset index `index`; out[2 (p+1)^4] real plane then imaginary plane,
- * column-major (the operators of fmm-tables/src/helmholtz/dense.rs). */
+ * child, 2 M2L of offset index `index`, 3 the same M2L composed from the pieces of the
+ * per-pair rotation path (rotation blocks, scaled coaxial blocks, rotation back; the
+ * RCR of fmm-exec/src/helmholtz/operator.rs:448-461), for tests; out[2 (p+1)^4] real
+ * plane then imaginary plane, column-major (the operators of
+ * fmm-tables/src/helmholtz/dense.rs). */
 int32_t fmmh_helmholtz_table(int32_t p, double kh_re, double kh_im, int32_t kind, int32_t index, double *out);
 /* Compensated f64 direct sum of q e^{ikr}/r and its gradient (ik r - 1) e^{ikr} d / r^3
  * (no 1/(4 pi)), nd_fmm_ref::helmholtz::p2p::direct_sum (fmm-ref/src/helmholtz/p2p.rs:155);

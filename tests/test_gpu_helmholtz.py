@@ -1,8 +1,9 @@
 """GPU parity of the Helmholtz path (kernel e^{ikr}/(4 pi r), complex charges) through the
 C ABI: the compensated direct sum against the oracle and the reference's FMM3D fixture;
 the plain f64 FMM against the oracle Helmholtz FMM on the same tree (1e-12 relative L2) on
-a uniform and an adaptive tree (W and X lists); the mixed precision (f32 P2P, bf16x3 M2L)
-against the direct sum at the degree rule's p; real, complex and imaginary (Yukawa) k."""
+a uniform and an adaptive tree (W and X lists), with the dense M2L and with the per-pair
+rotation M2L (the reference's RCR); the mixed precision (f32 P2P, bf16x3 M2L) against the
+direct sum at the degree rule's p; real, complex and imaginary (Yukawa) k."""
 import os
 import sys
 
@@ -87,6 +88,72 @@ def test_plain_f64_matches_oracle_fmm(kind, n, p, leaf, k):
     dp, dg = fmm_amd.helmholtz_direct_sum(k, pts, q, pts[s])
     _log(f"  vs direct: phi {_rel(out.potential[s], dp / (4 * np.pi)):.2e}")
     assert _rel(out.potential[s], dp / (4 * np.pi)) < 1e-3
+
+
+@pytest.mark.parametrize("kind,n,p,leaf,k", CASES)
+def test_rotation_m2l_matches_oracle_fmm(kind, n, p, leaf, k):
+    """The per-pair rotation M2L (rotation, coaxial shift, rotation back in f64) against the
+    oracle Helmholtz FMM (1e-12) and the dense path on the same plan."""
+    _log(f"case helmholtz rotation m2l {kind} n={n} p={p} leaf={leaf} k={k}")
+    pts = _points(kind, n)
+    q = _charges(n)
+    rot = fmm_amd.HelmholtzFmmBuilder(k, degree=p).max_points_per_leaf(leaf).m2l("rotation").build(pts)
+    dense = fmm_amd.HelmholtzFmm(rot.plan, fmm_amd.HelmholtzOperator(rot.plan, p, k))
+    out, d = rot.evaluate(q), dense.evaluate(q)
+    ref = O.Fmm(pts, p=p, leaf_points=leaf)
+    ophi, ograd = ref.evaluate_helmholtz(k, q)
+    ep, eg = _rel(out.potential, ophi), _rel(out.gradient, ograd)
+    _log(f"  vs oracle: phi {ep:.2e} grad {eg:.2e}; vs dense {_rel(out.potential, d.potential):.2e}")
+    assert ep < 1e-12 and eg < 1e-12
+    assert _rel(out.potential, d.potential) < 1e-12 and _rel(out.gradient, d.gradient) < 1e-12
+    # the translation tables: O(p^3) per distance instead of O(p^4) per offset and level
+    assert rot.operator.stats()["device_bytes"] < dense.operator.stats()["device_bytes"]
+
+
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+def test_dense_budget_mixes_dense_and_rotation_levels(precision):
+    """HelmholtzM2l::Dense under a byte budget (DenseM2l::select): dense tables on the finest
+    levels that fit, per-pair rotation on the coarser ones; the mix against every level
+    dense and against the oracle."""
+    n, p, leaf, k = 6000, 10, 32, 14.0 + 0.2j
+    _log(f"case helmholtz dense budget n={n} p={p} {precision}")
+    pts = _points("uniform", n)
+    q = _charges(n)
+    level = 316 * 2 * (p + 1) ** 4 * 8
+    b = fmm_amd.HelmholtzFmmBuilder(k, degree=p).max_points_per_leaf(leaf).precision(precision)
+    mix = b.dense_budget(level).build(pts)  # the finest level dense, the others rotation
+    assert mix.plan.nlevels >= 4
+    full = fmm_amd.HelmholtzFmm(mix.plan, fmm_amd.HelmholtzOperator(mix.plan, p, k, precision, dense_budget=-1))
+    a, f = mix.evaluate(q), full.evaluate(q)
+    bar = 1e-12 if precision == "f64" else 1e-6
+    _log(f"  mix vs all dense: phi {_rel(a.potential, f.potential):.2e} grad {_rel(a.gradient, f.gradient):.2e}")
+    assert _rel(a.potential, f.potential) < bar and _rel(a.gradient, f.gradient) < bar
+    if precision == "f64":
+        ophi, ograd = O.Fmm(pts, p=p, leaf_points=leaf).evaluate_helmholtz(k, q)
+        assert _rel(a.potential, ophi) < 1e-12 and _rel(a.gradient, ograd) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+def test_rotation_m2l_at_the_degree_rule(precision):
+    """The rotation M2L at the degree rule for 1e-6 (p = 15 at k = 25 here, the 256-point
+    leaves of the reference's default): within the tolerance of the direct sum, and equal
+    to the dense path's plain f64 result to 1e-10."""
+    n, k = 20000, 25.0
+    _log(f"case helmholtz rotation m2l at the degree rule n={n} {precision}")
+    pts = _points("uniform", n)
+    q = _charges(n)
+    b = fmm_amd.HelmholtzFmmBuilder(k, tolerance=1e-6).near_wx("device").precision(precision)
+    rot = b.m2l("rotation").build(pts)
+    s = np.arange(0, n, 97)
+    dp, dg = fmm_amd.helmholtz_direct_sum(k, pts, q, pts[s])
+    out = rot.evaluate(q)
+    ep, eg = _rel(out.potential[s], dp / (4 * np.pi)), _rel(out.gradient[s], dg / (4 * np.pi))
+    _log(f"  p={rot.p}: vs direct phi {ep:.2e} grad {eg:.2e}")
+    assert ep < 1e-6 and eg < 1e-6
+    if precision == "f64":
+        dense = fmm_amd.HelmholtzFmm(rot.plan, fmm_amd.HelmholtzOperator(rot.plan, rot.p, k))
+        d = dense.evaluate(q)
+        assert _rel(out.potential, d.potential) < 1e-10 and _rel(out.gradient, d.gradient) < 1e-10
 
 
 def test_near_rule_matches_oracle_fmm():

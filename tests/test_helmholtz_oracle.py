@@ -191,6 +191,19 @@ def test_product_tables_match_oracle_tables():
             assert np.abs(A - B).max() <= 1e-13 * np.abs(B).max(), (p, kh, kind, i)
 
 
+def test_rotation_pieces_compose_the_oracle_tables():
+    # the device's per-pair M2L pieces (rotation blocks per offset, coaxial blocks per
+    # distance with the stored forms' scales, the transpose as the rotation back) composed
+    # on the host: the oracle's M2L table of every checked offset, to rounding
+    import fmm_amd
+
+    for p, kh in [(4, 0.9), (8, 1.3 + 0.2j), (12, 3.1), (15, 2.2), (10, 0.05), (8, 2.0j)]:
+        for i in (0, 37, 73, 150, 211, 315):
+            A = fmm_amd.helmholtz_table(p, kh, "m2l_rcr", i)
+            B = O.h_m2l(p, kh, O.offset_of_index(i))
+            assert np.abs(A - B).max() <= 1e-13 * np.abs(B).max(), (p, kh, i)
+
+
 def test_degree_rule_matches_the_reference_examples():
     """fmm-exec/src/helmholtz/degree.rs:63-76: the plateau 15 at 1e-6 and kD = 100 on the
     unit cube giving 35 and 22 on levels 2 and 3."""
