@@ -405,7 +405,7 @@ HelmholtzOperator::HelmholtzOperator(const Plan &plan, const HelmholtzOptions &o
       const uint32_t nblocks = static_cast<uint32_t>(blk_offset.size());
       blk_first.push_back(npairs);
       // chunks of whole blocks, at most chunk_pairs pairs each (at least one block)
-      const size_t chunk_pairs = std::max<size_t>(size_t(pb), kRotScratchBytes / (2 * size_t(nc_) * sizeof(double)));
+      const size_t chunk_pairs = std::max<size_t>(size_t(pb), dev::kRotScratchBytes / (2 * size_t(nc_) * sizeof(double)));
       rot_chunk_blk_.assign(1, 0);
       for (uint32_t bk = 0; bk < nblocks; ++bk)
         if (blk_first[bk + 1] - blk_first[rot_chunk_blk_.back()] > chunk_pairs) rot_chunk_blk_.push_back(bk);

@@ -35,8 +35,6 @@ struct HelmholtzOptions {
 
 // DEFAULT_DENSE_BUDGET of fmm-exec/src/helmholtz/operator.rs:783, 1 GiB
 constexpr int64_t kHelmholtzDenseBudget = int64_t(1) << 30;
-// scratch rows of the per-pair rotation M2L: at most this many bytes at once
-constexpr size_t kRotScratchBytes = size_t(256) << 20;
 
 class HelmholtzOperator {
  public:

@@ -13,6 +13,9 @@ constexpr int kMaxCoef = (kMaxDegree + 1) * (kMaxDegree + 1);
 constexpr int kMaxCoefGrad = (kMaxDegree + 2) * (kMaxDegree + 2);
 constexpr int COLS = 128;  // output columns (boxes) per contraction task: 8 waves x 16
 constexpr int KC = 4;      // MFMA k-steps per LDS chunk of a table
+// scratch rows of the per-pair rotation M2L (one or two rows per pair): at most this many
+// bytes at once; the pairs run in chunks, each followed by its ordered reduction
+constexpr size_t kRotScratchBytes = size_t(512) << 20;
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2: block b
 // runs on XCD b % 8.  With a chunk C > 0, xcd_index deals chunks of C consecutive items
@@ -62,15 +65,17 @@ void build_source_records(const double *sx, const double *sy, const double *sz, 
 // Rotation M2L (rotation.hip): blocks of at most rotation_pairs_per_block pairs of one
 // offset (blk_offset), pairs blk_first[b] .. blk_first[b + 1] with source boxes pair_src;
 // tables per offset (fwd, bwd: rb values) and per distance (coax: cb values); pair q's
-// result goes to scratch row q of S (nc values), added to the locals by reduce_slots.
+// result goes to scratch row q - q_base of S (nc values; blk_first and blk_offset start
+// at the launch's first block, its pairs from q_base on), added to the locals by
+// reduce_slots.
 template <typename T>
 int rotation_pairs_per_block(int p, int rb, int cb);
 template <typename T>
 hipError_t rotation_allow_lds(int p, int rb, int cb);
 template <typename T>
-void m2l_rotation(hipStream_t s, int p, uint32_t nblocks, const uint32_t *blk_first, const uint16_t *blk_offset,
-                  const uint32_t *pair_src, const uint16_t *dist_of, const T *fwd, const T *bwd, const T *coax, int rb,
-                  int cb, const T *M, T *S);
+void m2l_rotation(hipStream_t s, int p, uint32_t nblocks, uint32_t q_base, const uint32_t *blk_first,
+                  const uint16_t *blk_offset, const uint32_t *pair_src, const uint16_t *dist_of, const T *fwd,
+                  const T *bwd, const T *coax, int rb, int cb, const T *M, T *S);
 // b[i] = f32(a[i]), i < n
 void to_f32(hipStream_t s, size_t n, const double *a, float *b);
 // Helmholtz rotation M2L (rotation.hip): blocks of at most h_rotation_pairs_per_block pairs

@@ -630,12 +630,57 @@ void DeviceOperator::build_rotation(const Plan &plan) {
   rot_blk_first_.upload(blk_first);
   rot_blk_offset_.upload(blk_offset);
   rot_pair_src_.upload(pair_src);
-  rot_pair_of_.upload(pair_of);
+  // chunks of whole blocks within each level, at most chunk_pairs pairs of scratch rows
+  const size_t chunk_pairs =
+      std::max<size_t>(size_t(pb), dev::kRotScratchBytes / (size_t(nc_) * (mixed ? sizeof(float) : sizeof(double))));
+  rot_chunk_blk_.clear();
+  rot_chunk_level_.assign(nlevels_ + 1, 0);
+  for (int l = 0; l < nlevels_; ++l) {
+    rot_chunk_level_[l] = static_cast<uint32_t>(rot_chunk_blk_.size());
+    for (uint32_t bk = rot_blk_level_[l]; bk < rot_blk_level_[l + 1]; ++bk)
+      if (bk == rot_blk_level_[l] || blk_first[bk + 1] - blk_first[rot_chunk_blk_.back()] > chunk_pairs)
+        rot_chunk_blk_.push_back(bk);
+  }
+  rot_chunk_level_[nlevels_] = static_cast<uint32_t>(rot_chunk_blk_.size());
+  const size_t nchunks = rot_chunk_blk_.size();
+  rot_chunk_blk_.push_back(static_cast<uint32_t>(blk_offset.size()));
+  rot_chunk_q_.assign(nchunks + 1, npairs);
+  for (size_t c = 0; c < nchunks; ++c) rot_chunk_q_[c] = blk_first[rot_chunk_blk_[c]];
+  // per chunk, its targets' rows (level by level, row order) with that chunk's pairs in V
+  // order, relative to the chunk's first pair
+  std::vector<std::vector<uint32_t>> cid(nchunks), coff(nchunks), cent(nchunks);
+  for (size_t c = 0; c < nchunks; ++c) coff[c].push_back(0);
+  for (uint32_t r = 0; r + 1 < off.size(); ++r) {
+    std::map<size_t, std::vector<uint32_t>> by_chunk;
+    for (uint32_t e = off[r]; e < off[r + 1]; ++e) {
+      const uint32_t q = pair_of[e];
+      const size_t c = size_t(std::upper_bound(rot_chunk_q_.begin(), rot_chunk_q_.begin() + nchunks, q) -
+                              rot_chunk_q_.begin()) - 1;
+      by_chunk[c].push_back(q - rot_chunk_q_[c]);
+    }
+    for (const auto &kv : by_chunk) {
+      cid[kv.first].push_back(box[r]);
+      cent[kv.first].insert(cent[kv.first].end(), kv.second.begin(), kv.second.end());
+      coff[kv.first].push_back(static_cast<uint32_t>(cent[kv.first].size()));
+    }
+  }
+  std::vector<uint32_t> red_id, red_off{0}, red_ent;
+  rot_chunk_row_.assign(1, 0);
+  size_t most = 0;
+  for (size_t c = 0; c < nchunks; ++c) {
+    const uint32_t base = static_cast<uint32_t>(red_ent.size());
+    red_id.insert(red_id.end(), cid[c].begin(), cid[c].end());
+    for (size_t k = 1; k < coff[c].size(); ++k) red_off.push_back(base + coff[c][k]);
+    red_ent.insert(red_ent.end(), cent[c].begin(), cent[c].end());
+    rot_chunk_row_.push_back(static_cast<uint32_t>(red_id.size()));
+    most = std::max<size_t>(most, rot_chunk_q_[c + 1] - rot_chunk_q_[c]);
+  }
+  upload_rows(rot_red_, red_id, {}, red_off, red_ent);
   if (mixed) {
-    rot_slots32_.alloc(size_t(npairs) * nc_);
+    rot_slots32_.alloc(most * nc_);
     mult32_.alloc(nbox_total_ * nc_);
   } else {
-    rot_slots_.alloc(size_t(npairs) * nc_);
+    rot_slots_.alloc(most * nc_);
   }
 }
 
@@ -667,27 +712,30 @@ void DeviceOperator::launch_m2l(int level_first, int level_last) {
     return;
   }
   if (opt_.m2l == kM2lRotation) {
-    const uint32_t b0 = rot_blk_level_[level_first], b1 = rot_blk_level_[level_last + 1];
-    const uint32_t r0 = rot_level_[level_first], r1 = rot_level_[level_last + 1];
     if (opt_.precision == kMixed) {
       // the multipoles of these levels in f32, read by every pair that uses them
       const size_t x0 = box_base_[level_first], x1 = box_base_[level_last + 1];
       dev::to_f32(stream_, (x1 - x0) * nc_, mult_.ptr + x0 * nc_, mult32_.ptr + x0 * nc_);
     }
-    timed(kKindM2lContract, [&] {
-      if (opt_.precision == kMixed)
-        dev::m2l_rotation<float>(stream_, opt_.p, b1 - b0, rot_blk_first_.ptr + b0, rot_blk_offset_.ptr + b0,
-                                 rot_pair_src_.ptr, rot_dist_.ptr, rot_fwd32_.ptr, rot_bwd32_.ptr, rot_coax32_.ptr,
-                                 rot_rb_, rot_cb_, mult32_.ptr, rot_slots32_.ptr);
-      else
-        dev::m2l_rotation<double>(stream_, opt_.p, b1 - b0, rot_blk_first_.ptr + b0, rot_blk_offset_.ptr + b0,
-                                  rot_pair_src_.ptr, rot_dist_.ptr, rot_fwd_.ptr, rot_bwd_.ptr, rot_coax_.ptr, rot_rb_,
-                                  rot_cb_, mult_.ptr, rot_slots_.ptr);
-    });
-    // each target adds its pairs' rows in the order of its V list
-    if (r1 > r0)
-      dev::reduce_slots(stream_, opt_.p, r1 - r0, rot_box_.ptr + r0, rot_off_.ptr + r0, rot_pair_of_.ptr, 0,
-                        rot_slots_.ptr, opt_.precision == kMixed ? rot_slots32_.ptr : nullptr, loc_.ptr);
+    // chunk by chunk: the pairs' rows, then each target adds that chunk's rows in the
+    // order of its V list
+    for (uint32_t c = rot_chunk_level_[level_first]; c < rot_chunk_level_[level_last + 1]; ++c) {
+      const uint32_t c0 = rot_chunk_blk_[c], c1 = rot_chunk_blk_[c + 1], q0 = rot_chunk_q_[c];
+      timed(kKindM2lContract, [&] {
+        if (opt_.precision == kMixed)
+          dev::m2l_rotation<float>(stream_, opt_.p, c1 - c0, q0, rot_blk_first_.ptr + c0, rot_blk_offset_.ptr + c0,
+                                   rot_pair_src_.ptr, rot_dist_.ptr, rot_fwd32_.ptr, rot_bwd32_.ptr, rot_coax32_.ptr,
+                                   rot_rb_, rot_cb_, mult32_.ptr, rot_slots32_.ptr);
+        else
+          dev::m2l_rotation<double>(stream_, opt_.p, c1 - c0, q0, rot_blk_first_.ptr + c0, rot_blk_offset_.ptr + c0,
+                                    rot_pair_src_.ptr, rot_dist_.ptr, rot_fwd_.ptr, rot_bwd_.ptr, rot_coax_.ptr,
+                                    rot_rb_, rot_cb_, mult_.ptr, rot_slots_.ptr);
+      });
+      const uint32_t w0 = rot_chunk_row_[c], w1 = rot_chunk_row_[c + 1];
+      dev::reduce_slots(stream_, opt_.p, w1 - w0, rot_red_.row_id.ptr + w0, rot_red_.row_off.ptr + w0,
+                        rot_red_.row_ent.ptr, 0, rot_slots_.ptr,
+                        opt_.precision == kMixed ? rot_slots32_.ptr : nullptr, loc_.ptr);
+    }
     return;
   }
   const uint32_t t0 = m2l_level_task_[level_first], t1 = m2l_level_task_[level_last + 1];

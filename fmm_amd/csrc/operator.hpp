@@ -221,7 +221,12 @@ class DeviceOperator {
   int rot_rb_ = 0, rot_cb_ = 0;
   // rotation M2L: the pairs grouped by (level, offset) in blocks of one offset; pair_of
   // maps each V entry (rot rows' order) to its pair's scratch row
-  DevBuf<uint32_t> rot_blk_first_, rot_pair_src_, rot_pair_of_;
+  DevBuf<uint32_t> rot_blk_first_, rot_pair_src_;
+  // rotation M2L in chunks of whole blocks of one level (scratch rows within
+  // kRotScratchBytes): per chunk its first block, first pair and first reduction row; the
+  // chunks of level l from rot_chunk_level_[l]
+  std::vector<uint32_t> rot_chunk_blk_, rot_chunk_q_, rot_chunk_row_, rot_chunk_level_;
+  Rows rot_red_;
   DevBuf<uint16_t> rot_blk_offset_;
   std::vector<uint32_t> rot_blk_level_;
   DevBuf<double> rot_slots_;

@@ -37,7 +37,7 @@ static size_t rot_lds(int p, int rb, int cb, int pb) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(kRotThreads) void k_m2l_rot_pairs(int p, int pb, uint32_t nblocks,
+__global__ __launch_bounds__(kRotThreads) void k_m2l_rot_pairs(int p, int pb, uint32_t nblocks, uint32_t q_base,
                                                                const uint32_t *__restrict__ blk_first,
                                                                const uint16_t *__restrict__ blk_offset,
                                                                const uint32_t *__restrict__ pair_src,
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(kRotThreads) void k_m2l_rot_pairs(int p, int pb, ui
 #pragma unroll
     for (int jj = 0; jj < kRotJ; ++jj) {
       const uint32_t j = uint32_t(jg * kRotJ + jj);
-      if (j < np) S[size_t(q0 + j) * nc + k] = y[jj];
+      if (j < np) S[size_t(q0 - q_base + j) * nc + k] = y[jj];
     }
   }
 }
@@ -175,13 +175,13 @@ void to_f32(hipStream_t s, size_t n, const double *a, float *b) {
 }
 
 template <typename T>
-void m2l_rotation(hipStream_t s, int p, uint32_t nblocks, const uint32_t *blk_first, const uint16_t *blk_offset,
-                  const uint32_t *pair_src, const uint16_t *dist_of, const T *fwd, const T *bwd, const T *coax, int rb,
-                  int cb, const T *M, T *S) {
+void m2l_rotation(hipStream_t s, int p, uint32_t nblocks, uint32_t q_base, const uint32_t *blk_first,
+                  const uint16_t *blk_offset, const uint32_t *pair_src, const uint16_t *dist_of, const T *fwd,
+                  const T *bwd, const T *coax, int rb, int cb, const T *M, T *S) {
   if (nblocks == 0) return;
   const int pb = rotation_pairs_per_block<T>(p, rb, cb);
   hipLaunchKernelGGL((k_m2l_rot_pairs<T>), dim3(xcd_grid(nblocks)), dim3(kRotThreads), rot_lds<T>(p, rb, cb, pb), s,
-                     p, pb, nblocks, blk_first, blk_offset, pair_src, dist_of, fwd, bwd, coax, rb, cb, M, S);
+                     p, pb, nblocks, q_base, blk_first, blk_offset, pair_src, dist_of, fwd, bwd, coax, rb, cb, M, S);
 }
 
 // ---- Helmholtz: the same three steps on complex expansions ------------------------------------
@@ -341,12 +341,12 @@ template int rotation_pairs_per_block<double>(int, int, int);
 template int rotation_pairs_per_block<float>(int, int, int);
 template hipError_t rotation_allow_lds<double>(int, int, int);
 template hipError_t rotation_allow_lds<float>(int, int, int);
-template void m2l_rotation<double>(hipStream_t, int, uint32_t, const uint32_t *, const uint16_t *, const uint32_t *,
-                                   const uint16_t *, const double *, const double *, const double *, int, int,
-                                   const double *, double *);
-template void m2l_rotation<float>(hipStream_t, int, uint32_t, const uint32_t *, const uint16_t *, const uint32_t *,
-                                  const uint16_t *, const float *, const float *, const float *, int, int,
-                                  const float *, float *);
+template void m2l_rotation<double>(hipStream_t, int, uint32_t, uint32_t, const uint32_t *, const uint16_t *,
+                                   const uint32_t *, const uint16_t *, const double *, const double *, const double *,
+                                   int, int, const double *, double *);
+template void m2l_rotation<float>(hipStream_t, int, uint32_t, uint32_t, const uint32_t *, const uint16_t *,
+                                  const uint32_t *, const uint16_t *, const float *, const float *, const float *, int,
+                                  int, const float *, float *);
 
 }  // namespace dev
 }  // namespace fmmh

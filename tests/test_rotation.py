@@ -84,3 +84,24 @@ def test_rotation_fmm_matches_oracle(_gpu, precision, bar):
     e1, e2 = _rel(out.potential, rphi), _rel(out.gradient, rgrad)
     print(f"rotation fmm {precision} vs oracle: phi {e1:.3e} grad {e2:.3e}", flush=True)
     assert e1 < bar and e2 < bar
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,bar", [("f64", 1e-12), ("mixed", 2e-6)])
+def test_rotation_in_several_scratch_chunks_equals_dense(_gpu, precision, bar):
+    # enough V pairs (1M points, p = 10: ~3M pairs) that the rotation M2L runs in several
+    # chunks of scratch rows (512 MB each), each followed by its ordered reduction; the
+    # result against the dense M2L on the same plan (itself pinned to the oracle)
+    rng = O.Rng(95)
+    pts = rng.cube(1_000_000)
+    q = rng.normals(len(pts))
+    plan = fmm_amd.Plan(pts, leaf_points=64)
+    pairs = sum(len(plan.list("v", l)[1]) for l in range(plan.nlevels))
+    per_chunk = (512 << 20) // (121 * (8 if precision == "f64" else 4))
+    assert pairs > 2 * per_chunk
+    rot = fmm_amd.Fmm(plan, fmm_amd.Operator(plan, 10, precision, True, 0, "rotation")).evaluate(q)
+    dense = fmm_amd.Fmm(plan, fmm_amd.Operator(plan, 10, "f64", True, 0, "dense")).evaluate(q)
+    e1 = O.error_norms(rot.potential, dense.potential)[0]
+    e2 = O.error_norms(rot.gradient, dense.gradient, True)[0]
+    print(f"rotation {precision} in {pairs / per_chunk:.1f} chunks vs dense f64: {e1:.2e} {e2:.2e}", flush=True)
+    assert e1 < bar and e2 < bar
