@@ -103,6 +103,7 @@ class Plan:
         h = C.c_void_p()
         _lib.check(_lib.lib().fmmh_plan_build(_lib.ptr(src), len(src), _lib.ptr(tgt), len(tgt), C.byref(o), C.byref(h)))
         self._h = h
+        self._free = _lib.lib().fmmh_plan_free
 
     @classmethod
     def forest(cls, clouds, leaf_points=64, max_level=16, near_rule=None):
@@ -121,12 +122,16 @@ class Plan:
         _lib.check(_lib.lib().fmmh_forest_build(_lib.ptr(pts), _lib.ptr(counts), len(counts), C.byref(o),
                                                 C.byref(h)))
         self._h = h
+        self._free = _lib.lib().fmmh_plan_free
         return self
 
     def __del__(self):
+        # the library's free function held by the object: at interpreter exit the module's
+        # globals may already be gone
         h = getattr(self, "_h", None)
-        if h:
-            _lib.lib().fmmh_plan_free(h)
+        free = getattr(self, "_free", None)
+        if h and free is not None:
+            free(h)
             self._h = None
 
     @property
@@ -212,14 +217,18 @@ class Operator:
         t = C.byref(tuning) if tuning is not None else None
         _lib.check(_lib.lib().fmmh_operator_create_tuned(plan.handle, C.byref(o), t, C.byref(h)))
         self._h = h
+        self._free = _lib.lib().fmmh_operator_free
         self.tuning = tuning
         self.ntargets = len(plan.targets)
         self.nsources = len(plan.sources)
 
     def __del__(self):
+        # the library's free function held by the object: at interpreter exit the module's
+        # globals may already be gone
         h = getattr(self, "_h", None)
-        if h:
-            _lib.lib().fmmh_operator_free(h)
+        free = getattr(self, "_free", None)
+        if h and free is not None:
+            free(h)
             self._h = None
 
     def stats(self) -> dict:
@@ -582,13 +591,17 @@ class HelmholtzOperator:
         h = C.c_void_p()
         _lib.check(_lib.lib().fmmh_helmholtz_create(plan.handle, C.byref(o), C.byref(h)))
         self._h = h
+        self._free = _lib.lib().fmmh_helmholtz_free
         self.ntargets = len(plan.targets)
         self.nsources = len(plan.sources)
 
     def __del__(self):
+        # the library's free function held by the object: at interpreter exit the module's
+        # globals may already be gone
         h = getattr(self, "_h", None)
-        if h:
-            _lib.lib().fmmh_helmholtz_free(h)
+        free = getattr(self, "_free", None)
+        if h and free is not None:
+            free(h)
             self._h = None
 
     def stats(self) -> dict:
