@@ -16,7 +16,7 @@ import copy
 import ctypes as C
 import os
 from dataclasses import dataclass
-from typing import Optional
+from typing import NamedTuple, Optional
 
 import numpy as np
 
@@ -509,22 +509,51 @@ class FmmBuilder:
         return Fmm(plan, self._operator(plan))
 
 
-def plane_wave_quadrature(p: int, eps: float):
-    """(lambda, weight, alpha counts, H) of the plane-wave quadrature for degree p."""
-    n, h = C.c_int32(0), C.c_int32(0)
-    lam, w, na = np.zeros(400), np.zeros(400), np.zeros(400, np.int32)
-    _lib.check(_lib.lib().fmmh_plane_wave_quadrature(int(p), float(eps), C.byref(n), C.byref(h), _lib.ptr(lam),
-                                                     _lib.ptr(w), _lib.ptr(na)))
+class PlaneWaveQuadrature(NamedTuple):
+    """FMM3D's plane-wave quadrature as the reference uses it (quadrature.rs Quadrature)."""
+    lambdas: np.ndarray
+    weights: np.ndarray
+    alpha_nodes: np.ndarray  # FMM3D's N_k (odd counts included)
+    fourier: np.ndarray      # modes kept at degree p: min(F_k, p + 1)
+    half_terms: int          # H = sum_k N_k' / 2, N_k' = N_k rounded up to even
+    max_error: float         # FMM3D's stated maximum error
+
+
+def plane_wave_quadrature(p: int, nodes: int = 0) -> PlaneWaveQuadrature:
+    """The plane-wave quadrature of degree p: FMM3D's table with ``nodes`` lambda nodes, or
+    the reference's count for degree p when nodes is 0."""
+    n, h, err = C.c_int32(0), C.c_int32(0), C.c_double(0.0)
+    lam, w = np.zeros(40), np.zeros(40)
+    na, fo = np.zeros(40, np.int32), np.zeros(40, np.int32)
+    _lib.check(_lib.lib().fmmh_plane_wave_quadrature(int(p), int(nodes), C.byref(n), C.byref(h), _lib.ptr(lam),
+                                                     _lib.ptr(w), _lib.ptr(na), _lib.ptr(fo), C.byref(err)))
     k = n.value
-    return lam[:k].copy(), w[:k].copy(), na[:k].copy(), h.value
+    return PlaneWaveQuadrature(lam[:k].copy(), w[:k].copy(), na[:k].copy(), fo[:k].copy(), h.value, err.value)
 
 
-def plane_wave_m2l(p: int, eps: float, offset: int, multipole) -> np.ndarray:
-    """One V pair's plane-wave M2L on the host (scaled storage): the local it adds."""
+def plane_wave_m2l(p: int, offset: int, multipole, direction: int = -1, nodes: int = 0) -> np.ndarray:
+    """One V pair's plane-wave M2L on the host (scaled storage): the local it adds, in the
+    given direction (-1: the offset's own)."""
     x = np.ascontiguousarray(multipole, dtype=np.float64)
     out = np.zeros((p + 1) ** 2)
-    _lib.check(_lib.lib().fmmh_plane_wave_m2l(int(p), float(eps), int(offset), _lib.ptr(x), _lib.ptr(out)))
+    _lib.check(_lib.lib().fmmh_plane_wave_m2l(int(p), int(nodes), int(offset), int(direction), _lib.ptr(x),
+                                              _lib.ptr(out)))
     return out
+
+
+def plane_wave_merge(p: int, slot_multipoles, present, nodes: int = 0) -> np.ndarray:
+    """One parent's merge and shift on the host: 216 slot multipoles (rows), which are held,
+    and the 8 children's locals (rows) it adds."""
+    x = np.ascontiguousarray(slot_multipoles, dtype=np.float64).reshape(216, (p + 1) ** 2)
+    pr = np.ascontiguousarray(present, dtype=np.uint8).reshape(216)
+    out = np.zeros((8, (p + 1) ** 2))
+    _lib.check(_lib.lib().fmmh_plane_wave_merge(int(p), int(nodes), _lib.ptr(x), _lib.ptr(pr), _lib.ptr(out)))
+    return out
+
+
+def plane_wave_slot_direction(slot: int, child: int) -> int:
+    """The direction (0..5) of the merge-and-shift pair slot -> child, -1 if adjacent."""
+    return int(_lib.lib().fmmh_plane_wave_slot_direction(int(slot), int(child)))
 
 
 # ---- Helmholtz --------------------------------------------------------------------------

@@ -41,7 +41,8 @@ EXPORTS = [
     "fmmh_helmholtz_load_charges", "fmmh_helmholtz_reset", "fmmh_helmholtz_evaluate",
     "fmmh_helmholtz_read_output", "fmmh_helmholtz_read_expansions", "fmmh_helmholtz_kind_timing",
     "fmmh_helmholtz_kind_times", "fmmh_helmholtz_table", "fmmh_helmholtz_direct_sum",
-    "fmmh_plane_wave_quadrature", "fmmh_plane_wave_m2l", "fmmh_stream_wait", "fmmh_helmholtz_stream_wait",
+    "fmmh_plane_wave_quadrature", "fmmh_plane_wave_m2l", "fmmh_plane_wave_merge", "fmmh_plane_wave_slot_direction",
+    "fmmh_stream_wait", "fmmh_helmholtz_stream_wait",
     "fmmh_direct_sum_noalloc", "fmmh_operator_create_tuned", "fmmh_autotune", "fmmh_inject_error",
     "fmmh_set_table_cache", "fmmh_table_cache_stats",
 ]
@@ -157,8 +158,10 @@ def lib():
     L.fmmh_kind_times.argtypes = [vp, vp, vp]
     L.fmmh_vjp.argtypes = [vp, vp, vp, vp, i64, i32, vp, vp]
     L.fmmh_forest_build.argtypes = [vp, vp, i32, C.POINTER(PlanOptions), C.POINTER(vp)]
-    L.fmmh_plane_wave_quadrature.argtypes = [i32, dbl, vp, vp, vp, vp, vp]
-    L.fmmh_plane_wave_m2l.argtypes = [i32, dbl, i32, vp, vp]
+    L.fmmh_plane_wave_quadrature.argtypes = [i32, i32, vp, vp, vp, vp, vp, vp, vp]
+    L.fmmh_plane_wave_m2l.argtypes = [i32, i32, i32, i32, vp, vp]
+    L.fmmh_plane_wave_merge.argtypes = [i32, i32, vp, vp, vp]
+    L.fmmh_plane_wave_slot_direction.argtypes = [i32, i32]
     L.fmmh_helmholtz_degree.argtypes = [dbl, dbl, dbl]
     L.fmmh_helmholtz_create.argtypes = [vp, C.POINTER(HelmholtzOptions), C.POINTER(vp)]
     L.fmmh_helmholtz_free.argtypes = [vp]

@@ -91,7 +91,7 @@ extern "C" {
 
 const char *fmmh_last_error(void) { return g_last_error.c_str(); }
 
-int32_t fmmh_abi_version(void) { return 5; }
+int32_t fmmh_abi_version(void) { return 6; }
 
 int32_t fmmh_plan_build(const double *sources, int64_t nsources, const double *targets, int64_t ntargets,
                         const fmmh_plan_options *options, fmmh_plan **out) {
@@ -685,35 +685,65 @@ int32_t fmmh_helmholtz_direct_sum(int32_t device, double k_re, double k_im, cons
   });
 }
 
-int32_t fmmh_plane_wave_quadrature(int32_t p, double eps, int32_t *nodes, int32_t *half_nodes, double *lambda,
-                                   double *weight, int32_t *nalpha) {
+int32_t fmmh_plane_wave_quadrature(int32_t p, int32_t nodes, int32_t *out_nodes, int32_t *half_nodes, double *lambda,
+                                   double *weight, int32_t *nalpha, int32_t *fourier, double *max_error) {
   return guard([&] {
-    need(nodes != nullptr, "fmmh_plane_wave_quadrature: NULL nodes");
-    const fmmh::PlaneWave pw = fmmh::build_plane_wave(p, eps);
-    *nodes = pw.nodes;
+    need(out_nodes != nullptr, "fmmh_plane_wave_quadrature: NULL nodes");
+    const fmmh::PlaneWave pw = fmmh::build_plane_wave(p, nodes);
+    *out_nodes = pw.nodes;
     if (half_nodes) *half_nodes = pw.H;
+    if (max_error) *max_error = pw.max_error;
     for (int k = 0; k < pw.nodes; ++k) {
       if (lambda) lambda[k] = pw.lambda[k];
       if (weight) weight[k] = pw.weight[k];
-      if (nalpha) nalpha[k] = pw.nalpha[k];
+      if (nalpha) nalpha[k] = pw.fmm3d_alpha[k];
+      if (fourier) fourier[k] = pw.fourier[k];
     }
   });
 }
 
-int32_t fmmh_plane_wave_m2l(int32_t p, double eps, int32_t offset, const double *in, double *out) {
+// The last quadrature built by the host restatements, kept for the next call of the same
+// (p, nodes).
+static const fmmh::PlaneWave &cached_plane_wave(int32_t p, int32_t nodes, std::unique_lock<std::mutex> &lock) {
+  static std::mutex mu;
+  static std::unique_ptr<fmmh::PlaneWave> last;
+  lock = std::unique_lock<std::mutex>(mu);
+  if (!last || last->p != p || last->nodes != (nodes == 0 ? fmmh::plane_wave_nodes_for_degree(p) : nodes))
+    last = std::make_unique<fmmh::PlaneWave>(fmmh::build_plane_wave(p, nodes));
+  return *last;
+}
+
+int32_t fmmh_plane_wave_m2l(int32_t p, int32_t nodes, int32_t offset, int32_t direction, const double *in,
+                            double *out) {
   return guard([&] {
     need(in && out, "fmmh_plane_wave_m2l: NULL argument");
     need(offset >= 0 && offset < 316, "fmmh_plane_wave_m2l: offset index 0..315");
-    // the last quadrature is kept for the next call of the same (p, eps)
-    static std::mutex mu;
-    static std::unique_ptr<fmmh::PlaneWave> last;
-    std::lock_guard<std::mutex> lock(mu);
-    if (!last || last->p != p || last->eps != eps) last = std::make_unique<fmmh::PlaneWave>(fmmh::build_plane_wave(p, eps));
-    const fmmh::PlaneWave &pw = *last;
+    need(direction >= -1 && direction < 6, "fmmh_plane_wave_m2l: direction -1..5");
+    std::unique_lock<std::mutex> lock;
+    const fmmh::PlaneWave &pw = cached_plane_wave(p, nodes, lock);
     int d[3];
     fmmh::offset_of_index(offset, d);
-    fmmh::plane_wave_m2l_host(pw, d, in, out);
+    fmmh::plane_wave_m2l_host(pw, d, direction, in, out);
   });
+}
+
+int32_t fmmh_plane_wave_merge(int32_t p, int32_t nodes, const double *slot_multipoles, const uint8_t *present,
+                              double *out) {
+  return guard([&] {
+    need(slot_multipoles && present && out, "fmmh_plane_wave_merge: NULL argument");
+    std::unique_lock<std::mutex> lock;
+    const fmmh::PlaneWave &pw = cached_plane_wave(p, nodes, lock);
+    const size_t nc = size_t(p + 1) * (p + 1);
+    std::vector<const double *> slots(fmmh::kPwSlots, nullptr);
+    for (int a = 0; a < fmmh::kPwSlots; ++a)
+      if (present[a]) slots[a] = slot_multipoles + a * nc;
+    fmmh::plane_wave_merge_host(pw, slots.data(), out);
+  });
+}
+
+int32_t fmmh_plane_wave_slot_direction(int32_t slot, int32_t child) {
+  if (slot < 0 || slot >= fmmh::kPwSlots || child < 0 || child >= fmmh::kPwChildren) return -2;
+  return fmmh::plane_wave_slot_direction(slot, child);
 }
 
 }  // extern "C"

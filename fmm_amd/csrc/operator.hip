@@ -243,7 +243,7 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
     upload_rows(p2l_rows_, id, cell, off, ent);
   }
   build_schedules(plan);
-  if (opt_.m2l == kM2lRotation || opt_.m2l == kM2lPlaneWave) build_rotation(plan);
+  if (opt_.m2l == kM2lRotation) build_rotation(plan);
   if (opt_.m2l == kM2lPlaneWave) build_plane_wave(plan);
   loc_.alloc(nbox_total_ * nc_);
   if (opt_.precision == kMixed)
@@ -512,51 +512,130 @@ void DeviceOperator::build_schedules(const Plan &plan) {
   upload_rows(m2l_red_, red_id, red_cell, red_off, red_ent);
 }
 
-// The plane-wave quadrature's tolerance at degree p: a tenth of the M2L truncation, about
-// 10^-(p/2 + 2) (the reference picks its node count per degree by measurement,
-// docs/CONVENTIONS.md:1335-1345).
-static double plane_wave_tolerance(int p) { return std::pow(10.0, -(0.5 * p + 2.0)); }
-
+// Plane-wave M2L by merge and shift (fmm-kernels/src/plane_wave.rs:1-60,
+// fmm-tables/src/plane_wave/geometry.rs): the quadrature of degree p, the accumulators and
+// shifts (the same for every parent), and per level the parents of its boxes with their
+// 216 slots and 8 children, read off the V lists: every V entry of a child is a slot of
+// its parent, and every pair (slot, child) that is not adjacent is a V entry when the slot
+// is held (checked here).
 void DeviceOperator::build_plane_wave(const Plan &plan) {
-  const PlaneWave pw = ::fmmh::build_plane_wave(opt_.p, plane_wave_tolerance(opt_.p));
+  const PlaneWave pw = ::fmmh::build_plane_wave(opt_.p, 0);
   require(pw.nodes <= dev::kPwMaxNodes, "plane wave: too many quadrature nodes", kUnsupportedLayout);
-  const size_t lds = dev::pw_incoming_lds(opt_.p, pw.H);
-  require(lds <= 160 * 1024, "plane wave: the quadrature does not fit in LDS", kUnsupportedLayout);
+  const size_t lds = dev::pw_local_lds(opt_.p, pw.H);
+  require(lds <= 64 * 1024, "plane wave: the quadrature does not fit in LDS", kUnsupportedLayout);
   hip_check(dev::pw_allow_lds(lds), "plane wave LDS");
-  pw_lambda_.upload(pw.lambda);
   pw_first_.upload(pw.first);
   pw_nalpha_.upload(pw.nalpha);
+  pw_fourier_.upload(pw.fourier);
+  pw_opow_.upload(pw.opow);
+  pw_ipow_.upload(pw.ipow);
   pw_hw_.upload(pw.hw);
   pw_hc_.upload(pw.hc);
   pw_hs_.upload(pw.hs);
   pw_hk_.upload(pw.hk);
   pw_fwd_.upload(pw.fwd);
   pw_bwd_.upload(pw.bwd);
-  // per offset: its direction and the shift T(d') of every half node
-  std::vector<int> dir(316);
-  std::vector<double> shift(size_t(316) * 2 * pw.H);
-  for (int t = 0; t < 316; ++t) {
-    int d[3], f, up;
-    double dp[3];
-    offset_of_index(t, d);
-    plane_wave_direction(d, f, up, dp);
-    dir[t] = 2 * f + (up > 0 ? 0 : 1);
-    for (int h = 0; h < pw.H; ++h) {
-      const double lam = pw.hl[h], amp = std::exp(-lam * std::fabs(dp[2]));
-      const double ph = lam * (dp[0] * pw.hc[h] + dp[1] * pw.hs[h]);
-      shift[size_t(t) * 2 * pw.H + h] = amp * std::cos(ph);
-      shift[size_t(t) * 2 * pw.H + pw.H + h] = amp * std::sin(ph);
+  // geometry: accumulators by direction, their entries with the source shift of each slot
+  // (one per distinct shift), the child shifts
+  const size_t H2 = 2 * size_t(pw.H);
+  const auto &accs = plane_wave_accumulators();
+  std::vector<int> acc_first{0}, acc_mask, ent_first{0}, ent_slot, ent_shift;
+  std::vector<double> src_shift, child_shift(size_t(kPwDirections) * kPwChildren * H2);
+  std::map<std::array<double, 4>, int> shift_of;
+  for (int dir = 0; dir < kPwDirections; ++dir) {
+    for (const PwAccumulator &a : accs[dir]) {
+      acc_mask.push_back(a.mask);
+      for (int slot : a.slots) {
+        const double d[3] = {2.5 - slot / 36, 2.5 - (slot / 6) % 6, 2.5 - slot % 6};
+        double dp[3];
+        plane_wave_rotate(dir, d, dp);
+        const std::array<double, 4> key{double(dir & 1), dp[0], dp[1], dp[2]};
+        auto it = shift_of.find(key);
+        if (it == shift_of.end()) {
+          it = shift_of.emplace(key, int(shift_of.size())).first;
+          src_shift.resize(src_shift.size() + H2);
+          plane_wave_shift(pw, dir, d, &src_shift[src_shift.size() - H2]);
+        }
+        ent_slot.push_back(slot);
+        ent_shift.push_back(it->second);
+      }
+      ent_first.push_back(static_cast<int>(ent_slot.size()));
+    }
+    acc_first.push_back(static_cast<int>(acc_mask.size()));
+    for (int c = 0; c < kPwChildren; ++c) {
+      const double d[3] = {((c >> 2) & 1) - 0.5, ((c >> 1) & 1) - 0.5, (c & 1) - 0.5};
+      plane_wave_shift(pw, dir, d, &child_shift[(size_t(kPwChildren) * dir + c) * H2]);
     }
   }
-  pw_dir_.upload(dir);
-  pw_shift_.upload(shift);
+  pw_acc_first_.upload(acc_first);
+  pw_acc_mask_.upload(acc_mask);
+  pw_ent_first_.upload(ent_first);
+  pw_ent_slot_.upload(ent_slot);
+  pw_ent_shift_.upload(ent_shift);
+  pw_src_shift_.upload(src_shift);
+  pw_child_shift_.upload(child_shift);
+  // parents by the level of their children, children in box order
+  constexpr uint32_t kNoBox = 0xffffffffu;
+  std::vector<uint32_t> slots, children, targets;
+  pw_par_level_.assign(nlevels_ + 1, 0);
+  pw_tgt_level_.assign(nlevels_ + 1, 0);
+  for (int l = 0; l < nlevels_; ++l) {
+    pw_par_level_[l] = static_cast<uint32_t>(children.size() / kPwChildren);
+    pw_tgt_level_[l] = static_cast<uint32_t>(targets.size());
+    if (l < 2) continue;
+    const Csr &V = plan.v[l];
+    for (uint32_t P = 0; P < plan.nboxes(l - 1); ++P) {
+      const uint32_t c0 = plan.first_child[l - 1][P];
+      if (c0 == kNone) continue;
+      const Cell cp = decode(plan.keys[l - 1][P]);
+      std::array<uint32_t, kPwSlots> sl;
+      std::array<uint32_t, kPwChildren> ch;
+      sl.fill(kNoBox);
+      ch.fill(kNoBox);
+      uint64_t entries = 0;
+      for (uint32_t b = c0; b < c0 + kPwChildren; ++b) {
+        const Cell cc = decode(plan.keys[l][b]);
+        const int64_t bx = cc.i - 2 * cp.i, by = cc.j - 2 * cp.j, bz = cc.k - 2 * cp.k;
+        require(bx >= 0 && bx <= 1 && by >= 0 && by <= 1 && bz >= 0 && bz <= 1 && plan.parent[l][b] == P,
+                "plane wave: a parent's children are not its eight octants", kUnsupportedLayout);
+        ch[4 * bx + 2 * by + bz] = static_cast<uint32_t>(box_base_[l] + b);
+        for (uint32_t e = V.off[b]; e < V.off[b + 1]; ++e) {
+          const Cell cs = decode(plan.keys[l][V.ent[e]]);
+          const int64_t ax = cs.i - 2 * cp.i + 2, ay = cs.j - 2 * cp.j + 2, az = cs.k - 2 * cp.k + 2;
+          require(ax >= 0 && ax < 6 && ay >= 0 && ay < 6 && az >= 0 && az < 6,
+                  "plane wave: a V entry outside the parent's slots", kUnsupportedLayout);
+          const uint32_t src = static_cast<uint32_t>(box_base_[l] + V.ent[e]);
+          uint32_t &slot = sl[36 * ax + 6 * ay + az];
+          require(slot == kNoBox || slot == src, "plane wave: two boxes in one slot", kUnsupportedLayout);
+          slot = src;
+          ++entries;
+        }
+      }
+      if (entries == 0) continue;
+      uint64_t pairs = 0;
+      for (int a = 0; a < kPwSlots; ++a)
+        for (int c = 0; sl[a] != kNoBox && c < kPwChildren; ++c)
+          if (ch[c] != kNoBox && plane_wave_slot_direction(a, c) >= 0) ++pairs;
+      require(pairs == entries, "plane wave: the V lists are not the merge-and-shift pairs", kUnsupportedLayout);
+      slots.insert(slots.end(), sl.begin(), sl.end());
+      children.insert(children.end(), ch.begin(), ch.end());
+      for (uint32_t b = c0; b < c0 + kPwChildren; ++b) targets.push_back(static_cast<uint32_t>(box_base_[l] + b));
+    }
+  }
+  pw_par_level_[nlevels_] = static_cast<uint32_t>(children.size() / kPwChildren);
+  pw_tgt_level_[nlevels_] = static_cast<uint32_t>(targets.size());
+  pw_slots_.upload(slots);
+  pw_children_.upload(children);
+  pw_targets_.upload(targets);
   pw_box0_ = nlevels_ > 2 ? box_base_[2] : 0;
   pw_nbox_ = nlevels_ > 2 ? static_cast<uint32_t>(box_base_[nlevels_] - box_base_[2]) : 0;
   pw_waves_.alloc(size_t(pw_nbox_) * 12 * pw.H);
-  pw_ = dev::PwArgs{opt_.p,        pw.nodes,      pw.H,          rotation_block_size(opt_.p), pw_lambda_.ptr,
-                    pw_first_.ptr, pw_nalpha_.ptr, pw_hw_.ptr,   pw_hc_.ptr,                  pw_hs_.ptr,
-                    pw_hk_.ptr,    pw_fwd_.ptr,   pw_bwd_.ptr,   pw_shift_.ptr,               pw_dir_.ptr};
-  (void)plan;
+  pw_inc_.alloc(size_t(pw_nbox_) * 12 * pw.H);
+  pw_ = dev::PwArgs{opt_.p,         pw.nodes,     pw.H,       rotation_block_size(opt_.p), pw_first_.ptr,
+                    pw_nalpha_.ptr, pw_fourier_.ptr, pw_opow_.ptr, pw_ipow_.ptr,              pw_hw_.ptr,
+                    pw_hc_.ptr,     pw_hs_.ptr,   pw_hk_.ptr, pw_fwd_.ptr,                 pw_bwd_.ptr};
+  pw_merge_ = dev::PwMerge{pw_acc_first_.ptr, pw_acc_mask_.ptr,  pw_ent_first_.ptr,  pw_ent_slot_.ptr,
+                           pw_ent_shift_.ptr, pw_src_shift_.ptr, pw_child_shift_.ptr};
 }
 
 void DeviceOperator::build_rotation(const Plan &plan) {
@@ -686,7 +765,8 @@ void DeviceOperator::build_rotation(const Plan &plan) {
 
 size_t DeviceOperator::device_bytes() const {
   return mult_.bytes() + loc_.bytes() + slots_.bytes() + slots32_.bytes() + t_m2l_.bytes() + t_m2m_.bytes() +
-         t_l2l_.bytes() + sx_.bytes() * 8 + src_rec_.bytes();
+         t_l2l_.bytes() + sx_.bytes() * 8 + src_rec_.bytes() + rot_slots_.bytes() + rot_slots32_.bytes() +
+         mult32_.bytes() + pw_waves_.bytes() + pw_inc_.bytes() + pw_src_shift_.bytes() + pw_child_shift_.bytes();
 }
 
 void DeviceOperator::launch_contract(const Schedule &s, uint32_t task_first, uint32_t ntasks, const double *X,
@@ -700,15 +780,18 @@ void DeviceOperator::launch_contract(const Schedule &s, uint32_t task_first, uin
 // reduction of the clustered-mode levels.
 void DeviceOperator::launch_m2l(int level_first, int level_last) {
   if (opt_.m2l == kM2lPlaneWave) {
-    const uint32_t r0 = rot_level_[level_first], r1 = rot_level_[level_last + 1];
-    if (r1 == r0) return;
-    // the waves of the source boxes of these levels, then the pairs of the target rows
+    const uint32_t q0 = pw_par_level_[level_first], q1 = pw_par_level_[level_last + 1];
+    if (q1 == q0) return;
+    // the waves of the boxes of these levels, the parents' merge and shift into their
+    // children's incoming waves, then the children's locals
     const uint32_t b0 = box_base_[level_first], b1 = box_base_[level_last + 1];
+    const uint32_t t0 = pw_tgt_level_[level_first], t1 = pw_tgt_level_[level_last + 1];
     dev::pw_outgoing(stream_, pw_, b1 - b0, b0, mult_.ptr, pw_waves_.ptr + size_t(b0 - pw_box0_) * 12 * pw_.H);
     timed(kKindM2lContract, [&] {
-      dev::pw_incoming(stream_, pw_, r1 - r0, rot_box_.ptr + r0, rot_off_.ptr + r0, rot_src_.ptr, rot_offset_.ptr,
-                       pw_box0_, pw_waves_.ptr, loc_.ptr);
+      dev::pw_merge(stream_, pw_, pw_merge_, q1 - q0, pw_slots_.ptr + size_t(q0) * kPwSlots,
+                    pw_children_.ptr + size_t(q0) * kPwChildren, pw_box0_, pw_waves_.ptr, pw_inc_.ptr);
     });
+    dev::pw_local(stream_, pw_, t1 - t0, pw_targets_.ptr + t0, pw_box0_, pw_inc_.ptr, loc_.ptr);
     return;
   }
   if (opt_.m2l == kM2lRotation) {

@@ -233,10 +233,17 @@ class DeviceOperator {
   DevBuf<float> rot_slots32_, mult32_;
   void build_rotation(const Plan &plan);
   void build_plane_wave(const Plan &plan);
-  // plane-wave M2L: quadrature tables, the waves of the boxes of levels >= 2
+  // plane-wave M2L: quadrature tables, the merge-and-shift geometry, per level (of the
+  // children) the parents' slots and children and the target boxes, the waves and the
+  // incoming waves of the boxes of levels >= 2
   dev::PwArgs pw_{};
-  DevBuf<double> pw_lambda_, pw_hw_, pw_hc_, pw_hs_, pw_fwd_, pw_bwd_, pw_shift_, pw_waves_;
-  DevBuf<int> pw_first_, pw_nalpha_, pw_hk_, pw_dir_;
+  dev::PwMerge pw_merge_{};
+  DevBuf<double> pw_opow_, pw_ipow_, pw_hw_, pw_hc_, pw_hs_, pw_fwd_, pw_bwd_, pw_src_shift_, pw_child_shift_;
+  DevBuf<double> pw_waves_, pw_inc_;
+  DevBuf<int> pw_first_, pw_nalpha_, pw_fourier_, pw_hk_;
+  DevBuf<int> pw_acc_first_, pw_acc_mask_, pw_ent_first_, pw_ent_slot_, pw_ent_shift_;
+  DevBuf<uint32_t> pw_slots_, pw_children_, pw_targets_;
+  std::vector<uint32_t> pw_par_level_, pw_tgt_level_;
   uint32_t pw_box0_ = 0, pw_nbox_ = 0;
   size_t nslots_ = 0;                  // clustered-mode scratch rows (slot ids follow the box ids)
   DevBuf<double> slots_;               // plain f64: scratch rows

@@ -262,14 +262,30 @@ int32_t fmmh_kind_times(fmmh_operator *op, double *ms, int64_t *launches);
  * device error whose message says "injected", which the operator then keeps. */
 int32_t fmmh_inject_error(fmmh_operator *op);
 
-/* The plane-wave quadrature built for degree p and tolerance eps (fmm-tables/src/plane_wave/
- * mod.rs:111 uses FMM3D's tables; this build compresses a Gauss-Legendre rule): node count,
- * half-node count H, and per node lambda, weight and alpha count (arrays of >= 400). */
-int32_t fmmh_plane_wave_quadrature(int32_t p, double eps, int32_t *nodes, int32_t *half_nodes, double *lambda,
-                                   double *weight, int32_t *nalpha);
-/* One V pair's plane-wave M2L on the host (the device kernels' steps; plane_wave::m2l,
- * fmm-kernels/src/plane_wave.rs:1159): out += M2L_pw(offset) in, scaled storage. */
-int32_t fmmh_plane_wave_m2l(int32_t p, double eps, int32_t offset, const double *in, double *out);
+/* The plane-wave quadrature of degree p (fmm-tables/src/plane_wave/quadrature.rs:48-104,
+ * Quadrature::for_degree / with_nodes): FMM3D's table with `nodes` lambda nodes (2..39), or
+ * with the reference's count for degree p (0..20) when nodes is 0.  Returns the node count,
+ * the half-node count H (sum of the alpha counts rounded up to even, halved), and per node
+ * lambda, weight, FMM3D's alpha count, the Fourier modes kept at degree p (min(F_k, p + 1))
+ * and FMM3D's stated maximum error (arrays of >= 39; any output may be NULL but nodes). */
+int32_t fmmh_plane_wave_quadrature(int32_t p, int32_t nodes, int32_t *out_nodes, int32_t *half_nodes, double *lambda,
+                                   double *weight, int32_t *nalpha, int32_t *fourier, double *max_error);
+/* One V pair's plane-wave M2L on the host (PlaneWaveTables::m2l, fmm-tables/src/plane_wave/
+ * mod.rs:52-58: one shift from the source to the target): out += M2L_pw(offset) in, scaled
+ * storage, in direction `direction` (0..5: up, down, north, south, east, west; -1: the first
+ * axis of z, y, x along which the offset is at least two boxes). */
+int32_t fmmh_plane_wave_m2l(int32_t p, int32_t nodes, int32_t offset, int32_t direction, const double *in,
+                            double *out);
+/* One parent's merge and shift on the host (fmm-kernels/src/plane_wave.rs:12-24, the device
+ * steps outgoing, merge, gather, local): slot_multipoles holds 216 multipoles (slot a =
+ * 36 a_x + 6 a_y + a_z is the box 2P - 2 + a one level down), present[a] says which are held;
+ * out (8 x (p+1)^2, child 4 b_x + 2 b_y + b_z) += the plane-wave M2L of every pair slot ->
+ * child that is not adjacent. */
+int32_t fmmh_plane_wave_merge(int32_t p, int32_t nodes, const double *slot_multipoles, const uint8_t *present,
+                              double *out);
+/* The direction (0..5) of the merge-and-shift pair slot -> child (geometry.rs
+ * Direction::for_slot), -1 if adjacent, -2 for an index out of range. */
+int32_t fmmh_plane_wave_slot_direction(int32_t slot, int32_t child);
 
 /* ---- Helmholtz (kernel e^{ikr} / (4 pi r), complex charges) ---------------------- */
 /* The reference's Helmholtz FMM (nd_fmm_exec::helmholtz) runs on the host only
