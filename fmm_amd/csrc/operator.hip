@@ -521,8 +521,8 @@ void DeviceOperator::build_schedules(const Plan &plan) {
 void DeviceOperator::build_plane_wave(const Plan &plan) {
   const PlaneWave pw = ::fmmh::build_plane_wave(opt_.p, 0);
   require(pw.nodes <= dev::kPwMaxNodes, "plane wave: too many quadrature nodes", kUnsupportedLayout);
-  const size_t lds = dev::pw_local_lds(opt_.p, pw.H);
-  require(lds <= 64 * 1024, "plane wave: the quadrature does not fit in LDS", kUnsupportedLayout);
+  const size_t lds = dev::pw_local_lds(opt_.p, pw.H, pw.nodes);
+  require(lds <= 160 * 1024, "plane wave: the quadrature does not fit in LDS", kUnsupportedLayout);
   hip_check(dev::pw_allow_lds(lds), "plane wave LDS");
   pw_first_.upload(pw.first);
   pw_nalpha_.upload(pw.nalpha);
@@ -535,6 +535,13 @@ void DeviceOperator::build_plane_wave(const Plan &plan) {
   pw_hk_.upload(pw.hk);
   pw_fwd_.upload(pw.fwd);
   pw_bwd_.upload(pw.bwd);
+  // the (node, order) sums of the local step, by decreasing alpha count
+  std::vector<int> gitem;
+  for (int k = 0; k < pw.nodes; ++k)
+    for (int m = 0; m < pw.fourier[k]; ++m) gitem.push_back(k * (opt_.p + 1) + m);
+  std::stable_sort(gitem.begin(), gitem.end(),
+                   [&](int x, int y) { return pw.nalpha[x / (opt_.p + 1)] > pw.nalpha[y / (opt_.p + 1)]; });
+  pw_gitem_.upload(gitem);
   // geometry: accumulators by direction, their entries with the source shift of each slot
   // (one per distinct shift), the child shifts
   const size_t H2 = 2 * size_t(pw.H);
@@ -543,6 +550,7 @@ void DeviceOperator::build_plane_wave(const Plan &plan) {
   std::vector<double> src_shift, child_shift(size_t(kPwDirections) * kPwChildren * H2);
   std::map<std::array<double, 4>, int> shift_of;
   for (int dir = 0; dir < kPwDirections; ++dir) {
+    require(accs[dir].size() < size_t(dev::kPwAccStride), "plane wave: too many accumulators", kUnsupportedLayout);
     for (const PwAccumulator &a : accs[dir]) {
       acc_mask.push_back(a.mask);
       for (int slot : a.slots) {
@@ -567,16 +575,17 @@ void DeviceOperator::build_plane_wave(const Plan &plan) {
       plane_wave_shift(pw, dir, d, &child_shift[(size_t(kPwChildren) * dir + c) * H2]);
     }
   }
+  require(shift_of.size() <= 128, "plane wave: too many source shifts", kUnsupportedLayout);
   pw_acc_first_.upload(acc_first);
   pw_acc_mask_.upload(acc_mask);
-  pw_ent_first_.upload(ent_first);
-  pw_ent_slot_.upload(ent_slot);
-  pw_ent_shift_.upload(ent_shift);
   pw_src_shift_.upload(src_shift);
   pw_child_shift_.upload(child_shift);
   // parents by the level of their children, children in box order
   constexpr uint32_t kNoBox = 0xffffffffu;
-  std::vector<uint32_t> slots, children, targets;
+  const uint32_t box0 = nlevels_ > 2 ? static_cast<uint32_t>(box_base_[2]) : 0;
+  require(nlevels_ <= 2 || (box_base_[nlevels_] - box0) * 6 < (uint64_t(1) << 25),
+          "plane wave: too many boxes for the merge's entry format", kTooLarge);
+  std::vector<uint32_t> pe_first, pe, children, targets;
   pw_par_level_.assign(nlevels_ + 1, 0);
   pw_tgt_level_.assign(nlevels_ + 1, 0);
   for (int l = 0; l < nlevels_; ++l) {
@@ -617,25 +626,40 @@ void DeviceOperator::build_plane_wave(const Plan &plan) {
         for (int c = 0; sl[a] != kNoBox && c < kPwChildren; ++c)
           if (ch[c] != kNoBox && plane_wave_slot_direction(a, c) >= 0) ++pairs;
       require(pairs == entries, "plane wave: the V lists are not the merge-and-shift pairs", kUnsupportedLayout);
-      slots.insert(slots.end(), sl.begin(), sl.end());
+      // the held entries of each direction's accumulators, in the accumulators' order
+      for (int dir = 0; dir < kPwDirections; ++dir) {
+        const size_t base = pe_first.size();
+        pe_first.resize(base + dev::kPwAccStride, 0);
+        for (int a = acc_first[dir]; a < acc_first[dir + 1]; ++a) {
+          pe_first[base + (a - acc_first[dir])] = static_cast<uint32_t>(pe.size());
+          for (int e = ent_first[a]; e < ent_first[a + 1]; ++e) {
+            const uint32_t src = sl[ent_slot[e]];
+            if (src == kNoBox) continue;
+            pe.push_back(((src - box0) * 6 + dir) << 7 | static_cast<uint32_t>(ent_shift[e]));
+          }
+        }
+        for (int a = acc_first[dir + 1] - acc_first[dir]; a < dev::kPwAccStride; ++a)
+          pe_first[base + a] = static_cast<uint32_t>(pe.size());
+      }
       children.insert(children.end(), ch.begin(), ch.end());
       for (uint32_t b = c0; b < c0 + kPwChildren; ++b) targets.push_back(static_cast<uint32_t>(box_base_[l] + b));
     }
   }
   pw_par_level_[nlevels_] = static_cast<uint32_t>(children.size() / kPwChildren);
   pw_tgt_level_[nlevels_] = static_cast<uint32_t>(targets.size());
-  pw_slots_.upload(slots);
+  pw_pe_first_.upload(pe_first);
+  pw_pe_.upload(pe);
   pw_children_.upload(children);
   pw_targets_.upload(targets);
-  pw_box0_ = nlevels_ > 2 ? box_base_[2] : 0;
+  pw_box0_ = box0;
   pw_nbox_ = nlevels_ > 2 ? static_cast<uint32_t>(box_base_[nlevels_] - box_base_[2]) : 0;
   pw_waves_.alloc(size_t(pw_nbox_) * 12 * pw.H);
   pw_inc_.alloc(size_t(pw_nbox_) * 12 * pw.H);
   pw_ = dev::PwArgs{opt_.p,         pw.nodes,     pw.H,       rotation_block_size(opt_.p), pw_first_.ptr,
                     pw_nalpha_.ptr, pw_fourier_.ptr, pw_opow_.ptr, pw_ipow_.ptr,              pw_hw_.ptr,
-                    pw_hc_.ptr,     pw_hs_.ptr,   pw_hk_.ptr, pw_fwd_.ptr,                 pw_bwd_.ptr};
-  pw_merge_ = dev::PwMerge{pw_acc_first_.ptr, pw_acc_mask_.ptr,  pw_ent_first_.ptr,  pw_ent_slot_.ptr,
-                           pw_ent_shift_.ptr, pw_src_shift_.ptr, pw_child_shift_.ptr};
+                    pw_hc_.ptr,     pw_hs_.ptr,   pw_hk_.ptr, pw_fwd_.ptr,                 pw_bwd_.ptr,
+                    static_cast<int>(gitem.size()), pw_gitem_.ptr};
+  pw_merge_ = dev::PwMerge{pw_acc_first_.ptr, pw_acc_mask_.ptr, pw_src_shift_.ptr, pw_child_shift_.ptr};
 }
 
 void DeviceOperator::build_rotation(const Plan &plan) {
@@ -788,8 +812,8 @@ void DeviceOperator::launch_m2l(int level_first, int level_last) {
     const uint32_t t0 = pw_tgt_level_[level_first], t1 = pw_tgt_level_[level_last + 1];
     dev::pw_outgoing(stream_, pw_, b1 - b0, b0, mult_.ptr, pw_waves_.ptr + size_t(b0 - pw_box0_) * 12 * pw_.H);
     timed(kKindM2lContract, [&] {
-      dev::pw_merge(stream_, pw_, pw_merge_, q1 - q0, pw_slots_.ptr + size_t(q0) * kPwSlots,
-                    pw_children_.ptr + size_t(q0) * kPwChildren, pw_box0_, pw_waves_.ptr, pw_inc_.ptr);
+      dev::pw_merge(stream_, pw_, pw_merge_, q1 - q0, pw_pe_first_.ptr + size_t(q0) * 6 * dev::kPwAccStride,
+                    pw_pe_.ptr, pw_children_.ptr + size_t(q0) * kPwChildren, pw_box0_, pw_waves_.ptr, pw_inc_.ptr);
     });
     dev::pw_local(stream_, pw_, t1 - t0, pw_targets_.ptr + t0, pw_box0_, pw_inc_.ptr, loc_.ptr);
     return;

@@ -234,15 +234,15 @@ class DeviceOperator {
   void build_rotation(const Plan &plan);
   void build_plane_wave(const Plan &plan);
   // plane-wave M2L: quadrature tables, the merge-and-shift geometry, per level (of the
-  // children) the parents' slots and children and the target boxes, the waves and the
-  // incoming waves of the boxes of levels >= 2
+  // children) the parents' held accumulator entries and children and the target boxes,
+  // the waves and the incoming waves of the boxes of levels >= 2
   dev::PwArgs pw_{};
   dev::PwMerge pw_merge_{};
   DevBuf<double> pw_opow_, pw_ipow_, pw_hw_, pw_hc_, pw_hs_, pw_fwd_, pw_bwd_, pw_src_shift_, pw_child_shift_;
   DevBuf<double> pw_waves_, pw_inc_;
-  DevBuf<int> pw_first_, pw_nalpha_, pw_fourier_, pw_hk_;
-  DevBuf<int> pw_acc_first_, pw_acc_mask_, pw_ent_first_, pw_ent_slot_, pw_ent_shift_;
-  DevBuf<uint32_t> pw_slots_, pw_children_, pw_targets_;
+  DevBuf<int> pw_first_, pw_nalpha_, pw_fourier_, pw_hk_, pw_gitem_;
+  DevBuf<int> pw_acc_first_, pw_acc_mask_;
+  DevBuf<uint32_t> pw_pe_first_, pw_pe_, pw_children_, pw_targets_;
   std::vector<uint32_t> pw_par_level_, pw_tgt_level_;
   uint32_t pw_box0_ = 0, pw_nbox_ = 0;
   size_t nslots_ = 0;                  // clustered-mode scratch rows (slot ids follow the box ids)

@@ -2,22 +2,24 @@
 // reference fmm-kernels/src/plane_wave.rs:1-60: outgoing, merge, gather, local).
 //
 // Three kernels, all levels of a call in one launch each:
-//   k_pw_outgoing: one 256-thread block per source box.  For each frame f the multipole is
-//     rotated into the frame (thread = slot), A_k(m) formed for up and down (thread =
-//     (node, order), orders m < F_k'), and the waves X of the H half nodes written for the
-//     six directions (thread = half node): waves[box][2 f + down][re, im][H].
+//   k_pw_outgoing: one 512-thread block per source box, the three frames at once: the
+//     multipole rotated into each frame (thread = (frame, slot)), A_k(m) formed for up and
+//     down (thread = (frame, node, order), orders m < F_k'), and the waves X of the H half
+//     nodes written for the six directions (thread = (frame, half node)):
+//     waves[box][2 f + down][re, im][H].
 //   k_pw_merge: one block per (parent, direction), a thread per half node.  Each of the
 //     direction's accumulators sums its slots' waves times their shifts to the parent's
 //     centre; each child adds the accumulators that serve it and multiplies by its shift
 //     from the parent's centre: inc[child][dir][re, im][H].  The merge and the gather of
 //     the reference in one pass, the accumulators in registers.
-//   k_pw_local: one 256-thread block per target box.  Per direction G_k(m) from the
-//     incoming waves (thread = (node, order), the e^{-i m a_j} by a recurrence in j), the
-//     frame's local L'_n^m (thread = slot); after up and down the frame's local is rotated
-//     back and added to the slot's sum, which is added to the box's local at the end.
+//   k_pw_local: one 512-thread block per target box, the six directions at once: G_k(m)
+//     from the incoming waves (thread = (direction, node, order), the e^{-i m a_j} by a
+//     recurrence in j), every frame's local L'_n^m from its up and down sums (thread =
+//     (frame, n, m)), the three rotations back summed per slot and added to the box's local.
 // f64 throughout (the plane-wave M2L is an f64 operator in the reference).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "kernels.hpp"
@@ -27,8 +29,15 @@ namespace fmmh {
 namespace dev {
 namespace {
 
-constexpr int kPwThreads = 256;
+constexpr int kPwLocalThreads = 512;
 constexpr uint32_t kNoBox = 0xffffffffu;
+
+// LDS words of k_pw_local's waves region (the six directions' waves, later the three
+// frames' locals)
+__host__ __device__ inline size_t pw_local_wave_words(int p, int H) {
+  const size_t a = 12 * size_t(H), b = 3 * size_t(p + 1) * (p + 1);
+  return a > b ? a : b;
+}
 
 // slot k -> degree
 __device__ __forceinline__ int degree_of(int k) {
@@ -37,118 +46,129 @@ __device__ __forceinline__ int degree_of(int k) {
   return n;
 }
 
-__global__ __launch_bounds__(kPwThreads) void k_pw_outgoing(PwArgs a, uint32_t nboxes, uint32_t box0,
-                                                            const double *__restrict__ M, double *__restrict__ waves) {
-  __shared__ double Ms[kMaxCoef], xs[kMaxCoef];
-  __shared__ double Au[kPwMaxNodes][kMaxDegree + 1][2], Ad[kPwMaxNodes][kMaxDegree + 1][2];
+// One block per source box, the three frames at once: the rotations into the frames
+// (thread = (frame, slot)), the modes A_k(m) up and down (thread = (frame, node, order)),
+// the waves of the six directions (thread = (frame, half node)).
+__global__ __launch_bounds__(kPwLocalThreads) void k_pw_outgoing(PwArgs a, uint32_t nboxes, uint32_t box0,
+                                                                 const double *__restrict__ M,
+                                                                 double *__restrict__ waves) {
+  extern __shared__ double sm[];
+  const int p = a.p, P1 = p + 1, nc = P1 * P1, H = a.H, s = a.nodes;
+  double *Ms = sm, *xs = sm + nc;          // the multipole; its rotations into the frames [3][nc]
+  double *A = xs + 3 * nc;                 // [3][s][P1][up re, up im, down re, down im]
   const uint32_t item = xcd_index(nboxes);
   if (item >= nboxes) return;
   const size_t box = size_t(box0) + item;
-  const int p = a.p, P1 = p + 1, nc = P1 * P1, H = a.H, s = a.nodes;
   const int t = threadIdx.x;
-  for (int k = t; k < nc; k += kPwThreads) Ms[k] = M[box * nc + k];
+  for (int k = t; k < nc; k += kPwLocalThreads) Ms[k] = M[box * nc + k];
+  __syncthreads();
+  // rotate into the frames: x_n = F_n M_n (column-major blocks)
+  for (int it = t; it < 3 * nc; it += kPwLocalThreads) {
+    const int f = it / nc, k = it % nc;
+    const int n = degree_of(k), w = 2 * n + 1, r = k - n * n;
+    const double *blk = a.fwd + size_t(f) * a.rb + n * (2 * n - 1) * (2 * n + 1) / 3;
+    double v = 0.0;
+    for (int c = 0; c < w; ++c) v = fma(blk[r + c * w], Ms[n * n + c], v);
+    xs[f * nc + k] = v;
+  }
+  __syncthreads();
+  // A_k(m) = sum_{n >= m} lambda^n s^(n-m) 2^-(n+1) x_n^m, up (s = 1) and down (s = -1)
+  for (int it = t; it < 3 * a.nitem; it += kPwLocalThreads) {
+    const int f = it % 3, q = a.gitem[it / 3], k = q / P1, m = q % P1;
+    const double *pw = a.opow + k * P1, *x = xs + f * nc;
+    double ur = 0.0, ui = 0.0, dr = 0.0, di = 0.0, sg = 1.0;
+    for (int n = m; n <= p; ++n) {
+      const double ln = pw[n], re = x[n * n + n + m], im = m > 0 ? x[n * n + n - m] : 0.0;
+      ur = fma(ln, re, ur);
+      ui = fma(ln, im, ui);
+      dr = fma(sg * ln, re, dr);
+      di = fma(sg * ln, im, di);
+      sg = -sg;
+    }
+    double *o = A + ((f * s + k) * P1 + m) * 4;
+    o[0] = ur;
+    o[1] = ui;
+    o[2] = dr;
+    o[3] = di;
+  }
+  __syncthreads();
+  // X(k, j) = (w_k / N_k) [A(0) + sum_{1 <= m < F_k'} 2 (-i)^m Re(A(m) e^{i m a_j})]
   double *wv = waves + size_t(item) * 12 * H;
-  for (int f = 0; f < 3; ++f) {
-    __syncthreads();
-    // rotate into the frame: x_n = F_n M_n (column-major blocks)
-    for (int k = t; k < nc; k += kPwThreads) {
-      const int n = degree_of(k), w = 2 * n + 1, r = k - n * n;
-      const double *blk = a.fwd + size_t(f) * a.rb + n * (2 * n - 1) * (2 * n + 1) / 3;
-      double v = 0.0;
-      for (int c = 0; c < w; ++c) v = fma(blk[r + c * w], Ms[n * n + c], v);
-      xs[k] = v;
-    }
-    __syncthreads();
-    // A_k(m) = sum_{n >= m} lambda^n s^(n-m) 2^-(n+1) x_n^m, up (s = 1) and down (s = -1)
-    for (int q = t; q < s * P1; q += kPwThreads) {
-      const int k = q / P1, m = q % P1;
-      if (m >= a.fourier[k]) continue;
-      const double *pw = a.opow + k * P1;
-      double ur = 0.0, ui = 0.0, dr = 0.0, di = 0.0, sg = 1.0;
-      for (int n = m; n <= p; ++n) {
-        const double ln = pw[n], re = xs[n * n + n + m], im = m > 0 ? xs[n * n + n - m] : 0.0;
-        ur = fma(ln, re, ur);
-        ui = fma(ln, im, ui);
-        dr = fma(sg * ln, re, dr);
-        di = fma(sg * ln, im, di);
-        sg = -sg;
+  for (int it = t; it < 3 * H; it += kPwLocalThreads) {
+    const int f = it / H, h = it % H;
+    const int k = a.hk[h], F = a.fourier[k];
+    const double *Ak = A + (f * s + k) * P1 * 4;
+    const double c1 = a.hc[h], s1 = a.hs[h];
+    double cr = 1.0, ci = 0.0;  // e^{i m a}
+    double xur = Ak[0], xui = 0.0, xdr = Ak[2], xdi = 0.0;
+    for (int m = 1; m < F; ++m) {
+      const double nr = cr * c1 - ci * s1, ni = cr * s1 + ci * c1;
+      cr = nr;
+      ci = ni;
+      // Re(A e^{i m a}) for up and down
+      const double ru = 2.0 * (Ak[4 * m] * cr - Ak[4 * m + 1] * ci);
+      const double rd = 2.0 * (Ak[4 * m + 2] * cr - Ak[4 * m + 3] * ci);
+      // (-i)^m: 1, -i, -1, i
+      switch (m & 3) {
+        case 0:
+          xur += ru;
+          xdr += rd;
+          break;
+        case 1:
+          xui -= ru;
+          xdi -= rd;
+          break;
+        case 2:
+          xur -= ru;
+          xdr -= rd;
+          break;
+        default:
+          xui += ru;
+          xdi += rd;
+          break;
       }
-      Au[k][m][0] = ur;
-      Au[k][m][1] = ui;
-      Ad[k][m][0] = dr;
-      Ad[k][m][1] = di;
     }
-    __syncthreads();
-    // X(k, j) = (w_k / N_k) [A(0) + sum_{1 <= m < F_k'} 2 (-i)^m Re(A(m) e^{i m a_j})]
-    for (int h = t; h < H; h += kPwThreads) {
-      const int k = a.hk[h], F = a.fourier[k];
-      const double c1 = a.hc[h], s1 = a.hs[h];
-      double cr = 1.0, ci = 0.0;  // e^{i m a}
-      double xur = Au[k][0][0], xui = 0.0, xdr = Ad[k][0][0], xdi = 0.0;
-      for (int m = 1; m < F; ++m) {
-        const double nr = cr * c1 - ci * s1, ni = cr * s1 + ci * c1;
-        cr = nr;
-        ci = ni;
-        // Re(A e^{i m a}) for up and down
-        const double ru = 2.0 * (Au[k][m][0] * cr - Au[k][m][1] * ci);
-        const double rd = 2.0 * (Ad[k][m][0] * cr - Ad[k][m][1] * ci);
-        // (-i)^m: 1, -i, -1, i
-        switch (m & 3) {
-          case 0:
-            xur += ru;
-            xdr += rd;
-            break;
-          case 1:
-            xui -= ru;
-            xdi -= rd;
-            break;
-          case 2:
-            xur -= ru;
-            xdr -= rd;
-            break;
-          default:
-            xui += ru;
-            xdi += rd;
-            break;
-        }
-      }
-      const double wn = a.hw[h];
-      wv[(2 * f) * 2 * H + h] = wn * xur;
-      wv[(2 * f) * 2 * H + H + h] = wn * xui;
-      wv[(2 * f + 1) * 2 * H + h] = wn * xdr;
-      wv[(2 * f + 1) * 2 * H + H + h] = wn * xdi;
-    }
+    const double wn = a.hw[h];
+    wv[(2 * f) * 2 * H + h] = wn * xur;
+    wv[(2 * f) * 2 * H + H + h] = wn * xui;
+    wv[(2 * f + 1) * 2 * H + h] = wn * xdr;
+    wv[(2 * f + 1) * 2 * H + H + h] = wn * xdi;
   }
 }
 
-// One block per (parent, direction); threads stride over the half nodes.
+// One block per (parent, direction); threads stride over the half nodes.  The held
+// entries of the parent's accumulators of the direction are listed ahead (pe_first per
+// (parent, direction) and accumulator, pe = wave block << 7 | source shift), so the loads
+// of successive entries do not wait on one another.
 __global__ __launch_bounds__(1024) void k_pw_merge(PwArgs a, PwMerge g, uint32_t nparents,
-                                                   const uint32_t *__restrict__ slots,
+                                                   const uint32_t *__restrict__ pe_first,
+                                                   const uint32_t *__restrict__ pe,
                                                    const uint32_t *__restrict__ children, uint32_t box0,
                                                    const double *__restrict__ waves, double *__restrict__ inc) {
   const uint32_t item = xcd_index(nparents * 6);
   if (item >= nparents * 6) return;
   const uint32_t parent = item / 6;
   const int dir = int(item % 6), H = a.H;
-  const uint32_t *sl = slots + size_t(parent) * 216;
+  const uint32_t *fo = pe_first + size_t(item) * kPwAccStride;
   const uint32_t *ch = children + size_t(parent) * 8;
-  const int a0 = g.acc_first[dir], a1 = g.acc_first[dir + 1];
+  const int a0 = g.acc_first[dir], na = g.acc_first[dir + 1] - a0;
   for (int h = threadIdx.x; h < H; h += blockDim.x) {
     double vr[8], vi[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) vr[c] = vi[c] = 0.0;
-    for (int ac = a0; ac < a1; ++ac) {
+    for (int ac = 0; ac < na; ++ac) {
       double sr = 0.0, si = 0.0;
-      for (int e = g.ent_first[ac]; e < g.ent_first[ac + 1]; ++e) {
-        const uint32_t src = sl[g.ent_slot[e]];
-        if (src == kNoBox) continue;
-        const double *X = waves + (size_t(src - box0) * 6 + dir) * 2 * H;
-        const double *T = g.src_shift + size_t(g.ent_shift[e]) * 2 * H;
+      const uint32_t e1 = fo[ac + 1];
+#pragma unroll 4
+      for (uint32_t e = fo[ac]; e < e1; ++e) {
+        const uint32_t w = pe[e];
+        const double *X = waves + size_t(w >> 7) * 2 * H;
+        const double *T = g.src_shift + size_t(w & 127u) * 2 * H;
         const double xr = X[h], xi = X[H + h], tr = T[h], ti = T[H + h];
         sr = fma(xr, tr, fma(-xi, ti, sr));
         si = fma(xr, ti, fma(xi, tr, si));
       }
-      const int mask = g.acc_mask[ac];
+      const int mask = g.acc_mask[a0 + ac];
 #pragma unroll
       for (int c = 0; c < 8; ++c)
         if ((mask >> c) & 1) {
@@ -169,122 +189,130 @@ __global__ __launch_bounds__(1024) void k_pw_merge(PwArgs a, PwMerge g, uint32_t
   }
 }
 
-__global__ __launch_bounds__(kPwThreads) void k_pw_local(PwArgs a, uint32_t ntargets,
+// One block per target box.  All six directions at once: the incoming waves staged in LDS,
+// then every (direction, node, order) sum G (items by decreasing alpha count, dealt to the
+// threads in turn), then every frame's local (thread = (frame, n, m)), stored over the
+// waves, then the rotations back (thread = slot).
+__global__ __launch_bounds__(kPwLocalThreads) void k_pw_local(PwArgs a, uint32_t ntargets,
                                                          const uint32_t *__restrict__ target, uint32_t box0,
                                                          const double *__restrict__ inc, double *__restrict__ L) {
   extern __shared__ double sm[];
   const int p = a.p, P1 = p + 1, nc = P1 * P1, H = a.H, s = a.nodes;
-  double *Vr = sm, *Vi = sm + H;                           // incoming waves of one direction
-  double *Gr = Vi + H, *Gi = Gr + kPwMaxNodes * P1;       // G_k(m)
-  double *Lr = Gi + kPwMaxNodes * P1, *Li = Lr + P1 * P1;  // the frame's local, [n][m]
-  double *ys = Li + P1 * P1;                              // real storage of the frame's local
+  double *V = sm;                      // [6][re, im][H], later the frames' locals ys[3][nc]
+  double *Gr = sm + pw_local_wave_words(p, H);  // [6][s][P1]
+  double *Gi = Gr + 6 * s * P1;
+  double *ip = Gi + 6 * s * P1;        // the incoming powers [s][P1]
+  int *kf = reinterpret_cast<int *>(ip + s * P1);  // per node: first half node, N_k, F_k'
+  int *gq = kf + 3 * kPwMaxNodes;      // the (node, order) items
   const uint32_t row = xcd_index(ntargets);
   if (row >= ntargets) return;
   const int t = threadIdx.x;
   const uint32_t box = target[row];
   const double *in = inc + size_t(box - box0) * 12 * H;
-  double acc = 0.0;  // slot t's sum over the frames
-  for (int f = 0; f < 3; ++f) {
-    for (int q = t; q < P1 * P1; q += kPwThreads) Lr[q] = Li[q] = 0.0;
-    for (int ud = 0; ud < 2; ++ud) {
-      const int dir = 2 * f + ud;
-      __syncthreads();
-      for (int h = t; h < 2 * H; h += kPwThreads) Vr[h] = in[size_t(dir) * 2 * H + h];
-      __syncthreads();
-      // G_k(m) = 2 sum_j Re V e^{-i m a_j} (m even), 2 i sum_j Im V e^{-i m a_j} (m odd)
-      for (int q = t; q < s * P1; q += kPwThreads) {
-        const int k = q / P1, m = q % P1;
-        if (m >= a.fourier[k]) continue;
-        const int h0 = a.first[k], nh = a.nalpha[k] / 2;
-        double sr, si;
-        sincos(-2.0 * 3.141592653589793 * m / a.nalpha[k], &si, &sr);  // e^{-i m 2 pi / N_k}
-        double er = 1.0, ei = 0.0, gr = 0.0, gi = 0.0;
-        const bool even = (m & 1) == 0;
-        const double *v = even ? Vr + h0 : Vi + h0;
-        for (int j = 0; j < nh; ++j) {
-          gr = fma(v[j], er, gr);
-          gi = fma(v[j], ei, gi);
-          const double nr = er * sr - ei * si, ni = er * si + ei * sr;
-          er = nr;
-          ei = ni;
-        }
-        // even: 2 (gr + i gi); odd: 2 i (gr + i gi) = 2 (-gi + i gr)
-        Gr[k * P1 + m] = even ? 2.0 * gr : -2.0 * gi;
-        Gi[k * P1 + m] = even ? 2.0 * gi : 2.0 * gr;
-      }
-      __syncthreads();
-      // L'_n^m += sum_{k: m < F_k'} 2^-n lambda_k^n (-s)^(n-m) i^m G_k(m)   (s = 1 up, -1 down)
-      for (int q = t; q < P1 * P1; q += kPwThreads) {
-        const int n = q / P1, m = q % P1;
-        if (m > n) continue;
-        double sr = 0.0, si = 0.0;
-        for (int k = 0; k < s; ++k) {
-          if (m >= a.fourier[k]) continue;
-          const double c = a.ipow[k * P1 + n];
-          sr = fma(c, Gr[k * P1 + m], sr);
-          si = fma(c, Gi[k * P1 + m], si);
-        }
-        if (ud == 0 && ((n - m) & 1)) {
-          sr = -sr;
-          si = -si;
-        }
-        // times i^m
-        double rr = sr, ri = si;
-        switch (m & 3) {
-          case 1:
-            rr = -si;
-            ri = sr;
-            break;
-          case 2:
-            rr = -sr;
-            ri = -si;
-            break;
-          case 3:
-            rr = si;
-            ri = -sr;
-            break;
-          default:
-            break;
-        }
-        Lr[q] += rr;
-        Li[q] += ri;
-      }
-    }
-    __syncthreads();
-    for (int k = t; k < nc; k += kPwThreads) {
-      const int n = degree_of(k), m = k - n * n - n;
-      ys[k] = m >= 0 ? Lr[n * P1 + m] : Li[n * P1 - m];
-    }
-    __syncthreads();
-    if (t < nc) {
-      const int n = degree_of(t), w = 2 * n + 1, r = t - n * n;
-      const double *blk = a.bwd + size_t(f) * a.rb + n * (2 * n - 1) * (2 * n + 1) / 3;
-      double v = 0.0;
-      for (int c = 0; c < w; ++c) v = fma(blk[r + c * w], ys[n * n + c], v);
-      acc += v;
-    }
+  for (int h = t; h < 12 * H; h += kPwLocalThreads) V[h] = in[h];
+  for (int q = t; q < s * P1; q += kPwLocalThreads) ip[q] = a.ipow[q];
+  for (int k = t; k < s; k += kPwLocalThreads) {
+    kf[3 * k] = a.first[k];
+    kf[3 * k + 1] = a.nalpha[k];
+    kf[3 * k + 2] = a.fourier[k];
   }
-  if (t < nc) L[size_t(box) * nc + t] += acc;
+  for (int q = t; q < a.nitem; q += kPwLocalThreads) gq[q] = a.gitem[q];
+  __syncthreads();
+  // G_k(m) = 2 sum_j Re V e^{-i m a_j} (m even), 2 i sum_j Im V e^{-i m a_j} (m odd)
+  for (int it = t; it < 6 * a.nitem; it += kPwLocalThreads) {
+    const int dir = it % 6, q = gq[it / 6], k = q / P1, m = q % P1;
+    const int h0 = kf[3 * k], nh = kf[3 * k + 1] / 2;
+    double sr, si;
+    sincos(-2.0 * 3.141592653589793 * m / kf[3 * k + 1], &si, &sr);  // e^{-i m 2 pi / N_k}
+    double er = 1.0, ei = 0.0, gr = 0.0, gi = 0.0;
+    const bool even = (m & 1) == 0;
+    const double *v = V + dir * 2 * H + (even ? 0 : H) + h0;
+    for (int j = 0; j < nh; ++j) {
+      gr = fma(v[j], er, gr);
+      gi = fma(v[j], ei, gi);
+      const double nr = er * sr - ei * si, ni = er * si + ei * sr;
+      er = nr;
+      ei = ni;
+    }
+    // even: 2 (gr + i gi); odd: 2 i (gr + i gi) = 2 (-gi + i gr)
+    Gr[(dir * s + k) * P1 + m] = even ? 2.0 * gr : -2.0 * gi;
+    Gi[(dir * s + k) * P1 + m] = even ? 2.0 * gi : 2.0 * gr;
+  }
+  __syncthreads();
+  // the frame's local: L'_n^m = sum_{up, down} sum_{k: m < F_k'} 2^-n lambda_k^n (-s)^(n-m) i^m
+  // G_k(m) (s = 1 up, -1 down), stored as the real coefficients of the frame
+  double *ys = V;
+  const int tri = P1 * (P1 + 1) / 2;
+  for (int it = t; it < 3 * tri; it += kPwLocalThreads) {
+    const int f = it / tri;
+    int r = it % tri, n = 0;
+    while (r > n) r -= ++n;
+    const int m = r;
+    double ur = 0.0, ui = 0.0, dr = 0.0, di = 0.0;
+    for (int k = 0; k < s; ++k) {
+      if (m >= kf[3 * k + 2]) continue;
+      const double c = ip[k * P1 + n];
+      const int gu = ((2 * f) * s + k) * P1 + m, gd = ((2 * f + 1) * s + k) * P1 + m;
+      ur = fma(c, Gr[gu], ur);
+      ui = fma(c, Gi[gu], ui);
+      dr = fma(c, Gr[gd], dr);
+      di = fma(c, Gi[gd], di);
+    }
+    // up carries (-1)^(n-m)
+    const double sr = ((n - m) & 1) ? dr - ur : dr + ur, si = ((n - m) & 1) ? di - ui : di + ui;
+    // times i^m
+    double rr = sr, ri = si;
+    switch (m & 3) {
+      case 1:
+        rr = -si;
+        ri = sr;
+        break;
+      case 2:
+        rr = -sr;
+        ri = -si;
+        break;
+      case 3:
+        rr = si;
+        ri = -sr;
+        break;
+      default:
+        break;
+    }
+    ys[f * nc + n * n + n + m] = rr;
+    if (m > 0) ys[f * nc + n * n + n - m] = ri;
+  }
+  __syncthreads();
+  if (t < nc) {
+    const int n = degree_of(t), w = 2 * n + 1, r = t - n * n;
+    double acc = 0.0;
+    for (int f = 0; f < 3; ++f) {
+      const double *blk = a.bwd + size_t(f) * a.rb + n * (2 * n - 1) * (2 * n + 1) / 3;
+      for (int c = 0; c < w; ++c) acc = fma(blk[r + c * w], ys[f * nc + n * n + c], acc);
+    }
+    L[size_t(box) * nc + t] += acc;
+  }
 }
 
 }  // namespace
 
-size_t pw_local_lds(int p, int H) {
-  const int P1 = p + 1;
-  return sizeof(double) * (2 * size_t(H) + 2 * size_t(kPwMaxNodes) * P1 + 2 * size_t(P1) * P1 + size_t(P1) * P1);
+size_t pw_local_lds(int p, int H, int nodes) {
+  const size_t P1 = p + 1;
+  return sizeof(double) * (pw_local_wave_words(p, H) + 13 * size_t(nodes) * P1) +
+         sizeof(int) * (3 * size_t(kPwMaxNodes) + size_t(kPwMaxNodes) * P1);
 }
 
 void pw_outgoing(hipStream_t s, const PwArgs &a, uint32_t nboxes, uint32_t box0, const double *M, double *waves) {
   if (nboxes == 0) return;
-  hipLaunchKernelGGL(k_pw_outgoing, dim3(xcd_grid(nboxes)), dim3(kPwThreads), 0, s, a, nboxes, box0, M, waves);
+  const size_t P1 = a.p + 1, lds = sizeof(double) * (4 * P1 * P1 + 12 * size_t(a.nodes) * P1);
+  hipLaunchKernelGGL(k_pw_outgoing, dim3(xcd_grid(nboxes)), dim3(kPwLocalThreads), lds, s, a, nboxes, box0, M, waves);
 }
 
-void pw_merge(hipStream_t s, const PwArgs &a, const PwMerge &g, uint32_t nparents, const uint32_t *slots,
-              const uint32_t *children, uint32_t box0, const double *waves, double *inc) {
+void pw_merge(hipStream_t s, const PwArgs &a, const PwMerge &g, uint32_t nparents, const uint32_t *pe_first,
+              const uint32_t *pe, const uint32_t *children, uint32_t box0, const double *waves, double *inc) {
   if (nparents == 0) return;
   const int threads = a.H >= 1024 ? 1024 : (a.H + 63) / 64 * 64;
-  hipLaunchKernelGGL(k_pw_merge, dim3(xcd_grid(nparents * 6)), dim3(threads), 0, s, a, g, nparents, slots, children,
-                     box0, waves, inc);
+  hipLaunchKernelGGL(k_pw_merge, dim3(xcd_grid(nparents * 6)), dim3(threads), 0, s, a, g, nparents, pe_first, pe,
+                     children, box0, waves, inc);
 }
 
 hipError_t pw_allow_lds(size_t bytes) {
@@ -295,7 +323,7 @@ hipError_t pw_allow_lds(size_t bytes) {
 void pw_local(hipStream_t s, const PwArgs &a, uint32_t ntargets, const uint32_t *target, uint32_t box0,
               const double *inc, double *L) {
   if (ntargets == 0) return;
-  hipLaunchKernelGGL(k_pw_local, dim3(xcd_grid(ntargets)), dim3(kPwThreads), pw_local_lds(a.p, a.H), s, a, ntargets,
+  hipLaunchKernelGGL(k_pw_local, dim3(xcd_grid(ntargets)), dim3(kPwLocalThreads), pw_local_lds(a.p, a.H, a.nodes), s, a, ntargets,
                      target, box0, inc, L);
 }
 
