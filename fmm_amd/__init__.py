@@ -595,6 +595,63 @@ def helmholtz_direct_sum(k, sources, charges, targets, gradients=True, device=0)
     return phi.view(np.complex128), (grad.view(np.complex128).reshape(-1, 3) if gradients else None)
 
 
+class HelmholtzPlaneWaveQuadrature(NamedTuple):
+    """FMM3D's iprec-2 quadrature of kappa = k w as the reference maps it (quadrature.rs)."""
+    domain: int              # 1-based, 0 outside the tables
+    stated_error: float
+    lambdas: np.ndarray      # complex
+    t: np.ndarray            # complex, sqrt(lambda^2 + kappa^2)
+    weights: np.ndarray      # complex
+    alpha_counts: np.ndarray
+    fourier_counts: np.ndarray
+
+
+def helmholtz_plane_wave_quadrature(kappa) -> HelmholtzPlaneWaveQuadrature:
+    """The exponential M2L's quadrature of kappa = k w (box widths)."""
+    cap = 128
+    dom, n, err = C.c_int32(0), C.c_int32(0), C.c_double(0.0)
+    lam, t, w = np.zeros(2 * cap), np.zeros(2 * cap), np.zeros(2 * cap)
+    al, fo = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+    _lib.check(_lib.lib().fmmh_helmholtz_plane_wave_quadrature(
+        float(np.real(kappa)), float(np.imag(kappa)), cap, C.byref(dom), C.byref(n), C.byref(err), _lib.ptr(lam),
+        _lib.ptr(t), _lib.ptr(w), _lib.ptr(al), _lib.ptr(fo)))
+    k = n.value
+    c = lambda v: v[: 2 * k].view(np.complex128).copy()
+    return HelmholtzPlaneWaveQuadrature(dom.value, err.value, c(lam), c(t), c(w), al[:k].copy(), fo[:k].copy())
+
+
+def bessel_j_norms(z, top: int) -> np.ndarray:
+    """|J_0(z)| .. |J_top(z)| of complex z (Miller's recurrence, normalised by e^{iz})."""
+    out = np.zeros(top + 1)
+    _lib.check(_lib.lib().fmmh_bessel_j_norms(float(np.real(z)), float(np.imag(z)), int(top), _lib.ptr(out)))
+    return out
+
+
+def helmholtz_plane_wave_m2l(p: int, kh, offset: int, multipole, direction: int = -1) -> np.ndarray:
+    """One V pair's exponential M2L on the host for a level with kh = k r_l: the stored
+    local (complex, (p+1)^2) it adds to a stored multipole (complex)."""
+    n = (p + 1) ** 2
+    m = np.asarray(multipole, dtype=np.complex128).reshape(n)
+    x = np.ascontiguousarray(np.concatenate([m.real, m.imag]))
+    out = np.zeros(2 * n)
+    _lib.check(_lib.lib().fmmh_helmholtz_plane_wave_m2l(int(p), float(np.real(kh)), float(np.imag(kh)), int(offset),
+                                                        int(direction), _lib.ptr(x), _lib.ptr(out)))
+    return out[:n] + 1j * out[n:]
+
+
+def helmholtz_plane_wave_merge(p: int, kh, slot_multipoles, present) -> np.ndarray:
+    """One parent's merge and shift on the host: 216 complex slot multipoles (rows), which
+    are held; the 8 children's complex locals (rows) it adds."""
+    n = (p + 1) ** 2
+    m = np.asarray(slot_multipoles, dtype=np.complex128).reshape(216, n)
+    x = np.ascontiguousarray(np.concatenate([m.real, m.imag], axis=1))
+    pr = np.ascontiguousarray(present, dtype=np.uint8).reshape(216)
+    out = np.zeros((8, 2 * n))
+    _lib.check(_lib.lib().fmmh_helmholtz_plane_wave_merge(int(p), float(np.real(kh)), float(np.imag(kh)),
+                                                          _lib.ptr(x), _lib.ptr(pr), _lib.ptr(out)))
+    return out[:, :n] + 1j * out[:, n:]
+
+
 class HelmholtzOperator:
     """The Helmholtz device operator over a plan (complex charges, kernel e^{ikr}/(4 pi r))."""
 

@@ -42,6 +42,8 @@ EXPORTS = [
     "fmmh_helmholtz_read_output", "fmmh_helmholtz_read_expansions", "fmmh_helmholtz_kind_timing",
     "fmmh_helmholtz_kind_times", "fmmh_helmholtz_table", "fmmh_helmholtz_direct_sum",
     "fmmh_plane_wave_quadrature", "fmmh_plane_wave_m2l", "fmmh_plane_wave_merge", "fmmh_plane_wave_slot_direction",
+    "fmmh_helmholtz_plane_wave_quadrature", "fmmh_bessel_j_norms", "fmmh_helmholtz_plane_wave_m2l",
+    "fmmh_helmholtz_plane_wave_merge",
     "fmmh_stream_wait", "fmmh_helmholtz_stream_wait",
     "fmmh_direct_sum_noalloc", "fmmh_operator_create_tuned", "fmmh_autotune", "fmmh_inject_error",
     "fmmh_set_table_cache", "fmmh_table_cache_stats",
@@ -162,6 +164,10 @@ def lib():
     L.fmmh_plane_wave_m2l.argtypes = [i32, i32, i32, i32, vp, vp]
     L.fmmh_plane_wave_merge.argtypes = [i32, i32, vp, vp, vp]
     L.fmmh_plane_wave_slot_direction.argtypes = [i32, i32]
+    L.fmmh_helmholtz_plane_wave_quadrature.argtypes = [dbl, dbl, i32, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.fmmh_bessel_j_norms.argtypes = [dbl, dbl, i32, vp]
+    L.fmmh_helmholtz_plane_wave_m2l.argtypes = [i32, dbl, dbl, i32, i32, vp, vp]
+    L.fmmh_helmholtz_plane_wave_merge.argtypes = [i32, dbl, dbl, vp, vp, vp]
     L.fmmh_helmholtz_degree.argtypes = [dbl, dbl, dbl]
     L.fmmh_helmholtz_create.argtypes = [vp, C.POINTER(HelmholtzOptions), C.POINTER(vp)]
     L.fmmh_helmholtz_free.argtypes = [vp]

@@ -9,6 +9,7 @@
 
 #include "../../include/fmm_amd.h"
 #include "hoperator.hpp"
+#include "hpwtables.hpp"
 #include "htables.hpp"
 #include "operator.hpp"
 #include "plan.hpp"
@@ -289,6 +290,75 @@ int32_t fmmh_device_describe(int32_t device, char *name, int32_t capacity) {
     const std::string s = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount) + "cu";
     need(int64_t(s.size()) < capacity, "fmmh_device_describe: name buffer too small");
     std::memcpy(name, s.c_str(), s.size() + 1);
+  });
+}
+
+int32_t fmmh_helmholtz_plane_wave_quadrature(double kappa_re, double kappa_im, int32_t capacity, int32_t *domain,
+                                             int32_t *nodes, double *stated_error, double *lambda, double *t,
+                                             double *weight, int32_t *alpha, int32_t *fourier) {
+  return guard([&] {
+    need(domain && nodes, "fmmh_helmholtz_plane_wave_quadrature: NULL domain or nodes");
+    const fmmh::HpwQuadrature q = fmmh::hpw_quadrature(fmmh::cplx(kappa_re, kappa_im));
+    *domain = q.domain;
+    *nodes = static_cast<int32_t>(q.lambda.size());
+    if (stated_error) *stated_error = q.stated_error;
+    need(int(q.lambda.size()) <= capacity, "fmmh_helmholtz_plane_wave_quadrature: capacity below the node count");
+    for (size_t k = 0; k < q.lambda.size(); ++k) {
+      if (lambda) lambda[2 * k] = q.lambda[k].real(), lambda[2 * k + 1] = q.lambda[k].imag();
+      if (t) t[2 * k] = q.t[k].real(), t[2 * k + 1] = q.t[k].imag();
+      if (weight) weight[2 * k] = q.weight[k].real(), weight[2 * k + 1] = q.weight[k].imag();
+      if (alpha) alpha[k] = q.alpha[k];
+      if (fourier) fourier[k] = q.fourier[k];
+    }
+  });
+}
+
+int32_t fmmh_bessel_j_norms(double z_re, double z_im, int32_t top, double *out) {
+  return guard([&] {
+    need(out != nullptr && top >= 0, "fmmh_bessel_j_norms: NULL out or negative top");
+    const std::vector<double> v = fmmh::hpw_bessel_j_norms(fmmh::cplx(z_re, z_im), top);
+    std::copy(v.begin(), v.end(), out);
+  });
+}
+
+// The last Helmholtz plane-wave level built by the host restatements, kept for the next
+// call of the same (p, kh).
+static const fmmh::HpwLevel &cached_hpw_level(int32_t p, fmmh::cplx kh, std::unique_lock<std::mutex> &lock) {
+  static std::mutex mu;
+  static std::unique_ptr<fmmh::HpwLevel> last;
+  lock = std::unique_lock<std::mutex>(mu);
+  if (!last || last->p != p || last->kh != kh) {
+    need(p >= 0 && p <= 30, "helmholtz plane wave: p must lie in 0..30");
+    last = std::make_unique<fmmh::HpwLevel>(fmmh::hpw_level(p, kh, fmmh::hpw_quadrature(2.0 * kh)));
+  }
+  return *last;
+}
+
+int32_t fmmh_helmholtz_plane_wave_m2l(int32_t p, double kh_re, double kh_im, int32_t offset, int32_t direction,
+                                      const double *in, double *out) {
+  return guard([&] {
+    need(in && out, "fmmh_helmholtz_plane_wave_m2l: NULL argument");
+    need(offset >= 0 && offset < 316, "fmmh_helmholtz_plane_wave_m2l: offset index 0..315");
+    need(direction >= -1 && direction < 6, "fmmh_helmholtz_plane_wave_m2l: direction -1..5");
+    std::unique_lock<std::mutex> lock;
+    const fmmh::HpwLevel &L = cached_hpw_level(p, fmmh::cplx(kh_re, kh_im), lock);
+    int d[3];
+    fmmh::offset_of_index(offset, d);
+    fmmh::hpw_m2l_host(L, d, direction, in, out);
+  });
+}
+
+int32_t fmmh_helmholtz_plane_wave_merge(int32_t p, double kh_re, double kh_im, const double *slot_multipoles,
+                                        const uint8_t *present, double *out) {
+  return guard([&] {
+    need(slot_multipoles && present && out, "fmmh_helmholtz_plane_wave_merge: NULL argument");
+    std::unique_lock<std::mutex> lock;
+    const fmmh::HpwLevel &L = cached_hpw_level(p, fmmh::cplx(kh_re, kh_im), lock);
+    const size_t nc = size_t(p + 1) * (p + 1);
+    std::vector<const double *> slots(fmmh::kPwSlots, nullptr);
+    for (int a = 0; a < fmmh::kPwSlots; ++a)
+      if (present[a]) slots[a] = slot_multipoles + a * 2 * nc;
+    fmmh::hpw_merge_host(L, slots.data(), out);
   });
 }
 

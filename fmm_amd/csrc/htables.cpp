@@ -351,6 +351,13 @@ void h_l2l_tables(int p, cplx kh, std::vector<double> &out) {
   }
 }
 
+void h_rotation_blocks(int p, const double Q[9], std::vector<double> &out) {
+  std::vector<double> norm;
+  h_norm_factors(p, norm);
+  std::vector<int> boff;
+  rotation_blocks(p, Q, norm, out, boff);
+}
+
 int h_rcr_rotation_size(int p) { return (p + 1) * (2 * p + 1) * (2 * p + 3) / 3; }
 
 int h_rcr_coaxial_size(int p) {

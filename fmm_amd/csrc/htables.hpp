@@ -43,6 +43,11 @@ int h_rcr_coaxial_size(int p);
 void h_rcr_rotations(int p, std::vector<double> &out);
 void h_rcr_coaxial(int p, cplx kh, const std::vector<double> &dist, std::vector<double> &out);
 
+// The degree blocks O^n(Q) of a rotation Q (Y_n(Q x) = O^n Y_n(x)) of the orthonormal real
+// harmonics, row-major ((r, c) at r (2n+1) + c) from n (2n-1)(2n+1)/3 on: a coefficient
+// vector goes into the frame x' = Q x by O^n and back by its transpose.
+void h_rotation_blocks(int p, const double Q[9], std::vector<double> &out);
+
 // Level constants in a level's units: s = min(|kh| sqrt 3, 1), beta = Im(kh) sqrt 3.
 inline double h_scale(cplx kh) {
   const double v = std::abs(kh) * 1.7320508075688772;

@@ -353,6 +353,26 @@ int32_t fmmh_helmholtz_direct_sum(int32_t device, double k_re, double k_im, cons
                                   const double *charges, int64_t nsources, const double *targets, int64_t ntargets,
                                   double *phi, double *grad, int32_t on_device);
 
+/* The Helmholtz exponential (plane-wave) M2L on the host (fmm-tables/src/helmholtz/
+ * plane_wave/{quadrature,pair,merge}.rs).  Quadrature: FMM3D's iprec-2 table of the domain
+ * of kappa = k w (quadrature.rs Quadrature::new), its domain (0 outside the tables), node
+ * count, stated error, per node lambda, t and the weight (complex, interleaved re/im), the
+ * alpha and Fourier counts (at most `capacity` nodes). */
+int32_t fmmh_helmholtz_plane_wave_quadrature(double kappa_re, double kappa_im, int32_t capacity, int32_t *domain,
+                                             int32_t *nodes, double *stated_error, double *lambda, double *t,
+                                             double *weight, int32_t *alpha, int32_t *fourier);
+/* |J_0(z)| .. |J_top(z)| of complex z (quadrature.rs bessel_j_norms): top + 1 values. */
+int32_t fmmh_bessel_j_norms(double z_re, double z_im, int32_t top, double *out);
+/* One V pair of a level with kh = k r_l (pair.rs PlaneWaveLevel::m2l): out (two planes) +=
+ * the exponential M2L of the stored multipole in (two planes) along `direction` (0..5, -1:
+ * the offset's own), the level's quadrature at iprec 2. */
+int32_t fmmh_helmholtz_plane_wave_m2l(int32_t p, double kh_re, double kh_im, int32_t offset, int32_t direction,
+                                      const double *in, double *out);
+/* One parent's merge and shift (merge.rs): 216 slot multipoles of two planes each, which
+ * are held, and out (8 x two planes) += the M2L of every pair slot -> child. */
+int32_t fmmh_helmholtz_plane_wave_merge(int32_t p, double kh_re, double kh_im, const double *slot_multipoles,
+                                        const uint8_t *present, double *out);
+
 #ifdef __cplusplus
 }
 #endif
