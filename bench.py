@@ -644,6 +644,8 @@ def helmholtz(args):
     mixed = run("mixed")
     plain = run("f64")
     rot = run("f64", "rotation")
+    pw = run("f64", "plane_wave")
+    pw_mixed = run("mixed", "plane_wave")
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -670,6 +672,13 @@ def helmholtz(args):
                       "device_bytes": plain[3]["device_bytes"]},
         "plain_f64_rotation_m2l": {"ms_per_step": rot[0] * 1e3, "value": n / rot[0] / 1e6, "errors": rot[1],
                                    "m2l_ms": rot[2]["m2l"][0], "device_bytes": rot[3]["device_bytes"]},
+        # the exponential M2L (merge and shift, f64) on every level the reference's rule
+        # gives a quadrature: the configuration's M2L in the reference
+        "plain_f64_plane_wave_m2l": {"ms_per_step": pw[0] * 1e3, "value": n / pw[0] / 1e6, "errors": pw[1],
+                                     "m2l_ms": pw[2]["m2l"][0], "device_bytes": pw[3]["device_bytes"]},
+        "mixed_plane_wave_m2l": {"ms_per_step": pw_mixed[0] * 1e3, "value": n / pw_mixed[0] / 1e6,
+                                 "errors": pw_mixed[1], "m2l_ms": pw_mixed[2]["m2l"][0],
+                                 "device_bytes": pw_mixed[3]["device_bytes"]},
         "device_bytes": stats["device_bytes"],
         "kinds_ms": {k: v[0] for k, v in kinds.items() if v[1]},
         "interactions_per_s": stats["p2p_pairs"] / (p2p_ms * 1e-3) if p2p_ms > 0 else None,

@@ -660,11 +660,14 @@ class HelmholtzOperator:
         """m2l (HelmholtzM2l, fmm-exec/src/helmholtz/operator.rs:448-479): "dense", one
         complex table per offset on the MFMA contraction on the levels whose tables fit
         dense_budget bytes (316 x 2 (p+1)^4 f64 per level, from the finest level up; 0 the
-        reference's 1 GiB, negative no limit) and per-pair rotation on the others, or
+        reference's 1 GiB, negative no limit) and per-pair rotation on the others,
         "rotation", per pair rotation, coaxial shift, rotation back in f64 (RCR) on every
-        level, with tables O(p^3) per distance."""
-        if m2l not in ("dense", "rotation"):
-            raise FmmError(7, f"Helmholtz M2L: dense or rotation, not {m2l!r}")
+        level, with tables O(p^3) per distance, or "plane_wave", the exponential M2L by merge
+        and shift in f64 on the levels the reference's rule gives a quadrature (degree <= 15,
+        levels >= 2 whose kappa = k w lies in FMM3D's tables; HelmholtzM2l::PlaneWave) and
+        per-pair rotation on the others."""
+        if m2l not in ("dense", "rotation", "plane_wave"):
+            raise FmmError(7, f"Helmholtz M2L: dense, rotation or plane_wave, not {m2l!r}")
         self.plan = plan
         self.p = p
         self.k = complex(k)
@@ -801,10 +804,12 @@ class HelmholtzFmmBuilder:
         return self
 
     def m2l(self, method: str):
-        """The M2L: "dense" (the default) or "rotation" (the reference's RCR per pair,
-        HelmholtzM2l in fmm-exec/src/helmholtz/operator.rs:448-479)."""
-        if method not in ("dense", "rotation"):
-            raise FmmError(7, f"Helmholtz M2L: dense or rotation, not {method!r}")
+        """The M2L: "dense" (the default), "rotation" (the reference's RCR per pair) or
+        "plane_wave" (the exponential M2L by merge and shift where the reference's rule gives
+        a level a quadrature, RCR elsewhere); HelmholtzM2l in
+        fmm-exec/src/helmholtz/operator.rs:448-479."""
+        if method not in ("dense", "rotation", "plane_wave"):
+            raise FmmError(7, f"Helmholtz M2L: dense, rotation or plane_wave, not {method!r}")
         self._m2l = method
         return self
 

@@ -17,7 +17,9 @@
 
 #include "hkernels.hpp"
 #include "hoperator.hpp"
+#include "hpwtables.hpp"
 #include "htables.hpp"
+#include "pwtables.hpp"
 #include "kernels.hpp"
 #include "schedule.hpp"
 
@@ -53,7 +55,8 @@ bool all_zero(const double *v, size_t n) {
 HelmholtzOperator::HelmholtzOperator(const Plan &plan, const HelmholtzOptions &options) : opt_(options) {
   require(opt_.p >= 0 && opt_.p <= dev::kMaxDegree, "the device path supports 0 <= p <= 15", kUnsupportedLayout);
   require(opt_.precision == kPlainF64 || opt_.precision == kMixed, "unknown precision", kUnsupportedPrecision);
-  require(opt_.m2l == kM2lDense || opt_.m2l == kM2lRotation, "Helmholtz: M2L dense or rotation", kUnsupportedLayout);
+  require(opt_.m2l == kM2lDense || opt_.m2l == kM2lRotation || opt_.m2l == kM2lPlaneWave,
+          "Helmholtz: M2L dense, rotation or plane wave", kUnsupportedLayout);
   require(std::isfinite(opt_.kre) && std::isfinite(opt_.kim) && (opt_.kre != 0.0 || opt_.kim != 0.0),
           "Helmholtz: the wavenumber must be finite and nonzero");
   require(opt_.kre >= 0.0 && opt_.kim >= 0.0, "Helmholtz: Re k >= 0 and Im k >= 0 (outgoing, not growing)");
@@ -342,6 +345,13 @@ HelmholtzOperator::HelmholtzOperator(const Plan &plan, const HelmholtzOptions &o
         dense_level[l] = 1;
       }
   }
+  // under the exponential M2L (HelmholtzM2l::PlaneWave, operator.rs:462-468) the levels
+  // that the reference's rule gives a quadrature (merge.rs select without a tolerance:
+  // iprec 2 up to degree 15, levels >= 2 whose kappa = 2 kh lies in FMM3D's tables), the
+  // per-pair rotation on the others
+  std::vector<char> pw_level(nlevels_, 0);
+  if (opt_.m2l == kM2lPlaneWave)
+    for (int l = 2; l < nlevels_; ++l) pw_level[l] = plan.v[l].size() > 0 && hpw_level_selected(l, p, kh[l]);
   {
     // Per pair: rotate the source multipole so the offset lies along +z, shift coaxially,
     // rotate back (the reference's RCR).  Pairs by level, then offset: blocks of at most
@@ -356,7 +366,7 @@ HelmholtzOperator::HelmholtzOperator(const Plan &plan, const HelmholtzOptions &o
     std::vector<std::vector<uint32_t>> level_pairs(nlevels_);  // pair index of each V entry
     for (int l = 2; l < nlevels_; ++l) {
       const Csr &V = plan.v[l];
-      if (V.size() == 0 || dense_level[l]) continue;
+      if (V.size() == 0 || dense_level[l] || pw_level[l]) continue;
       if (rot_rb_ == 0) {
         rot_rb_ = h_rcr_rotation_size(p);
         rot_cb_ = h_rcr_coaxial_size(p);
@@ -606,7 +616,161 @@ HelmholtzOperator::HelmholtzOperator(const Plan &plan, const HelmholtzOptions &o
       t_m2l_.upload(m2l_sw);
     }
   }
+  if (opt_.m2l == kM2lPlaneWave) build_plane_wave(plan, kh, pw_level);
   hip_check(hipDeviceSynchronize(), "Helmholtz operator build");
+}
+
+// The exponential M2L's levels (operator_plane_wave.rs; merge.rs MergeLevel): per level its
+// quadrature and tables (hpw_level), its 104 source shifts and 48 child shifts, the merge's
+// lists with wave blocks counted from the level's first box, and the level's waves and
+// incoming waves (12 W f64 per box).
+void HelmholtzOperator::build_plane_wave(const Plan &plan, const std::vector<cplx> &kh,
+                                         const std::vector<char> &use) {
+  const int p = opt_.p;
+  const PwMergeGeometry &geo = plane_wave_merge_geometry();
+  static_assert(kPwAccStride == dev::kPwAccStride, "one accumulator stride");
+  std::vector<int> ints;
+  std::vector<double> tab, shifts;
+  struct Off {
+    size_t start, len, ord, mode_start, trig, factor, wk, item, cosm, sinm, out_re, out_im, in_re, in_im, frames;
+  };
+  std::vector<Off> offs;
+  size_t max_out = 0, max_local = 0, waves = 0;
+  pw_wave_off_.assign(nlevels_ + 1, 0);
+  pw_shift_off_.assign(nlevels_ + 1, 0);
+  for (int l = 0; l < nlevels_; ++l) {
+    pw_wave_off_[l] = waves;
+    pw_shift_off_[l] = shifts.size();
+    if (!use[l]) continue;
+    const HpwLevel L = hpw_level(p, kh[l], hpw_quadrature(2.0 * kh[l]));
+    Off o{};
+    auto put_i = [&](const std::vector<int> &v) {
+      const size_t at = ints.size();
+      ints.insert(ints.end(), v.begin(), v.end());
+      return at;
+    };
+    auto put_d = [&](const std::vector<double> &v) {
+      const size_t at = tab.size();
+      tab.insert(tab.end(), v.begin(), v.end());
+      return at;
+    };
+    std::vector<int> wk(L.W), item;
+    for (int k = 0; k < L.s; ++k) {
+      for (int j = 0; j < L.len[k]; ++j) wk[L.start[k] + j] = k;
+      for (int m = 0; m < L.ord[k]; ++m) item.push_back(k * (p + 1) + m);
+    }
+    std::stable_sort(item.begin(), item.end(),
+                     [&](int a, int b) { return L.len[a / (p + 1)] > L.len[b / (p + 1)]; });
+    o.start = put_i(L.start);
+    o.len = put_i(L.len);
+    o.ord = put_i(L.ord);
+    o.mode_start = put_i(L.mode_start);
+    o.trig = put_i(L.trig);
+    o.factor = put_i(L.factor);
+    o.wk = put_i(wk);
+    o.item = put_i(item);
+    o.cosm = put_d(L.cosm);
+    o.sinm = put_d(L.sinm);
+    o.out_re = put_d(L.out_re);
+    o.out_im = put_d(L.out_im);
+    o.in_re = put_d(L.in_re);
+    o.in_im = put_d(L.in_im);
+    o.frames = put_d(L.frame1);
+    put_d(L.frame2);
+    offs.push_back(o);
+    pw_levels_.push_back(l);
+    dev::HpwArgs a{};
+    a.p = p;
+    a.nodes = L.s;
+    a.W = L.W;
+    a.orders = L.orders;
+    a.rb = static_cast<int>(L.frame1.size());
+    a.nitem = static_cast<int>(item.size());
+    pw_args_.push_back(a);
+    max_out = std::max(max_out, dev::hpw_outgoing_lds(p, L.orders));
+    max_local = std::max(max_local, dev::hpw_local_lds(p, L.W, L.orders));
+    // the merge's shifts of this level: the source shifts, then the child shifts
+    const size_t W2 = 2 * size_t(L.W), s0 = shifts.size();
+    shifts.resize(s0 + (geo.shift_d.size() + size_t(kPwDirections) * kPwChildren) * W2);
+    for (size_t i = 0; i < geo.shift_d.size(); ++i) {
+      double dp[3];
+      plane_wave_rotate(geo.shift_dir[i], geo.shift_d[i].data(), dp);
+      hpw_shift(L, dp, (geo.shift_dir[i] & 1) == 0, &shifts[s0 + i * W2]);
+    }
+    for (int dir = 0; dir < kPwDirections; ++dir)
+      for (int c = 0; c < kPwChildren; ++c) {
+        const double d[3] = {((c >> 2) & 1) - 0.5, ((c >> 1) & 1) - 0.5, (c & 1) - 0.5};
+        double dp[3];
+        plane_wave_rotate(dir, d, dp);
+        hpw_shift(L, dp, (dir & 1) == 0, &shifts[s0 + (geo.shift_d.size() + size_t(kPwChildren) * dir + c) * W2]);
+      }
+    waves += size_t(plan.nboxes(l)) * 12 * L.W;
+    require(L.W * 2 <= (1 << 16) && waves < (size_t(1) << 40), "Helmholtz plane wave: too many waves", kTooLarge);
+  }
+  pw_wave_off_[nlevels_] = waves;
+  pw_shift_off_[nlevels_] = shifts.size();
+  if (pw_levels_.empty()) return;
+  require(max_out <= 160 * 1024 && max_local <= 160 * 1024, "Helmholtz plane wave: the level's tables exceed LDS",
+          kUnsupportedLayout);
+  hip_check(dev::hpw_allow_lds(max_out, max_local), "Helmholtz plane wave LDS");
+  pw_int_.upload(ints);
+  pw_tab_.upload(tab);
+  pw_shift_.upload(shifts);
+  for (size_t i = 0; i < pw_args_.size(); ++i) {
+    dev::HpwArgs &a = pw_args_[i];
+    const Off &o = offs[i];
+    a.start = pw_int_.ptr + o.start;
+    a.len = pw_int_.ptr + o.len;
+    a.ord = pw_int_.ptr + o.ord;
+    a.mode_start = pw_int_.ptr + o.mode_start;
+    a.trig = pw_int_.ptr + o.trig;
+    a.factor = pw_int_.ptr + o.factor;
+    a.wk = pw_int_.ptr + o.wk;
+    a.item = pw_int_.ptr + o.item;
+    a.cosm = pw_tab_.ptr + o.cosm;
+    a.sinm = pw_tab_.ptr + o.sinm;
+    a.out_re = pw_tab_.ptr + o.out_re;
+    a.out_im = pw_tab_.ptr + o.out_im;
+    a.in_re = pw_tab_.ptr + o.in_re;
+    a.in_im = pw_tab_.ptr + o.in_im;
+    a.frames = pw_tab_.ptr + o.frames;
+  }
+  pw_acc_first_.upload(geo.acc_first);
+  pw_acc_mask_.upload(geo.acc_mask);
+  const PwMergeLists lists = plane_wave_merge_lists(plan, box_base_, use, true);
+  pw_par_level_ = lists.par_level;
+  pw_tgt_level_ = lists.tgt_level;
+  pw_pe_first_.upload(lists.pe_first);
+  pw_pe_.upload(lists.pe);
+  pw_children_.upload(lists.children);
+  pw_targets_.upload(lists.targets);
+  pw_waves_.alloc(waves);
+  pw_inc_.alloc(waves);
+  for (int l : pw_levels_) m2l_pairs_ += plan.v[l].size();
+}
+
+// Per level with a quadrature: the waves of its boxes, the merge of its parents into their
+// children's incoming waves, the children's locals.
+void HelmholtzOperator::launch_plane_wave() {
+  const PwMergeGeometry &geo = plane_wave_merge_geometry();
+  for (size_t i = 0; i < pw_levels_.size(); ++i) {
+    const int l = pw_levels_[i];
+    const dev::HpwArgs &a = pw_args_[i];
+    const uint32_t first = box_base_[l], nb = box_base_[l + 1] - first;
+    double *waves = pw_waves_.ptr + pw_wave_off_[l], *inc = pw_inc_.ptr + pw_wave_off_[l];
+    dev::hpw_outgoing(stream_, a, nb, first, mult_.ptr, waves);
+    dev::PwArgs pa{};
+    pa.H = a.W;
+    const double *sh = pw_shift_.ptr + pw_shift_off_[l];
+    const dev::PwMerge g{pw_acc_first_.ptr, pw_acc_mask_.ptr, sh, sh + geo.shift_d.size() * 2 * size_t(a.W)};
+    const uint32_t q0 = pw_par_level_[l], q1 = pw_par_level_[l + 1];
+    timed(kKindM2lContract, [&] {
+      dev::pw_merge(stream_, pa, g, q1 - q0, pw_pe_first_.ptr + size_t(q0) * 6 * dev::kPwAccStride, pw_pe_.ptr,
+                    pw_children_.ptr + size_t(q0) * kPwChildren, first, waves, inc);
+    });
+    const uint32_t t0 = pw_tgt_level_[l], t1 = pw_tgt_level_[l + 1];
+    dev::hpw_local(stream_, a, t1 - t0, pw_targets_.ptr + t0, first, inc, loc_.ptr);
+  }
 }
 
 HelmholtzOperator::~HelmholtzOperator() {
@@ -623,7 +787,8 @@ void HelmholtzOperator::wait_for(hipStream_t producer) {
 size_t HelmholtzOperator::device_bytes() const {
   return mult_.bytes() + loc_.bytes() + t_m2m_.bytes() + t_l2l_.bytes() + t_m2l_.bytes() + t_m2l16_.bytes() +
          mult16_.bytes() + sx_.bytes() * 8 + slots_.bytes() + slots32_.bytes() + rot_fwd_.bytes() +
-         rot_coax_.bytes() + rot_slots_.bytes();
+         rot_coax_.bytes() + rot_slots_.bytes() + pw_tab_.bytes() + pw_shift_.bytes() + pw_waves_.bytes() +
+         pw_inc_.bytes();
 }
 
 void HelmholtzOperator::set_timing(bool enable) {
@@ -731,6 +896,7 @@ void HelmholtzOperator::evaluate() {
                           s.col_out.ptr, t_m2m_.ptr, mult_.ptr, mult_.ptr, none, nullptr, 0);
     });
   // M2L of every level in one launch
+  if (!pw_levels_.empty()) timed(3, [&] { launch_plane_wave(); });
   // the per-pair rotation levels, chunk by chunk
   for (size_t c = 0; c + 1 < rot_chunk_blk_.size(); ++c)
     timed(3, [&] {

@@ -15,8 +15,10 @@
 #include <cstdint>
 #include <vector>
 
+#include "hpwkernels.hpp"
 #include "operator.hpp"
 #include "plan.hpp"
+#include "pwkernels.hpp"
 
 namespace fmmh {
 
@@ -28,7 +30,9 @@ struct HelmholtzOptions {
   int device = 0;
   int m2l = 0;  // kM2lDense: dense complex tables per level on MFMA where they fit the
                 // budget, per-pair rotation elsewhere; kM2lRotation: per-pair rotation,
-                // coaxial shift, rotation back (f64) on every level, tables O(p^3) per distance
+                // coaxial shift, rotation back (f64) on every level, tables O(p^3) per distance;
+                // kM2lPlaneWave: the exponential M2L by merge and shift (f64) on the levels
+                // with a quadrature of the reference's rule, per-pair rotation elsewhere
   int64_t m2l_dense_budget = 0;  // bytes of dense tables (316 x 2 (p+1)^4 f64 per level): 0 the
                                  // reference's default (1 GiB), negative no limit
 };
@@ -105,6 +109,19 @@ class HelmholtzOperator {
   DevBuf<uint32_t> rot_blk_first_, rot_blk_coax_, rot_pair_src_;
   DevBuf<uint16_t> rot_blk_offset_;
   Rows rot_red_;
+  // exponential M2L: per level with a quadrature its tables (one buffer per kind, the levels
+  // after one another), its merge shifts, its waves and incoming waves, and the merge's
+  // parents, children and targets by level
+  std::vector<int> pw_levels_;
+  std::vector<dev::HpwArgs> pw_args_;
+  std::vector<size_t> pw_wave_off_, pw_shift_off_;
+  DevBuf<int> pw_int_;
+  DevBuf<double> pw_tab_, pw_shift_, pw_waves_, pw_inc_;
+  DevBuf<int> pw_acc_first_, pw_acc_mask_;
+  DevBuf<uint32_t> pw_pe_first_, pw_pe_, pw_children_, pw_targets_;
+  std::vector<uint32_t> pw_par_level_, pw_tgt_level_;
+  void build_plane_wave(const Plan &plan, const std::vector<std::complex<double>> &kh, const std::vector<char> &use);
+  void launch_plane_wave();
 
   bool timing_ = false;
   struct Mark {

@@ -542,115 +542,31 @@ void DeviceOperator::build_plane_wave(const Plan &plan) {
   std::stable_sort(gitem.begin(), gitem.end(),
                    [&](int x, int y) { return pw.nalpha[x / (opt_.p + 1)] > pw.nalpha[y / (opt_.p + 1)]; });
   pw_gitem_.upload(gitem);
-  // geometry: accumulators by direction, their entries with the source shift of each slot
-  // (one per distinct shift), the child shifts
+  // the merge's shifts: one table per distinct source shift, the child shifts
   const size_t H2 = 2 * size_t(pw.H);
-  const auto &accs = plane_wave_accumulators();
-  std::vector<int> acc_first{0}, acc_mask, ent_first{0}, ent_slot, ent_shift;
-  std::vector<double> src_shift, child_shift(size_t(kPwDirections) * kPwChildren * H2);
-  std::map<std::array<double, 4>, int> shift_of;
-  for (int dir = 0; dir < kPwDirections; ++dir) {
-    require(accs[dir].size() < size_t(dev::kPwAccStride), "plane wave: too many accumulators", kUnsupportedLayout);
-    for (const PwAccumulator &a : accs[dir]) {
-      acc_mask.push_back(a.mask);
-      for (int slot : a.slots) {
-        const double d[3] = {2.5 - slot / 36, 2.5 - (slot / 6) % 6, 2.5 - slot % 6};
-        double dp[3];
-        plane_wave_rotate(dir, d, dp);
-        const std::array<double, 4> key{double(dir & 1), dp[0], dp[1], dp[2]};
-        auto it = shift_of.find(key);
-        if (it == shift_of.end()) {
-          it = shift_of.emplace(key, int(shift_of.size())).first;
-          src_shift.resize(src_shift.size() + H2);
-          plane_wave_shift(pw, dir, d, &src_shift[src_shift.size() - H2]);
-        }
-        ent_slot.push_back(slot);
-        ent_shift.push_back(it->second);
-      }
-      ent_first.push_back(static_cast<int>(ent_slot.size()));
-    }
-    acc_first.push_back(static_cast<int>(acc_mask.size()));
+  const PwMergeGeometry &geo = plane_wave_merge_geometry();
+  static_assert(kPwAccStride == dev::kPwAccStride, "one accumulator stride");
+  std::vector<double> src_shift(geo.shift_d.size() * H2), child_shift(size_t(kPwDirections) * kPwChildren * H2);
+  for (size_t i = 0; i < geo.shift_d.size(); ++i)
+    plane_wave_shift(pw, geo.shift_dir[i], geo.shift_d[i].data(), &src_shift[i * H2]);
+  for (int dir = 0; dir < kPwDirections; ++dir)
     for (int c = 0; c < kPwChildren; ++c) {
       const double d[3] = {((c >> 2) & 1) - 0.5, ((c >> 1) & 1) - 0.5, (c & 1) - 0.5};
       plane_wave_shift(pw, dir, d, &child_shift[(size_t(kPwChildren) * dir + c) * H2]);
     }
-  }
-  require(shift_of.size() <= 128, "plane wave: too many source shifts", kUnsupportedLayout);
-  pw_acc_first_.upload(acc_first);
-  pw_acc_mask_.upload(acc_mask);
+  pw_acc_first_.upload(geo.acc_first);
+  pw_acc_mask_.upload(geo.acc_mask);
   pw_src_shift_.upload(src_shift);
   pw_child_shift_.upload(child_shift);
   // parents by the level of their children, children in box order
-  constexpr uint32_t kNoBox = 0xffffffffu;
+  const PwMergeLists lists = plane_wave_merge_lists(plan, box_base_, std::vector<char>(nlevels_, 1));
+  pw_par_level_ = lists.par_level;
+  pw_tgt_level_ = lists.tgt_level;
+  pw_pe_first_.upload(lists.pe_first);
+  pw_pe_.upload(lists.pe);
+  pw_children_.upload(lists.children);
+  pw_targets_.upload(lists.targets);
   const uint32_t box0 = nlevels_ > 2 ? static_cast<uint32_t>(box_base_[2]) : 0;
-  require(nlevels_ <= 2 || (box_base_[nlevels_] - box0) * 6 < (uint64_t(1) << 25),
-          "plane wave: too many boxes for the merge's entry format", kTooLarge);
-  std::vector<uint32_t> pe_first, pe, children, targets;
-  pw_par_level_.assign(nlevels_ + 1, 0);
-  pw_tgt_level_.assign(nlevels_ + 1, 0);
-  for (int l = 0; l < nlevels_; ++l) {
-    pw_par_level_[l] = static_cast<uint32_t>(children.size() / kPwChildren);
-    pw_tgt_level_[l] = static_cast<uint32_t>(targets.size());
-    if (l < 2) continue;
-    const Csr &V = plan.v[l];
-    for (uint32_t P = 0; P < plan.nboxes(l - 1); ++P) {
-      const uint32_t c0 = plan.first_child[l - 1][P];
-      if (c0 == kNone) continue;
-      const Cell cp = decode(plan.keys[l - 1][P]);
-      std::array<uint32_t, kPwSlots> sl;
-      std::array<uint32_t, kPwChildren> ch;
-      sl.fill(kNoBox);
-      ch.fill(kNoBox);
-      uint64_t entries = 0;
-      for (uint32_t b = c0; b < c0 + kPwChildren; ++b) {
-        const Cell cc = decode(plan.keys[l][b]);
-        const int64_t bx = cc.i - 2 * cp.i, by = cc.j - 2 * cp.j, bz = cc.k - 2 * cp.k;
-        require(bx >= 0 && bx <= 1 && by >= 0 && by <= 1 && bz >= 0 && bz <= 1 && plan.parent[l][b] == P,
-                "plane wave: a parent's children are not its eight octants", kUnsupportedLayout);
-        ch[4 * bx + 2 * by + bz] = static_cast<uint32_t>(box_base_[l] + b);
-        for (uint32_t e = V.off[b]; e < V.off[b + 1]; ++e) {
-          const Cell cs = decode(plan.keys[l][V.ent[e]]);
-          const int64_t ax = cs.i - 2 * cp.i + 2, ay = cs.j - 2 * cp.j + 2, az = cs.k - 2 * cp.k + 2;
-          require(ax >= 0 && ax < 6 && ay >= 0 && ay < 6 && az >= 0 && az < 6,
-                  "plane wave: a V entry outside the parent's slots", kUnsupportedLayout);
-          const uint32_t src = static_cast<uint32_t>(box_base_[l] + V.ent[e]);
-          uint32_t &slot = sl[36 * ax + 6 * ay + az];
-          require(slot == kNoBox || slot == src, "plane wave: two boxes in one slot", kUnsupportedLayout);
-          slot = src;
-          ++entries;
-        }
-      }
-      if (entries == 0) continue;
-      uint64_t pairs = 0;
-      for (int a = 0; a < kPwSlots; ++a)
-        for (int c = 0; sl[a] != kNoBox && c < kPwChildren; ++c)
-          if (ch[c] != kNoBox && plane_wave_slot_direction(a, c) >= 0) ++pairs;
-      require(pairs == entries, "plane wave: the V lists are not the merge-and-shift pairs", kUnsupportedLayout);
-      // the held entries of each direction's accumulators, in the accumulators' order
-      for (int dir = 0; dir < kPwDirections; ++dir) {
-        const size_t base = pe_first.size();
-        pe_first.resize(base + dev::kPwAccStride, 0);
-        for (int a = acc_first[dir]; a < acc_first[dir + 1]; ++a) {
-          pe_first[base + (a - acc_first[dir])] = static_cast<uint32_t>(pe.size());
-          for (int e = ent_first[a]; e < ent_first[a + 1]; ++e) {
-            const uint32_t src = sl[ent_slot[e]];
-            if (src == kNoBox) continue;
-            pe.push_back(((src - box0) * 6 + dir) << 7 | static_cast<uint32_t>(ent_shift[e]));
-          }
-        }
-        for (int a = acc_first[dir + 1] - acc_first[dir]; a < dev::kPwAccStride; ++a)
-          pe_first[base + a] = static_cast<uint32_t>(pe.size());
-      }
-      children.insert(children.end(), ch.begin(), ch.end());
-      for (uint32_t b = c0; b < c0 + kPwChildren; ++b) targets.push_back(static_cast<uint32_t>(box_base_[l] + b));
-    }
-  }
-  pw_par_level_[nlevels_] = static_cast<uint32_t>(children.size() / kPwChildren);
-  pw_tgt_level_[nlevels_] = static_cast<uint32_t>(targets.size());
-  pw_pe_first_.upload(pe_first);
-  pw_pe_.upload(pe);
-  pw_children_.upload(children);
-  pw_targets_.upload(targets);
   pw_box0_ = box0;
   pw_nbox_ = nlevels_ > 2 ? static_cast<uint32_t>(box_base_[nlevels_] - box_base_[2]) : 0;
   pw_waves_.alloc(size_t(pw_nbox_) * 12 * pw.H);

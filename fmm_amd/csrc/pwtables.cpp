@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <map>
 #include <mutex>
 
 #include "common.hpp"
@@ -271,6 +272,115 @@ const std::array<std::vector<PwAccumulator>, kPwDirections> &plane_wave_accumula
     }
   });
   return acc;
+}
+
+const PwMergeGeometry &plane_wave_merge_geometry() {
+  static PwMergeGeometry g;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    const auto &accs = plane_wave_accumulators();
+    g.acc_first.push_back(0);
+    g.ent_first.push_back(0);
+    std::map<std::array<double, 4>, int> shift_of;
+    for (int dir = 0; dir < kPwDirections; ++dir) {
+      require(accs[dir].size() < size_t(kPwAccStride), "plane wave: too many accumulators", kUnsupportedLayout);
+      for (const PwAccumulator &a : accs[dir]) {
+        g.acc_mask.push_back(a.mask);
+        for (int slot : a.slots) {
+          const double d[3] = {2.5 - slot / 36, 2.5 - (slot / 6) % 6, 2.5 - slot % 6};
+          double dp[3];
+          plane_wave_rotate(dir, d, dp);
+          const std::array<double, 4> key{double(dir & 1), dp[0], dp[1], dp[2]};
+          auto it = shift_of.find(key);
+          if (it == shift_of.end()) {
+            it = shift_of.emplace(key, int(shift_of.size())).first;
+            g.shift_dir.push_back(dir);
+            g.shift_d.push_back({d[0], d[1], d[2]});
+          }
+          g.ent_slot.push_back(slot);
+          g.ent_shift.push_back(it->second);
+        }
+        g.ent_first.push_back(static_cast<int>(g.ent_slot.size()));
+      }
+      g.acc_first.push_back(static_cast<int>(g.acc_mask.size()));
+    }
+    require(g.shift_d.size() <= 128, "plane wave: too many source shifts", kUnsupportedLayout);
+  });
+  return g;
+}
+
+PwMergeLists plane_wave_merge_lists(const Plan &plan, const std::vector<uint32_t> &box_base,
+                                    const std::vector<char> &use, bool per_level) {
+  constexpr uint32_t kNoBox = 0xffffffffu;
+  const PwMergeGeometry &g = plane_wave_merge_geometry();
+  const int nlevels = plan.nlevels;
+  const uint32_t box0 = nlevels > 2 ? box_base[2] : 0;
+  require(nlevels <= 2 || uint64_t(box_base[nlevels] - box0) * 6 < (uint64_t(1) << 25),
+          "plane wave: too many boxes for the merge's entry format", kTooLarge);
+  PwMergeLists out;
+  out.par_level.assign(nlevels + 1, 0);
+  out.tgt_level.assign(nlevels + 1, 0);
+  for (int l = 0; l < nlevels; ++l) {
+    out.par_level[l] = static_cast<uint32_t>(out.children.size() / kPwChildren);
+    out.tgt_level[l] = static_cast<uint32_t>(out.targets.size());
+    if (l < 2 || !use[l]) continue;
+    const Csr &V = plan.v[l];
+    const uint32_t base = per_level ? box_base[l] : box0;
+    for (uint32_t P = 0; P < plan.nboxes(l - 1); ++P) {
+      const uint32_t c0 = plan.first_child[l - 1][P];
+      if (c0 == kNone) continue;
+      const Cell cp = decode(plan.keys[l - 1][P]);
+      std::array<uint32_t, kPwSlots> sl;
+      std::array<uint32_t, kPwChildren> ch;
+      sl.fill(kNoBox);
+      ch.fill(kNoBox);
+      uint64_t entries = 0;
+      for (uint32_t b = c0; b < c0 + kPwChildren; ++b) {
+        const Cell cc = decode(plan.keys[l][b]);
+        const int64_t bx = cc.i - 2 * cp.i, by = cc.j - 2 * cp.j, bz = cc.k - 2 * cp.k;
+        require(bx >= 0 && bx <= 1 && by >= 0 && by <= 1 && bz >= 0 && bz <= 1 && plan.parent[l][b] == P,
+                "plane wave: a parent's children are not its eight octants", kUnsupportedLayout);
+        ch[4 * bx + 2 * by + bz] = box_base[l] + b;
+        for (uint32_t e = V.off[b]; e < V.off[b + 1]; ++e) {
+          const Cell cs = decode(plan.keys[l][V.ent[e]]);
+          const int64_t ax = cs.i - 2 * cp.i + 2, ay = cs.j - 2 * cp.j + 2, az = cs.k - 2 * cp.k + 2;
+          require(ax >= 0 && ax < 6 && ay >= 0 && ay < 6 && az >= 0 && az < 6,
+                  "plane wave: a V entry outside the parent's slots", kUnsupportedLayout);
+          const uint32_t src = box_base[l] + V.ent[e];
+          uint32_t &slot = sl[36 * ax + 6 * ay + az];
+          require(slot == kNoBox || slot == src, "plane wave: two boxes in one slot", kUnsupportedLayout);
+          slot = src;
+          ++entries;
+        }
+      }
+      if (entries == 0) continue;
+      uint64_t pairs = 0;
+      for (int a = 0; a < kPwSlots; ++a)
+        for (int c = 0; sl[a] != kNoBox && c < kPwChildren; ++c)
+          if (ch[c] != kNoBox && plane_wave_slot_direction(a, c) >= 0) ++pairs;
+      require(pairs == entries, "plane wave: the V lists are not the merge-and-shift pairs", kUnsupportedLayout);
+      // the held entries of each direction's accumulators, in the accumulators' order
+      for (int dir = 0; dir < kPwDirections; ++dir) {
+        const size_t at = out.pe_first.size();
+        out.pe_first.resize(at + kPwAccStride, 0);
+        for (int a = g.acc_first[dir]; a < g.acc_first[dir + 1]; ++a) {
+          out.pe_first[at + (a - g.acc_first[dir])] = static_cast<uint32_t>(out.pe.size());
+          for (int e = g.ent_first[a]; e < g.ent_first[a + 1]; ++e) {
+            const uint32_t src = sl[g.ent_slot[e]];
+            if (src == kNoBox) continue;
+            out.pe.push_back(((src - base) * 6 + dir) << 7 | static_cast<uint32_t>(g.ent_shift[e]));
+          }
+        }
+        for (int a = g.acc_first[dir + 1] - g.acc_first[dir]; a < kPwAccStride; ++a)
+          out.pe_first[at + a] = static_cast<uint32_t>(out.pe.size());
+      }
+      out.children.insert(out.children.end(), ch.begin(), ch.end());
+      for (uint32_t b = c0; b < c0 + kPwChildren; ++b) out.targets.push_back(box_base[l] + b);
+    }
+  }
+  out.par_level[nlevels] = static_cast<uint32_t>(out.children.size() / kPwChildren);
+  out.tgt_level[nlevels] = static_cast<uint32_t>(out.targets.size());
+  return out;
 }
 
 void plane_wave_m2l_host(const PlaneWave &pw, const int d[3], int dir, const double *in, double *out) {

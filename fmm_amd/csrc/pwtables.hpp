@@ -76,6 +76,36 @@ struct PwAccumulator {
 };
 const std::array<std::vector<PwAccumulator>, kPwDirections> &plane_wave_accumulators();
 
+// The merge's tables (the same for every parent and kernel): the accumulators of
+// direction d are acc_first[d] .. acc_first[d + 1], accumulator a serves the children of
+// acc_mask[a] and sums the entries ent_first[a] .. ent_first[a + 1], entry e the slot
+// ent_slot[e] shifted by the source shift ent_shift[e]; source shift i takes the waves of
+// direction shift_dir[i] by shift_d[i] (box widths, unrotated: a slot's offset to the
+// parent's centre); at most kPwAccStride - 1 accumulators per direction.
+constexpr int kPwAccStride = 11;
+struct PwMergeGeometry {
+  std::vector<int> acc_first, acc_mask, ent_first, ent_slot, ent_shift, shift_dir;
+  std::vector<std::array<double, 3>> shift_d;
+};
+const PwMergeGeometry &plane_wave_merge_geometry();
+
+struct Plan;
+// The merge's lists of the levels l with use[l] (l >= 2): per level (of the children) its
+// parents' held entries (per parent and direction kPwAccStride offsets into pe, pe = wave
+// block (box - box0) 6 + dir shifted left by 7, or'd with the source shift), the parents'
+// children (8 box ids per parent, child 4 b_x + 2 b_y + b_z) and the target boxes (the
+// children, in parent order); par_level[l] .. par_level[l + 1] the parents of level l's
+// children, tgt_level likewise.  Box ids are box_base[level] + index; the wave blocks count
+// from box0 = box_base[2] (one buffer for all levels) or, with per_level, from each level's
+// first box box_base[l] (one buffer per level).
+// Checks that every V entry of a child is a slot of its parent and every held, non-adjacent
+// (slot, child) pair a V entry.
+struct PwMergeLists {
+  std::vector<uint32_t> pe_first, pe, children, targets, par_level, tgt_level;
+};
+PwMergeLists plane_wave_merge_lists(const Plan &plan, const std::vector<uint32_t> &box_base,
+                                    const std::vector<char> &use, bool per_level = false);
+
 // Host restatement of one V pair's plane-wave M2L (scaled storage of the level): out +=
 // M2L_pw(d) in, in direction dir (-1: plane_wave_direction's).  The device kernels follow
 // the same steps.

@@ -206,3 +206,47 @@ def test_mixed_precision_within_tolerance(k):
         ep, eg = _rel(out.potential[s], dp), _rel(out.gradient[s], dg)
         _log(f"  {name} p={f.p}: phi {ep:.2e} grad {eg:.2e}")
         assert ep < bar and eg < bar
+
+
+@pytest.mark.parametrize("kind,n,p,leaf,k", [("uniform", 8000, 10, 32, 25.0), ("clustered", 8000, 12, 32, 14.0 + 0.2j)])
+def test_plane_wave_m2l_matches_oracle_fmm(kind, n, p, leaf, k):
+    """The exponential M2L by merge and shift (HelmholtzM2l::PlaneWave: the levels the rule
+    gives a quadrature, RCR on the others) against the oracle FMM (exact M2L) and the
+    direct sum: within the rotation path's error of the direct sum (the reference finds
+    iprec 2 within 1.05x of Rotation's) and within that error of the oracle."""
+    _log(f"case helmholtz plane-wave m2l {kind} n={n} p={p} leaf={leaf} k={k}")
+    pts = _points(kind, n)
+    q = _charges(n)
+    pw = fmm_amd.HelmholtzFmmBuilder(k, degree=p).max_points_per_leaf(leaf).m2l("plane_wave").build(pts)
+    rot = fmm_amd.HelmholtzFmm(pw.plan, fmm_amd.HelmholtzOperator(pw.plan, p, k, m2l="rotation"))
+    a, r = pw.evaluate(q), rot.evaluate(q)
+    ophi, ograd = O.Fmm(pts, p=p, leaf_points=leaf).evaluate_helmholtz(k, q)
+    s = np.arange(0, n, 29)
+    dp, dg = fmm_amd.helmholtz_direct_sum(k, pts, q, pts[s])
+    dp, dg = dp / (4 * np.pi), dg / (4 * np.pi)
+    t_pw, t_rot = _rel(a.potential[s], dp), _rel(r.potential[s], dp)
+    g_pw, g_rot = _rel(a.gradient[s], dg), _rel(r.gradient[s], dg)
+    e_pw, eg_pw = _rel(a.potential, ophi), _rel(a.gradient, ograd)
+    _log(f"  vs direct: phi {t_pw:.2e} (rotation {t_rot:.2e}) grad {g_pw:.2e} (rotation {g_rot:.2e}); "
+         f"vs oracle {e_pw:.2e} / {eg_pw:.2e}")
+    assert _rel(r.potential, ophi) < 1e-12
+    assert t_pw <= 1.5 * t_rot and g_pw <= 1.5 * g_rot
+    assert e_pw <= t_rot and eg_pw <= g_rot
+
+
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+def test_plane_wave_m2l_at_the_degree_rule(precision):
+    """The exponential M2L at the degree rule for 1e-6 (p = 15 at k = 25, iprec 2 on every
+    level with a V list): within the tolerance of the direct sum."""
+    n, k = 20000, 25.0
+    _log(f"case helmholtz plane-wave m2l at the degree rule n={n} {precision}")
+    pts = _points("uniform", n)
+    q = _charges(n)
+    b = fmm_amd.HelmholtzFmmBuilder(k, tolerance=1e-6).near_wx("device").precision(precision)
+    pw = b.m2l("plane_wave").build(pts)
+    s = np.arange(0, n, 97)
+    dp, dg = fmm_amd.helmholtz_direct_sum(k, pts, q, pts[s])
+    out = pw.evaluate(q)
+    ep, eg = _rel(out.potential[s], dp / (4 * np.pi)), _rel(out.gradient[s], dg / (4 * np.pi))
+    _log(f"  p={pw.p}: vs direct phi {ep:.2e} grad {eg:.2e}")
+    assert ep < 1e-6 and eg < 1e-6
