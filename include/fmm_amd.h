@@ -307,7 +307,10 @@ typedef struct {
                         whose tables fit m2l_dense_budget, from the finest level up
                         (DenseM2l::select), per-pair rotation on the others; 1 rotation:
                         per pair rotation, coaxial shift, rotation back in f64 (RCR) on every
-                        level, tables O(p^3) per distance instead of O(p^4) per offset */
+                        level, tables O(p^3) per distance instead of O(p^4) per offset; 2 plane
+                        wave: the exponential M2L by merge and shift in f64 on the levels the
+                        reference's rule gives a quadrature (iprec 2 to degree 15, levels >= 2
+                        whose kappa = k w lies in FMM3D's tables), RCR on the others */
   int64_t m2l_dense_budget; /* bytes of dense tables, counted as 316 x 2 (p+1)^4 f64 per level: 0
                                the reference's default (1 GiB), negative no limit */
 } fmmh_helmholtz_options;
