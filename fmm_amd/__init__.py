@@ -192,6 +192,24 @@ class Plan:
         _lib.check(_lib.lib().fmmh_plan_leaf_coordinates(self._h, int(targets), _lib.ptr(u), _lib.ptr(order)))
         return u, order
 
+    def leaf_cells(self) -> np.ndarray:
+        """(level, ix, iy, iz) of every leaf, uint32 [nleaves, 4]."""
+        out = np.zeros((self.nleaves, 4), np.uint32)
+        _lib.check(_lib.lib().fmmh_plan_leaf_cells(self._h, _lib.ptr(out)))
+        return out
+
+    def box_cells(self, level: int) -> np.ndarray:
+        """(level, ix, iy, iz) of every box of a level, uint32 [nboxes, 4]."""
+        out = np.zeros((len(self.box_keys(level)), 4), np.uint32)
+        _lib.check(_lib.lib().fmmh_plan_box_cells(self._h, int(level), _lib.ptr(out)))
+        return out
+
+    def point_starts(self, targets=False) -> np.ndarray:
+        """CSR of each leaf's points in leaf order (nleaves + 1 offsets)."""
+        out = np.zeros(self.nleaves + 1, np.uint32)
+        _lib.check(_lib.lib().fmmh_plan_point_starts(self._h, int(targets), _lib.ptr(out)))
+        return out
+
 
 @dataclass
 class Output:

@@ -32,6 +32,8 @@ EXPORTS = [
     "fmmh_last_error", "fmmh_abi_version", "fmmh_plan_build", "fmmh_plan_free", "fmmh_plan_nlevels",
     "fmmh_plan_nboxes", "fmmh_plan_nleaves", "fmmh_plan_domain", "fmmh_plan_box_keys", "fmmh_plan_leaf_keys",
     "fmmh_plan_list", "fmmh_plan_point_leaves", "fmmh_plan_leaf_coordinates", "fmmh_tables",
+    "fmmh_plan_leaf_cells", "fmmh_plan_box_cells", "fmmh_plan_point_starts", "fmmh_kernel_p2p", "fmmh_kernel_p2m", "fmmh_kernel_p2l",
+    "fmmh_kernel_l2p", "fmmh_kernel_m2p", "fmmh_kernel_zero", "fmmh_kernel_gather_output",
     "fmmh_device_count", "fmmh_device_describe", "fmmh_operator_create", "fmmh_operator_free", "fmmh_operator_stats",
     "fmmh_load_charges", "fmmh_reset", "fmmh_p2m", "fmmh_m2m", "fmmh_m2l", "fmmh_p2l", "fmmh_l2l",
     "fmmh_l2p", "fmmh_m2p", "fmmh_p2p", "fmmh_evaluate", "fmmh_read_output", "fmmh_sync",
@@ -79,6 +81,27 @@ class HelmholtzOptions(C.Structure):
     _fields_ = [("p", C.c_int32), ("k_re", C.c_double), ("k_im", C.c_double), ("precision", C.c_int32),
                 ("gradients", C.c_int32), ("device", C.c_int32), ("m2l", C.c_int32),
                 ("m2l_dense_budget", C.c_int64)]
+
+
+class P2pInputs(C.Structure):
+    """fmmh_p2p_inputs (device pointers)."""
+    _fields_ = [("leaf_first", C.c_uint32), ("count", C.c_uint32)] + [
+        (n, C.c_void_p) for n in ("near_off", "near_ent", "leaf_cell", "src_start", "tgt_start", "sx", "sy", "sz",
+                                  "q", "tx", "ty", "tz")]
+
+
+class SourceInputs(C.Structure):
+    """fmmh_source_inputs (device pointers)."""
+    _fields_ = [("rows", C.c_uint32)] + [
+        (n, C.c_void_p) for n in ("row_box", "row_cell", "row_off", "row_ent", "leaf_cell", "src_start", "sx", "sy",
+                                  "sz", "q")]
+
+
+class TargetInputs(C.Structure):
+    """fmmh_target_inputs (device pointers)."""
+    _fields_ = [("rows", C.c_uint32)] + [
+        (n, C.c_void_p) for n in ("row_leaf", "row_off", "row_box", "box_cell", "leaf_cell", "tgt_start", "tx", "ty",
+                                  "tz")]
 
 
 class Stats(C.Structure):
@@ -131,6 +154,16 @@ def lib():
     L.fmmh_plan_list.restype = i64
     L.fmmh_plan_point_leaves.argtypes = [vp, i32, vp]
     L.fmmh_plan_leaf_coordinates.argtypes = [vp, i32, vp, vp]
+    L.fmmh_plan_leaf_cells.argtypes = [vp, vp]
+    L.fmmh_plan_box_cells.argtypes = [vp, i32, vp]
+    L.fmmh_plan_point_starts.argtypes = [vp, i32, vp]
+    L.fmmh_kernel_p2p.argtypes = [i32, C.POINTER(P2pInputs), i32, vp, vp, vp]
+    L.fmmh_kernel_p2m.argtypes = [i32, i32, C.POINTER(SourceInputs), vp, vp]
+    L.fmmh_kernel_p2l.argtypes = [i32, i32, C.POINTER(SourceInputs), vp, vp]
+    L.fmmh_kernel_l2p.argtypes = [i32, i32, i32, C.POINTER(TargetInputs), vp, vp, vp, vp]
+    L.fmmh_kernel_m2p.argtypes = [i32, i32, i32, C.POINTER(TargetInputs), vp, vp, vp, vp]
+    L.fmmh_kernel_zero.argtypes = [i32, vp, C.c_int64, vp]
+    L.fmmh_kernel_gather_output.argtypes = [i32, C.c_uint32, vp, vp, vp, vp, vp, i32, vp, vp, vp]
     L.fmmh_tables.argtypes = [i32, vp, vp, vp]
     L.fmmh_set_table_cache.argtypes = [C.c_char_p]
     L.fmmh_table_cache_stats.argtypes = [vp, vp]

@@ -11,6 +11,7 @@
 #include "hoperator.hpp"
 #include "hpwtables.hpp"
 #include "htables.hpp"
+#include "kernels.hpp"
 #include "operator.hpp"
 #include "plan.hpp"
 #include "pwtables.hpp"
@@ -62,6 +63,10 @@ int32_t guard(F &&f) {
 }
 
 void need(bool ok, const char *what) {
+  if (!ok) throw fmmh::Error(fmmh::kInvalidArgument, what);
+}
+
+void need(bool ok, const std::string &what) {
   if (!ok) throw fmmh::Error(fmmh::kInvalidArgument, what);
 }
 
@@ -201,6 +206,43 @@ int32_t fmmh_plan_leaf_coordinates(const fmmh_plan *plan, int32_t targets, doubl
     const auto &o = targets ? p.tgt_order : p.src_order;
     if (coords) std::memcpy(coords, u.data(), u.size() * sizeof(double));
     if (order) std::memcpy(order, o.data(), o.size() * sizeof(uint32_t));
+  });
+}
+
+int32_t fmmh_plan_leaf_cells(const fmmh_plan *plan, uint32_t *out) {
+  return guard([&] {
+    need(plan && out, "fmmh_plan_leaf_cells: NULL argument");
+    const fmmh::Plan &p = plan->plan;
+    for (size_t i = 0; i < p.leaf_key.size(); ++i) {
+      const fmmh::Cell c = fmmh::decode(p.leaf_key[i]);
+      out[4 * i] = uint32_t(c.level);
+      out[4 * i + 1] = uint32_t(c.i);
+      out[4 * i + 2] = uint32_t(c.j);
+      out[4 * i + 3] = uint32_t(c.k);
+    }
+  });
+}
+
+int32_t fmmh_plan_box_cells(const fmmh_plan *plan, int32_t level, uint32_t *out) {
+  return guard([&] {
+    need(plan && out, "fmmh_plan_box_cells: NULL argument");
+    const fmmh::Plan &p = plan->plan;
+    need(level >= 0 && level < p.nlevels, "fmmh_plan_box_cells: level out of range");
+    for (size_t i = 0; i < p.keys[level].size(); ++i) {
+      const fmmh::Cell c = fmmh::decode(p.keys[level][i]);
+      out[4 * i] = uint32_t(c.level);
+      out[4 * i + 1] = uint32_t(c.i);
+      out[4 * i + 2] = uint32_t(c.j);
+      out[4 * i + 3] = uint32_t(c.k);
+    }
+  });
+}
+
+int32_t fmmh_plan_point_starts(const fmmh_plan *plan, int32_t targets, uint32_t *out) {
+  return guard([&] {
+    need(plan && out, "fmmh_plan_point_starts: NULL argument");
+    const auto &v = targets ? plan->plan.tgt_start : plan->plan.src_start;
+    std::memcpy(out, v.data(), v.size() * sizeof(uint32_t));
   });
 }
 
@@ -359,6 +401,102 @@ int32_t fmmh_helmholtz_plane_wave_merge(int32_t p, double kh_re, double kh_im, c
     for (int a = 0; a < fmmh::kPwSlots; ++a)
       if (present[a]) slots[a] = slot_multipoles + a * 2 * nc;
     fmmh::hpw_merge_host(L, slots.data(), out);
+  });
+}
+
+// ---- kernels: thin launches of the operator's own kernels on caller memory ------------
+
+static hipStream_t kernel_stream(int32_t device, void *stream) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw fmmh::Error(fmmh::kNoDevice, "no HIP device");
+  need(device >= 0 && device < n, "device ordinal out of range");
+  fmmh::hip_check(hipSetDevice(device), "hipSetDevice");
+  return static_cast<hipStream_t>(stream);
+}
+
+int32_t fmmh_kernel_p2p(int32_t device, const fmmh_p2p_inputs *in, int32_t gradients, double *phi_hat,
+                        double *grad_hat, void *stream) {
+  return guard([&] {
+    need(in && phi_hat && (!gradients || grad_hat), "fmmh_kernel_p2p: NULL argument");
+    need(in->near_off && in->near_ent && in->leaf_cell && in->src_start && in->tgt_start && in->sx && in->sy &&
+             in->sz && in->q && in->tx && in->ty && in->tz,
+         "fmmh_kernel_p2p: NULL input array");
+    const hipStream_t s = kernel_stream(device, stream);
+    fmmh::dev::p2p<double>(s, in->leaf_first, in->count, gradients ? 1 : 0, 0, in->near_off, in->near_ent, in->leaf_cell,
+                   in->src_start, in->tgt_start, in->sx, in->sy, in->sz, in->q, nullptr, nullptr, in->tx, in->ty,
+                   in->tz, phi_hat, gradients ? grad_hat : nullptr);
+    fmmh::hip_check(hipGetLastError(), "fmmh_kernel_p2p launch");
+  });
+}
+
+static int32_t kernel_expand(int32_t device, int32_t p, int irregular, const fmmh_source_inputs *in, double *coeffs,
+                             void *stream, const char *name) {
+  return guard([&] {
+    need(in && coeffs, std::string(name) + ": NULL argument");
+    need(p >= 0 && p <= fmmh::dev::kMaxDegree, std::string(name) + ": 0 <= p <= 15");
+    need(in->rows == 0 || (in->row_box && in->row_cell && in->row_off && in->row_ent && in->leaf_cell &&
+                           in->src_start && in->sx && in->sy && in->sz && in->q),
+         std::string(name) + ": NULL input array");
+    const hipStream_t s = kernel_stream(device, stream);
+    fmmh::dev::expand(s, p, irregular, in->rows, in->row_box, in->row_cell, in->row_off, in->row_ent, in->leaf_cell,
+                      in->src_start, in->sx, in->sy, in->sz, in->q, coeffs);
+    fmmh::hip_check(hipGetLastError(), (std::string(name) + " launch").c_str());
+  });
+}
+
+int32_t fmmh_kernel_p2m(int32_t device, int32_t p, const fmmh_source_inputs *in, double *coeffs, void *stream) {
+  return kernel_expand(device, p, 0, in, coeffs, stream, "fmmh_kernel_p2m");
+}
+
+int32_t fmmh_kernel_p2l(int32_t device, int32_t p, const fmmh_source_inputs *in, double *coeffs, void *stream) {
+  return kernel_expand(device, p, 1, in, coeffs, stream, "fmmh_kernel_p2l");
+}
+
+static int32_t kernel_evaluate(int32_t device, int32_t p, int irregular, int32_t gradients,
+                               const fmmh_target_inputs *in, const double *coeffs, double *phi_hat, double *grad_hat,
+                               void *stream, const char *name) {
+  return guard([&] {
+    need(in && coeffs && phi_hat && (!gradients || grad_hat), std::string(name) + ": NULL argument");
+    need(p >= 0 && p <= fmmh::dev::kMaxDegree, std::string(name) + ": 0 <= p <= 15");
+    need(in->rows == 0 || (in->row_leaf && in->row_off && in->row_box && in->box_cell && in->leaf_cell &&
+                           in->tgt_start && in->tx && in->ty && in->tz),
+         std::string(name) + ": NULL input array");
+    const hipStream_t s = kernel_stream(device, stream);
+    fmmh::dev::evaluate(s, p, irregular, gradients ? 1 : 0, in->rows, in->row_leaf, in->row_off, in->row_box,
+                        in->box_cell, in->leaf_cell, in->tgt_start, in->tx, in->ty, in->tz, coeffs, phi_hat,
+                        gradients ? grad_hat : nullptr);
+    fmmh::hip_check(hipGetLastError(), (std::string(name) + " launch").c_str());
+  });
+}
+
+int32_t fmmh_kernel_l2p(int32_t device, int32_t p, int32_t gradients, const fmmh_target_inputs *in,
+                        const double *coeffs, double *phi_hat, double *grad_hat, void *stream) {
+  return kernel_evaluate(device, p, 0, gradients, in, coeffs, phi_hat, grad_hat, stream, "fmmh_kernel_l2p");
+}
+
+int32_t fmmh_kernel_m2p(int32_t device, int32_t p, int32_t gradients, const fmmh_target_inputs *in,
+                        const double *coeffs, double *phi_hat, double *grad_hat, void *stream) {
+  return kernel_evaluate(device, p, 1, gradients, in, coeffs, phi_hat, grad_hat, stream, "fmmh_kernel_m2p");
+}
+
+int32_t fmmh_kernel_zero(int32_t device, double *values, int64_t count, void *stream) {
+  return guard([&] {
+    need(count >= 0 && (count == 0 || values), "fmmh_kernel_zero: NULL values or negative count");
+    const hipStream_t s = kernel_stream(device, stream);
+    fmmh::dev::fill(s, values, size_t(count), 0.0);
+    fmmh::hip_check(hipGetLastError(), "fmmh_kernel_zero launch");
+  });
+}
+
+int32_t fmmh_kernel_gather_output(int32_t device, uint32_t nleaves, const uint32_t *tgt_start, const double *leaf_r,
+                                  const uint32_t *order, const double *phi_hat, const double *grad_hat,
+                                  int32_t gradients, double *phi, double *grad, void *stream) {
+  return guard([&] {
+    need(nleaves == 0 || (tgt_start && leaf_r && order && phi_hat && phi && (!gradients || (grad_hat && grad))),
+         "fmmh_kernel_gather_output: NULL argument");
+    const hipStream_t s = kernel_stream(device, stream);
+    fmmh::dev::output_pass(s, nleaves, tgt_start, leaf_r, order, phi_hat, grad_hat, phi, grad, gradients ? 1 : 0);
+    fmmh::hip_check(hipGetLastError(), "fmmh_kernel_gather_output launch");
   });
 }
 

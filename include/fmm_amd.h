@@ -133,6 +133,13 @@ int32_t fmmh_plan_point_leaves(const fmmh_plan *plan, int32_t targets, uint32_t 
 /* Leaf-scaled coordinates (3 per point, leaf order) and the caller index of each. */
 int32_t fmmh_plan_leaf_coordinates(const fmmh_plan *plan, int32_t targets, double *coords, uint32_t *order);
 
+/* Cell of every leaf (4 per leaf: level, ix, iy, iz; fmm-kernels/src/p2p.rs:320-337) and the
+ * points of every leaf (CSR over the leaves, nleaves + 1 offsets into leaf order). */
+int32_t fmmh_plan_leaf_cells(const fmmh_plan *plan, uint32_t *out);
+/* Cell of every box of one level (4 per box, the box index's order). */
+int32_t fmmh_plan_box_cells(const fmmh_plan *plan, int32_t level, uint32_t *out);
+int32_t fmmh_plan_point_starts(const fmmh_plan *plan, int32_t targets, uint32_t *out);
+
 /* Operator tables of degree p (column-major, 8 / 8 / 316 matrices of (p+1)^4 doubles),
  * through the table cache when one is set. */
 int32_t fmmh_tables(int32_t p, double *m2m, double *l2l, double *m2l);
@@ -286,6 +293,66 @@ int32_t fmmh_plane_wave_merge(int32_t p, int32_t nodes, const double *slot_multi
 /* The direction (0..5) of the merge-and-shift pair slot -> child (geometry.rs
  * Direction::for_slot), -1 if adjacent, -2 for an index out of range. */
 int32_t fmmh_plane_wave_slot_direction(int32_t slot, int32_t child);
+
+/* ---- kernels (the per-level launch wrappers of fmm-kernels: p2p.rs:1244, leaf.rs:1851-1951,
+ * movement.rs:411,482) ------------------------------------------------------------- */
+/* Device pointers, f64, queued on `stream` (NULL: the default stream); nothing is allocated
+ * and nothing synchronises.  Every call adds (+=) into its outputs, each output written by
+ * one thread in a fixed order (the same bits on every call).  Preconditions, as the
+ * reference's kernels take them (fmm-kernels/src/error.rs:7-9): offsets ascending, indices
+ * within their arrays, leaf cells of the points' leaves; they are not checked on the device. */
+typedef struct {
+  uint32_t leaf_first, count;   /* target leaves leaf_first .. leaf_first + count (global indices) */
+  const uint32_t *near_off;     /* near CSR over all leaves (global): source leaves of each target leaf */
+  const uint32_t *near_ent;
+  const uint32_t *leaf_cell;    /* 4 per leaf: level, ix, iy, iz */
+  const uint32_t *src_start;    /* CSR of each leaf's sources (leaf order) */
+  const uint32_t *tgt_start;    /* CSR of each leaf's targets (leaf order) */
+  const double *sx, *sy, *sz;   /* sources in leaf order: leaf-scaled coordinates */
+  const double *q;              /* charges in leaf order */
+  const double *tx, *ty, *tz;   /* targets in leaf order: leaf-scaled coordinates */
+} fmmh_p2p_inputs;
+/* P2P (p2p::p2p, fmm-kernels/src/p2p.rs:1244): phi_hat[t] += sum q / |u_t - y^|, grad_hat[3t..]
+ * -= sum q (u_t - y^) / |u_t - y^|^3 over each target leaf's near row in row order, in the
+ * target leaf's units (relative frames from the cells); grad_hat may be NULL without gradients. */
+int32_t fmmh_kernel_p2p(int32_t device, const fmmh_p2p_inputs *in, int32_t gradients, double *phi_hat,
+                        double *grad_hat, void *stream);
+typedef struct {
+  uint32_t rows;                /* output expansions */
+  const uint32_t *row_box;      /* output slot of each row: coeffs[row_box[r] (p+1)^2 ..] */
+  const uint32_t *row_cell;     /* 4 per row: the output box's level, ix, iy, iz */
+  const uint32_t *row_off;      /* CSR of each row's source leaves */
+  const uint32_t *row_ent;
+  const uint32_t *leaf_cell, *src_start;
+  const double *sx, *sy, *sz, *q;
+} fmmh_source_inputs;
+/* P2M (leaf::p2m) and P2L (leaf::p2l, fmm-kernels/src/leaf.rs:1851,1873): coeffs += the
+ * multipole at each row's box of its leaves' sources, or the local of its X-list leaves'. */
+int32_t fmmh_kernel_p2m(int32_t device, int32_t p, const fmmh_source_inputs *in, double *coeffs, void *stream);
+int32_t fmmh_kernel_p2l(int32_t device, int32_t p, const fmmh_source_inputs *in, double *coeffs, void *stream);
+typedef struct {
+  uint32_t rows;                /* target leaves with work */
+  const uint32_t *row_leaf;     /* the target leaf of each row */
+  const uint32_t *row_off;      /* CSR of each row's boxes */
+  const uint32_t *row_box;      /* box slots (coefficients at coeffs[box (p+1)^2 ..]) */
+  const uint32_t *box_cell;     /* 4 per box slot: level, ix, iy, iz */
+  const uint32_t *leaf_cell, *tgt_start;
+  const double *tx, *ty, *tz;
+} fmmh_target_inputs;
+/* L2P (leaf::l2p; one box per row, the leaf's own) and M2P (leaf::m2p, the W-list boxes),
+ * fmm-kernels/src/leaf.rs:1902,1933: phi_hat, grad_hat += the expansions at the row's targets. */
+int32_t fmmh_kernel_l2p(int32_t device, int32_t p, int32_t gradients, const fmmh_target_inputs *in,
+                        const double *coeffs, double *phi_hat, double *grad_hat, void *stream);
+int32_t fmmh_kernel_m2p(int32_t device, int32_t p, int32_t gradients, const fmmh_target_inputs *in,
+                        const double *coeffs, double *phi_hat, double *grad_hat, void *stream);
+/* movement::zero (movement.rs:482): values[0 .. count) = 0. */
+int32_t fmmh_kernel_zero(int32_t device, double *values, int64_t count, void *stream);
+/* movement::gather_output (movement.rs:411): phi[order[i]] = phi_hat[i] / (4 pi r), grad[3 order[i]
+ * ..] = grad_hat[3 i ..] / (4 pi r^2) for the targets of each of the nleaves leaves, r the
+ * leaf's radius in the caller's units (leaf_r[leaf]). */
+int32_t fmmh_kernel_gather_output(int32_t device, uint32_t nleaves, const uint32_t *tgt_start, const double *leaf_r,
+                                  const uint32_t *order, const double *phi_hat, const double *grad_hat,
+                                  int32_t gradients, double *phi, double *grad, void *stream);
 
 /* ---- Helmholtz (kernel e^{ikr} / (4 pi r), complex charges) ---------------------- */
 /* The reference's Helmholtz FMM (nd_fmm_exec::helmholtz) runs on the host only
