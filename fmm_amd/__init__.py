@@ -91,9 +91,11 @@ def tables(p: int):
 
 
 class Plan:
-    """Tree, box index, lists and leaf-ordered points (one rank), built on the CPU."""
+    """Tree, box index, lists and leaf-ordered points (one rank), built on the CPU, or with
+    device set its per-point stages (bounding box, Morton keys and their sort, the points'
+    leaves, order and coordinates) on that GPU: the same plan bit for bit."""
 
-    def __init__(self, sources, targets=None, leaf_points=64, max_level=16, near_rule=None):
+    def __init__(self, sources, targets=None, leaf_points=64, max_level=16, near_rule=None, device=None):
         src = np.ascontiguousarray(sources, dtype=np.float64).reshape(-1, 3)
         tgt = src if targets is None else np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 3)
         self.sources, self.targets = src, tgt
@@ -101,7 +103,12 @@ class Plan:
         if near_rule is not None:
             o.near_w, o.near_x = float(near_rule[0]), float(near_rule[1])
         h = C.c_void_p()
-        _lib.check(_lib.lib().fmmh_plan_build(_lib.ptr(src), len(src), _lib.ptr(tgt), len(tgt), C.byref(o), C.byref(h)))
+        if device is None:
+            _lib.check(_lib.lib().fmmh_plan_build(_lib.ptr(src), len(src), _lib.ptr(tgt), len(tgt), C.byref(o),
+                                                  C.byref(h)))
+        else:
+            _lib.check(_lib.lib().fmmh_plan_build_device(int(device), _lib.ptr(src), len(src), _lib.ptr(tgt), len(tgt),
+                                                         C.byref(o), C.byref(h)))
         self._h = h
         self._free = _lib.lib().fmmh_plan_free
 

@@ -32,7 +32,7 @@ EXPORTS = [
     "fmmh_last_error", "fmmh_abi_version", "fmmh_plan_build", "fmmh_plan_free", "fmmh_plan_nlevels",
     "fmmh_plan_nboxes", "fmmh_plan_nleaves", "fmmh_plan_domain", "fmmh_plan_box_keys", "fmmh_plan_leaf_keys",
     "fmmh_plan_list", "fmmh_plan_point_leaves", "fmmh_plan_leaf_coordinates", "fmmh_tables",
-    "fmmh_plan_leaf_cells", "fmmh_plan_box_cells", "fmmh_plan_point_starts", "fmmh_kernel_p2p", "fmmh_kernel_p2m", "fmmh_kernel_p2l",
+    "fmmh_plan_build_device", "fmmh_plan_leaf_cells", "fmmh_plan_box_cells", "fmmh_plan_point_starts", "fmmh_kernel_p2p", "fmmh_kernel_p2m", "fmmh_kernel_p2l",
     "fmmh_kernel_l2p", "fmmh_kernel_m2p", "fmmh_kernel_zero", "fmmh_kernel_gather_output",
     "fmmh_device_count", "fmmh_device_describe", "fmmh_operator_create", "fmmh_operator_free", "fmmh_operator_stats",
     "fmmh_load_charges", "fmmh_reset", "fmmh_p2m", "fmmh_m2m", "fmmh_m2l", "fmmh_p2l", "fmmh_l2l",
@@ -155,6 +155,7 @@ def lib():
     L.fmmh_plan_point_leaves.argtypes = [vp, i32, vp]
     L.fmmh_plan_leaf_coordinates.argtypes = [vp, i32, vp, vp]
     L.fmmh_plan_leaf_cells.argtypes = [vp, vp]
+    L.fmmh_plan_build_device.argtypes = [i32, vp, i64, vp, i64, C.POINTER(PlanOptions), C.POINTER(vp)]
     L.fmmh_plan_box_cells.argtypes = [vp, i32, vp]
     L.fmmh_plan_point_starts.argtypes = [vp, i32, vp]
     L.fmmh_kernel_p2p.argtypes = [i32, C.POINTER(P2pInputs), i32, vp, vp, vp]

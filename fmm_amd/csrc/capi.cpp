@@ -119,6 +119,26 @@ int32_t fmmh_plan_build(const double *sources, int64_t nsources, const double *t
   });
 }
 
+int32_t fmmh_plan_build_device(int32_t device, const double *sources, int64_t nsources, const double *targets,
+                               int64_t ntargets, const fmmh_plan_options *options, fmmh_plan **out) {
+  return guard([&] {
+    need(out != nullptr, "fmmh_plan_build_device: out is NULL");
+    need(nsources == 0 || sources != nullptr, "fmmh_plan_build_device: sources is NULL");
+    need(ntargets == 0 || targets != nullptr, "fmmh_plan_build_device: targets is NULL");
+    fmmh::PlanOptions o;
+    if (options) {
+      o.leaf_points = options->leaf_points;
+      o.max_level = options->max_level;
+      o.near_w = options->near_w;
+      o.near_x = options->near_x;
+    }
+    need(o.max_level >= 0 && o.max_level <= fmmh::kDeepest, "fmmh_plan_build_device: max_level must lie in 0..16");
+    auto p = std::make_unique<fmmh_plan>();
+    p->plan = fmmh::build_plan_device(device, sources, nsources, targets, ntargets, o);
+    *out = p.release();
+  });
+}
+
 int32_t fmmh_forest_build(const double *points, const int64_t *counts, int32_t nclouds,
                           const fmmh_plan_options *options, fmmh_plan **out) {
   return guard([&] {

@@ -272,8 +272,7 @@ void apply_near_rule(Plan &plan) {
 
 }  // namespace
 
-Plan build_plan(const double *sources, int64_t ns, const double *targets, int64_t nt,
-                const PlanOptions &options) {
+Plan plan_begin(int64_t ns, int64_t nt, bool same_points, const PlanOptions &options) {
   require(ns >= 0 && nt >= 0 && ns + nt > 0, "build_plan: no points");
   require(ns + nt < (int64_t(1) << 32), "build_plan: too many points", kTooLarge);
   require(options.leaf_points >= 1, "build_plan: leaf_points must be positive");
@@ -281,19 +280,11 @@ Plan build_plan(const double *sources, int64_t ns, const double *targets, int64_
   plan.options = options;
   plan.ns = ns;
   plan.nt = nt;
-  plan.same_points = sources == targets && ns == nt;
-  const int64_t n = ns + nt;
-  auto point = [&](int64_t i) { return i < ns ? sources + 3 * i : targets + 3 * (i - ns); };
+  plan.same_points = same_points;
+  return plan;
+}
 
-  // Domain: compute_global_bounding_box (octree/src/octree.rs:1278-1369).
-  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int64_t i = 0; i < n; ++i)
-    for (int k = 0; k < 3; ++k) {
-      const double v = point(i)[k];
-      require(std::isfinite(v), "build_plan: a coordinate is not finite");
-      lo[k] = std::min(lo[k], v);
-      hi[k] = std::max(hi[k], v);
-    }
+void plan_domain(Plan &plan, const double lo[3], const double hi[3]) {
   double diam[3], mean[3];
   for (int k = 0; k < 3; ++k) {
     diam[k] = hi[k] - lo[k];
@@ -309,14 +300,12 @@ Plan build_plan(const double *sources, int64_t ns, const double *targets, int64_
   }
   const double sides[3] = {plan.box[3] - plan.box[0], plan.box[4] - plan.box[1], plan.box[5] - plan.box[2]};
   plan.side = std::max(std::max(sides[0], sides[1]), sides[2]);
+}
 
-  // Distinct level-16 keys of sources then targets.
-  std::vector<uint64_t> fine(n);
-#pragma omp parallel for schedule(static)
-  for (int64_t i = 0; i < n; ++i) fine[i] = fine_key(point(i), plan.box);
-  std::sort(fine.begin(), fine.end());
-  fine.erase(std::unique(fine.begin(), fine.end()), fine.end());
+uint64_t plan_fine_key(const Plan &plan, const double *x) { return fine_key(x, plan.box); }
 
+void plan_tree(Plan &plan, std::vector<uint64_t> fine) {
+  const PlanOptions &options = plan.options;
   // Refinement from the root, then 2:1 balance.
   std::vector<uint64_t> leaves;
   refine(fine.data(), fine.size(), 0ULL, std::min(options.max_level, kDeepest), options.leaf_points, leaves);
@@ -548,22 +537,57 @@ Plan build_plan(const double *sources, int64_t ns, const double *targets, int64_
     }
   }
 
-  // Points in leaf order; the leaf of a point is the last leaf at or before its fine key
-  // in Morton order (leaves are disjoint and cover the domain).
-  std::vector<uint64_t> morton = leaves;  // sorted by key
-  std::vector<uint32_t> morton_to_leaf(morton.size());
-  for (size_t i = 0; i < morton.size(); ++i) {
+  // the leaves in Morton order and their leaf indices, for the points' assignment
+  plan.morton_leaves = leaves;  // sorted by key
+  plan.morton_to_leaf.resize(leaves.size());
+  for (size_t i = 0; i < leaves.size(); ++i) {
     uint32_t b;
     bool lf;
-    held(morton[i], b, lf);
-    morton_to_leaf[i] = plan.box_leaf[key_level(morton[i])][b];
+    held(leaves[i], b, lf);
+    plan.morton_to_leaf[i] = plan.box_leaf[key_level(leaves[i])][b];
   }
-  assign_points(sources, ns, plan, morton, morton_to_leaf, plan.src_start, plan.src_order, plan.us);
-  assign_points(targets, nt, plan, morton, morton_to_leaf, plan.tgt_start, plan.tgt_order, plan.ut);
+}
+
+void plan_finish(Plan &plan) {
   apply_near_rule(plan);
   plan.leaf_radius.resize(plan.nleaves());
-  for (int l = 0; l < L; ++l)
+  for (int l = 0; l < plan.nlevels; ++l)
     for (uint32_t i = plan.leaf_off[l]; i < plan.leaf_off[l + 1]; ++i) plan.leaf_radius[i] = plan.radius(l);
+  std::vector<uint64_t>().swap(plan.morton_leaves);
+  std::vector<uint32_t>().swap(plan.morton_to_leaf);
+}
+
+void plan_leaf_coordinates(const Plan &plan, const double *x, uint32_t leaf, double *u) {
+  leaf_coordinates(x, plan.leaf_key[leaf], plan, u);
+}
+
+Plan build_plan(const double *sources, int64_t ns, const double *targets, int64_t nt,
+                const PlanOptions &options) {
+  Plan plan = plan_begin(ns, nt, sources == targets && ns == nt, options);
+  const int64_t n = ns + nt;
+  auto point = [&](int64_t i) { return i < ns ? sources + 3 * i : targets + 3 * (i - ns); };
+  // Domain: compute_global_bounding_box (octree/src/octree.rs:1278-1369).
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      const double v = point(i)[k];
+      require(std::isfinite(v), "build_plan: a coordinate is not finite");
+      lo[k] = std::min(lo[k], v);
+      hi[k] = std::max(hi[k], v);
+    }
+  plan_domain(plan, lo, hi);
+  // Distinct level-16 keys of sources then targets.
+  std::vector<uint64_t> fine(n);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) fine[i] = fine_key(point(i), plan.box);
+  std::sort(fine.begin(), fine.end());
+  fine.erase(std::unique(fine.begin(), fine.end()), fine.end());
+  plan_tree(plan, std::move(fine));
+  // Points in leaf order; the leaf of a point is the last leaf at or before its fine key
+  // in Morton order (leaves are disjoint and cover the domain).
+  assign_points(sources, ns, plan, plan.morton_leaves, plan.morton_to_leaf, plan.src_start, plan.src_order, plan.us);
+  assign_points(targets, nt, plan, plan.morton_leaves, plan.morton_to_leaf, plan.tgt_start, plan.tgt_order, plan.ut);
+  plan_finish(plan);
   return plan;
 }
 

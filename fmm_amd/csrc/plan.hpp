@@ -75,6 +75,10 @@ struct Plan {
   // Radius of each leaf in domain units (side / 2^(level + 1); per cloud in a forest).
   std::vector<double> leaf_radius;
 
+  // During a build only: the leaves in Morton order and the leaf index of each.
+  std::vector<uint64_t> morton_leaves;
+  std::vector<uint32_t> morton_to_leaf;
+
   uint32_t nleaves() const { return static_cast<uint32_t>(leaf_key.size()); }
   uint32_t nboxes(int level) const { return static_cast<uint32_t>(keys[level].size()); }
   double radius(int level) const;  // r_l = w / 2^(l+1)
@@ -83,6 +87,23 @@ struct Plan {
 // Builds the plan of sources and targets (3 doubles per point).
 Plan build_plan(const double *sources, int64_t ns, const double *targets, int64_t nt,
                 const PlanOptions &options);
+// The same build on the GPU for its per-point stages (plan_device.hip): the bounding box,
+// the fine keys, their sort and distinct set, and the points' leaves, order and leaf
+// coordinates; the tree and lists from the distinct keys on the host.  The same plan.
+Plan build_plan_device(int device, const double *sources, int64_t ns, const double *targets, int64_t nt,
+                       const PlanOptions &options);
+
+// The stages of a build: the checked settings, the domain of the points' bounding box
+// [lo, hi], the fine (level-16) key of a point, the leaves, boxes and lists from the sorted
+// distinct fine keys (leaving morton_leaves / morton_to_leaf for the points), a point's
+// leaf-scaled coordinates in a leaf, and the near rule and leaf radii once the points are
+// placed.
+Plan plan_begin(int64_t ns, int64_t nt, bool same_points, const PlanOptions &options);
+void plan_domain(Plan &plan, const double lo[3], const double hi[3]);
+uint64_t plan_fine_key(const Plan &plan, const double *x);
+void plan_tree(Plan &plan, std::vector<uint64_t> fine);
+void plan_leaf_coordinates(const Plan &plan, const double *x, uint32_t leaf, double *u);
+void plan_finish(Plan &plan);
 
 // A forest: independent clouds (each its own domain and tree, sources = targets) as one
 // plan whose levels hold every cloud's boxes and leaves, cloud after cloud; lists never

@@ -106,6 +106,13 @@ int32_t fmmh_abi_version(void);
 /* sources/targets: 3 doubles per point (x, y, z); targets may equal sources. */
 int32_t fmmh_plan_build(const double *sources, int64_t nsources, const double *targets, int64_t ntargets,
                         const fmmh_plan_options *options, fmmh_plan **out);
+/* fmmh_plan_build with its per-point stages on GPU `device` (octree/src/octree/
+ * implementation.rs:682-716,1014): the bounding box, the points' level-16 Morton keys, their
+ * sort and distinct set (device radix sort and unique), then, from the tree and lists the
+ * host derives from those keys, each point's leaf, the points in leaf order and their
+ * leaf-scaled coordinates.  Host point arrays; the same plan bit for bit. */
+int32_t fmmh_plan_build_device(int32_t device, const double *sources, int64_t nsources, const double *targets,
+                               int64_t ntargets, const fmmh_plan_options *options, fmmh_plan **out);
 /* Batched independent clouds as one plan (a forest): nclouds clouds of counts[c] points,
  * concatenated in points (3 doubles each), sources = targets within each cloud.  Each
  * cloud is refined and listed as if alone (its own domain); one operator on the forest
