@@ -129,8 +129,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy 256 MiB device to device (a known-size kernel for counter scaling)")
     ap.add_argument("--m2l", default="dense", choices=("dense", "rotation", "plane_wave"),
-                    help="M2L formulation of both legs (rotation: f32 in the mixed leg; plane_wave: the plain "
-                         "leg only, the mixed leg keeps the dense bf16x3 M2L)")
+                    help="M2L formulation of both legs (rotation: f32 in the mixed leg; plane_wave: f64 waves by "
+                         "merge and shift in both legs)")
     ap.add_argument("--workload", default="cube", choices=("cube", "batched", "helmholtz", "nonuniform", "grid"),
                     help="cube: 4M uniform points (the headline); batched: 256 independent Gaussian clouds, "
                          "p = 6, one forest evaluation replayed as a hipGraph, plus its adjoint; helmholtz: "
@@ -207,7 +207,7 @@ def main():
     tune_dir = args.tuning_cache or tempfile.mkdtemp(prefix="fmm_amd_tune")
 
     def run(precision):
-        m2l = "dense" if (args.m2l == "plane_wave" and precision == "mixed") else args.m2l
+        m2l = args.m2l
         tuning, tinfo = None, None
         if args.tune_budget > 0:
             tuning, tinfo = tune.resolve(plan, args.p, precision, True, 0, m2l, tune_dir, args.tune_budget)

@@ -496,7 +496,8 @@ class FmmBuilder:
     def m2l(self, method: str):
         """The M2L formulation (fmm-exec's M2lMethod): "dense" (tables on MFMA), "rotation"
         (rotate, coaxial shift, rotate back; f32 in the mixed precision) or "plane_wave"
-        (waves in six directions, f64, to a quadrature tolerance of about 10^-(p/2 + 2))."""
+        (waves in six directions by merge and shift, f64 in both precisions, on FMM3D's
+        quadrature of the degree: its error about a tenth of the truncation's)."""
         if method not in M2L_METHODS:
             raise ValueError(f"unknown M2L method {method!r}")
         self._m2l = method

@@ -26,8 +26,6 @@ DeviceOperator::DeviceOperator(const Plan &plan, const OperatorOptions &options)
   require(opt_.precision == kPlainF64 || opt_.precision == kMixed, "unknown precision", kUnsupportedPrecision);
   require(opt_.m2l == kM2lDense || opt_.m2l == kM2lRotation || opt_.m2l == kM2lPlaneWave, "unknown M2L method",
           kUnsupportedLayout);
-  require(opt_.m2l != kM2lPlaneWave || opt_.precision == kPlainF64, "the plane-wave M2L runs in f64",
-          kUnsupportedPrecision);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(kNoDevice, "no HIP device");
   require(opt_.device >= 0 && opt_.device < ndev, "device ordinal out of range", kNoDevice);

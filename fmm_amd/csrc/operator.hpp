@@ -90,7 +90,8 @@ struct Rows {
 enum M2lMethod : int {
   kM2lDense = 0,     // dense tables, grouped contraction on MFMA
   kM2lRotation = 1,  // rotation, coaxial shift, rotation back (f32 in the mixed precision)
-  kM2lPlaneWave = 2, // plane waves in six directions (f64; approximate to the quadrature's tolerance)
+  kM2lPlaneWave = 2, // plane waves in six directions by merge and shift (f64 in both precisions;
+                     // approximate to the quadrature's tolerance)
 };
 
 constexpr int kTuneLevels = 17;  // levels 0..16

@@ -208,12 +208,15 @@ def test_mixed_precision_within_tolerance(k):
         assert ep < bar and eg < bar
 
 
-@pytest.mark.parametrize("kind,n,p,leaf,k", [("uniform", 8000, 10, 32, 25.0), ("clustered", 8000, 12, 32, 14.0 + 0.2j)])
+@pytest.mark.parametrize("kind,n,p,leaf,k", [("uniform", 8000, 10, 32, 25.0), ("clustered", 8000, 12, 32, 14.0 + 0.2j),
+                                              ("uniform", 8000, 10, 32, 12.0 + 8.0j)])
 def test_plane_wave_m2l_matches_oracle_fmm(kind, n, p, leaf, k):
     """The exponential M2L by merge and shift (HelmholtzM2l::PlaneWave: the levels the rule
     gives a quadrature, RCR on the others) against the oracle FMM (exact M2L) and the
     direct sum: within the rotation path's error of the direct sum (the reference finds
-    iprec 2 within 1.05x of Rotation's) and within that error of the oracle."""
+    iprec 2 within 1.05x of Rotation's) and within that error of the oracle.  k = 12 + 8i
+    puts the coarse levels in FMM3D's real-t domains (Im kappa >= pi/2), the others on
+    the contour."""
     _log(f"case helmholtz plane-wave m2l {kind} n={n} p={p} leaf={leaf} k={k}")
     pts = _points(kind, n)
     q = _charges(n)

@@ -197,7 +197,20 @@ def test_plane_wave_fmm_matches_oracle(kind, n, p, leaf):
     assert e_pw <= t_dense
 
 
-def test_plane_wave_is_f64_only():
-    pts = np.random.default_rng(0).uniform(0, 1, (500, 3))
-    with pytest.raises(fmm_amd.FmmError):
-        fmm_amd.FmmBuilder(6).precision("mixed").m2l("plane_wave").build(pts)
+@pytest.mark.gpu
+def test_plane_wave_in_the_mixed_operator():
+    """The mixed operator with the plane-wave M2L (f64 waves beside the f32 P2P, as the
+    reference runs its plane-wave tables in either precision): within the mixed bar of the
+    f64 plane-wave operator on the same plan."""
+    if fmm_amd.device_count() < 1:
+        pytest.skip("no GPU")
+    print("case plane-wave fmm mixed", flush=True)
+    pts = O.Rng(11).cube(12000)
+    q = O.Rng(5).normals(12000)
+    f64 = fmm_amd.FmmBuilder(8).max_points_per_leaf(40).m2l("plane_wave").build(pts)
+    mixed = fmm_amd.Fmm(f64.plan, fmm_amd.Operator(f64.plan, 8, "mixed", m2l="plane_wave"))
+    a, b = f64.evaluate(q), mixed.evaluate(q)
+    e = np.linalg.norm(b.potential - a.potential) / np.linalg.norm(a.potential)
+    eg = np.linalg.norm(b.gradient - a.gradient) / np.linalg.norm(a.gradient)
+    print(f"  mixed vs f64 plane wave: {e:.2e} / {eg:.2e}", flush=True)
+    assert e < 1e-6 and eg < 1e-6
