@@ -7,13 +7,15 @@
 //   k_hpw_outgoing: one 512-thread block per box of the level.  The stored multipole (real
 //     and imaginary planes) rotated into the three frames (thread = (frame, plane, slot)),
 //     the modes A_k(m), B_k(m) up and down (thread = (frame, node, order)), the waves of the
-//     six directions (thread = (frame, wave value)): waves[box][dir][re, im][W].
+//     six directions (thread = (frame, wave value), cos and sin m a_j by a recurrence in m):
+//     waves[box][dir][re, im][W].
 //   the merge: k_pw_merge of pwave.hip (the accumulators and child shifts are complex in
 //     both families; W in place of H).
 //   k_hpw_local: one 512-thread block per target box, frame by frame: its up and down
 //     incoming waves staged in LDS, the sums over the wave values per (up/down, node,
-//     order), the frame's local per (n, m) from both, the rotation back per (plane, slot),
-//     summed over the frames and added to the box's local (planes 0 and 1).
+//     order; cos and sin m a_j by a recurrence in j), the frame's local per (n, m) from
+//     both, the rotation back per (plane, slot), summed over the frames and added to the
+//     box's local (planes 0 and 1).
 // f64 throughout (the reference's exponential M2L is f64 only).
 #include <hip/hip_runtime.h>
 
@@ -89,12 +91,18 @@ __global__ __launch_bounds__(kHpwThreads) void k_hpw_outgoing(HpwArgs a, uint32_
   // waves: X(k, j) = sum_m A(m) cos m a_j + B(m) sin m a_j, up and down
   double *wv = waves + size_t(item) * 12 * W;
   for (int it = t; it < 3 * W; it += kHpwThreads) {
-    const int f = it / W, j = it % W, k = a.wk[j], jj = j - a.start[k], N = a.len[k];
-    const double *cs = a.cosm + a.trig[k] + jj, *sn = a.sinm + a.trig[k] + jj;
+    const int f = it / W, j = it % W, k = a.wk[j], jj = j - a.start[k], N = a.len[k], om = a.ord[k];
     const double *mk = md + (size_t(f) * O + a.mode_start[k]) * 8;
+    // cos and sin of m a_j by the recurrence in m from row 1 of the node's table (a_j)
+    const double c1 = om > 1 ? a.cosm[a.trig[k] + N + jj] : 1.0, s1 = om > 1 ? a.sinm[a.trig[k] + N + jj] : 0.0;
+    double c = 1.0, s = 0.0;
     double ur = 0.0, ui = 0.0, dr = 0.0, di = 0.0;
-    for (int m = 0; m < a.ord[k]; ++m) {
-      const double c = cs[m * N], s = sn[m * N];
+    for (int m = 0; m < om; ++m) {
+      if (m > 0) {
+        const double nc_ = c * c1 - s * s1;
+        s = s * c1 + c * s1;
+        c = nc_;
+      }
       const double *mm = mk + 8 * m;
       ur = fma(mm[0], c, fma(mm[2], s, ur));
       ui = fma(mm[1], c, fma(mm[3], s, ui));
@@ -130,13 +138,18 @@ __global__ __launch_bounds__(kHpwThreads) void k_hpw_local(HpwArgs a, uint32_t n
       const int ud = it % 2, q = a.item[it / 2], k = q / P1, m = q % P1;
       const int N = a.len[k], st = a.start[k];
       const double *vr = V + ud * 2 * W + st, *vi = vr + W;
-      const double *cs = a.cosm + a.trig[k] + m * N, *sn = a.sinm + a.trig[k] + m * N;
+      // cos and sin of m a_j by the recurrence in j from the step m a_1 (the table's row m)
+      const double cd = N > 1 ? a.cosm[a.trig[k] + m * N + 1] : 1.0, sd = N > 1 ? a.sinm[a.trig[k] + m * N + 1] : 0.0;
+      double c = 1.0, sv = 0.0;
       double cr = 0.0, ci = 0.0, sr = 0.0, si = 0.0;
       for (int j = 0; j < N; ++j) {
-        cr = fma(vr[j], cs[j], cr);
-        ci = fma(vi[j], cs[j], ci);
-        sr = fma(vr[j], sn[j], sr);
-        si = fma(vi[j], sn[j], si);
+        cr = fma(vr[j], c, cr);
+        ci = fma(vi[j], c, ci);
+        sr = fma(vr[j], sv, sr);
+        si = fma(vi[j], sv, si);
+        const double nc_ = c * cd - sv * sd;
+        sv = sv * cd + c * sd;
+        c = nc_;
       }
       double *o = g + (size_t(ud) * O + a.mode_start[k] + m) * 4;
       o[0] = cr;
