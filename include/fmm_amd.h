@@ -49,8 +49,9 @@ typedef enum {
 
 /* The M2L formulation (M2lMethod): dense tables (fmm-kernels/src/translate.rs:1703),
  * rotation, coaxial shift and rotation back (fmm-kernels/src/rotation.rs:441; in the mixed
- * precision in f32), or plane waves in six directions (fmm-kernels/src/plane_wave.rs:1159;
- * f64, approximate to a quadrature tolerance of about 10^-(p/2 + 2)). */
+ * precision in f32), or plane waves in six directions by merge and shift
+ * (fmm-kernels/src/plane_wave.rs:1-60; f64 in both precisions, on FMM3D's quadrature of
+ * the degree, whose error is about a tenth of the truncation's). */
 typedef enum { FMMH_M2L_DENSE = 0, FMMH_M2L_ROTATION = 1, FMMH_M2L_PLANE_WAVE = 2 } fmmh_m2l_method;
 
 typedef enum { FMMH_LIST_NEAR = 0, FMMH_LIST_V = 1, FMMH_LIST_W = 2, FMMH_LIST_X = 3 } fmmh_list_kind;
