@@ -36,6 +36,21 @@ __device__ __forceinline__ int degree_of(int k) {
   return n;
 }
 
+// sum_j row[j * stride] x[j], j < w: four independent partial sums (their table loads in
+// flight together), added in a fixed order
+__device__ __forceinline__ double strided_dot(const double *__restrict__ row, int stride, int w, const double *x) {
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  int j = 0;
+  for (; j + 3 < w; j += 4) {
+    v0 = fma(row[j * stride], x[j], v0);
+    v1 = fma(row[(j + 1) * stride], x[j + 1], v1);
+    v2 = fma(row[(j + 2) * stride], x[j + 2], v2);
+    v3 = fma(row[(j + 3) * stride], x[j + 3], v3);
+  }
+  for (; j < w; ++j) v0 = fma(row[j * stride], x[j], v0);
+  return (v0 + v1) + (v2 + v3);
+}
+
 // first factor of order m of a node whose factors start at f0: per order m' < m the
 // degrees m'..p
 __device__ __forceinline__ int factor_of(int f0, int p, int m) { return f0 + m * (p + 1) - m * (m - 1) / 2; }
@@ -58,10 +73,7 @@ __global__ __launch_bounds__(kHpwThreads) void k_hpw_outgoing(HpwArgs a, uint32_
     const int f = 1 + it / (2 * nc), r = it % (2 * nc), pl = r / nc, k = r % nc;
     const int n = degree_of(k), w = 2 * n + 1, i = k - n * n;
     const double *blk = a.frames + size_t(f - 1) * a.rb + n * (2 * n - 1) * (2 * n + 1) / 3 + i * w;
-    const double *in = x + pl * nc + n * n;
-    double v = 0.0;
-    for (int j = 0; j < w; ++j) v = fma(blk[j], in[j], v);
-    x[(2 * f + pl) * nc + k] = v;
+    x[(2 * f + pl) * nc + k] = strided_dot(blk, 1, w, x + pl * nc + n * n);
   }
   __syncthreads();
   // modes: per order the sums over n of P^out_k(n, m) M^{+-m}_n, even and odd n - m apart
@@ -195,9 +207,7 @@ __global__ __launch_bounds__(kHpwThreads) void k_hpw_local(HpwArgs a, uint32_t n
         acc += y[jj];
       } else {
         const double *blk = a.frames + size_t(f - 1) * a.rb + n * (2 * n - 1) * (2 * n + 1) / 3 + jj;
-        double v = 0.0;
-        for (int i = 0; i < w; ++i) v = fma(blk[i * w], y[i], v);
-        acc += v;
+        acc += strided_dot(blk, w, w, y);
       }
     }
     __syncthreads();

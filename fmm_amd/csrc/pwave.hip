@@ -33,10 +33,25 @@ constexpr int kPwLocalThreads = 512;
 constexpr uint32_t kNoBox = 0xffffffffu;
 
 // LDS words of k_pw_local's waves region (the six directions' waves, later the three
-// frames' locals)
+// frames' locals and their rotations back)
 __host__ __device__ inline size_t pw_local_wave_words(int p, int H) {
-  const size_t a = 12 * size_t(H), b = 3 * size_t(p + 1) * (p + 1);
+  const size_t a = 12 * size_t(H), b = 6 * size_t(p + 1) * (p + 1);  // later ys[3][nc], part[3][nc]
   return a > b ? a : b;
+}
+
+// sum_c blk[c * w] x[c], c < w: four independent partial sums (their table loads in
+// flight together), added in a fixed order
+__device__ __forceinline__ double column_dot(const double *__restrict__ blk, int w, const double *x) {
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  int c = 0;
+  for (; c + 3 < w; c += 4) {
+    v0 = fma(blk[c * w], x[c], v0);
+    v1 = fma(blk[(c + 1) * w], x[c + 1], v1);
+    v2 = fma(blk[(c + 2) * w], x[c + 2], v2);
+    v3 = fma(blk[(c + 3) * w], x[c + 3], v3);
+  }
+  for (; c < w; ++c) v0 = fma(blk[c * w], x[c], v0);
+  return (v0 + v1) + (v2 + v3);
 }
 
 // slot k -> degree
@@ -67,9 +82,7 @@ __global__ __launch_bounds__(kPwLocalThreads) void k_pw_outgoing(PwArgs a, uint3
     const int f = it / nc, k = it % nc;
     const int n = degree_of(k), w = 2 * n + 1, r = k - n * n;
     const double *blk = a.fwd + size_t(f) * a.rb + n * (2 * n - 1) * (2 * n + 1) / 3;
-    double v = 0.0;
-    for (int c = 0; c < w; ++c) v = fma(blk[r + c * w], Ms[n * n + c], v);
-    xs[f * nc + k] = v;
+    xs[f * nc + k] = column_dot(blk + r, w, Ms + n * n);
   }
   __syncthreads();
   // A_k(m) = sum_{n >= m} lambda^n s^(n-m) 2^-(n+1) x_n^m, up (s = 1) and down (s = -1)
@@ -282,15 +295,15 @@ __global__ __launch_bounds__(kPwLocalThreads) void k_pw_local(PwArgs a, uint32_t
     if (m > 0) ys[f * nc + n * n + n - m] = ri;
   }
   __syncthreads();
-  if (t < nc) {
-    const int n = degree_of(t), w = 2 * n + 1, r = t - n * n;
-    double acc = 0.0;
-    for (int f = 0; f < 3; ++f) {
-      const double *blk = a.bwd + size_t(f) * a.rb + n * (2 * n - 1) * (2 * n + 1) / 3;
-      for (int c = 0; c < w; ++c) acc = fma(blk[r + c * w], ys[f * nc + n * n + c], acc);
-    }
-    L[size_t(box) * nc + t] += acc;
+  // each frame's rotation back in its own thread, then the three added per slot in frame order
+  double *part = ys + 3 * nc;
+  for (int it = t; it < 3 * nc; it += kPwLocalThreads) {
+    const int f = it / nc, k = it % nc, n = degree_of(k), w = 2 * n + 1, r = k - n * n;
+    const double *blk = a.bwd + size_t(f) * a.rb + n * (2 * n - 1) * (2 * n + 1) / 3;
+    part[it] = column_dot(blk + r, w, ys + f * nc + n * n);
   }
+  __syncthreads();
+  if (t < nc) L[size_t(box) * nc + t] += (part[t] + part[nc + t]) + part[2 * nc + t];
 }
 
 }  // namespace
