@@ -562,6 +562,7 @@ def nonuniform(args):
     rot = run("mixed", "rotation")
     dense = run("mixed", "dense")
     plain = run("f64", "dense")
+    pw = run("mixed", "plane_wave")
     line = {
         "metric": "non-uniform Laplace FMM Mpts/s (2M sphere + 1M Plummer, p = 10, rotation M2L f32)",
         "value": rot["value"], "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -570,7 +571,7 @@ def nonuniform(args):
         "config": {"workload": "nonuniform-sphere-plummer", "n": n, "p": p, "leaf_points": leaf, "max_level": 16,
                    "near_rule": near, "leaf_levels": hist, "wx_pairs": wx, "levels": int(plan.nlevels)},
         "errors": rot["errors"], "kind_ms": rot["kind_ms"],
-        "mixed_dense_bf16x3": dense, "plain_f64_dense": plain,
+        "mixed_dense_bf16x3": dense, "plain_f64_dense": plain, "mixed_plane_wave_f64": pw,
     }
     print(json.dumps(line), flush=True)
 
