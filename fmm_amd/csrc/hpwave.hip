@@ -11,10 +11,9 @@
 //     waves[box][dir][re, im][W].
 //   the merge: k_pw_merge of pwave.hip (the accumulators and child shifts are complex in
 //     both families; W in place of H).
-//   k_hpw_local: one 512-thread block per target box, frame by frame: its up and down
-//     incoming waves staged in LDS, the sums over the wave values per (up/down, node,
-//     order; cos and sin m a_j by a recurrence in j), the frame's local per (n, m) from
-//     both, the rotation back per (plane, slot), summed over the frames and added to the
+//   k_hpw_local: one 512-thread block per target box, frame by frame: its up, then its
+//     down incoming waves staged in LDS and summed over the wave values per (node, order;
+//     cos and sin m a_j by a recurrence in j), the frame's local per (n, m) from both, the rotation back per (plane, slot), summed over the frames and added to the
 //     box's local (planes 0 and 1).
 // f64 throughout (the reference's exponential M2L is f64 only).
 #include <hip/hip_runtime.h>
@@ -133,8 +132,8 @@ __global__ __launch_bounds__(kHpwThreads) void k_hpw_local(HpwArgs a, uint32_t n
                                                            const double *__restrict__ inc, double *__restrict__ L) {
   extern __shared__ double sm[];
   const int p = a.p, P1 = p + 1, nc = P1 * P1, W = a.W, O = a.orders, s = a.nodes;
-  double *V = sm;              // the frame's up then down waves [2][re, im][W]
-  double *g = V + 4 * W;       // [up/down][order][cos re, cos im, sin re, sin im]
+  double *V = sm;              // one direction's waves [re, im][W]
+  double *g = V + 2 * W;       // [up/down][order][cos re, cos im, sin re, sin im]
   double *ys = g + 8 * O;      // the frame's local [plane][nc]
   const uint32_t row = xcd_index(ntargets);
   if (row >= ntargets) return;
@@ -142,14 +141,16 @@ __global__ __launch_bounds__(kHpwThreads) void k_hpw_local(HpwArgs a, uint32_t n
   const uint32_t box = target[row];
   const double *in = inc + size_t(box - box0) * 12 * W;
   double acc = 0.0;  // plane t / nc, slot t % nc
-  for (int f = 0; f < 3; ++f) {
-    for (int q = t; q < 4 * W; q += kHpwThreads) V[q] = in[size_t(2 * f) * 2 * W + q];
+  for (int f = 0; f < 3; ++f)
+  for (int ud = 0; ud < 2; ++ud) {
+    // one direction's waves at a time (half the LDS of both: two blocks per CU)
+    for (int q = t; q < 2 * W; q += kHpwThreads) V[q] = in[size_t(2 * f + ud) * 2 * W + q];
     __syncthreads();
     // the sums over the wave values of V cos m a_j and V sin m a_j
-    for (int it = t; it < 2 * a.nitem; it += kHpwThreads) {
-      const int ud = it % 2, q = a.item[it / 2], k = q / P1, m = q % P1;
+    for (int it = t; it < a.nitem; it += kHpwThreads) {
+      const int q = a.item[it], k = q / P1, m = q % P1;
       const int N = a.len[k], st = a.start[k];
-      const double *vr = V + ud * 2 * W + st, *vi = vr + W;
+      const double *vr = V + st, *vi = vr + W;
       // cos and sin of m a_j by the recurrence in j from the step m a_1 (the table's row m)
       const double cd = N > 1 ? a.cosm[a.trig[k] + m * N + 1] : 1.0, sd = N > 1 ? a.sinm[a.trig[k] + m * N + 1] : 0.0;
       double c = 1.0, sv = 0.0;
@@ -170,6 +171,7 @@ __global__ __launch_bounds__(kHpwThreads) void k_hpw_local(HpwArgs a, uint32_t n
       o[3] = si;
     }
     __syncthreads();
+    if (ud == 0) continue;
     // the frame's local: slot +m from the cos sums, -m from the sin sums, up and down
     const int tri = P1 * (P1 + 1) / 2;
     for (int it = t; it < tri; it += kHpwThreads) {
@@ -224,7 +226,7 @@ size_t hpw_outgoing_lds(int p, int orders) {
 
 size_t hpw_local_lds(int p, int W, int orders) {
   const size_t nc = size_t(p + 1) * (p + 1);
-  return sizeof(double) * (4 * size_t(W) + 8 * size_t(orders) + 2 * nc);
+  return sizeof(double) * (2 * size_t(W) + 8 * size_t(orders) + 2 * nc);
 }
 
 hipError_t hpw_allow_lds(size_t outgoing, size_t local) {
